@@ -1,0 +1,243 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) for the tests.
+
+The oracle is test infrastructure: it is the checker the HIP path is compared
+against, never a fallback for it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+LIB_PATH = ROOT / "oracle" / "liboracle.so"
+
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+
+class ScaleInfo(C.Structure):
+    _fields_ = [("orig_w", C.c_int), ("orig_h", C.c_int), ("work_w", C.c_int), ("work_h", C.c_int),
+                ("sx", C.c_double), ("sy", C.c_double), ("D", C.c_double * 9)]
+
+
+class Top2(C.Structure):
+    _fields_ = [("first_distance", C.c_double), ("second_distance", C.c_double),
+                ("first_index", C.c_int), ("second_index", C.c_int)]
+
+
+class Point(C.Structure):
+    _fields_ = [("ax", C.c_double), ("ay", C.c_double), ("bx", C.c_double), ("by", C.c_double)]
+
+
+class HomographyResult(C.Structure):
+    _fields_ = [("H", C.c_double * 9), ("H_s", C.c_double * 9), ("num_matches", C.c_int), ("num_inliers", C.c_int),
+                ("inlier_ratio", C.c_double), ("inlier_coverage", C.c_double), ("reproj_residual", C.c_double),
+                ("status", C.c_int)]
+
+
+class Model(C.Structure):
+    _fields_ = [("h", C.c_double * 9), ("loss", C.c_double), ("hypothesis", C.c_int), ("valid", C.c_int)]
+
+
+class S5State(C.Structure):
+    _fields_ = [("best", Model), ("evaluated", C.c_int), ("status", C.c_int)]
+
+
+class Geometry(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("origin_x", C.c_int), ("origin_y", C.c_int),
+                ("width_a", C.c_int), ("height_a", C.c_int), ("width_b", C.c_int), ("height_b", C.c_int),
+                ("denominator_tolerance", C.c_double), ("snap_tolerance", C.c_double),
+                ("inverse", C.c_double * 9)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            subprocess.run(["make", "-s", "-C", str(LIB_PATH.parent)], check=True)
+        L = C.CDLL(str(LIB_PATH))
+        L.orc_make_scale_info.argtypes = [C.c_int, C.c_int, C.c_double, C.POINTER(ScaleInfo)]
+        L.orc_s1_preprocess.argtypes = [u8p, C.c_int, C.c_int, C.c_int, C.c_double, u8p, C.c_int,
+                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_s1_gradient.argtypes = [u8p, C.c_int, C.c_int, C.c_int]
+        L.orc_s1_gradient.restype = C.c_double
+        L.orc_fast_score.argtypes = [u8p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_harris_response.argtypes = [u8p, C.c_int, C.c_int, C.c_int, C.c_float]
+        L.orc_harris_response.restype = C.c_float
+        L.orc_s2_detect.argtypes = [u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                                    f32p, f32p, f32p, C.c_void_p, C.c_void_p]
+        L.orc_s3_blur.argtypes = [u8p, C.c_int, C.c_int, C.c_int, u8p]
+        L.orc_fast_atan2.argtypes = [C.c_float, C.c_float]
+        L.orc_fast_atan2.restype = C.c_float
+        L.orc_s3_describe.argtypes = [u8p, C.c_int, C.c_int, C.c_int, f32p, f32p, C.c_int, C.c_void_p, u8p]
+        L.orc_s4_top2.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                  C.POINTER(Top2)]
+        L.orc_s4_match.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                   C.c_double, C.c_int, i32p, i32p, f32p, C.c_void_p]
+        L.orc_s5_random_word.argtypes = [C.c_uint64]
+        L.orc_s5_random_word.restype = C.c_uint64
+        L.orc_s5_sample_four.argtypes = [C.c_int, C.c_uint64, C.c_int, i32p]
+        L.orc_s5_nondegenerate.argtypes = [C.POINTER(Point)]
+        L.orc_s5_fit.argtypes = [C.c_int, C.POINTER(Point), C.c_void_p, C.POINTER(C.c_double)]
+        L.orc_s5_normalize.argtypes = [C.POINTER(C.c_double)]
+        L.orc_s5_residual.argtypes = [C.POINTER(C.c_double), Point]
+        L.orc_s5_residual.restype = C.c_double
+        L.orc_s5_valid_domain.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int]
+        L.orc_s5_tree_loss.argtypes = [C.c_int, C.POINTER(Point), C.POINTER(C.c_double), C.c_double]
+        L.orc_s5_tree_loss.restype = C.c_double
+        L.orc_s5_serial_loss.argtypes = [C.c_int, C.POINTER(Point), C.POINTER(C.c_double), C.c_double]
+        L.orc_s5_serial_loss.restype = C.c_double
+        L.orc_s5_estimate.argtypes = [C.c_int, C.POINTER(Point), C.POINTER(ScaleInfo), C.POINTER(ScaleInfo),
+                                      C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, C.POINTER(S5State),
+                                      u8p, C.POINTER(HomographyResult)]
+        L.orc_s5_estimate_kp.argtypes = [f32p, f32p, C.c_int, C.c_int, f32p, f32p, C.c_int, C.c_int, i32p, i32p,
+                                         C.c_int, C.c_int, C.POINTER(ScaleInfo), C.POINTER(ScaleInfo), C.c_int,
+                                         C.c_int, C.c_int, C.c_double, C.c_uint64, C.POINTER(S5State), u8p,
+                                         C.POINTER(HomographyResult)]
+        L.orc_s6_prepare_geometry.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_double,
+                                              C.c_double, C.c_double, C.c_ulonglong, C.POINTER(Geometry)]
+        L.orc_s6_effective_reductions.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.orc_s6_render.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(Geometry), C.c_int, u8p, u8p,
+                                    C.POINTER(C.c_ulonglong)]
+        _lib = L
+    return _lib
+
+
+# ---------------------------------------------------------------- wrappers
+
+def scale_info(w, h, s):
+    si = ScaleInfo()
+    st = lib().orc_make_scale_info(w, h, s, C.byref(si))
+    return st, si
+
+
+def s1_preprocess(rgb: np.ndarray, scale: float) -> np.ndarray:
+    h, w, _ = rgb.shape
+    st, si = scale_info(w, h, scale)
+    assert st == 0
+    out = np.zeros((si.work_h, si.work_w), np.uint8)
+    ww, wh = C.c_int(), C.c_int()
+    assert lib().orc_s1_preprocess(np.ascontiguousarray(rgb), w, h, 3 * w, scale, out, si.work_w,
+                                   C.byref(ww), C.byref(wh)) == 0
+    return out
+
+
+def s1_gradient(gray: np.ndarray) -> float:
+    h, w = gray.shape
+    return lib().orc_s1_gradient(np.ascontiguousarray(gray), w, h, w)
+
+
+def s2_detect(gray: np.ndarray, fast_threshold: int, k: int, harris_k: float = 0.04, maps=False):
+    h, w = gray.shape
+    xs = np.zeros(max(k, 1), np.float32)
+    ys = np.zeros_like(xs)
+    rs = np.zeros_like(xs)
+    scores = np.zeros((h, w), np.uint8) if maps else None
+    maxima = np.zeros((h, w), np.uint8) if maps else None
+    n = lib().orc_s2_detect(np.ascontiguousarray(gray), w, h, w, fast_threshold, k, harris_k, xs, ys, rs,
+                            scores.ctypes.data if maps else None, maxima.ctypes.data if maps else None)
+    assert n >= 0
+    kp = np.stack([xs[:n], ys[:n], rs[:n]], axis=1)
+    return (kp, scores, maxima) if maps else kp
+
+
+def s3_blur(gray: np.ndarray) -> np.ndarray:
+    h, w = gray.shape
+    out = np.zeros((h, w), np.uint8)
+    lib().orc_s3_blur(np.ascontiguousarray(gray), w, h, w, out)
+    return out
+
+
+def s3_describe(gray: np.ndarray, kp: np.ndarray) -> np.ndarray:
+    h, w = gray.shape
+    n = kp.shape[0]
+    out = np.zeros((max(n, 1), 32), np.uint8)
+    xs = np.ascontiguousarray(kp[:, 0], np.float32) if n else np.zeros(1, np.float32)
+    ys = np.ascontiguousarray(kp[:, 1], np.float32) if n else np.zeros(1, np.float32)
+    lib().orc_s3_describe(np.ascontiguousarray(gray), w, h, w, xs, ys, n, None, out)
+    return out[:n]
+
+
+def s4_top2(da: np.ndarray, db: np.ndarray):
+    binary = da.dtype == np.uint8
+    na, dim = da.shape
+    nb = db.shape[0]
+    esz = 1 if binary else 4
+    out = (Top2 * max(na, 1))()
+    da = np.ascontiguousarray(da)
+    db = np.ascontiguousarray(db)
+    lib().orc_s4_top2(int(binary), da.ctypes.data, na, dim * esz, db.ctypes.data, nb, dim * esz, dim, out)
+    return [(out[i].first_distance, out[i].second_distance, out[i].first_index, out[i].second_index)
+            for i in range(na)]
+
+
+def s4_match(da: np.ndarray, db: np.ndarray, ratio: float, cross_check: bool):
+    binary = da.dtype == np.uint8
+    na, dim = da.shape
+    nb = db.shape[0]
+    esz = 1 if binary else 4
+    ia = np.zeros(max(na, 1), np.int32)
+    ib = np.zeros_like(ia)
+    dist = np.zeros(max(na, 1), np.float32)
+    da = np.ascontiguousarray(da)
+    db = np.ascontiguousarray(db)
+    n = lib().orc_s4_match(int(binary), da.ctypes.data, na, dim * esz, db.ctypes.data, nb, dim * esz, dim,
+                           ratio, int(cross_check), ia, ib, dist, None)
+    return ia[:n], ib[:n], dist[:n]
+
+
+def points(arr: np.ndarray):
+    """arr (n, 4) float64 [ax, ay, bx, by] -> ctypes Point array."""
+    n = arr.shape[0]
+    pts = (Point * max(n, 1))()
+    for i in range(n):
+        pts[i] = Point(*map(float, arr[i]))
+    return pts
+
+
+def s5_estimate_kp(ax, ay, bx, by, idx_a, idx_b, sa, sb, hypotheses, refinement_steps, threshold, seed,
+                   previous_hypotheses=0, state=None, counts=None, caps=None):
+    ax, ay, bx, by = (np.ascontiguousarray(v, np.float32) for v in (ax, ay, bx, by))
+    idx_a = np.ascontiguousarray(idx_a, np.int32)
+    idx_b = np.ascontiguousarray(idx_b, np.int32)
+    na, nb, nm = counts if counts else (len(ax), len(bx), len(idx_a))
+    ca, cb, cm = caps if caps else (len(ax), len(bx), len(idx_a))
+    state = state if state is not None else S5State()
+    mask = np.zeros(max(cm, 1), np.uint8)
+    res = HomographyResult()
+    lib().orc_s5_estimate_kp(ax if len(ax) else np.zeros(1, np.float32), ay if len(ay) else np.zeros(1, np.float32),
+                             na, ca, bx if len(bx) else np.zeros(1, np.float32),
+                             by if len(by) else np.zeros(1, np.float32), nb, cb,
+                             idx_a if len(idx_a) else np.zeros(1, np.int32),
+                             idx_b if len(idx_b) else np.zeros(1, np.int32), nm, cm, C.byref(sa), C.byref(sb),
+                             hypotheses, previous_hypotheses, refinement_steps, threshold, seed, C.byref(state),
+                             mask, C.byref(res))
+    return res, mask[:cm], state
+
+
+def prepare_geometry(wa, ha, wb, hb, h, den_tol=1e-12, inv_tol=1e-12, snap_tol=1e-10, max_pixels=16777216):
+    g = Geometry()
+    harr = (C.c_double * 9)(*[float(v) for v in np.asarray(h, np.float64).ravel()])
+    st = lib().orc_s6_prepare_geometry(wa, ha, wb, hb, harr, den_tol, inv_tol, snap_tol, max_pixels, C.byref(g))
+    return st, g
+
+
+MODES = {"feather": 0, "multiband_3": 1, "multiband_5": 2, "multiband_6": 3, "multiband_7": 4}
+
+
+def s6_render(a: np.ndarray, b: np.ndarray, g: Geometry, mode: int):
+    out = np.zeros((g.height, g.width, 3), np.uint8)
+    mask = np.zeros((g.height, g.width), np.uint8)
+    stats = (C.c_ulonglong * 2)()
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    st = lib().orc_s6_render(a, a.shape[1] * 3, b, b.shape[1] * 3, C.byref(g), mode, out, mask, stats)
+    assert st == 0
+    return out, mask, (stats[0], stats[1])
