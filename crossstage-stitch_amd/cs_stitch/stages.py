@@ -1,0 +1,231 @@
+"""Python mirror of the stage APIs (S1..S6) over the C ABI.
+
+Argument meaning, status codes and ownership follow the reference's stage
+headers (src/s1/s1.hpp, src/s2_s3/s2.hpp, src/s2_s3/s3.hpp, src/s4/s4.hpp,
+src/s5/s5.hpp, src/s6/s6.hpp): the caller owns device buffers, workspaces
+are queried then supplied, work is enqueued on the current stream and
+counts stay on the device. torch tensors are used only as device memory.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import torch
+
+from . import _ffi as F
+
+
+class StageError(RuntimeError):
+    def __init__(self, what: str, status: int):
+        super().__init__(f"{what} failed: {F.load().cs_status_name(status).decode()} ({status})")
+        self.status = status
+
+
+def _check(status: int, what: str) -> None:
+    if status != F.OK:
+        raise StageError(what, status)
+
+
+def stream_handle(stream: torch.cuda.Stream | None = None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def image(t: torch.Tensor, fmt: int | None = None) -> F.Image:
+    """View of a uint8 device tensor (H, W, 3) RGB or (H, W) gray; rows may be pitched."""
+    assert t.dtype == torch.uint8 and t.is_cuda
+    if t.dim() == 3:
+        assert t.shape[2] == 3 and t.stride(2) == 1 and t.stride(1) == 3
+        return F.Image(t.data_ptr(), t.shape[1], t.shape[0], t.stride(0), F.RGB8 if fmt is None else fmt)
+    assert t.dim() == 2 and t.stride(1) == 1
+    return F.Image(t.data_ptr(), t.shape[1], t.shape[0], t.stride(0), F.GRAY8 if fmt is None else fmt)
+
+
+def pitched(h: int, w: int, channels: int, pad: int = 64, device="cuda") -> torch.Tensor:
+    """Device image with row pitch w*channels + pad (the reference harness pads rows by 64 B)."""
+    base = torch.zeros((h, w * channels + pad), dtype=torch.uint8, device=device)
+    v = base[:, : w * channels]
+    return v.view(h, w, channels) if channels == 3 else v
+
+
+def workspace(nbytes: int, device="cuda") -> torch.Tensor:
+    """256-byte aligned device workspace (torch's allocator aligns to at least 512 B)."""
+    t = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+    assert t.data_ptr() % 256 == 0
+    return t
+
+
+def scale_info(w: int, h: int, s: float) -> F.ScaleInfo:
+    si = F.ScaleInfo()
+    _check(F.load().cs_make_scale_info(w, h, s, C.byref(si)), "make_scale_info")
+    return si
+
+
+# ---------------------------------------------------------------- S1
+
+def s1_preprocess(a: torch.Tensor, b: torch.Tensor, scale: float, pad: int = 64):
+    L = F.load()
+    sa, sb = scale_info(a.shape[1], a.shape[0], scale), scale_info(b.shape[1], b.shape[0], scale)
+    ga, gb = pitched(sa.work_h, sa.work_w, 1, pad), pitched(sb.work_h, sb.work_w, 1, pad)
+    ia, ib, oa, ob = image(a), image(b), image(ga), image(gb)
+    _check(L.cs_s1_preprocess(C.byref(ia), C.byref(ib), scale, C.byref(oa), C.byref(ob), C.byref(sa), C.byref(sb),
+                              stream_handle()), "s1_preprocess")
+    return ga, gb, sa, sb
+
+
+def s1_preprocess_single(a: torch.Tensor, scale: float, pad: int = 64):
+    L = F.load()
+    si = scale_info(a.shape[1], a.shape[0], scale)
+    g = pitched(si.work_h, si.work_w, 1, pad)
+    ia, og = image(a), image(g)
+    _check(L.cs_s1_preprocess_single(C.byref(ia), scale, C.byref(og), C.byref(si), stream_handle()),
+           "s1_preprocess_single")
+    return g, si
+
+
+def s1_compute_gradient(ga: torch.Tensor, gb: torch.Tensor):
+    L = F.load()
+    ws = workspace(32)
+    pinned = torch.zeros(2, dtype=torch.float64, pin_memory=True)
+    ia, ib = image(ga), image(gb)
+    _check(L.cs_s1_compute_gradient(C.byref(ia), C.byref(ib), ws.data_ptr(), pinned.data_ptr(),
+                                    pinned.data_ptr() + 8, stream_handle()), "s1_compute_gradient")
+    torch.cuda.current_stream().synchronize()
+    return float(pinned[0]), float(pinned[1])
+
+
+# ---------------------------------------------------------------- S2 / S3
+
+@dataclass
+class KeypointSet:
+    x: torch.Tensor
+    y: torch.Tensor
+    response: torch.Tensor | None
+    count: torch.Tensor  # device int32[1]
+
+    @property
+    def capacity(self) -> int:
+        return self.x.numel()
+
+    def c(self, capacity: int | None = None) -> F.Keypoints:
+        cap = self.capacity if capacity is None else capacity
+        return F.Keypoints(self.x.data_ptr(), self.y.data_ptr(),
+                           self.response.data_ptr() if self.response is not None else None, cap,
+                           self.count.data_ptr())
+
+    @staticmethod
+    def empty(capacity: int, with_response: bool = True) -> "KeypointSet":
+        n = max(capacity, 1)
+        return KeypointSet(torch.zeros(n, dtype=torch.float32, device="cuda"),
+                           torch.zeros(n, dtype=torch.float32, device="cuda"),
+                           torch.zeros(n, dtype=torch.float32, device="cuda") if with_response else None,
+                           torch.zeros(1, dtype=torch.int32, device="cuda"))
+
+    def host(self):
+        n = int(self.count.item())
+        return n, self.x[:n].cpu(), self.y[:n].cpu(), (self.response[:n].cpu() if self.response is not None else None)
+
+
+def s2_detect(gray: torch.Tensor, fast_threshold: int, max_keypoints: int, harris_k: float = 0.04,
+              out: KeypointSet | None = None, ws: torch.Tensor | None = None) -> KeypointSet:
+    L = F.load()
+    cfg = F.S2Config(fast_threshold, max_keypoints, harris_k)
+    nbytes = C.c_size_t()
+    _check(L.cs_s2_workspace_bytes(gray.shape[1], gray.shape[0], C.byref(cfg), C.byref(nbytes)), "s2_workspace_bytes")
+    ws = ws if ws is not None else workspace(nbytes.value)
+    out = out if out is not None else KeypointSet.empty(max_keypoints)
+    img, kp = image(gray), out.c(max_keypoints)
+    _check(L.cs_s2_detect(C.byref(img), C.byref(cfg), C.byref(kp), ws.data_ptr(), ws.numel(), stream_handle()),
+           "s2_detect")
+    return out
+
+
+def s3_describe(gray: torch.Tensor, kp: KeypointSet, capacity: int | None = None, stride: int = 32,
+                ws: torch.Tensor | None = None) -> torch.Tensor:
+    L = F.load()
+    cap = kp.capacity if capacity is None else capacity
+    nbytes = C.c_size_t()
+    _check(L.cs_s3_workspace_bytes(gray.shape[1], gray.shape[0], C.byref(nbytes)), "s3_workspace_bytes")
+    ws = ws if ws is not None else workspace(nbytes.value)
+    desc = torch.zeros((max(cap, 1), stride), dtype=torch.uint8, device="cuda")
+    d = F.Descriptors(F.BINARY_U8, 32, stride, desc.data_ptr())
+    img, k = image(gray), kp.c(cap)
+    _check(L.cs_s3_describe(C.byref(img), C.byref(k), C.byref(d), ws.data_ptr(), ws.numel(), stream_handle()),
+           "s3_describe")
+    return desc
+
+
+# ---------------------------------------------------------------- S4
+
+def descriptors(t: torch.Tensor) -> F.Descriptors:
+    """(rows, dim) uint8 (binary) or float32 device tensor with unit column stride."""
+    assert t.is_cuda and t.dim() == 2 and t.stride(1) == 1
+    if t.dtype == torch.uint8:
+        return F.Descriptors(F.BINARY_U8, t.shape[1], t.stride(0), t.data_ptr())
+    assert t.dtype == torch.float32
+    return F.Descriptors(F.FLOAT32, t.shape[1], t.stride(0) * 4, t.data_ptr())
+
+
+@dataclass
+class MatchSet:
+    idx_a: torch.Tensor
+    idx_b: torch.Tensor
+    distance: torch.Tensor
+    count: torch.Tensor
+
+    def c(self) -> F.Matches:
+        return F.Matches(self.idx_a.data_ptr(), self.idx_b.data_ptr(), self.distance.data_ptr(), self.idx_a.numel(),
+                         self.count.data_ptr())
+
+    @staticmethod
+    def empty(capacity: int) -> "MatchSet":
+        n = max(capacity, 1)
+        return MatchSet(torch.full((n,), -7, dtype=torch.int32, device="cuda"),
+                        torch.full((n,), -7, dtype=torch.int32, device="cuda"),
+                        torch.zeros(n, dtype=torch.float32, device="cuda"),
+                        torch.zeros(1, dtype=torch.int32, device="cuda"))
+
+    def host(self):
+        n = int(self.count.item())
+        return self.idx_a[:n].cpu(), self.idx_b[:n].cpu(), self.distance[:n].cpu()
+
+
+def _kp_view(count: torch.Tensor, capacity: int) -> F.Keypoints:
+    return F.Keypoints(None, None, None, capacity, count.data_ptr())
+
+
+def s4_match(da: torch.Tensor, count_a: torch.Tensor, db: torch.Tensor, count_b: torch.Tensor, ratio: float,
+             cross_check: bool, capacity_a: int | None = None, capacity_b: int | None = None,
+             out: MatchSet | None = None, ws: torch.Tensor | None = None):
+    L = F.load()
+    ca = da.shape[0] if capacity_a is None else capacity_a
+    cb = db.shape[0] if capacity_b is None else capacity_b
+    cfg = F.S4Config(ratio, int(cross_check))
+    nbytes = C.c_size_t()
+    _check(L.cs_s4_workspace_bytes(ca, cb, C.byref(cfg), C.byref(nbytes)), "s4_workspace_bytes")
+    ws = ws if ws is not None else workspace(nbytes.value)
+    out = out if out is not None else MatchSet.empty(ca)
+    ka, kb = _kp_view(count_a, ca), _kp_view(count_b, cb)
+    ea, eb, m = descriptors(da), descriptors(db), out.c()
+    _check(L.cs_s4_match(C.byref(ka), C.byref(ea), C.byref(kb), C.byref(eb), C.byref(cfg), C.byref(m),
+                         ws.data_ptr(), ws.numel(), stream_handle()), "s4_match")
+    return out, ws
+
+
+def s4_knn_top2(da: torch.Tensor, count_a: torch.Tensor, db: torch.Tensor, count_b: torch.Tensor):
+    L = F.load()
+    ca, cb = da.shape[0], db.shape[0]
+    out = torch.zeros((max(ca, 1), 3), dtype=torch.float64, device="cuda")  # 24-byte Top2 records
+    ka, kb = _kp_view(count_a, ca), _kp_view(count_b, cb)
+    ea, eb = descriptors(da), descriptors(db)
+    _check(L.cs_s4_knn_top2(C.byref(ka), C.byref(ea), C.byref(kb), C.byref(eb), out.data_ptr(), stream_handle()),
+           "s4_knn_top2")
+    return out
+
+
+def top2_records(t: torch.Tensor, n: int):
+    """Decodes n 24-byte Top2 records from a device tensor to Python tuples."""
+    raw = t.reshape(-1)[: 3 * n].cpu().numpy().tobytes()
+    arr = (F.Top2 * n).from_buffer_copy(raw)
+    return [(r.first_distance, r.second_distance, r.first_index, r.second_index) for r in arr]

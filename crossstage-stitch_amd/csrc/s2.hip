@@ -1,0 +1,518 @@
+// S2: single-level ORB detection for any number of images per launch.
+// FAST-9/16 score -> strict 3x3 maxima inside the 31-px margin -> the 2K
+// highest FAST scores (ties to the smaller row-major index) -> Harris response
+// (integer sums, FP32 without contraction) -> the K highest responses (ties to
+// the smaller index). Semantics: src/s2_s3/s2.cu:39-387 and
+// tools/s2_s3/reference.py:50-147.
+//
+// MI355X layout: FAST and NMS run fused on LDS tiles with halos; the
+// pre-selection is an ordered compaction with wave ballots; the top-K order
+// is a rank count split over many workgroups instead of one O(n^2) block.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+
+#include "stitch/jobs.hpp"
+
+namespace stitch {
+namespace {
+
+using jobs::S2Job;
+using jobs::SelectParams;
+using jobs::kS2Chunk;
+using jobs::kS2RankKeys;
+using jobs::kS2RankSegment;
+using jobs::kS2TileH;
+using jobs::kS2TileW;
+
+constexpr int kEdge = s2::kEdge;
+constexpr int kHalo = 4;  // 3 for the FAST circle + 1 for the NMS neighbourhood
+constexpr int kGrayW = kS2TileW + 2 * kHalo, kGrayH = kS2TileH + 2 * kHalo;
+constexpr int kScoreW = kS2TileW + 2, kScoreH = kS2TileH + 2;
+
+__constant__ int kCircleX[16] = {0, 1, 2, 3, 3, 3, 2, 1, 0, -1, -2, -3, -3, -3, -2, -1};
+__constant__ int kCircleY[16] = {3, 3, 2, 1, 0, -1, -2, -3, -3, -3, -2, -1, 0, 1, 2, 3};
+
+__global__ void s2_init(dev::JobSpan<S2Job> span) {
+    const S2Job& j = dev::job_at(span, blockIdx.x);
+    if (!j.active) {
+        if (threadIdx.x == 0) *j.d_count = 0;
+        return;
+    }
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) j.hist[i] = 0;
+    for (int i = threadIdx.x; i < j.candidates; i += blockDim.x) j.ranks[i] = 0;
+}
+
+// cornerScore<16> on an LDS tile; g points at the centre pixel, row stride kGrayW.
+__device__ int fast_score_lds(const std::uint8_t* g, int threshold) {
+    const int v = g[0];
+    int brighter = 0, darker = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k += 4) {
+        const int p = g[kCircleY[k] * kGrayW + kCircleX[k]];
+        brighter += p > v + threshold;
+        darker += p < v - threshold;
+    }
+    if (brighter < 2 && darker < 2) return 0;
+    int d[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) d[k] = v - g[kCircleY[k] * kGrayW + kCircleX[k]];
+    int best = INT_MIN;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        int lo = INT_MAX, hi = INT_MIN;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const int e = d[(k + j) & 15];
+            lo = min(lo, e);
+            hi = max(hi, e);
+        }
+        best = max(best, max(lo, -hi));
+    }
+    return best > threshold ? best - 1 : 0;
+}
+
+// FAST + NMS + histogram on 64x16 tiles with a 4-pixel gray halo in LDS.
+__global__ void __launch_bounds__(256) s2_fast_nms(dev::JobSpan<S2Job> span, long long total) {
+    __shared__ std::uint8_t gray[kGrayH * kGrayW];
+    __shared__ std::uint8_t score[kScoreH * kScoreW];
+    __shared__ std::uint32_t hist[256];
+    const int tid = threadIdx.x;
+    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+        const int ji = dev::find_job(span, t, &S2Job::fast_begin);
+        const S2Job& j = dev::job_at(span, ji);
+        const long long local = t - j.fast_begin;
+        const int x0 = static_cast<int>(local % j.tiles_x) * kS2TileW;
+        const int y0 = static_cast<int>(local / j.tiles_x) * kS2TileH;
+        hist[tid] = 0;
+        for (int i = tid; i < kGrayH * kGrayW; i += 256) {
+            const int gx = x0 - kHalo + i % kGrayW, gy = y0 - kHalo + i / kGrayW;
+            gray[i] = (gx >= 0 && gx < j.w && gy >= 0 && gy < j.h) ? j.img[static_cast<size_t>(gy) * j.pitch + gx] : 0;
+        }
+        __syncthreads();
+        for (int i = tid; i < kScoreH * kScoreW; i += 256) {
+            const int sx = i % kScoreW, sy = i / kScoreW;
+            const int gx = x0 - 1 + sx, gy = y0 - 1 + sy;
+            int s = 0;
+            if (gx >= 3 && gx < j.w - 3 && gy >= 3 && gy < j.h - 3)
+                s = fast_score_lds(&gray[(sy + kHalo - 1) * kGrayW + sx + kHalo - 1], j.threshold);
+            score[i] = static_cast<std::uint8_t>(s);
+        }
+        __syncthreads();
+        for (int i = tid; i < kS2TileW * kS2TileH; i += 256) {
+            const int lx = i % kS2TileW, ly = i / kS2TileW;
+            const int gx = x0 + lx, gy = y0 + ly;
+            if (gx >= j.w || gy >= j.h) continue;
+            const std::uint8_t* c = &score[(ly + 1) * kScoreW + lx + 1];
+            const int s = c[0];
+            bool keep = s > 0 && gx >= kEdge && gx < j.w - kEdge && gy >= kEdge && gy < j.h - kEdge;
+            if (keep)
+                keep = s > c[-kScoreW - 1] && s > c[-kScoreW] && s > c[-kScoreW + 1] && s > c[-1] && s > c[1] &&
+                       s > c[kScoreW - 1] && s > c[kScoreW] && s > c[kScoreW + 1];
+            j.maxima[static_cast<size_t>(gy) * j.w + gx] = keep ? static_cast<std::uint8_t>(s) : 0;
+            if (keep) atomicAdd(&hist[s], 1u);
+        }
+        __syncthreads();
+        if (hist[tid]) atomicAdd(&j.hist[tid], hist[tid]);
+        __syncthreads();
+    }
+}
+
+// One wave per job: suffix[s] = #maxima with score >= s; the boundary score
+// of the 2K pre-selection is the largest s with suffix[s] >= 2K.
+__global__ void __launch_bounds__(64) s2_select(dev::JobSpan<S2Job> span) {
+    const S2Job& j = dev::job_at(span, blockIdx.x);
+    if (!j.active) return;
+    __shared__ std::uint32_t suffix[257];
+    const int lane = threadIdx.x;
+    std::uint32_t h[4], sum = 0;
+    for (int k = 0; k < 4; ++k) {
+        const int s = 4 * lane + k;
+        h[k] = s ? j.hist[s] : 0;
+        sum += h[k];
+    }
+    // Inclusive suffix scan of the per-lane sums over lanes (lane 63 is the top).
+    std::uint32_t incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const std::uint32_t o = __shfl_down(incl, off, 64);
+        if (lane + off < 64) incl += o;
+    }
+    std::uint32_t run = incl - sum;  // sum over higher lanes
+    for (int k = 3; k >= 0; --k) {
+        run += h[k];
+        suffix[4 * lane + k] = run;
+    }
+    if (lane == 0) suffix[256] = 0;
+    __syncthreads();
+    const std::uint32_t cap = static_cast<std::uint32_t>(j.candidates);
+    const std::uint32_t total = suffix[1];
+    for (int k = 0; k < 4; ++k) {
+        const int s = 4 * lane + k;
+        SelectParams p{};
+        bool writer = false;
+        if (total <= cap) {
+            writer = s == 1;
+            p = {1u, total - suffix[1] + suffix[2], suffix[1] - suffix[2], total};
+        } else if (s >= 1 && suffix[s] >= cap && suffix[s + 1] < cap) {
+            writer = true;
+            p = {static_cast<std::uint32_t>(s), suffix[s + 1], cap - suffix[s + 1], cap};
+        }
+        if (writer) {
+            *j.params = p;
+            *j.d_count = static_cast<int>(min(p.selected, static_cast<std::uint32_t>(j.K)));
+        }
+    }
+}
+
+struct Flags {
+    std::uint32_t above, tie;
+};
+
+// Counts (above, tie) over 16 consecutive pixels starting at p.
+__device__ Flags count16(const std::uint8_t* maxima, long long pixels, long long p, std::uint32_t thr,
+                         std::uint32_t bytes[4]) {
+    Flags f{0, 0};
+    if (p + 16 <= pixels) {
+        const uint4 v = *reinterpret_cast<const uint4*>(maxima + p);
+        bytes[0] = v.x; bytes[1] = v.y; bytes[2] = v.z; bytes[3] = v.w;
+    } else {
+        for (int w = 0; w < 4; ++w) {
+            std::uint32_t word = 0;
+            for (int b = 0; b < 4; ++b) {
+                const long long q = p + 4 * w + b;
+                if (q < pixels) word |= static_cast<std::uint32_t>(maxima[q]) << (8 * b);
+            }
+            bytes[w] = word;
+        }
+    }
+    for (int i = 0; i < 16; ++i) {
+        const std::uint32_t s = (bytes[i >> 2] >> (8 * (i & 3))) & 0xffu;
+        f.above += s > thr;
+        f.tie += (s && s == thr);
+    }
+    return f;
+}
+
+// Block-wide exclusive scan of (above, tie); returns the block totals.
+__device__ Flags block_scan(Flags v, Flags* excl) {
+    __shared__ Flags wave_tot[4];
+    const int lane = dev::lane_id(), wave = threadIdx.x / 64;
+    Flags incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const std::uint32_t a = __shfl_up(incl.above, off, 64), t = __shfl_up(incl.tie, off, 64);
+        if (lane >= off) { incl.above += a; incl.tie += t; }
+    }
+    if (lane == 63) wave_tot[wave] = incl;
+    __syncthreads();
+    Flags base{0, 0}, total{0, 0};
+    for (int w = 0; w < 4; ++w) {
+        if (w < wave) { base.above += wave_tot[w].above; base.tie += wave_tot[w].tie; }
+        total.above += wave_tot[w].above;
+        total.tie += wave_tot[w].tie;
+    }
+    __syncthreads();
+    excl->above = base.above + incl.above - v.above;
+    excl->tie = base.tie + incl.tie - v.tie;
+    return total;
+}
+
+__global__ void __launch_bounds__(256) s2_count(dev::JobSpan<S2Job> span, long long total) {
+    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+        const int ji = dev::find_job(span, t, &S2Job::chunk_begin);
+        const S2Job& j = dev::job_at(span, ji);
+        const long long c = t - j.chunk_begin;
+        const long long pixels = static_cast<long long>(j.w) * j.h;
+        const std::uint32_t thr = j.params->threshold;
+        std::uint32_t bytes[4];
+        const Flags f = count16(j.maxima, pixels, c * kS2Chunk + threadIdx.x * 16, thr, bytes);
+        Flags excl;
+        const Flags tot = block_scan(f, &excl);
+        if (threadIdx.x == 0) {
+            j.chunk_counts[2 * c] = tot.above;
+            j.chunk_counts[2 * c + 1] = tot.tie;
+        }
+    }
+}
+
+// One workgroup per job: exclusive prefix over the job's chunk totals.
+__global__ void __launch_bounds__(256) s2_scan(dev::JobSpan<S2Job> span) {
+    const S2Job& j = dev::job_at(span, blockIdx.x);
+    if (!j.active) return;
+    const int per = (j.chunks + 255) / 256;
+    const int first = threadIdx.x * per;
+    Flags sum{0, 0};
+    for (int i = 0; i < per && first + i < j.chunks; ++i) {
+        sum.above += j.chunk_counts[2 * (first + i)];
+        sum.tie += j.chunk_counts[2 * (first + i) + 1];
+    }
+    Flags run;
+    block_scan(sum, &run);
+    for (int i = 0; i < per && first + i < j.chunks; ++i) {
+        j.chunk_offsets[2 * (first + i)] = run.above;
+        j.chunk_offsets[2 * (first + i) + 1] = run.tie;
+        run.above += j.chunk_counts[2 * (first + i)];
+        run.tie += j.chunk_counts[2 * (first + i) + 1];
+    }
+}
+
+// Writes the pixel index of every pre-selected maximum to its candidate slot:
+// all points above the boundary score, then the first `ties` points at it.
+__global__ void __launch_bounds__(256) s2_scatter(dev::JobSpan<S2Job> span, long long total) {
+    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+        const int ji = dev::find_job(span, t, &S2Job::chunk_begin);
+        const S2Job& j = dev::job_at(span, ji);
+        const long long c = t - j.chunk_begin;
+        const long long pixels = static_cast<long long>(j.w) * j.h;
+        const SelectParams p = *j.params;
+        const long long first = c * kS2Chunk + threadIdx.x * 16;
+        std::uint32_t bytes[4];
+        const Flags f = count16(j.maxima, pixels, first, p.threshold, bytes);
+        Flags rank;
+        block_scan(f, &rank);
+        rank.above += j.chunk_offsets[2 * c];
+        rank.tie += j.chunk_offsets[2 * c + 1];
+        if (f.above | f.tie) {
+            for (int i = 0; i < 16; ++i) {
+                const std::uint32_t s = (bytes[i >> 2] >> (8 * (i & 3))) & 0xffu;
+                const std::uint64_t px = static_cast<std::uint64_t>(first + i);
+                if (s > p.threshold) j.keys[rank.above++] = px;
+                else if (s && s == p.threshold) {
+                    if (rank.tie < p.ties) j.keys[p.above + rank.tie] = px;
+                    ++rank.tie;
+                }
+            }
+        }
+    }
+}
+
+__device__ std::uint32_t order_bits(float value) {
+    const std::uint32_t u = __float_as_uint(value);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ float from_order_bits(std::uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+// OpenCV HarrisResponses: 7x7 Sobel sums in integers, FP32 formula.
+__device__ float harris_response(const std::uint8_t* image, int pitch, int x0, int y0, float k) {
+    int a = 0, b = 0, c = 0;
+    for (int i = 0; i < 7; ++i) {
+        const std::uint8_t* row = image + static_cast<size_t>(y0 - 3 + i) * pitch + (x0 - 3);
+#pragma unroll
+        for (int jx = 0; jx < 7; ++jx) {
+            const std::uint8_t* q = row + jx;
+            const int ix = (q[1] - q[-1]) * 2 + (q[-pitch + 1] - q[-pitch - 1]) + (q[pitch + 1] - q[pitch - 1]);
+            const int iy = (q[pitch] - q[-pitch]) * 2 + (q[pitch - 1] - q[-pitch - 1]) + (q[pitch + 1] - q[-pitch + 1]);
+            a += ix * ix;
+            b += iy * iy;
+            c += ix * iy;
+        }
+    }
+    const float fa = static_cast<float>(a), fb = static_cast<float>(b), fc = static_cast<float>(c);
+    const float trace = fa + fb;
+    const float det = fa * fb - fc * fc;
+    const float penalty = (k * trace) * trace;
+    return (det - penalty) * 0x1.bb9da2p-52f;
+}
+
+__global__ void __launch_bounds__(256) s2_harris(dev::JobSpan<S2Job> span, long long total) {
+    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+        const int ji = dev::find_job(span, t, &S2Job::cand_begin);
+        const S2Job& j = dev::job_at(span, ji);
+        const int slot = static_cast<int>(t - j.cand_begin) * 256 + threadIdx.x;
+        if (slot >= j.candidates) continue;
+        if (static_cast<std::uint32_t>(slot) >= j.params->selected) {
+            j.keys[slot] = 0;
+            continue;
+        }
+        const auto px = static_cast<std::uint32_t>(j.keys[slot]);
+        const int x = static_cast<int>(px % static_cast<std::uint32_t>(j.w));
+        const int y = static_cast<int>(px / static_cast<std::uint32_t>(j.w));
+        const float r = harris_response(j.img, j.pitch, x, y, j.harris_k);
+        j.keys[slot] = (static_cast<std::uint64_t>(order_bits(r)) << 32) | (0xffffffffu - px);
+    }
+}
+
+// rank(key) = #keys greater. Tile = 64 keys x one 2048-key segment; 4 waves
+// split the segment and the partial counts are added atomically.
+__global__ void __launch_bounds__(256) s2_rank(dev::JobSpan<S2Job> span, long long total) {
+    __shared__ std::uint64_t seg[kS2RankSegment];
+    __shared__ std::uint32_t part[4][kS2RankKeys];
+    const int lane = dev::lane_id(), wave = threadIdx.x / 64;
+    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+        const int ji = dev::find_job(span, t, &S2Job::rank_begin);
+        const S2Job& j = dev::job_at(span, ji);
+        const long long local = t - j.rank_begin;
+        const int segs = (j.candidates + kS2RankSegment - 1) / kS2RankSegment;
+        const int kb = static_cast<int>(local / segs), sg = static_cast<int>(local % segs);
+        const int selected = static_cast<int>(j.params->selected);
+        const int key0 = kb * kS2RankKeys, seg0 = sg * kS2RankSegment;
+        if (key0 >= selected || seg0 >= selected) continue;
+        const int n = min(kS2RankSegment, selected - seg0);
+        for (int i = threadIdx.x; i < n; i += 256) seg[i] = j.keys[seg0 + i];
+        __syncthreads();
+        const int slot = key0 + lane;
+        const std::uint64_t key = slot < selected ? j.keys[slot] : ~0ull;
+        const int per = (n + 3) / 4;
+        const int b = wave * per, e = min(n, b + per);
+        std::uint32_t rank = 0;
+        for (int i = b; i < e; ++i) rank += seg[i] > key;
+        part[wave][lane] = rank;
+        __syncthreads();
+        if (wave == 0 && slot < selected) {
+            const std::uint32_t r = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+            if (r) atomicAdd(&j.ranks[slot], r);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) s2_write(dev::JobSpan<S2Job> span, long long total) {
+    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+        const int ji = dev::find_job(span, t, &S2Job::cand_begin);
+        const S2Job& j = dev::job_at(span, ji);
+        const int slot = static_cast<int>(t - j.cand_begin) * 256 + threadIdx.x;
+        if (slot >= j.candidates || static_cast<std::uint32_t>(slot) >= j.params->selected) continue;
+        const std::uint32_t rank = j.ranks[slot];
+        if (rank >= static_cast<std::uint32_t>(j.K)) continue;
+        const std::uint64_t key = j.keys[slot];
+        const std::uint32_t px = 0xffffffffu - static_cast<std::uint32_t>(key);
+        j.out_x[rank] = static_cast<float>(px % static_cast<std::uint32_t>(j.w));
+        j.out_y[rank] = static_cast<float>(px / static_cast<std::uint32_t>(j.w));
+        if (j.out_resp) j.out_resp[rank] = from_order_bits(static_cast<std::uint32_t>(key >> 32));
+    }
+}
+
+bool valid_image_shape(int w, int h) { return w > 0 && h > 0 && w <= INT_MAX / h; }
+
+bool valid_config(const s2::Config& c) {
+    return c.fast_threshold >= 1 && c.fast_threshold <= 254 && c.max_keypoints >= 0 &&
+           c.max_keypoints <= INT_MAX / s2::kCandidateFactor && std::isfinite(c.harris_k) && c.harris_k >= 0.0f;
+}
+
+std::size_t align256(std::size_t v) { return (v + 255) / 256 * 256; }
+
+}  // namespace
+
+namespace jobs {
+
+// Offsets: maxima, hist, params, chunk_counts, chunk_offsets, keys, ranks.
+std::size_t s2_layout_bytes(int w, int h, int K, std::size_t off[7]) {
+    const std::size_t pixels = static_cast<std::size_t>(w) * h;
+    const std::size_t chunks = (pixels + kS2Chunk - 1) / kS2Chunk;
+    const std::size_t cands = static_cast<std::size_t>(s2::kCandidateFactor) * K;
+    const std::size_t sizes[7] = {pixels + 16, 256 * 4, sizeof(SelectParams), 8 * chunks, 8 * chunks, 8 * cands,
+                                  4 * cands};
+    std::size_t cur = 0;
+    for (int i = 0; i < 7; ++i) {
+        cur = align256(cur);
+        off[i] = cur;
+        cur += sizes[i];
+    }
+    return cur;
+}
+
+Status make_s2_job(const ImageView& image, const s2::Config& cfg, const Keypoints& out, void* ws,
+                   std::size_t ws_bytes, S2Job* job) {
+    if (!image.data || image.format != PixelFormat::GRAY8 || !valid_image_shape(image.width, image.height) ||
+        image.pitch_bytes < image.width || !valid_config(cfg) || !out.d_count ||
+        reinterpret_cast<std::uintptr_t>(out.d_count) % alignof(int) || out.capacity < cfg.max_keypoints ||
+        (cfg.max_keypoints > 0 && (!out.x || !out.y)))
+        return Status::INVALID_ARGUMENT;
+    S2Job j{};
+    j.img = image.data;
+    j.w = image.width;
+    j.h = image.height;
+    j.pitch = image.pitch_bytes;
+    j.threshold = cfg.fast_threshold;
+    j.K = cfg.max_keypoints;
+    j.candidates = s2::kCandidateFactor * cfg.max_keypoints;
+    j.harris_k = cfg.harris_k;
+    j.out_x = out.x;
+    j.out_y = out.y;
+    j.out_resp = out.response;
+    j.d_count = out.d_count;
+    j.active = cfg.max_keypoints > 0 && image.width > 2 * s2::kEdge && image.height > 2 * s2::kEdge;
+    if (j.active) {
+        std::size_t off[7];
+        const std::size_t need = s2_layout_bytes(j.w, j.h, j.K, off);
+        if (!ws || reinterpret_cast<std::uintptr_t>(ws) % 256) return Status::INVALID_ARGUMENT;
+        if (ws_bytes < need) return Status::INSUFFICIENT_WORKSPACE;
+        auto* base = static_cast<std::uint8_t*>(ws);
+        j.maxima = base + off[0];
+        j.hist = reinterpret_cast<std::uint32_t*>(base + off[1]);
+        j.params = reinterpret_cast<SelectParams*>(base + off[2]);
+        j.chunk_counts = reinterpret_cast<std::uint32_t*>(base + off[3]);
+        j.chunk_offsets = reinterpret_cast<std::uint32_t*>(base + off[4]);
+        j.keys = reinterpret_cast<std::uint64_t*>(base + off[5]);
+        j.ranks = reinterpret_cast<std::uint32_t*>(base + off[6]);
+        j.tiles_x = (j.w + kS2TileW - 1) / kS2TileW;
+        j.tiles = j.tiles_x * ((j.h + kS2TileH - 1) / kS2TileH);
+        j.chunks = static_cast<int>((static_cast<long long>(j.w) * j.h + kS2Chunk - 1) / kS2Chunk);
+    }
+    *job = j;
+    return Status::OK;
+}
+
+S2Totals s2_prefix(S2Job* js, int n) {
+    S2Totals t{n, 0, 0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        S2Job& j = js[i];
+        j.fast_begin = t.fast;
+        j.chunk_begin = t.chunks;
+        j.cand_begin = t.cands;
+        j.rank_begin = t.rank;
+        if (!j.active) continue;
+        t.fast += j.tiles;
+        t.chunks += j.chunks;
+        t.cands += (j.candidates + 255) / 256;
+        t.rank += static_cast<long long>((j.candidates + kS2RankKeys - 1) / kS2RankKeys) *
+                  ((j.candidates + kS2RankSegment - 1) / kS2RankSegment);
+    }
+    return t;
+}
+
+hipError_t launch_s2(const S2Job* table, int count, const S2Totals& t, const S2Job* single, hipStream_t stream) {
+    dev::JobSpan<S2Job> span{table, count, single ? *single : S2Job{}};
+    s2_init<<<count, 256, 0, stream>>>(span);
+    if (t.fast > 0) {
+        s2_fast_nms<<<persistent_grid(t.fast), 256, 0, stream>>>(span, t.fast);
+        s2_select<<<count, 64, 0, stream>>>(span);
+        s2_count<<<persistent_grid(t.chunks), 256, 0, stream>>>(span, t.chunks);
+        s2_scan<<<count, 256, 0, stream>>>(span);
+        s2_scatter<<<persistent_grid(t.chunks), 256, 0, stream>>>(span, t.chunks);
+        s2_harris<<<persistent_grid(t.cands), 256, 0, stream>>>(span, t.cands);
+        s2_rank<<<persistent_grid(t.rank), 256, 0, stream>>>(span, t.rank);
+        s2_write<<<persistent_grid(t.cands), 256, 0, stream>>>(span, t.cands);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace jobs
+
+namespace s2 {
+
+Status s2_workspace_bytes(int width, int height, const Config& config, std::size_t* bytes) {
+    if (!bytes) return Status::INVALID_ARGUMENT;
+    *bytes = 0;
+    if (!valid_config(config) || !valid_image_shape(width, height)) return Status::INVALID_ARGUMENT;
+    if (config.max_keypoints == 0) return Status::OK;
+    std::size_t off[7];
+    *bytes = jobs::s2_layout_bytes(width, height, config.max_keypoints, off);
+    return Status::OK;
+}
+
+Status s2_detect(const ImageView& image, const Config& config, Keypoints output, void* workspace,
+                 std::size_t workspace_bytes, hipStream_t stream) {
+    jobs::S2Job job{};
+    const Status s = jobs::make_s2_job(image, config, output, workspace, workspace_bytes, &job);
+    if (s != Status::OK) return s;
+    const jobs::S2Totals t = jobs::s2_prefix(&job, 1);
+    return jobs::launch_s2(nullptr, 1, t, &job, stream) == hipSuccess ? Status::OK : Status::CUDA_ERROR;
+}
+
+}  // namespace s2
+}  // namespace stitch

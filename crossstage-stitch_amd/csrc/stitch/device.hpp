@@ -1,0 +1,82 @@
+// Device-side helpers shared by the stage kernels: job spans for ragged
+// multi-image launches, wave64 reductions and launch bookkeeping.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace stitch::dev {
+
+constexpr int kWave = 64;
+
+// A launch covers `count` jobs (images, pairs or canvases). One job may be
+// passed by value (single-image stage calls, no upload needed); batches pass a
+// device table. Each job records, per kernel, the first tile it owns in that
+// kernel's flattened tile range, so one persistent grid walks every job.
+template <class Job>
+struct JobSpan {
+    const Job* table;  // device table, or nullptr to use `single`
+    int count;
+    Job single;
+};
+
+template <class Job>
+__device__ inline const Job& job_at(const JobSpan<Job>& span, int i) {
+    return span.table ? span.table[i] : span.single;
+}
+
+// Index of the job owning flattened tile `t` for the kernel whose prefix is
+// the member `begin` (jobs are stored in increasing prefix order).
+template <class Job>
+__device__ inline int find_job(const JobSpan<Job>& span, long long t, long long Job::*begin) {
+    if (!span.table) return 0;
+    int lo = 0, hi = span.count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (span.table[mid].*begin <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ inline int lane_id() { return __lane_id(); }
+
+template <class T>
+__device__ inline T wave_sum(T v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ inline unsigned wave_min_u32(unsigned v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned o = __shfl_xor(v, off, kWave);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// Exclusive prefix of a per-lane predicate within the wave, and the wave total.
+__device__ inline int wave_prefix(bool pred, int* total) {
+    const unsigned long long mask = __ballot(pred);
+    *total = __popcll(mask);
+    return __popcll(mask & ((1ull << lane_id()) - 1ull));
+}
+
+}  // namespace stitch::dev
+
+// Grid size for a persistent launch over `tiles` tiles of `threads` threads:
+// enough workgroups to fill every CU a few times, never more than there are tiles.
+inline unsigned persistent_grid(long long tiles, int per_cu = 8) {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipDeviceProp_t prop{};
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cus = prop.multiProcessorCount;
+        if (cus <= 0) cus = 256;
+    }
+    const long long cap = static_cast<long long>(cus) * per_cu;
+    const long long g = tiles < cap ? tiles : cap;
+    return static_cast<unsigned>(g > 0 ? g : 1);
+}

@@ -1,0 +1,115 @@
+/* C ABI of the MI355X stitching path (libcsstitch.so).
+ *
+ * Plain pointers and sizes only; every function returns a status code
+ * (cs_status) and never throws. Device pointers are HIP device memory;
+ * `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *
+ * Each entry point replaces one interface of the reference (paths relative to
+ * the reference repository root); the C++ API behind it keeps the
+ * reference's names and argument meaning:
+ *   cs_s1_*  src/s1/s1.hpp:12-66           cs_s2_*  src/s2_s3/s2.hpp:26-39
+ *   cs_s3_*  src/s2_s3/s3.hpp:14-26        cs_s4_*  src/s4/s4.hpp:23-60
+ *   cs_s5_*  src/s5/s5.hpp:21-39           cs_s6_*  src/s6/s6.hpp:21-47, src/s6/geometry.hpp:84-93
+ *   cs_runner_* harness/pipeline_runner.hpp:32-81 (PipelineRunner)
+ *   cs_controller_*, cs_plan_retry  src/controller/controller.hpp:60-93,
+ *       src/controller/reuse.hpp:20, harness/controller_policy.hpp:208-226
+ * The batched (cs_batch_*) and served (cs_server_*) forms have no
+ * counterpart in the reference; they run many pairs per launch.
+ */
+#ifndef CS_STITCH_H
+#define CS_STITCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: values of stitch::Status (src/common/status.hpp:6-13). */
+enum {
+    CS_OK = 0,
+    CS_INVALID_ARGUMENT = 1,
+    CS_INSUFFICIENT_WORKSPACE = 2,
+    CS_DEVICE_ERROR = 3,
+    CS_NOT_ENOUGH_POINTS = 4,
+    CS_DEGENERATE = 5
+};
+
+enum { CS_RGB8 = 0, CS_GRAY8 = 1 };
+enum { CS_BINARY_U8 = 0, CS_FLOAT32 = 1 };
+enum { CS_FEATHER = 0, CS_MULTIBAND3 = 1, CS_MULTIBAND5 = 2, CS_MULTIBAND6 = 3, CS_MULTIBAND7 = 4 };
+
+typedef struct { uint8_t* data; int width, height, pitch_bytes, format; } cs_image;
+typedef struct { int orig_w, orig_h, work_w, work_h; double sx, sy; double D[9]; } cs_scale_info;
+typedef struct { float* x; float* y; float* response; int capacity; int* d_count; } cs_keypoints;
+typedef struct { int type, dim, row_stride_bytes; void* data; } cs_descriptors;
+typedef struct { int32_t* idx_a; int32_t* idx_b; float* distance; int capacity; int* d_count; } cs_matches;
+typedef struct {
+    double H[9];
+    double H_s[9];
+    int num_matches, num_inliers;
+    double inlier_ratio, inlier_coverage, reproj_residual;
+    int status;
+} cs_homography_result;
+typedef struct { double first_distance, second_distance; int first_index, second_index; } cs_top2;
+typedef struct { int fast_threshold, max_keypoints; float harris_k; } cs_s2_config;
+typedef struct { double ratio_threshold; int cross_check; } cs_s4_config;
+typedef struct {
+    int hypotheses, previous_hypotheses, batch_size, refinement_steps;
+    double threshold;
+    uint64_t seed, generation;
+} cs_s5_config;
+typedef struct {
+    double denominator_tolerance, inverse_tolerance, snap_tolerance;
+    uint64_t max_canvas_pixels;
+} cs_geometry_policy;
+typedef struct {
+    int width, height, origin_x, origin_y, width_a, height_a, width_b, height_b;
+    double denominator_tolerance, snap_tolerance;
+    double inverse[9];
+} cs_geometry;
+typedef struct { uint64_t union_pixels, overlap_pixels; } cs_render_statistics;
+
+/* Library ----------------------------------------------------------------- */
+const char* cs_version(void);
+const char* cs_status_name(int status);
+int cs_make_scale_info(int width, int height, double scale, cs_scale_info* out);
+
+/* S1: RGB -> GRAY + area resize; probe gradient. */
+int cs_s1_workspace_bytes(int orig_w, int orig_h, double scale, size_t* bytes);
+int cs_s1_preprocess(const cs_image* a, const cs_image* b, double scale, cs_image* out_a, cs_image* out_b,
+                     cs_scale_info* scale_a, cs_scale_info* scale_b, void* stream);
+int cs_s1_preprocess_single(const cs_image* in, double scale, cs_image* out, cs_scale_info* scale_info,
+                            void* stream);
+int cs_s1_gradient_workspace_bytes(size_t* bytes);
+int cs_s1_compute_gradient(const cs_image* gray_a, const cs_image* gray_b, void* workspace,
+                           double* pinned_a, double* pinned_b, void* stream);
+
+/* S2: FAST + Harris top-K. */
+int cs_s2_workspace_bytes(int width, int height, const cs_s2_config* config, size_t* bytes);
+int cs_s2_detect(const cs_image* image, const cs_s2_config* config, const cs_keypoints* output,
+                 void* workspace, size_t workspace_bytes, void* stream);
+
+/* S3: rBRIEF descriptors. */
+int cs_s3_workspace_bytes(int width, int height, size_t* bytes);
+int cs_s3_describe(const cs_image* image, const cs_keypoints* keypoints, const cs_descriptors* output,
+                   void* workspace, size_t workspace_bytes, void* stream);
+
+/* S4: brute-force top-2 + ratio + cross-check. */
+int cs_s4_workspace_bytes(int capacity_a, int capacity_b, const cs_s4_config* config, size_t* bytes);
+int cs_s4_knn_top2(const cs_keypoints* a, const cs_descriptors* da, const cs_keypoints* b,
+                   const cs_descriptors* db, cs_top2* output, void* stream);
+int cs_s4_match(const cs_keypoints* a, const cs_descriptors* da, const cs_keypoints* b,
+                const cs_descriptors* db, const cs_s4_config* config, const cs_matches* output,
+                void* workspace, size_t workspace_bytes, void* stream);
+int cs_s4_forward_neighbors(int capacity_a, int capacity_b, const cs_s4_config* config, const void* workspace,
+                            size_t workspace_bytes, const cs_top2** output);
+/* Train rows per LDS segment of the S4 fast path (64..512, multiple of 64). */
+int cs_s4_set_segment(int rows);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CS_STITCH_H */
