@@ -83,6 +83,27 @@ class RenderStatistics(C.Structure):
     _fields_ = [("union_pixels", C.c_uint64), ("overlap_pixels", C.c_uint64)]
 
 
+class HarnessConfig(C.Structure):
+    _fields_ = [("S", C.c_double), ("K", C.c_int), ("R", C.c_int), ("blend_mode", C.c_int),
+                ("fast_threshold", C.c_int), ("harris_k", C.c_float), ("ratio_threshold", C.c_double),
+                ("cross_check", C.c_int), ("msac_batch_size", C.c_int), ("msac_refinements", C.c_int),
+                ("msac_threshold", C.c_double), ("seed", C.c_uint64)]
+
+
+class RunRecord(C.Structure):
+    _fields_ = [("S", C.c_double), ("K", C.c_int), ("R", C.c_int), ("blend_mode", C.c_int),
+                ("t_s1", C.c_double), ("t_s2", C.c_double), ("t_s3", C.c_double), ("t_s4", C.c_double),
+                ("t_s5", C.c_double), ("t_s6", C.c_double), ("t_controller", C.c_double), ("t_e2e", C.c_double),
+                ("t_probe", C.c_double), ("t_stream_wait", C.c_double), ("t_gpu_span", C.c_double),
+                ("probe_gradient_a", C.c_double), ("probe_gradient_b", C.c_double),
+                ("attempt_count", C.c_int), ("num_kp_a", C.c_int), ("num_kp_b", C.c_int), ("num_matches", C.c_int),
+                ("num_inliers", C.c_int), ("inlier_ratio", C.c_double), ("inlier_coverage", C.c_double),
+                ("reproj_residual", C.c_double), ("H", C.c_double * 9), ("canvas_pixels", C.c_uint64),
+                ("overlap_pixels", C.c_uint64), ("canvas_width", C.c_int), ("canvas_height", C.c_int),
+                ("num_levels", C.c_int), ("cache_hit", C.c_int), ("status", C.c_char * 32),
+                ("fail_stage", C.c_char * 16)]
+
+
 P = C.POINTER
 vp = C.c_void_p
 sz = C.c_size_t
@@ -108,6 +129,36 @@ SIGNATURES = {
                               vp, sz, vp]),
     "cs_s4_forward_neighbors": (C.c_int, [C.c_int, C.c_int, P(S4Config), vp, sz, P(vp)]),
     "cs_s4_set_segment": (C.c_int, [C.c_int]),
+    "cs_s5_workspace_bytes": (C.c_int, [P(S5Config), P(sz)]),
+    "cs_s5_estimate": (C.c_int, [P(Keypoints), P(Keypoints), P(Matches), P(ScaleInfo), P(ScaleInfo), P(S5Config),
+                                 vp, vp, vp, sz, vp]),
+    "cs_s6_prepare_geometry": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_double), P(GeometryPolicy),
+                                         P(Geometry)]),
+    "cs_s6_effective_reductions": (C.c_int, [P(Geometry), C.c_int]),
+    "cs_s6_workspace_bytes": (C.c_int, [P(Geometry), C.c_int, P(sz)]),
+    "cs_s6_render": (C.c_int, [P(Image), P(Image), P(Geometry), C.c_int, P(Image), P(Image), vp, sz, vp,
+                               P(RenderStatistics)]),
+    "cs_runner_create": (vp, []),
+    "cs_runner_destroy": (None, [vp]),
+    "cs_runner_initialize": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.c_uint64]),
+    "cs_runner_query_memory": (C.c_int, [C.c_int] * 7 + [P(C.c_uint64)] * 3),
+    "cs_runner_upload_pair": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, C.c_int, C.c_int]),
+    "cs_runner_use_device_pair": (C.c_int, [vp, P(Image), P(Image)]),
+    "cs_runner_execute_run": (C.c_int, [vp, P(HarnessConfig), P(RunRecord)]),
+    "cs_runner_execute_adaptive_run": (C.c_int, [vp, vp, C.c_uint64, P(RunRecord)]),
+    "cs_runner_set_flags": (C.c_int, [vp, C.c_int, C.c_int]),
+    "cs_runner_outputs": (C.c_int, [vp, P(Image), P(Image), P(Keypoints), P(Keypoints), P(Descriptors),
+                                    P(Descriptors), P(Matches), P(vp), P(HomographyResult)]),
+    "cs_runner_stream": (vp, [vp]),
+    "cs_controller_create_legacy": (vp, [C.c_int]),
+    "cs_controller_destroy": (None, [vp]),
+    "cs_plan_retry": (C.c_int, [C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                                P(C.c_int)]),
+    "cs_controller_walk": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     P(HomographyResult), P(C.c_int), C.c_int, P(C.c_double)]),
+    "cs_synth_pair": (C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, vp, vp,
+                                P(C.c_double)]),
 }
 
 _lib = None

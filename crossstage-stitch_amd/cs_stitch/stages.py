@@ -229,3 +229,59 @@ def top2_records(t: torch.Tensor, n: int):
     raw = t.reshape(-1)[: 3 * n].cpu().numpy().tobytes()
     arr = (F.Top2 * n).from_buffer_copy(raw)
     return [(r.first_distance, r.second_distance, r.first_index, r.second_index) for r in arr]
+
+
+# ---------------------------------------------------------------- S5
+
+def s5_workspace_bytes(hypotheses: int, previous: int = 0, batch_size: int = 64, refinement_steps: int = 1,
+                       threshold: float = 3.0, seed: int = 0, generation: int = 1) -> int:
+    cfg = F.S5Config(hypotheses, previous, batch_size, refinement_steps, threshold, seed, generation)
+    n = C.c_size_t()
+    _check(F.load().cs_s5_workspace_bytes(C.byref(cfg), C.byref(n)), "s5_workspace_bytes")
+    return n.value
+
+
+class S5Runner:
+    """Holds one S5 workspace, mask and pinned result so R extensions can reuse state."""
+
+    def __init__(self, max_hypotheses: int, match_capacity: int):
+        self.ws = workspace(s5_workspace_bytes(max_hypotheses))
+        self.mask = torch.zeros(max(match_capacity, 1), dtype=torch.uint8, device="cuda")
+        self.pinned = torch.zeros(C.sizeof(F.HomographyResult), dtype=torch.uint8, pin_memory=True)
+
+    def estimate(self, kp_a: KeypointSet, kp_b: KeypointSet, matches: MatchSet, sa: F.ScaleInfo, sb: F.ScaleInfo,
+                 hypotheses: int, previous: int = 0, batch_size: int = 64, refinement_steps: int = 1,
+                 threshold: float = 3.0, seed: int = 0, generation: int = 1, capacity_a=None, capacity_b=None):
+        L = F.load()
+        cfg = F.S5Config(hypotheses, previous, batch_size, refinement_steps, threshold, seed, generation)
+        a, b, m = kp_a.c(capacity_a), kp_b.c(capacity_b), matches.c()
+        _check(L.cs_s5_estimate(C.byref(a), C.byref(b), C.byref(m), C.byref(sa), C.byref(sb), C.byref(cfg),
+                                self.mask.data_ptr(), self.pinned.data_ptr(), self.ws.data_ptr(), self.ws.numel(),
+                                stream_handle()), "s5_estimate")
+        torch.cuda.current_stream().synchronize()
+        return F.HomographyResult.from_buffer_copy(self.pinned.numpy().tobytes())
+
+
+# ---------------------------------------------------------------- S6
+
+def prepare_geometry(wa: int, ha: int, wb: int, hb: int, h, policy: F.GeometryPolicy | None = None):
+    g = F.Geometry()
+    harr = (C.c_double * 9)(*[float(v) for v in (h.ravel() if hasattr(h, "ravel") else h)])
+    st = F.load().cs_s6_prepare_geometry(wa, ha, wb, hb, harr, C.byref(policy) if policy else None, C.byref(g))
+    return st, g
+
+
+def s6_render(a: torch.Tensor, b: torch.Tensor, g: F.Geometry, mode: int, stats: bool = True, pad: int = 64):
+    L = F.load()
+    n = C.c_size_t()
+    _check(L.cs_s6_workspace_bytes(C.byref(g), mode, C.byref(n)), "s6_workspace_bytes")
+    ws = workspace(n.value) if n.value else None
+    out = pitched(g.height, g.width, 3, pad)
+    mask = pitched(g.height, g.width, 1, pad)
+    counters = torch.zeros(2, dtype=torch.int64, device="cuda") if stats else None
+    ia, ib, io, im = image(a), image(b), image(out), image(mask)
+    _check(L.cs_s6_render(C.byref(ia), C.byref(ib), C.byref(g), mode, C.byref(io), C.byref(im),
+                          ws.data_ptr() if ws is not None else None, n.value, stream_handle(),
+                          C.cast(counters.data_ptr(), C.POINTER(F.RenderStatistics)) if stats else None),
+           "s6_render")
+    return out, mask, counters

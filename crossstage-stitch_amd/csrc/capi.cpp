@@ -30,6 +30,49 @@ void put_scale(const ScaleInfo& s, cs_scale_info* o) {
     for (int i = 0; i < 9; ++i) o->D[i] = s.D[i];
 }
 hipStream_t st(void* s) { return static_cast<hipStream_t>(s); }
+ScaleInfo scale(const cs_scale_info* s) {
+    ScaleInfo o{};
+    o.orig_w = s->orig_w;
+    o.orig_h = s->orig_h;
+    o.work_w = s->work_w;
+    o.work_h = s->work_h;
+    o.sx = s->sx;
+    o.sy = s->sy;
+    for (int i = 0; i < 9; ++i) o.D[i] = s->D[i];
+    return o;
+}
+s5::Config s5cfg(const cs_s5_config* c) {
+    return {c->hypotheses, c->previous_hypotheses, c->batch_size, c->refinement_steps, c->threshold, c->seed,
+            c->generation};
+}
+s6::Geometry geo(const cs_geometry* g) {
+    s6::Geometry o{};
+    o.width = g->width;
+    o.height = g->height;
+    o.origin_x = g->origin_x;
+    o.origin_y = g->origin_y;
+    o.width_a = g->width_a;
+    o.height_a = g->height_a;
+    o.width_b = g->width_b;
+    o.height_b = g->height_b;
+    o.denominator_tolerance = g->denominator_tolerance;
+    o.snap_tolerance = g->snap_tolerance;
+    for (int i = 0; i < 9; ++i) o.inverse[i] = g->inverse[i];
+    return o;
+}
+void put_geo(const s6::Geometry& g, cs_geometry* o) {
+    o->width = g.width;
+    o->height = g.height;
+    o->origin_x = g.origin_x;
+    o->origin_y = g.origin_y;
+    o->width_a = g.width_a;
+    o->height_a = g.height_a;
+    o->width_b = g.width_b;
+    o->height_b = g.height_b;
+    o->denominator_tolerance = g.denominator_tolerance;
+    o->snap_tolerance = g.snap_tolerance;
+    for (int i = 0; i < 9; ++i) o->inverse[i] = g.inverse[i];
+}
 
 template <class F>
 int guard(F&& f) {
@@ -155,6 +198,56 @@ int cs_s4_set_segment(int rows) {
     if (rows < 64 || rows > 512 || rows % 64) return CS_INVALID_ARGUMENT;
     jobs::s4_set_segment(rows);
     return CS_OK;
+}
+
+int cs_s5_workspace_bytes(const cs_s5_config* c, size_t* bytes) {
+    if (!c) return CS_INVALID_ARGUMENT;
+    return static_cast<int>(s5::s5_workspace_bytes(s5cfg(c), bytes));
+}
+
+int cs_s5_estimate(const cs_keypoints* a, const cs_keypoints* b, const cs_matches* m, const cs_scale_info* sa,
+                   const cs_scale_info* sb, const cs_s5_config* c, uint8_t* mask, cs_homography_result* out, void* ws,
+                   size_t bytes, void* stream) {
+    if (!a || !b || !m || !sa || !sb || !c) return CS_INVALID_ARGUMENT;
+    return guard([&] {
+        return s5::estimate(kps(a), kps(b), mt(m), scale(sa), scale(sb), s5cfg(c), mask,
+                            reinterpret_cast<HomographyResult*>(out), ws, bytes, st(stream));
+    });
+}
+
+int cs_s6_prepare_geometry(int wa, int ha, int wb, int hb, const double* h, const cs_geometry_policy* p,
+                           cs_geometry* out) {
+    if (!out) return CS_INVALID_ARGUMENT;
+    s6::GeometryPolicy pol{};
+    if (p) {
+        pol.denominator_tolerance = p->denominator_tolerance;
+        pol.inverse_tolerance = p->inverse_tolerance;
+        pol.snap_tolerance = p->snap_tolerance;
+        pol.max_canvas_pixels = static_cast<std::size_t>(p->max_canvas_pixels);
+    }
+    s6::Geometry g{};
+    const Status r = s6::prepare_geometry(wa, ha, wb, hb, h, pol, &g);
+    put_geo(g, out);
+    return static_cast<int>(r);
+}
+
+int cs_s6_effective_reductions(const cs_geometry* g, int mode) {
+    if (!g) return -1;
+    return s6::effective_reductions(geo(g), static_cast<s6::BlendMode>(mode));
+}
+
+int cs_s6_workspace_bytes(const cs_geometry* g, int mode, size_t* bytes) {
+    if (!g) return CS_INVALID_ARGUMENT;
+    return static_cast<int>(s6::s6_workspace_bytes(geo(g), static_cast<s6::BlendMode>(mode), bytes));
+}
+
+int cs_s6_render(const cs_image* a, const cs_image* b, const cs_geometry* g, int mode, const cs_image* out,
+                 const cs_image* mask, void* ws, size_t bytes, void* stream, cs_render_statistics* stats) {
+    if (!a || !b || !g || !out || !mask) return CS_INVALID_ARGUMENT;
+    return guard([&] {
+        return s6::render(img(a), img(b), geo(g), static_cast<s6::BlendMode>(mode), img(out), img(mask), ws, bytes,
+                          st(stream), reinterpret_cast<s6::RenderStatistics*>(stats));
+    });
 }
 
 }  // extern "C"

@@ -175,6 +175,8 @@ struct S5State {
     Status status;
 };
 constexpr std::size_t kS5ModelOffset = (sizeof(S5State) + 255) / 256 * 256;
+// Hypotheses evaluated per launch; the workspace holds at most this many models.
+constexpr int kS5MaxModels = 8192;
 
 struct S5Job {
     const float* ax;

@@ -108,6 +108,86 @@ int cs_s4_forward_neighbors(int capacity_a, int capacity_b, const cs_s4_config* 
 /* Train rows per LDS segment of the S4 fast path (64..512, multiple of 64). */
 int cs_s4_set_segment(int rows);
 
+/* S5: MSAC homography B -> A; output must be pinned host memory, valid after a
+ * stream synchronization. */
+int cs_s5_workspace_bytes(const cs_s5_config* config, size_t* bytes);
+int cs_s5_estimate(const cs_keypoints* a, const cs_keypoints* b, const cs_matches* matches,
+                   const cs_scale_info* scale_a, const cs_scale_info* scale_b, const cs_s5_config* config,
+                   uint8_t* inlier_mask, cs_homography_result* output, void* workspace, size_t workspace_bytes,
+                   void* stream);
+
+/* S6: host canvas geometry, then warp + feather / multiband render.
+ * policy may be NULL for the default GeometryPolicy. */
+int cs_s6_prepare_geometry(int width_a, int height_a, int width_b, int height_b, const double* h,
+                           const cs_geometry_policy* policy, cs_geometry* out);
+int cs_s6_effective_reductions(const cs_geometry* geometry, int mode);
+int cs_s6_workspace_bytes(const cs_geometry* geometry, int mode, size_t* bytes);
+int cs_s6_render(const cs_image* a, const cs_image* b, const cs_geometry* geometry, int mode,
+                 const cs_image* output, const cs_image* mask, void* workspace, size_t workspace_bytes,
+                 void* stream, cs_render_statistics* statistics);
+
+/* Plain per-pair runner: harness/pipeline_runner.hpp:32-81 (PipelineRunner),
+ * harness/harness_types.hpp:14-106 (HarnessConfig, RunRecord). */
+typedef struct cs_runner cs_runner;
+typedef struct cs_controller cs_controller;
+typedef struct {
+    double S;
+    int K, R, blend_mode, fast_threshold;
+    float harris_k;
+    double ratio_threshold;
+    int cross_check, msac_batch_size, msac_refinements;
+    double msac_threshold;
+    uint64_t seed;
+} cs_harness_config;
+typedef struct {
+    double S;
+    int K, R, blend_mode;
+    double t_s1, t_s2, t_s3, t_s4, t_s5, t_s6, t_controller, t_e2e, t_probe, t_stream_wait, t_gpu_span;
+    double probe_gradient_a, probe_gradient_b;
+    int attempt_count, num_kp_a, num_kp_b, num_matches, num_inliers;
+    double inlier_ratio, inlier_coverage, reproj_residual;
+    double H[9];
+    uint64_t canvas_pixels, overlap_pixels;
+    int canvas_width, canvas_height, num_levels, cache_hit;
+    char status[32];
+    char fail_stage[16];
+} cs_run_record;
+
+cs_runner* cs_runner_create(void);
+void cs_runner_destroy(cs_runner* runner);
+int cs_runner_initialize(cs_runner* runner, int max_width, int max_height, int max_k, int max_r, int canvas_width,
+                         int canvas_height, int workspace_mode, uint64_t device_reserve);
+int cs_runner_query_memory(int width, int height, int k, int r, int canvas_width, int canvas_height, int mode,
+                           uint64_t* device_bytes, uint64_t* workspace_bytes, uint64_t* s6_workspace_bytes);
+int cs_runner_upload_pair(cs_runner* runner, const uint8_t* rgb_a, int width_a, int height_a, const uint8_t* rgb_b,
+                          int width_b, int height_b);
+int cs_runner_use_device_pair(cs_runner* runner, const cs_image* a, const cs_image* b);
+int cs_runner_execute_run(cs_runner* runner, const cs_harness_config* config, cs_run_record* record);
+int cs_runner_execute_adaptive_run(cs_runner* runner, cs_controller* controller, uint64_t seed,
+                                   cs_run_record* record);
+int cs_runner_set_flags(cs_runner* runner, int reuse_enabled, int probe_enabled);
+int cs_runner_outputs(cs_runner* runner, cs_image* canvas, cs_image* mask, cs_keypoints* keypoints_a,
+                      cs_keypoints* keypoints_b, cs_descriptors* descriptors_a, cs_descriptors* descriptors_b,
+                      cs_matches* matches, uint8_t** inlier_mask, cs_homography_result* homography);
+void* cs_runner_stream(cs_runner* runner);
+
+/* Rule-based controller with the built-in legacy-v1 ladder (blind = 1: the
+ * probe-free S=0.5 variant), and plan_retry. cs_controller_walk runs
+ * begin_pair then observe over n feedbacks; trace rows are (action, reason,
+ * S, K, R), the first row the initial knobs. Returns the row count. */
+cs_controller* cs_controller_create_legacy(int blind);
+void cs_controller_destroy(cs_controller* controller);
+int cs_plan_retry(double s0, int k0, int r0, int b0, double s1, int k1, int r1, int b1, int* plan4);
+int cs_controller_walk(cs_controller* controller, double gradient_a, double gradient_b, int width_a, int height_a,
+                       int width_b, int height_b, const cs_homography_result* feedback, const int* exec_status,
+                       int n, double* trace);
+
+/* Seeded synthetic pair (no counterpart in the reference; bench and test
+ * input): host RGB8 buffers of width*height*3, overlap drawn in [lo, hi],
+ * kind 0 textured, 1 low texture, 2 repetitive; h_ba = true H (B -> A). */
+int cs_synth_pair(uint64_t seed, int width, int height, double overlap_lo, double overlap_hi, int kind,
+                  uint8_t* rgb_a, uint8_t* rgb_b, double* h_ba);
+
 #ifdef __cplusplus
 }
 #endif
