@@ -30,41 +30,17 @@ using jobs::kS6MaxLevels;
 
 constexpr int kBlock = 256;
 
-__device__ __forceinline__ int reflect101(long long x, long long n) {
-    if (n == 1) return 0;
-    const long long period = 2 * n - 2;
-    x %= period;
-    if (x < 0) x += period;
-    return static_cast<int>(x < n ? x : period - x);
+// Reflect-101 index for the filter footprints here, which overshoot a
+// border by at most 2: one fold is exact for n >= 3 (the reference's general
+// modulo form, s6.cu:122-128, gives the same index).
+__device__ __forceinline__ int reflect101(int x, int n) {
+    if (n <= 2) return n == 1 ? 0 : (x & 1);
+    return x < 0 ? -x : (x >= n ? 2 * n - 2 - x : x);
 }
 
 __device__ __forceinline__ float tap(int x) {
     const int t = x < 0 ? -x : x;
     return t == 0 ? 0.375f : (t == 1 ? 0.25f : 0.0625f);
-}
-
-// Upsampled value of a coarse level at fine pixel (x, y): zero-insertion on a
-// 2cw x 2ch grid, 5x5 filter, times 4 (s6.cu:133-146).
-__device__ __forceinline__ void expanded3(const float* src, int cw, int ch, int x, int y, float out[3]) {
-    float s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-    for (int dy = -2; dy <= 2; ++dy) {
-        const int yy = reflect101(static_cast<long long>(y) + dy, 2LL * ch);
-        if (yy & 1) continue;
-#pragma unroll
-        for (int dx = -2; dx <= 2; ++dx) {
-            const int xx = reflect101(static_cast<long long>(x) + dx, 2LL * cw);
-            if (xx & 1) continue;
-            const float w = tap(dx) * tap(dy);
-            const float* p = src + (static_cast<size_t>(yy / 2) * cw + xx / 2) * 3;
-            s0 += w * p[0];
-            s1 += w * p[1];
-            s2 += w * p[2];
-        }
-    }
-    out[0] = 4 * s0;
-    out[1] = 4 * s1;
-    out[2] = 4 * s2;
 }
 
 __device__ bool center_coordinate(double& x, int n, double tol) {
@@ -89,7 +65,9 @@ __device__ bool sample(const std::uint8_t* img, int w, int h, int pitch, double 
         const double v1 = (1 - ax) * r1[3 * x0 + c] + ax * r1[3 * x1 + c];
         rgb[c] = static_cast<float>(((1 - ay) * v0 + ay * v1) / 255.0);
     }
-    score = fmin(fmin((x + 0.5) / w, (w - 0.5 - x) / w), fmin((y + 0.5) / h, (h - 0.5 - y) / h));
+    // min((x+.5)/w, (w-.5-x)/w) == min(x+.5, w-.5-x)/w exactly: a correctly
+    // rounded division by a positive w is monotonic, so it commutes with min.
+    score = fmin(fmin(x + 0.5, w - 0.5 - x) / w, fmin(y + 0.5, h - 0.5 - y) / h);
     return true;
 }
 
@@ -109,6 +87,20 @@ __device__ int find_job_by(const dev::JobSpan<S6Job>& span, long long t, Begin b
     return lo;
 }
 
+// Row-segment tiles: tile = (row, 256-pixel segment); no per-thread division.
+__device__ __forceinline__ unsigned tiles_x(int w) { return (static_cast<unsigned>(w) + kBlock - 1) / kBlock; }
+
+__device__ __forceinline__ bool tile_pixel(long long local, int w, int& x, int& y) {
+    const unsigned lt = static_cast<unsigned>(local);
+    const unsigned tx = tiles_x(w);
+    const unsigned row = lt / tx;
+    x = static_cast<int>((lt - row * tx) * kBlock + threadIdx.x);
+    y = static_cast<int>(row);
+    return x < w;
+}
+
+constexpr unsigned kVA = 1u, kVB = 2u, kUA = 4u, kUB = 8u;
+
 __global__ void s6_clear_stats(dev::JobSpan<S6Job> span) {
     const S6Job& j = dev::job_at(span, blockIdx.x);
     if (threadIdx.x == 0 && j.stats) {
@@ -117,205 +109,369 @@ __global__ void s6_clear_stats(dev::JobSpan<S6Job> span) {
     }
 }
 
-// Canvas pixel -> A (integer shift) and B (FP64 inverse), level-0 buffers or
-// the feather output, union mask, optional union/overlap counters.
+__device__ void flush_stats(const S6Job& j, unsigned long long u, unsigned long long o,
+                            unsigned long long (&red)[2][kBlock / 64]) {
+    u = dev::wave_sum(u);
+    o = dev::wave_sum(o);
+    const int lane = dev::lane_id(), wave = threadIdx.x / 64;
+    if (lane == 0) {
+        red[0][wave] = u;
+        red[1][wave] = o;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long su = 0, so = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            su += red[0][w];
+            so += red[1][w];
+        }
+        if (su) atomicAdd(reinterpret_cast<unsigned long long*>(&j.stats->union_pixels), su);
+        if (so) atomicAdd(reinterpret_cast<unsigned long long*>(&j.stats->overlap_pixels), so);
+    }
+    __syncthreads();
+}
+
+// Canvas pixel -> A (integer shift) and B (FP64 inverse map), level-0 colours
+// and flags or the feather output, union mask, union/overlap counters.
+// A is sampled at integer positions, where the bilinear weights are exactly
+// (1, 0): its colour is float(v / 255.0) of the source byte, read from a
+// 256-entry table computed with the same expression.
 __global__ void __launch_bounds__(kBlock) s6_warp(dev::JobSpan<S6Job> span, long long total) {
     __shared__ unsigned long long red[2][kBlock / 64];
+    __shared__ float lut[256];
+    lut[threadIdx.x] = static_cast<float>(static_cast<double>(threadIdx.x) / 255.0);
+    __syncthreads();
+    int cur = -1;
+    unsigned long long uni = 0, ovl = 0;
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = find_job_by(span, t, [](const S6Job& j) { return j.warp_begin; });
+        if (ji != cur) {
+            if (cur >= 0 && dev::job_at(span, cur).stats) flush_stats(dev::job_at(span, cur), uni, ovl, red);
+            cur = ji;
+            uni = ovl = 0;
+        }
         const S6Job& j = dev::job_at(span, ji);
-        const long long p = (t - j.warp_begin) * kBlock + threadIdx.x;
-        const long long n = static_cast<long long>(j.width) * j.height;
-        bool ma = false, mb = false;
-        if (p < n) {
-            const int x = static_cast<int>(p % j.width), y = static_cast<int>(p / j.width);
-            const double xa = static_cast<double>(x) + j.origin_x, ya = static_cast<double>(y) + j.origin_y;
-            float ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
-            double sa = 0, sb = 0;
-            ma = sample(j.a, j.wa, j.ha, j.pitch_a, xa, ya, ra, sa, j.snap_tolerance);
-            const double* inv = j.inverse;
-            const double den = inv[6] * xa + inv[7] * ya + inv[8];
-            const double dsc = fabs(inv[6] * xa) + fabs(inv[7] * ya) + fabs(inv[8]);
-            if (isfinite(den) && dsc > 0 && fabs(den) > j.denominator_tolerance * dsc) {
-                const double xb = (inv[0] * xa + inv[1] * ya + inv[2]) / den;
-                const double yb = (inv[3] * xa + inv[4] * ya + inv[5]) / den;
-                mb = sample(j.b, j.wb, j.hb, j.pitch_b, xb, yb, rb, sb, j.snap_tolerance);
+        int x, y;
+        if (!tile_pixel(t - j.warp_begin, j.width, x, y)) continue;
+        const size_t p = static_cast<size_t>(y) * j.width + x;
+        const int ax = x + j.origin_x, ay = y + j.origin_y;
+        const double xa = static_cast<double>(x) + j.origin_x, ya = static_cast<double>(y) + j.origin_y;
+        float ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
+        double sa = 0, sb = 0;
+        const bool ma = ax >= 0 && ax <= j.wa - 1 && ay >= 0 && ay <= j.ha - 1;
+        if (ma) {
+            const std::uint8_t* q = j.a + static_cast<size_t>(ay) * j.pitch_a + 3 * ax;
+            ra[0] = lut[q[0]];
+            ra[1] = lut[q[1]];
+            ra[2] = lut[q[2]];
+            sa = fmin(fmin(xa + 0.5, j.wa - 0.5 - xa) / j.wa, fmin(ya + 0.5, j.ha - 0.5 - ya) / j.ha);
+        }
+        bool mb = false;
+        const double* inv = j.inverse;
+        const double den = inv[6] * xa + inv[7] * ya + inv[8];
+        const double dsc = fabs(inv[6] * xa) + fabs(inv[7] * ya) + fabs(inv[8]);
+        if (isfinite(den) && dsc > 0 && fabs(den) > j.denominator_tolerance * dsc) {
+            const double xb = (inv[0] * xa + inv[1] * ya + inv[2]) / den;
+            const double yb = (inv[3] * xa + inv[4] * ya + inv[5]) / den;
+            mb = sample(j.b, j.wb, j.hb, j.pitch_b, xb, yb, rb, sb, j.snap_tolerance);
+        }
+        j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = ma || mb ? 1 : 0;
+        if (j.top >= 0) {
+            const S6Level& L = j.lv[0];
+            unsigned f = 0;
+            if (ma) f |= kVA;
+            if (mb) f |= kVB;
+            if (ma && (!mb || sa >= sb)) f |= kUA;
+            if (mb && (!ma || sb > sa)) f |= kUB;
+            L.flags[p] = static_cast<std::uint8_t>(f);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                L.a[c * L.n + p] = ra[c];
+                L.b[c * L.n + p] = rb[c];
             }
-            j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = ma || mb ? 1 : 0;
-            if (j.top >= 0 && j.mode != 0) {
-                const S6Level& L = j.lv[0];
-                L.va[p] = ma ? 1.0f : 0.0f;
-                L.vb[p] = mb ? 1.0f : 0.0f;
-                L.ua[p] = ma && (!mb || sa >= sb) ? 1.0f : 0.0f;
-                L.ub[p] = mb && (!ma || sb > sa) ? 1.0f : 0.0f;
+        } else {
+            std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    L.a[3 * p + c] = ra[c];
-                    L.b[3 * p + c] = rb[c];
-                }
-            } else {
-                std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const float color =
-                        ma && mb ? static_cast<float>((sa * ra[c] + sb * rb[c]) / (sa + sb)) : (ma ? ra[c] : rb[c]);
-                    out[c] = quantize(color);
-                }
+            for (int c = 0; c < 3; ++c) {
+                const float color =
+                    ma && mb ? static_cast<float>((sa * ra[c] + sb * rb[c]) / (sa + sb)) : (ma ? ra[c] : rb[c]);
+                out[c] = quantize(color);
             }
         }
-        if (j.stats) {
-            const unsigned long long bu = __ballot(ma || mb), bo = __ballot(ma && mb);
-            const int lane = dev::lane_id(), wave = threadIdx.x / 64;
-            if (lane == 0) {
-                red[0][wave] = __popcll(bu);
-                red[1][wave] = __popcll(bo);
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                unsigned long long u = 0, o = 0;
-                for (int w = 0; w < kBlock / 64; ++w) {
-                    u += red[0][w];
-                    o += red[1][w];
-                }
-                if (u) atomicAdd(reinterpret_cast<unsigned long long*>(&j.stats->union_pixels), u);
-                if (o) atomicAdd(reinterpret_cast<unsigned long long*>(&j.stats->overlap_pixels), o);
-            }
-            __syncthreads();
-        }
+        uni += ma || mb;
+        ovl += ma && mb;
     }
+    if (cur >= 0 && dev::job_at(span, cur).stats) flush_stats(dev::job_at(span, cur), uni, ovl, red);
 }
 
-// 5x5 filter + 2x decimation of `channels` interleaved channels (s6.cu:236-253).
-__device__ __forceinline__ float reduce_at(const float* src, int w, int h, int x, int y, int channels, int c) {
-    float v = 0;
-#pragma unroll
-    for (int dy = -2; dy <= 2; ++dy) {
-        const int yy = reflect101(2LL * y + dy, h);
-#pragma unroll
-        for (int dx = -2; dx <= 2; ++dx) {
-            const int xx = reflect101(2LL * x + dx, w);
-            v += tap(dx) * tap(dy) * src[(static_cast<size_t>(yy) * w + xx) * channels + c];
-        }
-    }
-    return v;
-}
+// ---------------------------------------------------------------- reduce
+// 5x5 [1 4 6 4 1]/16 filter + 2x decimation (s6.cu:236-253) on LDS tiles:
+// a 32x8 tile of level-l outputs reads a 67x19 region of level l-1, loaded
+// once with coalesced rows and the reflect-101 borders applied at load time;
+// each output keeps the reference's dy-major, dx-minor sum per channel.
+constexpr int kRW = 32, kRH = 8, kRIW = 2 * kRW + 3, kRIH = 2 * kRH + 3;
 
-// kind 0: pull (A, B, VA, VB); kind 1: Gaussians of the filled images and of
-// the ownership (A, B, UA, UB). Destination level `l`.
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// kind 0: pull (A, B, VA, VB); kind 1: Gaussians of filled A, B and of UA, UB.
 __global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, long long total, int l, int kind) {
+    __shared__ float tile[8][kRIH][kRIW];
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = kind == 0 ? find_job_by(span, t, [l](const S6Job& j) { return j.pb[l]; })
-                                 : find_job_by(span, t, [l](const S6Job& j) { return j.mb[l]; });
+        const int ji = kind == 0 ? find_job_by(span, t, [l](const S6Job& j) { return j.rb[l]; })
+                                 : find_job_by(span, t, [l](const S6Job& j) { return j.gb[l]; });
         const S6Job& j = dev::job_at(span, ji);
-        const long long p = (t - (kind == 0 ? j.pb[l] : j.mb[l])) * kBlock + threadIdx.x;
         const S6Level& C = j.lv[l];
         const S6Level& F = j.lv[l - 1];
-        if (p >= static_cast<long long>(C.w) * C.h) continue;
-        const int x = static_cast<int>(p % C.w), y = static_cast<int>(p / C.w);
+        const unsigned local = static_cast<unsigned>(t - (kind == 0 ? j.rb[l] : j.gb[l]));
+        const unsigned tx = (static_cast<unsigned>(C.w) + kRW - 1) / kRW;
+        const int x0 = static_cast<int>(local % tx) * kRW, y0 = static_cast<int>(local / tx) * kRH;
+        const int ix0 = 2 * x0 - 2, iy0 = 2 * y0 - 2;
+        const float* w1 = kind == 0 ? F.va : F.ua;
+        const float* w2 = kind == 0 ? F.vb : F.ub;
+        const unsigned m1 = kind == 0 ? kVA : kUA, m2 = kind == 0 ? kVB : kUB;
+        for (int i = threadIdx.x; i < kRIW * kRIH; i += kBlock) {
+            const int ix = i % kRIW, iy = i / kRIW;
+            const int sx = clampi(reflect101(ix0 + ix, F.w), 0, F.w - 1);
+            const int sy = clampi(reflect101(iy0 + iy, F.h), 0, F.h - 1);
+            const size_t q = static_cast<size_t>(sy) * F.w + sx;
+            tile[0][iy][ix] = F.a[q];
+            tile[1][iy][ix] = F.a[F.n + q];
+            tile[2][iy][ix] = F.a[2 * F.n + q];
+            tile[3][iy][ix] = F.b[q];
+            tile[4][iy][ix] = F.b[F.n + q];
+            tile[5][iy][ix] = F.b[2 * F.n + q];
+            if (l == 1) {
+                const unsigned f = F.flags[q];
+                tile[6][iy][ix] = (f & m1) ? 1.0f : 0.0f;
+                tile[7][iy][ix] = (f & m2) ? 1.0f : 0.0f;
+            } else {
+                tile[6][iy][ix] = w1[q];
+                tile[7][iy][ix] = w2[q];
+            }
+        }
+        __syncthreads();
+        const int lx = threadIdx.x % kRW, ly = threadIdx.x / kRW;
+        const int x = x0 + lx, y = y0 + ly;
+        if (x < C.w && y < C.h) {
+            float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            C.a[3 * p + c] = reduce_at(F.a, F.w, F.h, x, y, 3, c);
-            C.b[3 * p + c] = reduce_at(F.b, F.w, F.h, x, y, 3, c);
+            for (int dy = 0; dy < 5; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 5; ++dx) {
+                    const float w = tap(dx - 2) * tap(dy - 2);
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) acc[c] += w * tile[c][2 * ly + dy][2 * lx + dx];
+                }
+            const size_t p = static_cast<size_t>(y) * C.w + x;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                C.a[c * C.n + p] = acc[c];
+                C.b[c * C.n + p] = acc[3 + c];
+            }
+            if (kind == 0) {
+                C.va[p] = acc[6];
+                C.vb[p] = acc[7];
+            } else {
+                C.ua[p] = acc[6];
+                C.ub[p] = acc[7];
+            }
         }
-        if (kind == 0) {
-            C.va[p] = reduce_at(F.va, F.w, F.h, x, y, 1, 0);
-            C.vb[p] = reduce_at(F.vb, F.w, F.h, x, y, 1, 0);
-        } else {
-            C.ua[p] = reduce_at(F.ua, F.w, F.h, x, y, 1, 0);
-            C.ub[p] = reduce_at(F.ub, F.w, F.h, x, y, 1, 0);
-        }
+        __syncthreads();
     }
 }
 
-// normalize_color + push_missing for level l (top-down): valid pixels keep
-// numerator / validity, missing ones take the expanded, already final, coarse
-// level (s6.cu:254-270, :399-410).
+// ---------------------------------------------------------------- expand
+// Upsampling reads a 3x3 coarse neighbourhood per fine pixel; a 64x4 fine
+// tile needs a 34x4 coarse region, staged in LDS. Taps whose reflected fine
+// position is odd carry weight 0 (adding 0 * v to the running sum leaves it
+// unchanged), so all lanes run the same 25-step loop without divergence.
+constexpr int kEW = 64, kEH = 4, kCW = kEW / 2 + 2, kCH = kEH / 2 + 2;
+
+struct ExpandTaps {
+    int xi[5], yi[5];      // LDS column/row per offset
+    float wx[5], wy[5];    // tap weight or 0 when the position is odd
+};
+
+__device__ __forceinline__ void expand_setup(int x, int y, int cw, int ch, int cx0, int cy0, ExpandTaps& e) {
+#pragma unroll
+    for (int d = 0; d < 5; ++d) {
+        const int rx = reflect101(x + d - 2, 2 * cw);
+        const int ry = reflect101(y + d - 2, 2 * ch);
+        e.xi[d] = (rx >> 1) - cx0;
+        e.yi[d] = (ry >> 1) - cy0;
+        e.wx[d] = (rx & 1) ? 0.0f : tap(d - 2);
+        e.wy[d] = (ry & 1) ? 0.0f : tap(d - 2);
+    }
+}
+
+// 4 * sum over dy, dx of (tap(dx) * tap(dy)) * coarse, for 3 interleaved channels.
+__device__ __forceinline__ void expand_lds(const float (&src)[3][kCH][kCW], const ExpandTaps& e, float out[3]) {
+    float s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for (int dy = 0; dy < 5; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 5; ++dx) {
+            const float w = e.wx[dx] * e.wy[dy];
+            s0 += w * src[0][e.yi[dy]][e.xi[dx]];
+            s1 += w * src[1][e.yi[dy]][e.xi[dx]];
+            s2 += w * src[2][e.yi[dy]][e.xi[dx]];
+        }
+    out[0] = 4 * s0;
+    out[1] = 4 * s1;
+    out[2] = 4 * s2;
+}
+
+__device__ __forceinline__ void load_coarse(const float* src, int cw, int ch, std::size_t cn, int cx0, int cy0,
+                                            float (&dst)[3][kCH][kCW]) {
+    for (int i = threadIdx.x; i < kCW * kCH; i += kBlock) {
+        const int cx = clampi(cx0 + i % kCW, 0, cw - 1), cy = clampi(cy0 + i / kCW, 0, ch - 1);
+        const size_t q = static_cast<size_t>(cy) * cw + cx;
+        dst[0][i / kCW][i % kCW] = src[q];
+        dst[1][i / kCW][i % kCW] = src[cn + q];
+        dst[2][i / kCW][i % kCW] = src[2 * cn + q];
+    }
+}
+
+// normalize_color + push_missing for level l, top-down (s6.cu:254-270,
+// :399-410). At level 0 the validity is exactly 0 or 1, so valid pixels keep
+// their colour (x / 1 == x) and only missing ones are written.
 __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, long long total, int l) {
+    __shared__ float ca[3][kCH][kCW], cb[3][kCH][kCW];
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = find_job_by(span, t, [l](const S6Job& j) { return j.pb[l]; });
         const S6Job& j = dev::job_at(span, ji);
-        const long long p = (t - j.pb[l]) * kBlock + threadIdx.x;
         const S6Level& L = j.lv[l];
-        if (p >= static_cast<long long>(L.w) * L.h) continue;
-        const int x = static_cast<int>(p % L.w), y = static_cast<int>(p / L.w);
+        const unsigned local = static_cast<unsigned>(t - j.pb[l]);
+        const unsigned tx = (static_cast<unsigned>(L.w) + kEW - 1) / kEW;
+        const int x0 = static_cast<int>(local % tx) * kEW, y0 = static_cast<int>(local / tx) * kEH;
         const bool coarse = l < j.top;
-        const float wa = L.va[p], wb = L.vb[p];
-        float e[3];
-        if (wa > 0) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) L.a[3 * p + c] = L.a[3 * p + c] / wa;
-        } else if (coarse) {
-            expanded3(j.lv[l + 1].a, j.lv[l + 1].w, j.lv[l + 1].h, x, y, e);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) L.a[3 * p + c] = e[c];
-        } else {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) L.a[3 * p + c] = 0;
+        const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 1;
+        if (coarse) {
+            const S6Level& C = j.lv[l + 1];
+            load_coarse(C.a, C.w, C.h, C.n, cx0, cy0, ca);
+            load_coarse(C.b, C.w, C.h, C.n, cx0, cy0, cb);
+            __syncthreads();
         }
-        if (wb > 0) {
+        const int x = x0 + static_cast<int>(threadIdx.x) % kEW, y = y0 + static_cast<int>(threadIdx.x) / kEW;
+        if (x < L.w && y < L.h) {
+            const size_t p = static_cast<size_t>(y) * L.w + x;
+            float wa, wb;
+            if (l == 0) {
+                const unsigned f = L.flags[p];
+                wa = (f & kVA) ? 1.0f : 0.0f;
+                wb = (f & kVB) ? 1.0f : 0.0f;
+            } else {
+                wa = L.va[p];
+                wb = L.vb[p];
+            }
+            ExpandTaps e;
+            if (coarse && (!(wa > 0) || !(wb > 0))) expand_setup(x, y, j.lv[l + 1].w, j.lv[l + 1].h, cx0, cy0, e);
+            float v[3];
+            if (wa > 0) {
+                if (l > 0)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) L.b[3 * p + c] = L.b[3 * p + c] / wb;
-        } else if (coarse) {
-            expanded3(j.lv[l + 1].b, j.lv[l + 1].w, j.lv[l + 1].h, x, y, e);
+                    for (int c = 0; c < 3; ++c) L.a[c * L.n + p] = L.a[c * L.n + p] / wa;
+            } else {
+                if (coarse) expand_lds(ca, e, v);
+                else v[0] = v[1] = v[2] = 0;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) L.b[3 * p + c] = e[c];
-        } else {
+                for (int c = 0; c < 3; ++c) L.a[c * L.n + p] = v[c];
+            }
+            if (wb > 0) {
+                if (l > 0)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) L.b[3 * p + c] = 0;
+                    for (int c = 0; c < 3; ++c) L.b[c * L.n + p] = L.b[c * L.n + p] / wb;
+            } else {
+                if (coarse) expand_lds(cb, e, v);
+                else v[0] = v[1] = v[2] = 0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) L.b[c * L.n + p] = v[c];
+            }
         }
+        if (coarse) __syncthreads();
     }
 }
 
 // mix_band + reconstruct (s6.cu:271-293) for level l, top-down; at level 0
-// also finish (s6.cu:294-306): single-source pixels keep their warped colour.
+// also finish (s6.cu:294-306), where single-source pixels keep their warped
+// colour and only overlap pixels need the band mix.
 __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, long long total, int l) {
+    __shared__ float ca[3][kCH][kCW], cb[3][kCH][kCW], cz[3][kCH][kCW];
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = find_job_by(span, t, [l](const S6Job& j) { return j.mb[l]; });
         const S6Job& j = dev::job_at(span, ji);
-        const long long p = (t - j.mb[l]) * kBlock + threadIdx.x;
         const S6Level& L = j.lv[l];
-        if (p >= static_cast<long long>(L.w) * L.h) continue;
-        const int x = static_cast<int>(p % L.w), y = static_cast<int>(p / L.w);
+        const unsigned local = static_cast<unsigned>(t - j.mb[l]);
+        const unsigned tx = (static_cast<unsigned>(L.w) + kEW - 1) / kEW;
+        const int x0 = static_cast<int>(local % tx) * kEW, y0 = static_cast<int>(local / tx) * kEH;
         const bool coarse = l < j.blend_top;
-        float la[3], lb[3], zc[3] = {0, 0, 0};
+        const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 1;
         if (coarse) {
             const S6Level& C = j.lv[l + 1];
-            float ea[3], eb[3];
-            expanded3(C.a, C.w, C.h, x, y, ea);
-            expanded3(C.b, C.w, C.h, x, y, eb);
-            expanded3(C.z, C.w, C.h, x, y, zc);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                la[c] = L.a[3 * p + c] - ea[c];
-                lb[c] = L.b[3 * p + c] - eb[c];
+            load_coarse(C.a, C.w, C.h, C.n, cx0, cy0, ca);
+            load_coarse(C.b, C.w, C.h, C.n, cx0, cy0, cb);
+            load_coarse(C.z, C.w, C.h, C.n, cx0, cy0, cz);
+            __syncthreads();
+        }
+        const int x = x0 + static_cast<int>(threadIdx.x) % kEW, y = y0 + static_cast<int>(threadIdx.x) / kEW;
+        if (x < L.w && y < L.h) {
+            const size_t p = static_cast<size_t>(y) * L.w + x;
+            float ua, ub;
+            bool both = true;
+            unsigned f = 0;
+            if (l == 0) {
+                f = L.flags[p];
+                ua = (f & kUA) ? 1.0f : 0.0f;
+                ub = (f & kUB) ? 1.0f : 0.0f;
+                both = (f & kVA) && (f & kVB);
+            } else {
+                ua = L.ua[p];
+                ub = L.ub[p];
             }
-        } else {
+            if (l == 0 && !both) {
+                std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                la[c] = L.a[3 * p + c] - 0.0f;
-                lb[c] = L.b[3 * p + c] - 0.0f;
+                for (int c = 0; c < 3; ++c)
+                    out[c] = quantize((f & kVA) ? L.a[c * L.n + p] : ((f & kVB) ? L.b[c * L.n + p] : 0));
+            } else {
+                float la[3], lb[3], zc[3] = {0, 0, 0};
+                if (coarse) {
+                    ExpandTaps e;
+                    expand_setup(x, y, j.lv[l + 1].w, j.lv[l + 1].h, cx0, cy0, e);
+                    float ea[3], eb[3];
+                    expand_lds(ca, e, ea);
+                    expand_lds(cb, e, eb);
+                    expand_lds(cz, e, zc);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        la[c] = L.a[c * L.n + p] - ea[c];
+                        lb[c] = L.b[c * L.n + p] - eb[c];
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        la[c] = L.a[c * L.n + p] - 0.0f;
+                        lb[c] = L.b[c * L.n + p] - 0.0f;
+                    }
+                }
+                const float tot = ua + ub;
+                float z[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const float m = tot > 0 ? (ua * la[c] + ub * lb[c]) / tot : 0;
+                    z[c] = coarse ? m + zc[c] : m;
+                }
+                if (l > 0) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) L.z[c * L.n + p] = z[c];
+                } else {
+                    std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) out[c] = quantize(z[c]);
+                }
             }
         }
-        const float ua = L.ua[p], ub = L.ub[p];
-        const float tot = ua + ub;
-        float z[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float m = tot > 0 ? (ua * la[c] + ub * lb[c]) / tot : 0;
-            z[c] = coarse ? m + zc[c] : m;
-        }
-        if (l > 0) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) L.z[3 * p + c] = z[c];
-        } else {
-            const float va = L.va[p], vb = L.vb[p];
-            std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float color = va > 0 && vb > 0 ? z[c] : (va > 0 ? L.a[3 * p + c] : (vb > 0 ? L.b[3 * p + c] : 0));
-                out[c] = quantize(color);
-            }
-        }
+        if (coarse) __syncthreads();
     }
 }
 
@@ -390,12 +546,18 @@ Status s6_layout(const s6::Geometry& g, s6::BlendMode mode, S6Layout* out) {
         const std::size_t n = static_cast<std::size_t>(w) * h;
         add(n, 3, o.a);
         add(n, 3, o.b);
-        add(n, 1, o.va);
-        add(n, 1, o.vb);
-        if (l <= requested) {
-            add(n, 1, o.ua);
-            add(n, 1, o.ub);
-            if (l > 0) add(n, 3, o.z);
+        if (l == 0) {
+            cur = align256(cur);
+            o.flags = cur;
+            cur += n;
+        } else {
+            add(n, 1, o.va);
+            add(n, 1, o.vb);
+            if (l <= requested) {
+                add(n, 1, o.ua);
+                add(n, 1, o.ub);
+                add(n, 3, o.z);
+            }
         }
         L.top = l;
         if (w == 1 && h == 1) break;
@@ -479,14 +641,19 @@ Status make_s6_job(const ImageView& a, const ImageView& b, const s6::Geometry& g
             S6Level& L = j.lv[l];
             L.w = o.w;
             L.h = o.h;
+            L.n = static_cast<std::size_t>(o.w) * o.h;
             L.a = f(o.a);
             L.b = f(o.b);
-            L.va = f(o.va);
-            L.vb = f(o.vb);
-            if (l <= layout.blend_top) {
-                L.ua = f(o.ua);
-                L.ub = f(o.ub);
-                L.z = l > 0 ? f(o.z) : nullptr;
+            if (l == 0) {
+                L.flags = base + o.flags;
+            } else {
+                L.va = f(o.va);
+                L.vb = f(o.vb);
+                if (l <= layout.blend_top) {
+                    L.ua = f(o.ua);
+                    L.ub = f(o.ub);
+                    L.z = f(o.z);
+                }
             }
         }
     }
@@ -501,14 +668,21 @@ S6Totals s6_prefix(S6Job* js, int n) {
     for (int i = 0; i < n; ++i) {
         S6Job& j = js[i];
         j.warp_begin = t.warp;
-        t.warp += (static_cast<long long>(j.width) * j.height + kBlock - 1) / kBlock;
+        t.warp += static_cast<long long>((j.width + kBlock - 1) / kBlock) * j.height;
         for (int l = 0; l < kS6MaxLevels; ++l) {
-            j.pb[l] = t.pull[l];
+            j.rb[l] = t.pull[l];
+            j.gb[l] = t.gauss[l];
+            j.pb[l] = t.push[l];
             j.mb[l] = t.mix[l];
-            if (j.top >= l) {
-                const long long tiles = (static_cast<long long>(j.lv[l].w) * j.lv[l].h + kBlock - 1) / kBlock;
-                t.pull[l] += tiles;
-                if (j.blend_top >= l) t.mix[l] += tiles;
+            if (j.top < l) continue;
+            const long long w = j.lv[l].w, h = j.lv[l].h;
+            const long long rtiles = ((w + kRW - 1) / kRW) * ((h + kRH - 1) / kRH);
+            const long long etiles = ((w + kEW - 1) / kEW) * ((h + kEH - 1) / kEH);
+            if (l >= 1) t.pull[l] += rtiles;
+            t.push[l] += etiles;
+            if (j.blend_top >= l) {
+                if (l >= 1) t.gauss[l] += rtiles;
+                t.mix[l] += etiles;
             }
         }
         t.top = std::max(t.top, j.top);
@@ -519,15 +693,15 @@ S6Totals s6_prefix(S6Job* js, int n) {
 
 hipError_t launch_s6(const S6Job* table, int count, const S6Totals& t, const S6Job* single, hipStream_t stream) {
     dev::JobSpan<S6Job> span{table, count, single ? *single : S6Job{}};
-    bool any_stats = single ? single->stats != nullptr : true;
+    const bool any_stats = single ? single->stats != nullptr : true;
     if (any_stats) s6_clear_stats<<<count, 64, 0, stream>>>(span);
     if (t.warp > 0) s6_warp<<<persistent_grid(t.warp), kBlock, 0, stream>>>(span, t.warp);
     for (int l = 1; l <= t.top; ++l)
         if (t.pull[l] > 0) s6_reduce<<<persistent_grid(t.pull[l]), kBlock, 0, stream>>>(span, t.pull[l], l, 0);
     for (int l = t.top; l >= 0; --l)
-        if (t.pull[l] > 0) s6_push<<<persistent_grid(t.pull[l]), kBlock, 0, stream>>>(span, t.pull[l], l);
+        if (t.push[l] > 0) s6_push<<<persistent_grid(t.push[l]), kBlock, 0, stream>>>(span, t.push[l], l);
     for (int l = 1; l <= t.blend_top; ++l)
-        if (t.mix[l] > 0) s6_reduce<<<persistent_grid(t.mix[l]), kBlock, 0, stream>>>(span, t.mix[l], l, 1);
+        if (t.gauss[l] > 0) s6_reduce<<<persistent_grid(t.gauss[l]), kBlock, 0, stream>>>(span, t.gauss[l], l, 1);
     for (int l = t.blend_top; l >= 0; --l)
         if (t.mix[l] > 0) s6_mix<<<persistent_grid(t.mix[l]), kBlock, 0, stream>>>(span, t.mix[l], l);
     return hipGetLastError();

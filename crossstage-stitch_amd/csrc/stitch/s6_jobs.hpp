@@ -9,11 +9,11 @@
 
 namespace stitch::jobs {
 
-constexpr int kS6MaxLevels = 32;
+constexpr int kS6MaxLevels = 24;  // canvases up to 2^23 pixels per side
 
 struct S6LevelOffsets {
     int w, h;
-    std::size_t a, b, va, vb, ua, ub, z;
+    std::size_t a, b, va, vb, ua, ub, z, flags;
 };
 
 struct S6Layout {
@@ -22,15 +22,19 @@ struct S6Layout {
     std::size_t bytes;
 };
 
+// Level 0 keeps validity and ownership as bits of one byte per pixel (they
+// are exactly 0 or 1 there); coarser levels hold FP32 planes.
 struct S6Level {
     int w, h;
-    float* a;   // RGB
-    float* b;   // RGB
-    float* va;
+    std::size_t n;  // w * h; colour planes are n floats apart
+    float* a;   // R, G, B planes
+    float* b;   // R, G, B planes
+    float* va;  // levels >= 1
     float* vb;
-    float* ua;
+    float* ua;  // levels 1..blend_top
     float* ub;
-    float* z;   // RGB, levels 1..blend_top
+    float* z;   // R, G, B planes, levels 1..blend_top
+    std::uint8_t* flags;  // level 0: bit 0 va, 1 vb, 2 ua, 3 ub
 };
 
 struct S6Job {
@@ -49,13 +53,17 @@ struct S6Job {
     int top, blend_top;  // -1 for feather
     S6Level lv[kS6MaxLevels];
     long long warp_begin;
-    long long pb[kS6MaxLevels];  // pull/push tiles of level l
-    long long mb[kS6MaxLevels];  // Gaussian/mix tiles of level l
+    long long rb[kS6MaxLevels];  // pull (reduce into level l) tiles
+    long long gb[kS6MaxLevels];  // Gaussian (reduce into level l) tiles
+    long long pb[kS6MaxLevels];  // push (fill level l) tiles
+    long long mb[kS6MaxLevels];  // mix/collapse (level l) tiles
 };
 
 struct S6Totals {
     long long warp;
     long long pull[kS6MaxLevels];
+    long long gauss[kS6MaxLevels];
+    long long push[kS6MaxLevels];
     long long mix[kS6MaxLevels];
     int top, blend_top;
 };

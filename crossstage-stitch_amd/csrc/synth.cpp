@@ -176,7 +176,10 @@ extern "C" int cs_synth_pair(std::uint64_t seed, int width, int height, double o
     const double W = width, H = height;
     const double target = rng.uniform(overlap_lo, overlap_hi);
     const double th = rng.uniform(-5, 5) * 3.141592653589793 / 180.0, s = rng.uniform(0.9, 1.1);
-    const double p1 = rng.uniform(-1e-4, 1e-4) * 1920.0 / W, p2 = rng.uniform(-1e-4, 1e-4) * 1080.0 / H;
+    // Perspective terms ~1e-4; above 1920 px the term is scaled down so the
+    // projective distortion across the frame stays that of a 1080p pair.
+    const double pscale = std::min(1.0, 1920.0 / W);
+    const double p1 = rng.uniform(-1e-4, 1e-4) * pscale, p2 = rng.uniform(-1e-4, 1e-4) * pscale;
     const double ty = rng.uniform(-0.05, 0.05) * H;
     double h[9] = {s * std::cos(th), -s * std::sin(th), 0, s * std::sin(th), s * std::cos(th), ty, p1, p2, 1};
     auto overlap = [&](double tx) {
