@@ -1,18 +1,22 @@
-"""Stitched pairs/s and p50/p99 latency on one MI355X: the plain per-pair GPU
-path (the reference's execute_run flow: one pair at a time, per-stage
-launches, two host synchronizations) beside the batched form, on the
-1920x1080 workload (ORB 5000, cross-checked 5000x5000 Hamming matching,
-RANSAC 4096, 6-band multiband blend, 30-50 % overlap), inputs resident in
-HBM before the timed region. Prints one JSON line.
+"""Stitched pairs/s and p50/p99 latency of the batched and served forms beside
+the plain per-pair GPU path, on one MI355X. Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-A step is one batch of --pairs synthetic pairs through S1-S6.
+Workload (value): the batched form on a ragged batch of 64 synthetic pairs
+- resolutions drawn from {640x480, 1280x720, 1920x1080, 3840x2160}, keypoint
+caps from U[500, 8000], overlap U[20 %, 80 %], cross-checked Hamming matching,
+RANSAC 4096, 6-band multiband blend. A step is that batch through S1-S6, with
+the inputs resident in HBM before the timed region. The same 64 pairs run
+through the plain per-pair runner (the reference's execute_run flow: one
+pair at a time, per-stage launches, two host synchronizations) for the ratio;
+the 1920x1080 subset is reported on its own. The served form replays Poisson
+arrival traces (adaptive legacy-v1 controller, 64 GB HBM budget, textured,
+low-texture, low-overlap and repetitive scenes) and reports p50/p99.
 """
 from __future__ import annotations
 
 import argparse
-import ctypes as C
 import json
 import os
 import sys
@@ -25,110 +29,257 @@ ROOT = Path(__file__).resolve().parent
 for p in (ROOT, ROOT / "crossstage-stitch_amd", ROOT / "tests"):
     sys.path.insert(0, str(p))
 
-W, H, K, R = 1920, 1080, 5000, 4096
-BLEND = "multiband_6"
+SIZES = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+R, BLEND = 4096, "multiband_6"
 SEED0 = 20261022
+POOL_BYTES = 48 << 30
+SERVE_BUDGET = 64 << 30
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--pairs", type=int, default=16, help="distinct synthetic pairs per step")
-    ap.add_argument("--cpu-pairs", type=int, default=1, help="pairs timed on the CPU oracle (0 = skip)")
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pairs", type=int, default=64, help="pairs per batch")
+    ap.add_argument("--serve-seconds", type=float, default=1.5, help="length of each arrival trace")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU oracle sample")
+    ap.add_argument("--tune-cache", default=None)
     return ap.parse_args()
 
 
-def make_pairs(n):
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def batch_workload(n):
+    """(w, h, K, overlap) per pair, seeded."""
+    rng = np.random.default_rng(SEED0)
+    out = []
+    for i in range(n):
+        w, h = SIZES[int(rng.integers(len(SIZES)))]
+        out.append((w, h, int(rng.integers(500, 8001)), float(rng.uniform(0.2, 0.8))))
+    return out
+
+
+def synth(seed, w, h, overlap, kind=0):
     from cs_stitch import pipeline as P
-    return [P.synth_pair(SEED0 + i, W, H, overlap=(0.3, 0.5)) for i in range(n)]
+    return P.synth_pair(seed, w, h, overlap=(overlap, overlap), kind=kind)
 
 
-def upload(pairs):
+def upload(a, b):
     import torch
     from cs_stitch import stages as st
-    dev = []
-    for a, b, _ in pairs:
-        ta, tb = st.pitched(H, W, 3), st.pitched(H, W, 3)
-        ta.copy_(torch.from_numpy(a).cuda())
-        tb.copy_(torch.from_numpy(b).cuda())
-        dev.append((ta, tb))
+    h, w = a.shape[:2]
+    ta, tb = st.pitched(h, w, 3), st.pitched(h, w, 3)
+    ta.copy_(torch.from_numpy(a).cuda())
+    tb.copy_(torch.from_numpy(b).cuda())
+    return ta, tb
+
+
+def specs_for(dev, work):
+    from cs_stitch import batch as B
+    return [B.pair_spec(ta, tb, S=1.0, K=k, R=R, blend=BLEND, cross_check=True) for (ta, tb), (_, _, k, _) in
+            zip(dev, work)]
+
+
+def time_batched(br, specs, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        br.run(specs)
     torch.cuda.synchronize()
-    return dev
+    times, last = [], None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s = time.perf_counter()
+        last, tm = br.run(specs)
+        times.append(time.perf_counter() - s)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    return total, times, last, tm
 
 
-def plain_path(dev_pairs, steps, warmup):
-    """One pair at a time through PipelineRunner.execute_run."""
+def plain_runners(dev, work, outcomes):
+    """One plain runner per canvas box, so every pair the batch stitched fits a
+    runner (a box may hold at most 16,777,216 pixels, like the reference)."""
     from cs_stitch import pipeline as P
-    runner = P.PipelineRunner(W, H, max_k=K, max_r=R, canvas_width=4 * W, canvas_height=2 * H,
-                              workspace_mode=P.F.BLEND_NAMES[BLEND])
-    cfg = P.harness_config(S=1.0, K=K, R=R, blend=BLEND, cross_check=True)
-    lat, ok, gpu_span = [], 0, []
-    stage = np.zeros(6)
+    boxes = {}
+    assign = []
+    for (ta, _), (w, h, k, _), o in zip(dev, work, outcomes):
+        cw = max(o.geometry.width, 1)
+        ch = max(o.geometry.height, 1)
+        key = (w, h, (cw + 255) // 256 * 256, (ch + 127) // 128 * 128)
+        if key[2] * key[3] > 16777216:
+            key = (w, h, cw, ch)
+        assign.append(key)
+        boxes.setdefault(key, 0)
+    runners = {}
+    for key in boxes:
+        w, h, cw, ch = key
+        runners[key] = P.PipelineRunner(w, h, max_k=8000, max_r=R, canvas_width=cw, canvas_height=ch,
+                                        workspace_mode=P.F.BLEND_NAMES[BLEND])
+    return runners, assign
+
+
+def time_plain(dev, work, runners, assign, passes=1):
+    import torch
+    from cs_stitch import pipeline as P
+    lat, ok = [], 0
     t_total = 0.0
-    for step in range(warmup + steps):
+    for rep in range(passes + 1):  # first pass warms every runner
         t0 = time.perf_counter()
-        for ta, tb in dev_pairs:
-            runner.use_device_pair(ta, tb)
-            rec = runner.execute_run(cfg)
-            if step >= warmup:
+        for (ta, tb), (_, _, k, _), key in zip(dev, work, assign):
+            r = runners[key]
+            r.use_device_pair(ta, tb)
+            rec = r.execute_run(P.harness_config(S=1.0, K=k, R=R, blend=BLEND, cross_check=True))
+            if rep:
                 lat.append(rec.t_e2e)
-                gpu_span.append(rec.t_gpu_span)
                 ok += rec.status == b"OK"
-                stage += [rec.t_s1, rec.t_s2, rec.t_s3, rec.t_s4, rec.t_s5, rec.t_s6]
-        if step >= warmup:
+        torch.cuda.synchronize()
+        if rep:
             t_total += time.perf_counter() - t0
-    n = steps * len(dev_pairs)
-    runner.close()
-    return dict(pairs_per_s=n / t_total, ms_per_step=1e3 * t_total / steps, p50_ms=float(np.percentile(lat, 50)),
-                p99_ms=float(np.percentile(lat, 99)), ok=ok, n=n, gpu_span_ms=float(np.median(gpu_span)),
-                stage_ms=dict(zip(["s1", "s2", "s3", "s4", "s5", "s6"], (stage / n).round(4).tolist())))
+    n = passes * len(dev)
+    return dict(pairs_per_s=n / t_total, ms_per_pair=1e3 * t_total / n, p50_ms=float(np.percentile(lat, 50)),
+                p99_ms=float(np.percentile(lat, 99)), ok=int(ok), n=n)
 
 
-def cpu_baseline(pairs, n):
+def serve_pool(n):
+    """Served scenes: sizes as the batch; half textured at 20-80 % overlap, the
+    rest hard: low texture (blobs / 10), low overlap (10-25 %), repetitive."""
+    rng = np.random.default_rng(SEED0 + 7)
+    out = []
+    for i in range(n):
+        w, h = SIZES[int(rng.integers(len(SIZES)))]
+        c = i % 6
+        kind, ov = (0, float(rng.uniform(0.2, 0.8))) if c < 3 else (
+            (1, float(rng.uniform(0.2, 0.8))) if c == 3 else ((0, float(rng.uniform(0.1, 0.25))) if c == 4 else
+                                                               (2, float(rng.uniform(0.2, 0.8)))))
+        out.append((w, h, kind, ov))
+    return out
+
+
+def run_serving(dev, rates, seconds):
+    from cs_stitch import batch as B
+    srv = B.Server(hbm_budget=SERVE_BUDGET, max_batch=64, max_live=256, adaptive=True)
+    res = {}
+    warm = [B.request(i, *dev[i % len(dev)], 0.0, seed=0) for i in range(min(8, len(dev)))]
+    srv.run_trace(warm)
+    for rate in rates:
+        n = max(8, int(rate * seconds))
+        arr = B.poisson_arrivals(rate, n, seed=int(rate))
+        reqs = [B.request(i, *dev[i % len(dev)], t, seed=0) for i, t in enumerate(arr)]
+        done, st = srv.run_trace(reqs)
+        statuses = {}
+        for c in done:
+            statuses[c.status.decode()] = statuses.get(c.status.decode(), 0) + 1
+        res[f"poisson_{int(rate)}"] = dict(requests=n, pairs_per_s=round(n / (st.makespan_ms / 1e3), 2),
+                                           p50_ms=round(st.p50_ms, 3), p99_ms=round(st.p99_ms, 3),
+                                           mean_feature_batch=round(st.mean_feature_batch, 2),
+                                           mean_render_batch=round(st.mean_render_batch, 2), statuses=statuses,
+                                           peak_pool_gb=round(st.peak_pool_bytes / 2 ** 30, 2))
+        log("served", rate, res[f"poisson_{int(rate)}"])
+    # Saturated: every request queued at t = 0 (capacity of the loop).
+    n = 4 * len(dev)
+    reqs = [B.request(i, *dev[i % len(dev)], 0.0, seed=0) for i in range(n)]
+    done, st = srv.run_trace(reqs)
+    res["saturated"] = dict(requests=n, pairs_per_s=round(n / (st.makespan_ms / 1e3), 2), p50_ms=round(st.p50_ms, 3),
+                            p99_ms=round(st.p99_ms, 3), mean_feature_batch=round(st.mean_feature_batch, 2),
+                            mean_render_batch=round(st.mean_render_batch, 2))
+    log("served saturated", res["saturated"])
+    srv.close()
+    return res
+
+
+def cpu_baseline(pairs, work, seconds):
+    """Single-thread CPU oracle on the first pairs of the batch until the budget is spent."""
     import pipeline_oracle
     t0 = time.perf_counter()
-    for a, b, _ in pairs[:n]:
-        pipeline_oracle.run(a, b, 1.0, K, R, BLEND, True)
+    n = 0
+    for (a, b, _), (w, h, k, _) in zip(pairs, work):
+        pipeline_oracle.run(a, b, 1.0, k, R, BLEND, True)
+        n += 1
+        if time.perf_counter() - t0 > seconds:
+            break
     dt = time.perf_counter() - t0
-    return dict(value=n / dt, unit="pairs/s", cores=1, kind="port",
-                sample=f"{n} pair(s) of {W}x{H}, K={K}, R={R}, cross-check, {BLEND}, single-thread CPU oracle "
-                       f"({dt:.1f} s)")
+    sizes = ", ".join(f"{w}x{h}" for (w, h, _, _) in work[:n])
+    return dict(value=round(n / dt, 4), unit="pairs/s", cores=1, kind="port",
+                sample=f"first {n} pair(s) of the batch ({sizes}; K caps {[k for _, _, k, _ in work[:n]]}, R={R}, "
+                       f"cross-check, {BLEND}) through the single-thread C oracle, {dt:.1f} s")
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): every pair is independent, "
-              "so the per-GPU figure is the measurement; see DESIGN.md for how the path shards.", file=sys.stderr)
+        print(f"bench.py measures one GPU (--gpus {args.gpus} requested): pairs are independent, so the "
+              "one-GPU figure is the measurement and the path shards by request across GPUs.", file=sys.stderr)
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     from cs_stitch import _ffi
+    from cs_stitch import batch as B
     _ffi.load()
-    pairs = make_pairs(args.pairs)
-    dev = upload(pairs)
-    plain = plain_path(dev, args.steps, args.warmup)
-    cpu = None if args.no_cpu or args.cpu_pairs <= 0 else cpu_baseline(pairs, args.cpu_pairs)
+    work = batch_workload(args.pairs)
+    log("synthesizing", len(work), "pairs")
+    pairs = [synth(SEED0 + i, w, h, ov) for i, (w, h, _, ov) in enumerate(work)]
+    dev = [upload(a, b) for a, b, _ in pairs]
+    torch.cuda.synchronize()
+    specs = specs_for(dev, work)
+    br = B.BatchRunner(max_pairs=args.pairs, pool_bytes=POOL_BYTES)
+    tune = br.autotune(specs, args.tune_cache)
+    log("tuning", tune.s4_segment, tune.workgroups_per_cu, tune.graphs)
+    total, times, outcomes, tm = time_batched(br, specs, args.steps, args.warmup)
+    ok = sum(o.status == 0 for o in outcomes)
+    batched = dict(pairs_per_s=args.pairs * args.steps / total, ms_per_step=1e3 * total / args.steps,
+                   p50_ms=1e3 * float(np.percentile(times, 50)), p99_ms=1e3 * float(np.percentile(times, 99)),
+                   feature_ms=tm.feature_ms, render_ms=tm.render_ms, host_ms=tm.host_ms, ok=ok)
+    log("batched", batched)
+    # 1920x1080 subset on its own.
+    idx = [i for i, (w, h, _, _) in enumerate(work) if (w, h) == (1920, 1080)]
+    sub_specs = [specs[i] for i in idx]
+    br.run(sub_specs)
+    t_sub, times_sub, _, _ = time_batched(br, sub_specs, args.steps, 0)
+    c1 = dict(pairs=len(idx), batched_pairs_per_s=len(idx) * args.steps / t_sub,
+              batched_p50_ms=1e3 * float(np.percentile(times_sub, 50)))
+    result = dict(batched=batched, c1_1080p=c1)
+    if not args.no_plain:
+        runners, assign = plain_runners(dev, work, outcomes)
+        plain = time_plain(dev, work, runners, assign)
+        log("plain", plain)
+        plain_sub = time_plain([dev[i] for i in idx], [work[i] for i in idx], runners, [assign[i] for i in idx])
+        c1["plain_pairs_per_s"] = plain_sub["pairs_per_s"]
+        c1["plain_p50_ms"] = plain_sub["p50_ms"]
+        c1["speedup"] = c1["batched_pairs_per_s"] / plain_sub["pairs_per_s"]
+        for r in runners.values():
+            r.close()
+        result["plain"] = plain
+        result["speedup_vs_plain"] = batched["pairs_per_s"] / plain["pairs_per_s"]
+    br.close()
+    if not args.no_serve:
+        pool = serve_pool(args.pairs)
+        sdev = [upload(*synth(SEED0 + 1000 + i, w, h, ov, kind)[:2]) for i, (w, h, kind, ov) in enumerate(pool)]
+        torch.cuda.synchronize()
+        result["served"] = run_serving(sdev, [50, 100, 200, 400], args.serve_seconds)
+    cpu = None if args.no_cpu else cpu_baseline(pairs, work, args.cpu_seconds)
     line = {
         "metric": "stitched pairs/s + p50/p99 latency, batched/served vs plain per-pair, 1 GPU",
-        "value": round(plain["pairs_per_s"], 3),
+        "value": round(batched["pairs_per_s"], 3),
         "unit": "pairs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(plain["ms_per_step"], 3),
+        "ms_per_step": round(batched["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8/int32/f32/f64",
         "data": "synthetic",
-        "config": {"workload": f"plain per-pair path, {W}x{H} RGB8 pairs, 30-50% overlap, ORB {K}, "
-                               f"cross-checked {K}x{K} Hamming, RANSAC {R}, {BLEND}",
-                   "pairs_per_step": args.pairs},
-        "plain": plain,
+        "config": {"workload": f"batched form, {args.pairs} pairs per step: sizes from "
+                               f"{{640x480,1280x720,1920x1080,3840x2160}}, K caps U[500,8000], overlap U[0.2,0.8], "
+                               f"cross-checked Hamming, RANSAC {R}, {BLEND}", "pairs_per_step": args.pairs},
+        **result,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -104,6 +104,56 @@ class RunRecord(C.Structure):
                 ("fail_stage", C.c_char * 16)]
 
 
+class PairSpec(C.Structure):
+    _fields_ = [("a", Image), ("b", Image), ("S", C.c_double), ("K", C.c_int), ("R", C.c_int),
+                ("blend_mode", C.c_int), ("fast_threshold", C.c_int), ("harris_k", C.c_float),
+                ("ratio_threshold", C.c_double), ("cross_check", C.c_int), ("msac_batch_size", C.c_int),
+                ("msac_refinements", C.c_int), ("msac_threshold", C.c_double), ("seed", C.c_uint64)]
+
+
+class PairOutcome(C.Structure):
+    _fields_ = [("status", C.c_int), ("fail_stage", C.c_char * 16), ("num_kp_a", C.c_int), ("num_kp_b", C.c_int),
+                ("scale_a", ScaleInfo), ("scale_b", ScaleInfo), ("homography", HomographyResult),
+                ("geometry", Geometry), ("statistics", RenderStatistics), ("canvas", Image), ("mask", Image),
+                ("keypoints_a", Keypoints), ("keypoints_b", Keypoints), ("descriptors_a", Descriptors),
+                ("descriptors_b", Descriptors), ("matches", Matches), ("inlier_mask", C.c_void_p)]
+
+
+class BatchTimings(C.Structure):
+    _fields_ = [("feature_ms", C.c_double), ("render_ms", C.c_double), ("host_ms", C.c_double),
+                ("total_ms", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("s4_segment", C.c_int), ("workgroups_per_cu", C.c_int), ("graphs", C.c_int)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_int), ("max_live", C.c_int), ("adaptive", C.c_int),
+                ("use_graphs", C.c_int), ("canvas_factor", C.c_double)]
+
+
+class Request(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("a", Image), ("b", Image), ("arrival_ms", C.c_double), ("seed", C.c_uint64),
+                ("fixed", PairSpec), ("keep_output", C.c_int)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_char * 32), ("fail_stage", C.c_char * 16),
+                ("arrival_ms", C.c_double), ("admitted_ms", C.c_double), ("done_ms", C.c_double),
+                ("attempts", C.c_int), ("S", C.c_double), ("K", C.c_int), ("R", C.c_int), ("blend_mode", C.c_int),
+                ("num_kp_a", C.c_int), ("num_kp_b", C.c_int), ("homography", HomographyResult),
+                ("geometry", Geometry), ("statistics", RenderStatistics), ("canvas", Image), ("mask", Image)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("completed", C.c_int), ("failed", C.c_int), ("makespan_ms", C.c_double), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("feature_launches", C.c_int),
+                ("render_launches", C.c_int), ("mean_feature_batch", C.c_double),
+                ("mean_render_batch", C.c_double), ("peak_pool_bytes", C.c_uint64),
+                ("feature_busy_ms", C.c_double), ("render_busy_ms", C.c_double)]
+
+
 P = C.POINTER
 vp = C.c_void_p
 sz = C.c_size_t
@@ -159,6 +209,16 @@ SIGNATURES = {
                                      P(HomographyResult), P(C.c_int), C.c_int, P(C.c_double)]),
     "cs_synth_pair": (C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, vp, vp,
                                 P(C.c_double)]),
+    "cs_batch_create": (C.c_int, [C.c_int, C.c_uint64, C.c_int, P(vp)]),
+    "cs_batch_destroy": (None, [vp]),
+    "cs_batch_run": (C.c_int, [vp, P(PairSpec), C.c_int, P(PairOutcome), P(BatchTimings)]),
+    "cs_batch_autotune": (C.c_int, [vp, P(PairSpec), C.c_int, C.c_char_p, P(Tuning)]),
+    "cs_batch_apply": (C.c_int, [vp, P(Tuning)]),
+    "cs_batch_pool": (C.c_int, [vp, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)]),
+    "cs_server_create": (C.c_int, [P(ServerConfig), P(vp)]),
+    "cs_server_destroy": (None, [vp]),
+    "cs_server_run_trace": (C.c_int, [vp, P(Request), C.c_int, P(Completion), P(ServeStats)]),
+    "cs_server_release": (C.c_int, [vp, C.c_uint64]),
 }
 
 _lib = None

@@ -58,7 +58,8 @@ __device__ std::uint8_t s1_pixel(const S1Job& p, int xd, int yd) {
 }
 
 // Persistent walk over 64 x 4 output tiles of every job; one pixel per thread.
-__global__ void __launch_bounds__(256) s1_kernel(dev::JobSpan<S1Job> span, long long total) {
+__global__ void __launch_bounds__(256) s1_kernel(dev::JobSpan<S1Job> span, dev::Total work) {
+    const long long total = work.get();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int j = dev::find_job(span, t, &S1Job::tile_begin);
         const S1Job& p = dev::job_at(span, j);
@@ -70,16 +71,11 @@ __global__ void __launch_bounds__(256) s1_kernel(dev::JobSpan<S1Job> span, long 
     }
 }
 
-struct GradJob {
-    const std::uint8_t* src;
-    int pitch, w, h;
-    unsigned long long* accum;
-    int tiles_x;
-    long long tile_begin;
-};
+using jobs::GradJob;
 
 // Sum of |I(x+1,y)-I(x,y)| + |I(x,y+1)-I(x,y)|: wave sums, one atomic per wave.
-__global__ void __launch_bounds__(256) gradient_kernel(dev::JobSpan<GradJob> span, long long total) {
+__global__ void __launch_bounds__(256) gradient_kernel(dev::JobSpan<GradJob> span, dev::Total work) {
+    const long long total = work.get();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int j = dev::find_job(span, t, &GradJob::tile_begin);
         const GradJob& g = dev::job_at(span, j);
@@ -102,6 +98,24 @@ __global__ void __launch_bounds__(256) gradient_kernel(dev::JobSpan<GradJob> spa
 // sequence stays safe to capture into graphs).
 __global__ void gradient_clear(unsigned long long* accum, int n) {
     if (threadIdx.x < n) accum[threadIdx.x] = 0;
+}
+
+// Per-job forms of the two helpers for job tables.
+__global__ void gradient_clear_jobs(dev::JobSpan<GradJob> span) {
+    const int jobs = span.jobs();
+    for (int ji = blockIdx.x; ji < jobs; ji += gridDim.x)
+        if (threadIdx.x == 0) *dev::job_at(span, ji).accum = 0;
+}
+
+__global__ void gradient_normalize_jobs(dev::JobSpan<GradJob> span) {
+    const int jobs = span.jobs();
+    for (int ji = blockIdx.x; ji < jobs; ji += gridDim.x) {
+        const GradJob& g = dev::job_at(span, ji);
+        if (threadIdx.x != 0) continue;
+        const unsigned long long edges =
+            static_cast<unsigned long long>(g.h) * (g.w - 1) + static_cast<unsigned long long>(g.w) * (g.h - 1);
+        *g.out = edges == 0 ? 0.0 : static_cast<double>(*g.accum) / (255.0 * static_cast<double>(edges));
+    }
 }
 
 __global__ void gradient_normalize(const unsigned long long* accum, int w0, int h0, int w1, int h1, int n,
@@ -149,10 +163,21 @@ Status make_s1_job(const ImageView& in, const ImageView& out, const ScaleInfo& s
     return Status::OK;
 }
 
-hipError_t launch_s1(const S1Job* table, int count, long long total, const S1Job* single, hipStream_t stream) {
+hipError_t launch_s1(const S1Job* table, int count, long long total, const S1Job* single, hipStream_t stream,
+                     const dev::Replay<long long>* replay) {
     if (total <= 0) return hipSuccess;
-    dev::JobSpan<S1Job> span{table, count, single ? *single : S1Job{}};
-    s1_kernel<<<persistent_grid(total), 256, 0, stream>>>(span, total);
+    dev::JobSpan<S1Job> span{table, count, single ? *single : S1Job{}, replay ? replay->count : nullptr};
+    s1_kernel<<<persistent_grid(total), 256, 0, stream>>>(span, dev::Total(total, replay ? replay->totals : nullptr));
+    return hipGetLastError();
+}
+
+hipError_t launch_gradient(const GradJob* table, int count, long long total, const GradJob* single,
+                           hipStream_t stream, const dev::Replay<long long>* replay) {
+    if (total <= 0) return hipSuccess;
+    dev::JobSpan<GradJob> span{table, count, single ? *single : GradJob{}, replay ? replay->count : nullptr};
+    gradient_clear_jobs<<<count, 64, 0, stream>>>(span);
+    gradient_kernel<<<persistent_grid(total), 256, 0, stream>>>(span, dev::Total(total, replay ? replay->totals : nullptr));
+    gradient_normalize_jobs<<<count, 64, 0, stream>>>(span);
     return hipGetLastError();
 }
 
@@ -206,7 +231,7 @@ Status gradient(const ImageView* views, int n, void* ws, double* const* pinned, 
     auto* out = reinterpret_cast<double*>(static_cast<std::uint8_t*>(ws) + 16);
     gradient_clear<<<1, 64, 0, stream>>>(accum, n);
     for (int i = 0; i < n; ++i) {
-        GradJob g{views[i].data, views[i].pitch_bytes, views[i].width, views[i].height, accum + i,
+        GradJob g{views[i].data, views[i].pitch_bytes, views[i].width, views[i].height, accum + i, out + i,
                   (views[i].width + kS1TileW - 1) / kS1TileW, 0};
         const long long tiles = static_cast<long long>(g.tiles_x) * ((g.h + kS1TileH - 1) / kS1TileH);
         gradient_kernel<<<persistent_grid(tiles), 256, 0, stream>>>(dev::JobSpan<GradJob>{nullptr, 1, g}, tiles);

@@ -35,14 +35,21 @@ constexpr int kScoreW = kS2TileW + 2, kScoreH = kS2TileH + 2;
 __constant__ int kCircleX[16] = {0, 1, 2, 3, 3, 3, 2, 1, 0, -1, -2, -3, -3, -3, -2, -1};
 __constant__ int kCircleY[16] = {3, 3, 2, 1, 0, -1, -2, -3, -3, -3, -2, -1, 0, 1, 2, 3};
 
-__global__ void s2_init(dev::JobSpan<S2Job> span) {
-    const S2Job& j = dev::job_at(span, blockIdx.x);
+__device__ __forceinline__ void s2_init_job(const S2Job& j) {
     if (!j.active) {
         if (threadIdx.x == 0) *j.d_count = 0;
         return;
     }
     for (int i = threadIdx.x; i < 256; i += blockDim.x) j.hist[i] = 0;
     for (int i = threadIdx.x; i < j.candidates; i += blockDim.x) j.ranks[i] = 0;
+}
+
+__global__ void s2_init(dev::JobSpan<S2Job> span) {
+    const int jobs = span.jobs();
+    for (int ji = blockIdx.x; ji < jobs; ji += gridDim.x) {
+        s2_init_job(dev::job_at(span, ji));
+        __syncthreads();
+    }
 }
 
 // cornerScore<16> on an LDS tile; g points at the centre pixel, row stride kGrayW.
@@ -75,7 +82,8 @@ __device__ int fast_score_lds(const std::uint8_t* g, int threshold) {
 }
 
 // FAST + NMS + histogram on 64x16 tiles with a 4-pixel gray halo in LDS.
-__global__ void __launch_bounds__(256) s2_fast_nms(dev::JobSpan<S2Job> span, long long total) {
+__global__ void __launch_bounds__(256) s2_fast_nms(dev::JobSpan<S2Job> span, dev::Total work) {
+    const long long total = work.get();
     __shared__ std::uint8_t gray[kGrayH * kGrayW];
     __shared__ std::uint8_t score[kScoreH * kScoreW];
     __shared__ std::uint32_t hist[256];
@@ -122,8 +130,7 @@ __global__ void __launch_bounds__(256) s2_fast_nms(dev::JobSpan<S2Job> span, lon
 
 // One wave per job: suffix[s] = #maxima with score >= s; the boundary score
 // of the 2K pre-selection is the largest s with suffix[s] >= 2K.
-__global__ void __launch_bounds__(64) s2_select(dev::JobSpan<S2Job> span) {
-    const S2Job& j = dev::job_at(span, blockIdx.x);
+__device__ __forceinline__ void s2_select_job(const S2Job& j) {
     if (!j.active) return;
     __shared__ std::uint32_t suffix[257];
     const int lane = threadIdx.x;
@@ -163,6 +170,14 @@ __global__ void __launch_bounds__(64) s2_select(dev::JobSpan<S2Job> span) {
             *j.params = p;
             *j.d_count = static_cast<int>(min(p.selected, static_cast<std::uint32_t>(j.K)));
         }
+    }
+}
+
+__global__ void __launch_bounds__(64) s2_select(dev::JobSpan<S2Job> span) {
+    const int jobs = span.jobs();
+    for (int ji = blockIdx.x; ji < jobs; ji += gridDim.x) {
+        s2_select_job(dev::job_at(span, ji));
+        __syncthreads();
     }
 }
 
@@ -218,7 +233,8 @@ __device__ Flags block_scan(Flags v, Flags* excl) {
     return total;
 }
 
-__global__ void __launch_bounds__(256) s2_count(dev::JobSpan<S2Job> span, long long total) {
+__global__ void __launch_bounds__(256) s2_count(dev::JobSpan<S2Job> span, dev::Total work) {
+    const long long total = work.get();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = dev::find_job(span, t, &S2Job::chunk_begin);
         const S2Job& j = dev::job_at(span, ji);
@@ -237,8 +253,7 @@ __global__ void __launch_bounds__(256) s2_count(dev::JobSpan<S2Job> span, long l
 }
 
 // One workgroup per job: exclusive prefix over the job's chunk totals.
-__global__ void __launch_bounds__(256) s2_scan(dev::JobSpan<S2Job> span) {
-    const S2Job& j = dev::job_at(span, blockIdx.x);
+__device__ __forceinline__ void s2_scan_job(const S2Job& j) {
     if (!j.active) return;
     const int per = (j.chunks + 255) / 256;
     const int first = threadIdx.x * per;
@@ -257,9 +272,18 @@ __global__ void __launch_bounds__(256) s2_scan(dev::JobSpan<S2Job> span) {
     }
 }
 
+__global__ void __launch_bounds__(256) s2_scan(dev::JobSpan<S2Job> span) {
+    const int jobs = span.jobs();
+    for (int ji = blockIdx.x; ji < jobs; ji += gridDim.x) {
+        s2_scan_job(dev::job_at(span, ji));
+        __syncthreads();
+    }
+}
+
 // Writes the pixel index of every pre-selected maximum to its candidate slot:
 // all points above the boundary score, then the first `ties` points at it.
-__global__ void __launch_bounds__(256) s2_scatter(dev::JobSpan<S2Job> span, long long total) {
+__global__ void __launch_bounds__(256) s2_scatter(dev::JobSpan<S2Job> span, dev::Total work) {
+    const long long total = work.get();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = dev::find_job(span, t, &S2Job::chunk_begin);
         const S2Job& j = dev::job_at(span, ji);
@@ -318,7 +342,8 @@ __device__ float harris_response(const std::uint8_t* image, int pitch, int x0, i
     return (det - penalty) * 0x1.bb9da2p-52f;
 }
 
-__global__ void __launch_bounds__(256) s2_harris(dev::JobSpan<S2Job> span, long long total) {
+__global__ void __launch_bounds__(256) s2_harris(dev::JobSpan<S2Job> span, dev::Total work) {
+    const long long total = work.get();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = dev::find_job(span, t, &S2Job::cand_begin);
         const S2Job& j = dev::job_at(span, ji);
@@ -338,7 +363,8 @@ __global__ void __launch_bounds__(256) s2_harris(dev::JobSpan<S2Job> span, long 
 
 // rank(key) = #keys greater. Tile = 64 keys x one 2048-key segment; 4 waves
 // split the segment and the partial counts are added atomically.
-__global__ void __launch_bounds__(256) s2_rank(dev::JobSpan<S2Job> span, long long total) {
+__global__ void __launch_bounds__(256) s2_rank(dev::JobSpan<S2Job> span, dev::Total work) {
+    const long long total = work.get();
     __shared__ std::uint64_t seg[kS2RankSegment];
     __shared__ std::uint32_t part[4][kS2RankKeys];
     const int lane = dev::lane_id(), wave = threadIdx.x / 64;
@@ -370,7 +396,8 @@ __global__ void __launch_bounds__(256) s2_rank(dev::JobSpan<S2Job> span, long lo
     }
 }
 
-__global__ void __launch_bounds__(256) s2_write(dev::JobSpan<S2Job> span, long long total) {
+__global__ void __launch_bounds__(256) s2_write(dev::JobSpan<S2Job> span, dev::Total work) {
+    const long long total = work.get();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = dev::find_job(span, t, &S2Job::cand_begin);
         const S2Job& j = dev::job_at(span, ji);
@@ -475,18 +502,22 @@ S2Totals s2_prefix(S2Job* js, int n) {
     return t;
 }
 
-hipError_t launch_s2(const S2Job* table, int count, const S2Totals& t, const S2Job* single, hipStream_t stream) {
-    dev::JobSpan<S2Job> span{table, count, single ? *single : S2Job{}};
+hipError_t launch_s2(const S2Job* table, int count, const S2Totals& t, const S2Job* single, hipStream_t stream,
+                     const dev::Replay<S2Totals>* replay) {
+    dev::JobSpan<S2Job> span{table, count, single ? *single : S2Job{}, replay ? replay->count : nullptr};
+    const auto total = [&](long long S2Totals::*m) {
+        return dev::Total(t.*m, replay ? &(replay->totals->*m) : nullptr);
+    };
     s2_init<<<count, 256, 0, stream>>>(span);
     if (t.fast > 0) {
-        s2_fast_nms<<<persistent_grid(t.fast), 256, 0, stream>>>(span, t.fast);
+        s2_fast_nms<<<persistent_grid(t.fast), 256, 0, stream>>>(span, total(&S2Totals::fast));
         s2_select<<<count, 64, 0, stream>>>(span);
-        s2_count<<<persistent_grid(t.chunks), 256, 0, stream>>>(span, t.chunks);
+        s2_count<<<persistent_grid(t.chunks), 256, 0, stream>>>(span, total(&S2Totals::chunks));
         s2_scan<<<count, 256, 0, stream>>>(span);
-        s2_scatter<<<persistent_grid(t.chunks), 256, 0, stream>>>(span, t.chunks);
-        s2_harris<<<persistent_grid(t.cands), 256, 0, stream>>>(span, t.cands);
-        s2_rank<<<persistent_grid(t.rank), 256, 0, stream>>>(span, t.rank);
-        s2_write<<<persistent_grid(t.cands), 256, 0, stream>>>(span, t.cands);
+        s2_scatter<<<persistent_grid(t.chunks), 256, 0, stream>>>(span, total(&S2Totals::chunks));
+        s2_harris<<<persistent_grid(t.cands), 256, 0, stream>>>(span, total(&S2Totals::cands));
+        s2_rank<<<persistent_grid(t.rank), 256, 0, stream>>>(span, total(&S2Totals::rank));
+        s2_write<<<persistent_grid(t.cands), 256, 0, stream>>>(span, total(&S2Totals::cands));
     }
     return hipGetLastError();
 }

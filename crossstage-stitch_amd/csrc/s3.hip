@@ -38,7 +38,8 @@ __device__ int reflect101(int i, int n) {
 
 // Row pass then column pass, taps accumulated 0..6, each product and sum
 // rounded separately; round half to even, clamp to [0, 255].
-__global__ void __launch_bounds__(256) s3_blur(dev::JobSpan<S3Job> span, long long total) {
+__global__ void __launch_bounds__(256) s3_blur(dev::JobSpan<S3Job> span, dev::Total work) {
+    const long long total = work.get();
     __shared__ float in[kInH * kInW];
     __shared__ float rows[kInH * kS3TileW];
     const int tid = threadIdx.x;
@@ -102,7 +103,8 @@ __device__ __forceinline__ int sample(const std::uint8_t* centre, int width, int
 // One wave per keypoint. Lanes 0..61 split the 31 patch rows in halves for the
 // moments (integer sums, order-free); lane l then evaluates pairs l, 64+l,
 // 128+l, 192+l and each ballot yields descriptor bytes 8r..8r+7.
-__global__ void __launch_bounds__(64 * kS3KeypointsPerBlock) s3_describe(dev::JobSpan<S3Job> span, long long total) {
+__global__ void __launch_bounds__(64 * kS3KeypointsPerBlock) s3_describe(dev::JobSpan<S3Job> span, dev::Total work) {
+    const long long total = work.get();
     const int lane = dev::lane_id();
     const int wave = threadIdx.x / 64;
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
@@ -199,11 +201,15 @@ S3Totals s3_prefix(S3Job* js, int n) {
     return t;
 }
 
-hipError_t launch_s3(const S3Job* table, int count, const S3Totals& t, const S3Job* single, hipStream_t stream) {
-    dev::JobSpan<S3Job> span{table, count, single ? *single : S3Job{}};
-    if (t.blur > 0) s3_blur<<<persistent_grid(t.blur), 256, 0, stream>>>(span, t.blur);
+hipError_t launch_s3(const S3Job* table, int count, const S3Totals& t, const S3Job* single, hipStream_t stream,
+                     const dev::Replay<S3Totals>* replay) {
+    dev::JobSpan<S3Job> span{table, count, single ? *single : S3Job{}, replay ? replay->count : nullptr};
+    const auto total = [&](long long S3Totals::*m) {
+        return dev::Total(t.*m, replay ? &(replay->totals->*m) : nullptr);
+    };
+    if (t.blur > 0) s3_blur<<<persistent_grid(t.blur), 256, 0, stream>>>(span, total(&S3Totals::blur));
     if (t.kp > 0)
-        s3_describe<<<persistent_grid(t.kp, 16), 64 * kS3KeypointsPerBlock, 0, stream>>>(span, t.kp);
+        s3_describe<<<persistent_grid(t.kp, 16), 64 * kS3KeypointsPerBlock, 0, stream>>>(span, total(&S3Totals::kp));
     return hipGetLastError();
 }
 

@@ -61,7 +61,8 @@ __device__ __forceinline__ unsigned wave_min_dpp(unsigned v) {
 
 // Distance tiles of the fast path: (query block of 512, train segment).
 template <bool kCrossAny>
-__global__ void __launch_bounds__(256) s4_dist(dev::JobSpan<S4Job> span, long long total) {
+__global__ void __launch_bounds__(256) s4_dist(dev::JobSpan<S4Job> span, dev::Total work) {
+    const long long total = work.get();
     __shared__ uint4 train[2 * kMaxSegment];
     __shared__ unsigned colmin[kMaxSegment];
     const int lane = dev::lane_id(), wave = threadIdx.x / 64;
@@ -139,7 +140,8 @@ __global__ void __launch_bounds__(256) s4_dist(dev::JobSpan<S4Job> span, long lo
 __device__ Top2 empty_top2() { return {INFINITY, INFINITY, -1, -1}; }
 
 // Forward merge over train segments, one thread per query; writes Top2.
-__global__ void __launch_bounds__(256) s4_qmerge(dev::JobSpan<S4Job> span, long long total) {
+__global__ void __launch_bounds__(256) s4_qmerge(dev::JobSpan<S4Job> span, dev::Total work) {
+    const long long total = work.get();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = dev::find_job(span, t, &S4Job::qmerge_begin);
         const S4Job& j = dev::job_at(span, ji);
@@ -169,7 +171,8 @@ __global__ void __launch_bounds__(256) s4_qmerge(dev::JobSpan<S4Job> span, long 
 }
 
 // Reverse top-1 per train row: minimum key over query blocks.
-__global__ void __launch_bounds__(256) s4_cmerge(dev::JobSpan<S4Job> span, long long total) {
+__global__ void __launch_bounds__(256) s4_cmerge(dev::JobSpan<S4Job> span, dev::Total work) {
+    const long long total = work.get();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = dev::find_job(span, t, &S4Job::cmerge_begin);
         const S4Job& j = dev::job_at(span, ji);
@@ -207,8 +210,9 @@ __device__ void insert_top2(Top2& p, double d, int i) {
 // Generic path (FLOAT32, other dimensions or alignments): one thread per
 // query, distances accumulated in FP64 in dimension order as the reference.
 // reverse == 1 searches B rows against A and writes reverse/rev_first.
-__global__ void __launch_bounds__(256) s4_generic(dev::JobSpan<S4Job> span, long long total, int reverse,
+__global__ void __launch_bounds__(256) s4_generic(dev::JobSpan<S4Job> span, dev::Total work, int reverse,
                                                   Top2* external) {
+    const long long total = work.get();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = dev::find_job(span, t, reverse ? &S4Job::grev_begin : &S4Job::gfwd_begin);
         const S4Job& j = dev::job_at(span, ji);
@@ -253,9 +257,8 @@ __global__ void __launch_bounds__(256) s4_generic(dev::JobSpan<S4Job> span, long
 }
 
 // Accept and compact in A order; one workgroup of 1024 threads per job.
-__global__ void __launch_bounds__(1024) s4_compact(dev::JobSpan<S4Job> span) {
+__device__ __forceinline__ void s4_compact_job(const S4Job& j) {
     __shared__ int wave_tot[16];
-    const S4Job& j = dev::job_at(span, blockIdx.x);
     const int tid = threadIdx.x, lane = dev::lane_id(), wave = tid / 64;
     const int per = (j.cap_a + 1023) / 1024;
     const int first = tid * per;
@@ -299,6 +302,14 @@ __global__ void __launch_bounds__(1024) s4_compact(dev::JobSpan<S4Job> span) {
     if (tid == 0) *j.d_count = total;
 }
 
+__global__ void __launch_bounds__(1024) s4_compact(dev::JobSpan<S4Job> span) {
+    const int jobs = span.jobs();
+    for (int ji = blockIdx.x; ji < jobs; ji += gridDim.x) {
+        s4_compact_job(dev::job_at(span, ji));
+        __syncthreads();
+    }
+}
+
 bool valid_input(const Keypoints& k, const Descriptors& d) {
     if (k.capacity < 0 || !k.d_count || reinterpret_cast<std::uintptr_t>(k.d_count) % alignof(int) != 0 ||
         d.dim <= 0 || d.row_stride_bytes <= 0)
@@ -331,6 +342,14 @@ void s4_set_segment(int rows) {
     if (rows >= kMinSegment && rows <= kMaxSegment && rows % 64 == 0) g_segment = rows;
 }
 int s4_segment() { return g_segment; }
+
+namespace {
+int g_workgroups = 8;
+}
+void set_workgroups_per_cu(int eighths) {
+    if (eighths >= 1 && eighths <= 64) g_workgroups = eighths;
+}
+int workgroups_per_cu() { return g_workgroups; }
 
 // Offsets: forward, reverse, rev_first, part_fwd, part_col.
 std::size_t s4_layout_bytes(int cap_a, int cap_b, bool cc, int segment, std::size_t off[5]) {
@@ -414,18 +433,24 @@ S4Totals s4_prefix(S4Job* js, int n) {
     return t;
 }
 
-hipError_t launch_s4(const S4Job* table, int count, const S4Totals& t, const S4Job* single, hipStream_t stream) {
-    dev::JobSpan<S4Job> span{table, count, single ? *single : S4Job{}};
+hipError_t launch_s4(const S4Job* table, int count, const S4Totals& t, const S4Job* single, hipStream_t stream,
+                     const dev::Replay<S4Totals>* replay) {
+    dev::JobSpan<S4Job> span{table, count, single ? *single : S4Job{}, replay ? replay->count : nullptr};
+    const auto total = [&](long long S4Totals::*m) {
+        return dev::Total(t.*m, replay ? &(replay->totals->*m) : nullptr);
+    };
     // Tables may mix jobs with and without cross-check; the kernel tests each job.
     const bool any_cross = single ? static_cast<bool>(single->cross_check) : true;
     if (t.dist > 0) {
-        if (any_cross) s4_dist<true><<<persistent_grid(t.dist), 256, 0, stream>>>(span, t.dist);
-        else s4_dist<false><<<persistent_grid(t.dist), 256, 0, stream>>>(span, t.dist);
-        s4_qmerge<<<persistent_grid(t.qmerge), 256, 0, stream>>>(span, t.qmerge);
-        if (t.cmerge > 0) s4_cmerge<<<persistent_grid(t.cmerge), 256, 0, stream>>>(span, t.cmerge);
+        if (any_cross) s4_dist<true><<<persistent_grid(t.dist), 256, 0, stream>>>(span, total(&S4Totals::dist));
+        else s4_dist<false><<<persistent_grid(t.dist), 256, 0, stream>>>(span, total(&S4Totals::dist));
+        s4_qmerge<<<persistent_grid(t.qmerge), 256, 0, stream>>>(span, total(&S4Totals::qmerge));
+        if (t.cmerge > 0) s4_cmerge<<<persistent_grid(t.cmerge), 256, 0, stream>>>(span, total(&S4Totals::cmerge));
     }
-    if (t.generic_fwd > 0) s4_generic<<<persistent_grid(t.generic_fwd), 256, 0, stream>>>(span, t.generic_fwd, 0, nullptr);
-    if (t.generic_rev > 0) s4_generic<<<persistent_grid(t.generic_rev), 256, 0, stream>>>(span, t.generic_rev, 1, nullptr);
+    if (t.generic_fwd > 0)
+        s4_generic<<<persistent_grid(t.generic_fwd), 256, 0, stream>>>(span, total(&S4Totals::generic_fwd), 0, nullptr);
+    if (t.generic_rev > 0)
+        s4_generic<<<persistent_grid(t.generic_rev), 256, 0, stream>>>(span, total(&S4Totals::generic_rev), 1, nullptr);
     s4_compact<<<count, 1024, 0, stream>>>(span);
     return hipGetLastError();
 }

@@ -253,8 +253,7 @@ __device__ bool same_scale(const ScaleInfo& a, const ScaleInfo& b) {
 }
 
 // prepare (s5.cu:42-83): state reset or reuse check, then validation.
-__global__ void __launch_bounds__(256) s5_prepare(dev::JobSpan<S5Job> span) {
-    const S5Job& j = dev::job_at(span, blockIdx.x);
+__device__ __forceinline__ void s5_prepare_job(const S5Job& j) {
     S5State* s = j.state;
     __shared__ int proceed, na_s, nb_s, nm_s;
     if (threadIdx.x == 0) {
@@ -304,8 +303,17 @@ __global__ void __launch_bounds__(256) s5_prepare(dev::JobSpan<S5Job> span) {
     }
 }
 
+__global__ void __launch_bounds__(256) s5_prepare(dev::JobSpan<S5Job> span) {
+    const int jobs = span.jobs();
+    for (int ji = blockIdx.x; ji < jobs; ji += gridDim.x) {
+        s5_prepare_job(dev::job_at(span, ji));
+        __syncthreads();
+    }
+}
+
 // One lane per hypothesis: sample, degeneracy test, 4-point solve, domain.
-__global__ void __launch_bounds__(256) s5_generate(dev::JobSpan<S5Job> span, long long total) {
+__global__ void __launch_bounds__(256) s5_generate(dev::JobSpan<S5Job> span, dev::Total work) {
+    const long long total = work.get();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = dev::find_job(span, t, &S5Job::gen_begin);
         const S5Job& j = dev::job_at(span, ji);
@@ -328,7 +336,8 @@ __global__ void __launch_bounds__(256) s5_generate(dev::JobSpan<S5Job> span, lon
 
 // One wave per hypothesis: lane l accumulates the reference's virtual lanes l
 // and l + 64 (strided by 128, in order), then the halving tree 64, 32, ..., 1.
-__global__ void __launch_bounds__(256) s5_score(dev::JobSpan<S5Job> span, long long total) {
+__global__ void __launch_bounds__(256) s5_score(dev::JobSpan<S5Job> span, dev::Total work) {
+    const long long total = work.get();
     const int lane = dev::lane_id(), wave = threadIdx.x / 64;
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = dev::find_job(span, t, &S5Job::score_begin);
@@ -372,8 +381,7 @@ __device__ __forceinline__ bool better(const S5Model& a, const S5Model& b) {
 }
 
 // Best model of this call merged into the persistent raw winner.
-__global__ void __launch_bounds__(256) s5_select(dev::JobSpan<S5Job> span) {
-    const S5Job& j = dev::job_at(span, blockIdx.x);
+__device__ __forceinline__ void s5_select_job(const S5Job& j) {
     __shared__ S5Model best[256];
     S5Model mine{};
     for (int i = threadIdx.x; i < j.count; i += blockDim.x)
@@ -386,6 +394,14 @@ __global__ void __launch_bounds__(256) s5_select(dev::JobSpan<S5Job> span) {
         __syncthreads();
     }
     if (threadIdx.x == 0 && j.state->status == Status::OK && better(best[0], j.state->best)) j.state->best = best[0];
+}
+
+__global__ void __launch_bounds__(256) s5_select(dev::JobSpan<S5Job> span) {
+    const int jobs = span.jobs();
+    for (int ji = blockIdx.x; ji < jobs; ji += gridDim.x) {
+        s5_select_job(dev::job_at(span, ji));
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------- finish
@@ -538,8 +554,7 @@ __device__ bool fit_inliers(const S5Job& j, int nm, const double* model, double 
     return ok;
 }
 
-__global__ void __launch_bounds__(256) s5_finish(dev::JobSpan<S5Job> span) {
-    const S5Job& j = dev::job_at(span, blockIdx.x);
+__device__ __forceinline__ void s5_finish_job(const S5Job& j) {
     S5State* s = j.state;
     __shared__ FinishShared sh;
     __shared__ HomographyResult result;
@@ -633,6 +648,14 @@ __global__ void __launch_bounds__(256) s5_finish(dev::JobSpan<S5Job> span) {
     }
 }
 
+__global__ void __launch_bounds__(256) s5_finish(dev::JobSpan<S5Job> span) {
+    const int jobs = span.jobs();
+    for (int ji = blockIdx.x; ji < jobs; ji += gridDim.x) {
+        s5_finish_job(dev::job_at(span, ji));
+        __syncthreads();
+    }
+}
+
 bool valid_keypoints(const Keypoints& p) { return p.capacity >= 0 && p.d_count && (p.capacity == 0 || (p.x && p.y)); }
 
 bool valid_scale(const ScaleInfo& s) {
@@ -694,11 +717,15 @@ S5Totals s5_prefix(S5Job* js, int n) {
     return t;
 }
 
-hipError_t launch_s5(const S5Job* table, int count, const S5Totals& t, const S5Job* single, hipStream_t stream) {
-    dev::JobSpan<S5Job> span{table, count, single ? *single : S5Job{}};
+hipError_t launch_s5(const S5Job* table, int count, const S5Totals& t, const S5Job* single, hipStream_t stream,
+                     const dev::Replay<S5Totals>* replay) {
+    dev::JobSpan<S5Job> span{table, count, single ? *single : S5Job{}, replay ? replay->count : nullptr};
+    const auto total = [&](long long S5Totals::*m) {
+        return dev::Total(t.*m, replay ? &(replay->totals->*m) : nullptr);
+    };
     s5_prepare<<<count, 256, 0, stream>>>(span);
-    if (t.gen > 0) s5_generate<<<persistent_grid(t.gen), 256, 0, stream>>>(span, t.gen);
-    if (t.score > 0) s5_score<<<persistent_grid(t.score, 16), 256, 0, stream>>>(span, t.score);
+    if (t.gen > 0) s5_generate<<<persistent_grid(t.gen), 256, 0, stream>>>(span, total(&S5Totals::gen));
+    if (t.score > 0) s5_score<<<persistent_grid(t.score, 16), 256, 0, stream>>>(span, total(&S5Totals::score));
     s5_select<<<count, 256, 0, stream>>>(span);
     s5_finish<<<count, 256, 0, stream>>>(span);
     return hipGetLastError();

@@ -78,7 +78,7 @@ __device__ __forceinline__ std::uint8_t quantize(float x) {
 template <class Begin>
 __device__ int find_job_by(const dev::JobSpan<S6Job>& span, long long t, Begin begin) {
     if (!span.table) return 0;
-    int lo = 0, hi = span.count - 1;
+    int lo = 0, hi = span.jobs() - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (begin(span.table[mid]) <= t) lo = mid;
@@ -101,11 +101,18 @@ __device__ __forceinline__ bool tile_pixel(long long local, int w, int& x, int& 
 
 constexpr unsigned kVA = 1u, kVB = 2u, kUA = 4u, kUB = 8u;
 
-__global__ void s6_clear_stats(dev::JobSpan<S6Job> span) {
-    const S6Job& j = dev::job_at(span, blockIdx.x);
+__device__ __forceinline__ void s6_clear_stats_job(const S6Job& j) {
     if (threadIdx.x == 0 && j.stats) {
         j.stats->union_pixels = 0;
         j.stats->overlap_pixels = 0;
+    }
+}
+
+__global__ void s6_clear_stats(dev::JobSpan<S6Job> span) {
+    const int jobs = span.jobs();
+    for (int ji = blockIdx.x; ji < jobs; ji += gridDim.x) {
+        s6_clear_stats_job(dev::job_at(span, ji));
+        __syncthreads();
     }
 }
 
@@ -136,7 +143,8 @@ __device__ void flush_stats(const S6Job& j, unsigned long long u, unsigned long 
 // A is sampled at integer positions, where the bilinear weights are exactly
 // (1, 0): its colour is float(v / 255.0) of the source byte, read from a
 // 256-entry table computed with the same expression.
-__global__ void __launch_bounds__(kBlock) s6_warp(dev::JobSpan<S6Job> span, long long total) {
+__global__ void __launch_bounds__(kBlock) s6_warp(dev::JobSpan<S6Job> span, dev::Total work) {
+    const long long total = work.get();
     __shared__ unsigned long long red[2][kBlock / 64];
     __shared__ float lut[256];
     lut[threadIdx.x] = static_cast<float>(static_cast<double>(threadIdx.x) / 255.0);
@@ -209,13 +217,19 @@ __global__ void __launch_bounds__(kBlock) s6_warp(dev::JobSpan<S6Job> span, long
 // a 32x8 tile of level-l outputs reads a 67x19 region of level l-1, loaded
 // once with coalesced rows and the reflect-101 borders applied at load time;
 // each output keeps the reference's dy-major, dx-minor sum per channel.
+// Tile rows hold the even columns first, then the odd ones, so the stride-2
+// reads of neighbouring lanes land in consecutive banks; a row pitch of 80
+// words puts the wave's two output rows 32 banks apart.
 constexpr int kRW = 32, kRH = 8, kRIW = 2 * kRW + 3, kRIH = 2 * kRH + 3;
+constexpr int kREven = (kRIW + 1) / 2, kRPitch = 80;
+static_assert(kRIW <= kRPitch && (2 * kRPitch) % 64 == 32, "reduce tile pitch");
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // kind 0: pull (A, B, VA, VB); kind 1: Gaussians of filled A, B and of UA, UB.
-__global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, long long total, int l, int kind) {
-    __shared__ float tile[8][kRIH][kRIW];
+__global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, dev::Total work, int l, int kind) {
+    const long long total = work.get();
+    __shared__ float tile[8][kRIH][kRPitch];
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = kind == 0 ? find_job_by(span, t, [l](const S6Job& j) { return j.rb[l]; })
                                  : find_job_by(span, t, [l](const S6Job& j) { return j.gb[l]; });
@@ -234,19 +248,20 @@ __global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, lo
             const int sx = clampi(reflect101(ix0 + ix, F.w), 0, F.w - 1);
             const int sy = clampi(reflect101(iy0 + iy, F.h), 0, F.h - 1);
             const size_t q = static_cast<size_t>(sy) * F.w + sx;
-            tile[0][iy][ix] = F.a[q];
-            tile[1][iy][ix] = F.a[F.n + q];
-            tile[2][iy][ix] = F.a[2 * F.n + q];
-            tile[3][iy][ix] = F.b[q];
-            tile[4][iy][ix] = F.b[F.n + q];
-            tile[5][iy][ix] = F.b[2 * F.n + q];
+            const int cx = (ix & 1) * kREven + (ix >> 1);
+            tile[0][iy][cx] = F.a[q];
+            tile[1][iy][cx] = F.a[F.n + q];
+            tile[2][iy][cx] = F.a[2 * F.n + q];
+            tile[3][iy][cx] = F.b[q];
+            tile[4][iy][cx] = F.b[F.n + q];
+            tile[5][iy][cx] = F.b[2 * F.n + q];
             if (l == 1) {
                 const unsigned f = F.flags[q];
-                tile[6][iy][ix] = (f & m1) ? 1.0f : 0.0f;
-                tile[7][iy][ix] = (f & m2) ? 1.0f : 0.0f;
+                tile[6][iy][cx] = (f & m1) ? 1.0f : 0.0f;
+                tile[7][iy][cx] = (f & m2) ? 1.0f : 0.0f;
             } else {
-                tile[6][iy][ix] = w1[q];
-                tile[7][iy][ix] = w2[q];
+                tile[6][iy][cx] = w1[q];
+                tile[7][iy][cx] = w2[q];
             }
         }
         __syncthreads();
@@ -260,7 +275,7 @@ __global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, lo
                 for (int dx = 0; dx < 5; ++dx) {
                     const float w = tap(dx - 2) * tap(dy - 2);
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) acc[c] += w * tile[c][2 * ly + dy][2 * lx + dx];
+                    for (int c = 0; c < 8; ++c) acc[c] += w * tile[c][2 * ly + dy][(dx & 1) * kREven + lx + (dx >> 1)];
                 }
             const size_t p = static_cast<size_t>(y) * C.w + x;
 #pragma unroll
@@ -335,7 +350,8 @@ __device__ __forceinline__ void load_coarse(const float* src, int cw, int ch, st
 // normalize_color + push_missing for level l, top-down (s6.cu:254-270,
 // :399-410). At level 0 the validity is exactly 0 or 1, so valid pixels keep
 // their colour (x / 1 == x) and only missing ones are written.
-__global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, long long total, int l) {
+__global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev::Total work, int l) {
+    const long long total = work.get();
     __shared__ float ca[3][kCH][kCW], cb[3][kCH][kCW];
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = find_job_by(span, t, [l](const S6Job& j) { return j.pb[l]; });
@@ -395,7 +411,8 @@ __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, long
 // mix_band + reconstruct (s6.cu:271-293) for level l, top-down; at level 0
 // also finish (s6.cu:294-306), where single-source pixels keep their warped
 // colour and only overlap pixels need the band mix.
-__global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, long long total, int l) {
+__global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::Total work, int l) {
+    const long long total = work.get();
     __shared__ float ca[3][kCH][kCW], cb[3][kCH][kCW], cz[3][kCH][kCW];
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = find_job_by(span, t, [l](const S6Job& j) { return j.mb[l]; });
@@ -691,19 +708,27 @@ S6Totals s6_prefix(S6Job* js, int n) {
     return t;
 }
 
-hipError_t launch_s6(const S6Job* table, int count, const S6Totals& t, const S6Job* single, hipStream_t stream) {
-    dev::JobSpan<S6Job> span{table, count, single ? *single : S6Job{}};
+hipError_t launch_s6(const S6Job* table, int count, const S6Totals& t, const S6Job* single, hipStream_t stream,
+                     const dev::Replay<S6Totals>* replay) {
+    dev::JobSpan<S6Job> span{table, count, single ? *single : S6Job{}, replay ? replay->count : nullptr};
+    const S6Totals* d = replay ? replay->totals : nullptr;
     const bool any_stats = single ? single->stats != nullptr : true;
     if (any_stats) s6_clear_stats<<<count, 64, 0, stream>>>(span);
-    if (t.warp > 0) s6_warp<<<persistent_grid(t.warp), kBlock, 0, stream>>>(span, t.warp);
+    if (t.warp > 0) s6_warp<<<persistent_grid(t.warp), kBlock, 0, stream>>>(span, dev::Total(t.warp, d ? &d->warp : nullptr));
     for (int l = 1; l <= t.top; ++l)
-        if (t.pull[l] > 0) s6_reduce<<<persistent_grid(t.pull[l]), kBlock, 0, stream>>>(span, t.pull[l], l, 0);
+        if (t.pull[l] > 0)
+            s6_reduce<<<persistent_grid(t.pull[l]), kBlock, 0, stream>>>(
+                span, dev::Total(t.pull[l], d ? &d->pull[l] : nullptr), l, 0);
     for (int l = t.top; l >= 0; --l)
-        if (t.push[l] > 0) s6_push<<<persistent_grid(t.push[l]), kBlock, 0, stream>>>(span, t.push[l], l);
+        if (t.push[l] > 0)
+            s6_push<<<persistent_grid(t.push[l]), kBlock, 0, stream>>>(span, dev::Total(t.push[l], d ? &d->push[l] : nullptr), l);
     for (int l = 1; l <= t.blend_top; ++l)
-        if (t.gauss[l] > 0) s6_reduce<<<persistent_grid(t.gauss[l]), kBlock, 0, stream>>>(span, t.gauss[l], l, 1);
+        if (t.gauss[l] > 0)
+            s6_reduce<<<persistent_grid(t.gauss[l]), kBlock, 0, stream>>>(
+                span, dev::Total(t.gauss[l], d ? &d->gauss[l] : nullptr), l, 1);
     for (int l = t.blend_top; l >= 0; --l)
-        if (t.mix[l] > 0) s6_mix<<<persistent_grid(t.mix[l]), kBlock, 0, stream>>>(span, t.mix[l], l);
+        if (t.mix[l] > 0)
+            s6_mix<<<persistent_grid(t.mix[l]), kBlock, 0, stream>>>(span, dev::Total(t.mix[l], d ? &d->mix[l] : nullptr), l);
     return hipGetLastError();
 }
 

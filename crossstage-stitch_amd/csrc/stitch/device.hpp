@@ -6,6 +6,8 @@
 
 #include <cstdint>
 
+#include "stitch/tuning.hpp"
+
 namespace stitch::dev {
 
 constexpr int kWave = 64;
@@ -19,6 +21,26 @@ struct JobSpan {
     const Job* table;  // device table, or nullptr to use `single`
     int count;
     Job single;
+    const int* count_dev = nullptr;  // graph replay: job count read on the device
+    __device__ int jobs() const { return count_dev ? *count_dev : count; }
+};
+
+// Work size (tiles) of one kernel launch. Direct launches pass the value;
+// a captured graph is replayed with new job tables, so its kernels read the
+// size from device memory written before each replay, and their grids are
+// sized for the capacity the graph was captured for.
+struct Total {
+    long long value;
+    const long long* dev;
+    __host__ __device__ Total(long long v, const long long* d = nullptr) : value(v), dev(d) {}
+    __device__ long long get() const { return dev ? *dev : value; }
+};
+
+// Device copies of a launch's job count and per-kernel totals (graph replay).
+template <class Totals>
+struct Replay {
+    const int* count;
+    const Totals* totals;
 };
 
 template <class Job>
@@ -31,7 +53,7 @@ __device__ inline const Job& job_at(const JobSpan<Job>& span, int i) {
 template <class Job>
 __device__ inline int find_job(const JobSpan<Job>& span, long long t, long long Job::*begin) {
     if (!span.table) return 0;
-    int lo = 0, hi = span.count - 1;
+    int lo = 0, hi = span.jobs() - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (span.table[mid].*begin <= t) lo = mid;
@@ -76,7 +98,8 @@ inline unsigned persistent_grid(long long tiles, int per_cu = 8) {
             cus = prop.multiProcessorCount;
         if (cus <= 0) cus = 256;
     }
-    const long long cap = static_cast<long long>(cus) * per_cu;
+    long long cap = static_cast<long long>(cus) * per_cu * stitch::jobs::workgroups_per_cu() / 8;
+    if (cap < cus) cap = cus;
     const long long g = tiles < cap ? tiles : cap;
     return static_cast<unsigned>(g > 0 ? g : 1);
 }

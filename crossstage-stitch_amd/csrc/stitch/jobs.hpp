@@ -32,7 +32,22 @@ inline long long s1_tiles(const S1Job& j) {
     return static_cast<long long>(j.tiles_x) * ((j.work_h + kS1TileH - 1) / kS1TileH);
 }
 hipError_t launch_s1(const S1Job* table, int count, long long total_tiles, const S1Job* single,
-                     hipStream_t stream);
+                     hipStream_t stream, const dev::Replay<long long>* replay = nullptr);
+
+// Controller probe gradient of one GRAY8 image (64 x 4 tiles, like S1).
+struct GradJob {
+    const std::uint8_t* src;
+    int pitch, w, h;
+    unsigned long long* accum;  // zeroed by the launch
+    double* out;                // mean absolute gradient in [0, 1]
+    int tiles_x;
+    long long tile_begin;
+};
+inline long long gradient_tiles(const GradJob& g) {
+    return static_cast<long long>(g.tiles_x) * ((g.h + kS1TileH - 1) / kS1TileH);
+}
+hipError_t launch_gradient(const GradJob* table, int count, long long total_tiles, const GradJob* single,
+                           hipStream_t stream, const dev::Replay<long long>* replay = nullptr);
 
 // ------------------------------------------------------------------ S2
 struct SelectParams {
@@ -73,7 +88,7 @@ struct S2Totals {
 // Fills the per-kernel tile prefixes of jobs[0..n) and returns the totals.
 S2Totals s2_prefix(S2Job* jobs, int n);
 hipError_t launch_s2(const S2Job* table, int count, const S2Totals& totals, const S2Job* single,
-                     hipStream_t stream);
+                     hipStream_t stream, const dev::Replay<S2Totals>* replay = nullptr);
 
 // ------------------------------------------------------------------ S3
 struct S3Job {
@@ -99,7 +114,7 @@ struct S3Totals {
 };
 S3Totals s3_prefix(S3Job* jobs, int n);
 hipError_t launch_s3(const S3Job* table, int count, const S3Totals& totals, const S3Job* single,
-                     hipStream_t stream);
+                     hipStream_t stream, const dev::Replay<S3Totals>* replay = nullptr);
 
 // ------------------------------------------------------------------ S4
 // Binary 256-bit fast path: per (query block, train segment) tile, 4 waves x
@@ -143,7 +158,7 @@ struct S4Totals {
 };
 S4Totals s4_prefix(S4Job* jobs, int n);
 hipError_t launch_s4(const S4Job* table, int count, const S4Totals& totals, const S4Job* single,
-                     hipStream_t stream);
+                     hipStream_t stream, const dev::Replay<S4Totals>* replay = nullptr);
 
 // ------------------------------------------------------------------ S5
 struct S5Model {
@@ -209,6 +224,6 @@ struct S5Totals {
 };
 S5Totals s5_prefix(S5Job* jobs, int n);
 hipError_t launch_s5(const S5Job* table, int count, const S5Totals& totals, const S5Job* single,
-                     hipStream_t stream);
+                     hipStream_t stream, const dev::Replay<S5Totals>* replay = nullptr);
 
 }  // namespace stitch::jobs

@@ -74,6 +74,6 @@ Status make_s6_job(const ImageView& a, const ImageView& b, const s6::Geometry& g
                    s6::RenderStatistics* stats, S6Job* job);
 S6Totals s6_prefix(S6Job* jobs, int n);
 hipError_t launch_s6(const S6Job* table, int count, const S6Totals& totals, const S6Job* single,
-                     hipStream_t stream);
+                     hipStream_t stream, const dev::Replay<S6Totals>* replay = nullptr);
 
 }  // namespace stitch::jobs

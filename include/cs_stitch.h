@@ -182,6 +182,97 @@ int cs_controller_walk(cs_controller* controller, double gradient_a, double grad
                        int width_b, int height_b, const cs_homography_result* feedback, const int* exec_status,
                        int n, double* trace);
 
+/* Batched form (no counterpart in the reference): ragged batches of pairs,
+ * each stage one persistent launch over the whole batch, captured in HIP
+ * graphs that are replayed with new job tables. Device memory comes from one
+ * pool of pool_bytes allocated at creation; the per-pair outputs in
+ * cs_pair_outcome are device views that stay valid until the next run.
+ * Knobs per pair mirror cs_harness_config; images are device RGB8. */
+typedef struct cs_batch cs_batch;
+typedef struct {
+    cs_image a, b;
+    double S;
+    int K, R, blend_mode, fast_threshold;
+    float harris_k;
+    double ratio_threshold;
+    int cross_check, msac_batch_size, msac_refinements;
+    double msac_threshold;
+    uint64_t seed;
+} cs_pair_spec;
+typedef struct {
+    int status;
+    char fail_stage[16];
+    int num_kp_a, num_kp_b;
+    cs_scale_info scale_a, scale_b;
+    cs_homography_result homography;
+    cs_geometry geometry;
+    cs_render_statistics statistics;
+    cs_image canvas, mask;
+    cs_keypoints keypoints_a, keypoints_b;
+    cs_descriptors descriptors_a, descriptors_b;
+    cs_matches matches;
+    uint8_t* inlier_mask;
+} cs_pair_outcome;
+typedef struct { double feature_ms, render_ms, host_ms, total_ms; } cs_batch_timings;
+typedef struct { int s4_segment, workgroups_per_cu, graphs; } cs_tuning;
+
+int cs_batch_create(int max_pairs, uint64_t pool_bytes, int use_graphs, cs_batch** out);
+void cs_batch_destroy(cs_batch* batch);
+int cs_batch_run(cs_batch* batch, const cs_pair_spec* pairs, int n, cs_pair_outcome* outcomes,
+                 cs_batch_timings* timings);
+/* Times the launch-shape knobs on this batch, keeps the fastest and returns
+ * it; cache_path (may be NULL) is a JSON file of earlier choices. */
+int cs_batch_autotune(cs_batch* batch, const cs_pair_spec* pairs, int n, const char* cache_path, cs_tuning* chosen);
+int cs_batch_apply(cs_batch* batch, const cs_tuning* tuning);
+int cs_batch_pool(cs_batch* batch, uint64_t* capacity, uint64_t* in_use, uint64_t* peak);
+
+/* Served form (no counterpart in the reference): a request loop replaying an
+ * arrival trace with continuous batching, HBM-budget admission and the
+ * legacy-v1 adaptive controller per request (or fixed knobs per request). */
+typedef struct cs_server cs_server;
+typedef struct {
+    uint64_t hbm_budget;
+    int max_batch, max_live, adaptive, use_graphs;
+    double canvas_factor;
+} cs_server_config;
+typedef struct {
+    uint64_t id;
+    cs_image a, b;
+    double arrival_ms;
+    uint64_t seed;
+    cs_pair_spec fixed;
+    int keep_output;
+} cs_request;
+typedef struct {
+    uint64_t id;
+    char status[32];
+    char fail_stage[16];
+    double arrival_ms, admitted_ms, done_ms;
+    int attempts;
+    double S;
+    int K, R, blend_mode;
+    int num_kp_a, num_kp_b;
+    cs_homography_result homography;
+    cs_geometry geometry;
+    cs_render_statistics statistics;
+    cs_image canvas, mask;
+} cs_completion;
+typedef struct {
+    int completed, failed;
+    double makespan_ms, p50_ms, p99_ms, mean_ms;
+    int feature_launches, render_launches;
+    double mean_feature_batch, mean_render_batch;
+    uint64_t peak_pool_bytes;
+    double feature_busy_ms, render_busy_ms;
+} cs_serve_stats;
+
+int cs_server_create(const cs_server_config* config, cs_server** out);
+void cs_server_destroy(cs_server* server);
+/* Requests sorted by arrival; completions (n of them) in completion order. */
+int cs_server_run_trace(cs_server* server, const cs_request* requests, int n, cs_completion* completions,
+                        cs_serve_stats* stats);
+int cs_server_release(cs_server* server, uint64_t id);
+
 /* Seeded synthetic pair (no counterpart in the reference; bench and test
  * input): host RGB8 buffers of width*height*3, overlap drawn in [lo, hi],
  * kind 0 textured, 1 low texture, 2 repetitive; h_ba = true H (B -> A). */
