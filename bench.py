@@ -109,8 +109,8 @@ def plain_runners(dev, work, outcomes):
     boxes = {}
     assign = []
     for (ta, _), (w, h, k, _), o in zip(dev, work, outcomes):
-        cw = max(o.geometry.width, 1)
-        ch = max(o.geometry.height, 1)
+        cw = max(o.geometry.width, w)
+        ch = max(o.geometry.height, h)
         key = (w, h, (cw + 255) // 256 * 256, (ch + 127) // 128 * 128)
         if key[2] * key[3] > 16777216:
             key = (w, h, cw, ch)
