@@ -4,13 +4,19 @@
 // (FP64 -> FP32 / 255), border-distance weights, ownership A if s_a >= s_b,
 // pull-push extension to 1x1, [1 4 6 4 1]/16 pyramids with reflect-101,
 // band mix by blurred ownership, collapse, floor(clamp(255 x) + 0.5).
-// The depth is generic: 3, 5, 6 or 7 reductions.
+// The depth is generic: 3, 5, 6 or 7 reductions. Filter taps, expansions and
+// the band mix accumulate with fused multiply-add.
 //
-// MI355X layout: FP32 pyramid levels in HBM, one launch per level covering
-// every canvas of the batch. Normalisation is fused into the push pass
-// (top-down, so the coarse level is final when it is expanded), and band
-// mixing is fused with the collapse - at level 0 also with the final
-// quantisation - which removes three full-resolution passes.
+// MI355X layout: the full-resolution level is the expensive one, so it is
+// never materialised as A colours (A is read from its source bytes through
+// a 256-entry table) and never filled in memory: the Gaussian reduce that
+// builds level 1 fills missing level-0 pixels on the fly from the pushed
+// level 1 staged in LDS. Level 0 keeps one flag byte per pixel and B's warped
+// colour; the final band mix writes the panorama and the union mask. Levels
+// >= 1 are FP32 planes. One launch per level covers every canvas of a batch;
+// normalisation is fused into the push pass and band mixing into the
+// collapse. Expansions visit only the taps whose reflected fine position has
+// the pixel's parity (reflect-101 keeps parity), 9 instead of 25.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,6 +49,8 @@ __device__ __forceinline__ float tap(int x) {
     return t == 0 ? 0.375f : (t == 1 ? 0.25f : 0.0625f);
 }
 
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
 __device__ bool center_coordinate(double& x, int n, double tol) {
     if (!isfinite(x)) return false;
     const double nearest = nearbyint(x);
@@ -51,7 +59,8 @@ __device__ bool center_coordinate(double& x, int n, double tol) {
     return x >= 0 && x <= n - 1.0;
 }
 
-__device__ bool sample(const std::uint8_t* img, int w, int h, int pitch, double x, double y, float* rgb, double& score,
+// Bilinear B sample (FP64, then FP32 / 255); the border score only when asked.
+__device__ bool sample(const std::uint8_t* img, int w, int h, int pitch, double x, double y, float* rgb, double* score,
                        double tol) {
     if (!center_coordinate(x, w, tol) || !center_coordinate(y, h, tol)) return false;
     const int x0 = static_cast<int>(floor(x)), y0 = static_cast<int>(floor(y));
@@ -67,7 +76,7 @@ __device__ bool sample(const std::uint8_t* img, int w, int h, int pitch, double 
     }
     // min((x+.5)/w, (w-.5-x)/w) == min(x+.5, w-.5-x)/w exactly: a correctly
     // rounded division by a positive w is monotonic, so it commutes with min.
-    score = fmin(fmin(x + 0.5, w - 0.5 - x) / w, fmin(y + 0.5, h - 0.5 - y) / h);
+    if (score) *score = fmin(fmin(x + 0.5, w - 0.5 - x) / w, fmin(y + 0.5, h - 0.5 - y) / h);
     return true;
 }
 
@@ -100,6 +109,15 @@ __device__ __forceinline__ bool tile_pixel(long long local, int w, int& x, int& 
 }
 
 constexpr unsigned kVA = 1u, kVB = 2u, kUA = 4u, kUB = 8u;
+
+// float(v / 255.0) for every byte: A's level-0 colour.
+__device__ __forceinline__ void fill_lut(float* lut) {
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) lut[t] = static_cast<float>(static_cast<double>(t) / 255.0);
+}
+
+__device__ __forceinline__ const std::uint8_t* a_pixel(const S6Job& j, int x, int y) {
+    return j.a + static_cast<size_t>(y + j.origin_y) * j.pitch_a + 3 * (x + j.origin_x);
+}
 
 __device__ __forceinline__ void s6_clear_stats_job(const S6Job& j) {
     if (threadIdx.x == 0 && j.stats) {
@@ -138,16 +156,16 @@ __device__ void flush_stats(const S6Job& j, unsigned long long u, unsigned long 
     __syncthreads();
 }
 
-// Canvas pixel -> A (integer shift) and B (FP64 inverse map), level-0 colours
-// and flags or the feather output, union mask, union/overlap counters.
-// A is sampled at integer positions, where the bilinear weights are exactly
-// (1, 0): its colour is float(v / 255.0) of the source byte, read from a
-// 256-entry table computed with the same expression.
+// ------------------------------------------------------------------ warp
+// Canvas pixel -> A (integer shift) and B (FP64 inverse map, only inside the
+// box that can hold B). Multiband: level-0 flags and B's colour where valid.
+// Feather: the output and the union mask. Union/overlap counters for both.
+// The border scores are only needed where both images are valid.
 __global__ void __launch_bounds__(kBlock) s6_warp(dev::JobSpan<S6Job> span, dev::Total work) {
     const long long total = work.get();
     __shared__ unsigned long long red[2][kBlock / 64];
     __shared__ float lut[256];
-    lut[threadIdx.x] = static_cast<float>(static_cast<double>(threadIdx.x) / 255.0);
+    fill_lut(lut);
     __syncthreads();
     int cur = -1;
     unsigned long long uni = 0, ovl = 0;
@@ -163,27 +181,28 @@ __global__ void __launch_bounds__(kBlock) s6_warp(dev::JobSpan<S6Job> span, dev:
         if (!tile_pixel(t - j.warp_begin, j.width, x, y)) continue;
         const size_t p = static_cast<size_t>(y) * j.width + x;
         const int ax = x + j.origin_x, ay = y + j.origin_y;
-        const double xa = static_cast<double>(x) + j.origin_x, ya = static_cast<double>(y) + j.origin_y;
+        const double xa = static_cast<double>(ax), ya = static_cast<double>(ay);
         float ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
-        double sa = 0, sb = 0;
         const bool ma = ax >= 0 && ax <= j.wa - 1 && ay >= 0 && ay <= j.ha - 1;
-        if (ma) {
-            const std::uint8_t* q = j.a + static_cast<size_t>(ay) * j.pitch_a + 3 * ax;
-            ra[0] = lut[q[0]];
-            ra[1] = lut[q[1]];
-            ra[2] = lut[q[2]];
-            sa = fmin(fmin(xa + 0.5, j.wa - 0.5 - xa) / j.wa, fmin(ya + 0.5, j.ha - 0.5 - ya) / j.ha);
-        }
         bool mb = false;
-        const double* inv = j.inverse;
-        const double den = inv[6] * xa + inv[7] * ya + inv[8];
-        const double dsc = fabs(inv[6] * xa) + fabs(inv[7] * ya) + fabs(inv[8]);
-        if (isfinite(den) && dsc > 0 && fabs(den) > j.denominator_tolerance * dsc) {
-            const double xb = (inv[0] * xa + inv[1] * ya + inv[2]) / den;
-            const double yb = (inv[3] * xa + inv[4] * ya + inv[5]) / den;
-            mb = sample(j.b, j.wb, j.hb, j.pitch_b, xb, yb, rb, sb, j.snap_tolerance);
+        double xb = 0, yb = 0;
+        if (x >= j.bx0 && x <= j.bx1 && y >= j.by0 && y <= j.by1) {
+            const double* inv = j.inverse;
+            const double den = inv[6] * xa + inv[7] * ya + inv[8];
+            const double dsc = fabs(inv[6] * xa) + fabs(inv[7] * ya) + fabs(inv[8]);
+            if (isfinite(den) && dsc > 0 && fabs(den) > j.denominator_tolerance * dsc) {
+                xb = (inv[0] * xa + inv[1] * ya + inv[2]) / den;
+                yb = (inv[3] * xa + inv[4] * ya + inv[5]) / den;
+                mb = sample(j.b, j.wb, j.hb, j.pitch_b, xb, yb, rb, nullptr, j.snap_tolerance);
+            }
         }
-        j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = ma || mb ? 1 : 0;
+        double sa = 0, sb = 0;
+        if (ma && mb) {
+            sa = fmin(fmin(xa + 0.5, j.wa - 0.5 - xa) / j.wa, fmin(ya + 0.5, j.ha - 0.5 - ya) / j.ha);
+            center_coordinate(xb, j.wb, j.snap_tolerance);
+            center_coordinate(yb, j.hb, j.snap_tolerance);
+            sb = fmin(fmin(xb + 0.5, j.wb - 0.5 - xb) / j.wb, fmin(yb + 0.5, j.hb - 0.5 - yb) / j.hb);
+        }
         if (j.top >= 0) {
             const S6Level& L = j.lv[0];
             unsigned f = 0;
@@ -192,12 +211,17 @@ __global__ void __launch_bounds__(kBlock) s6_warp(dev::JobSpan<S6Job> span, dev:
             if (ma && (!mb || sa >= sb)) f |= kUA;
             if (mb && (!ma || sb > sa)) f |= kUB;
             L.flags[p] = static_cast<std::uint8_t>(f);
+            if (mb) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                L.a[c * L.n + p] = ra[c];
-                L.b[c * L.n + p] = rb[c];
+                for (int c = 0; c < 3; ++c) L.b[c * L.n + p] = rb[c];
             }
         } else {
+            if (ma) {
+                const std::uint8_t* q = a_pixel(j, x, y);
+                ra[0] = lut[q[0]];
+                ra[1] = lut[q[1]];
+                ra[2] = lut[q[2]];
+            }
             std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -205,6 +229,7 @@ __global__ void __launch_bounds__(kBlock) s6_warp(dev::JobSpan<S6Job> span, dev:
                     ma && mb ? static_cast<float>((sa * ra[c] + sb * rb[c]) / (sa + sb)) : (ma ? ra[c] : rb[c]);
                 out[c] = quantize(color);
             }
+            j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = ma || mb ? 1 : 0;
         }
         uni += ma || mb;
         ovl += ma && mb;
@@ -212,57 +237,148 @@ __global__ void __launch_bounds__(kBlock) s6_warp(dev::JobSpan<S6Job> span, dev:
     if (cur >= 0 && dev::job_at(span, cur).stats) flush_stats(dev::job_at(span, cur), uni, ovl, red);
 }
 
+// ------------------------------------------------------------- expansion
+// 4 * sum of tap(dx) * tap(dy) * coarse over the taps whose reflected fine
+// position is even (s6.cu's expanded(), in its dy-major, dx-minor order).
+// Reflect-101 keeps parity, so those are the offsets with the parity of x
+// (and y): three for an even coordinate, two for an odd one, whose third
+// slot carries weight 0 - adding 0 * v leaves the sum unchanged, so every
+// lane runs the same 9 steps.
+struct Expand {
+    int cx[3], cy[3];    // LDS column / row per slot
+    float wx[3], wy[3];
+};
+
+__device__ __forceinline__ void expand_axis(int x, int cn, int c0, int cmax, int (&ci)[3], float (&w)[3]) {
+    const bool odd = x & 1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int d = odd ? (k == 0 ? -1 : 1) : 2 * k - 2;
+        const int r = reflect101(x + d, 2 * cn);
+        ci[k] = clampi((r >> 1) - c0, 0, cmax);
+        w[k] = (odd && k == 2) ? 0.0f : tap(d);
+    }
+}
+
+__device__ __forceinline__ void expand_setup(int x, int y, int cw, int ch, int cx0, int cy0, int cols, int rows,
+                                             Expand& e) {
+    expand_axis(x, cw, cx0, cols - 1, e.cx, e.wx);
+    expand_axis(y, ch, cy0, rows - 1, e.cy, e.wy);
+}
+
+// src: LDS plane with row pitch P.
+template <int P>
+__device__ __forceinline__ float expand_one(const float* src, const Expand& e) {
+    float s = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s = fmaf(e.wx[k] * e.wy[j], src[e.cy[j] * P + e.cx[k]], s);
+    return 4 * s;
+}
+
+// Stages rows [cy0, cy0 + rows) x cols [cx0, cx0 + cols) of `planes` planes
+// (n floats apart) of a coarse level, clamped to its extent.
+template <int P>
+__device__ __forceinline__ void stage_coarse(const float* src, int planes, std::size_t n, int cw, int ch, int cx0,
+                                             int cy0, int cols, int rows, float* dst) {
+    for (int i = threadIdx.x; i < cols * rows; i += blockDim.x) {
+        const int cx = clampi(cx0 + i % cols, 0, cw - 1), cy = clampi(cy0 + i / cols, 0, ch - 1);
+        const size_t q = static_cast<size_t>(cy) * cw + cx;
+        for (int c = 0; c < planes; ++c) dst[(c * rows + i / cols) * P + i % cols] = src[c * n + q];
+    }
+}
+
 // ---------------------------------------------------------------- reduce
 // 5x5 [1 4 6 4 1]/16 filter + 2x decimation (s6.cu:236-253) on LDS tiles:
 // a 32x8 tile of level-l outputs reads a 67x19 region of level l-1, loaded
-// once with coalesced rows and the reflect-101 borders applied at load time;
-// each output keeps the reference's dy-major, dx-minor sum per channel.
-// Tile rows hold the even columns first, then the odd ones, so the stride-2
-// reads of neighbouring lanes land in consecutive banks; a row pitch of 80
-// words puts the wave's two output rows 32 banks apart.
+// once with the reflect-101 borders applied at load time; each output keeps
+// the reference's dy-major, dx-minor sum per channel. Tile rows hold the
+// even columns first, then the odd ones, so the stride-2 reads of
+// neighbouring lanes land in consecutive banks; a row pitch of 80 words puts
+// the wave's two output rows 32 banks apart.
 constexpr int kRW = 32, kRH = 8, kRIW = 2 * kRW + 3, kRIH = 2 * kRH + 3;
 constexpr int kREven = (kRIW + 1) / 2, kRPitch = 80;
 static_assert(kRIW <= kRPitch && (2 * kRPitch) % 64 == 32, "reduce tile pitch");
+// Pushed level 1 behind a tile of level 0 (Gaussian of level 1, mode 3).
+constexpr int kFCols = (kRIW - 1 + 4) / 2 + 1, kFRows = (kRIH - 1 + 4) / 2 + 1;
 
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-// kind 0: pull (A, B, VA, VB); kind 1: Gaussians of filled A, B and of UA, UB.
-__global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, dev::Total work, int l, int kind) {
+// Modes: 0 pull at l >= 2, 1 Gaussian at l >= 2, 2 pull at l = 1 (from the
+// virtual level 0), 3 Gaussian at l = 1 (level 0 filled on the fly).
+template <int kMode>
+__global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, dev::Total work, int l) {
     const long long total = work.get();
+    constexpr bool kFromZero = kMode >= 2;
     __shared__ float tile[8][kRIH][kRPitch];
+    __shared__ float lut[kFromZero ? 256 : 1];
+    __shared__ float fill[kMode == 3 ? 6 : 1][kMode == 3 ? kFRows : 1][kMode == 3 ? kFCols : 1];
+    if (kFromZero) {
+        fill_lut(lut);
+        __syncthreads();
+    }
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = kind == 0 ? find_job_by(span, t, [l](const S6Job& j) { return j.rb[l]; })
-                                 : find_job_by(span, t, [l](const S6Job& j) { return j.gb[l]; });
+        const int ji = (kMode == 0 || kMode == 2) ? find_job_by(span, t, [l](const S6Job& j) { return j.rb[l]; })
+                                                  : find_job_by(span, t, [l](const S6Job& j) { return j.gb[l]; });
         const S6Job& j = dev::job_at(span, ji);
         const S6Level& C = j.lv[l];
         const S6Level& F = j.lv[l - 1];
-        const unsigned local = static_cast<unsigned>(t - (kind == 0 ? j.rb[l] : j.gb[l]));
+        const unsigned local = static_cast<unsigned>(t - ((kMode == 0 || kMode == 2) ? j.rb[l] : j.gb[l]));
         const unsigned tx = (static_cast<unsigned>(C.w) + kRW - 1) / kRW;
         const int x0 = static_cast<int>(local % tx) * kRW, y0 = static_cast<int>(local / tx) * kRH;
         const int ix0 = 2 * x0 - 2, iy0 = 2 * y0 - 2;
-        const float* w1 = kind == 0 ? F.va : F.ua;
-        const float* w2 = kind == 0 ? F.vb : F.ub;
-        const unsigned m1 = kind == 0 ? kVA : kUA, m2 = kind == 0 ? kVB : kUB;
+        const int fx0 = (ix0 - 2) >> 1, fy0 = (iy0 - 2) >> 1;  // first staged coarse column / row (mode 3)
+        if (kMode == 3) {
+            stage_coarse<kFCols>(C.a, 3, C.n, C.w, C.h, fx0, fy0, kFCols, kFRows, &fill[0][0][0]);
+            stage_coarse<kFCols>(C.b, 3, C.n, C.w, C.h, fx0, fy0, kFCols, kFRows, &fill[3][0][0]);
+            __syncthreads();
+        }
         for (int i = threadIdx.x; i < kRIW * kRIH; i += kBlock) {
             const int ix = i % kRIW, iy = i / kRIW;
             const int sx = clampi(reflect101(ix0 + ix, F.w), 0, F.w - 1);
             const int sy = clampi(reflect101(iy0 + iy, F.h), 0, F.h - 1);
             const size_t q = static_cast<size_t>(sy) * F.w + sx;
             const int cx = (ix & 1) * kREven + (ix >> 1);
-            tile[0][iy][cx] = F.a[q];
-            tile[1][iy][cx] = F.a[F.n + q];
-            tile[2][iy][cx] = F.a[2 * F.n + q];
-            tile[3][iy][cx] = F.b[q];
-            tile[4][iy][cx] = F.b[F.n + q];
-            tile[5][iy][cx] = F.b[2 * F.n + q];
-            if (l == 1) {
+            float v[8];
+            if (kFromZero) {
                 const unsigned f = F.flags[q];
-                tile[6][iy][cx] = (f & m1) ? 1.0f : 0.0f;
-                tile[7][iy][cx] = (f & m2) ? 1.0f : 0.0f;
+                Expand e;
+                if (kMode == 3 && (f & (kVA | kVB)) != (kVA | kVB))
+                    expand_setup(sx, sy, C.w, C.h, fx0, fy0, kFCols, kFRows, e);
+                if (f & kVA) {
+                    const std::uint8_t* s = a_pixel(j, sx, sy);
+                    v[0] = lut[s[0]];
+                    v[1] = lut[s[1]];
+                    v[2] = lut[s[2]];
+                } else if (kMode == 3) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) v[c] = expand_one<kFCols>(&fill[c][0][0], e);
+                } else {
+                    v[0] = v[1] = v[2] = 0;
+                }
+                if (f & kVB) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) v[3 + c] = F.b[c * F.n + q];
+                } else if (kMode == 3) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) v[3 + c] = expand_one<kFCols>(&fill[3 + c][0][0], e);
+                } else {
+                    v[3] = v[4] = v[5] = 0;
+                }
+                v[6] = (f & (kMode == 2 ? kVA : kUA)) ? 1.0f : 0.0f;
+                v[7] = (f & (kMode == 2 ? kVB : kUB)) ? 1.0f : 0.0f;
             } else {
-                tile[6][iy][cx] = w1[q];
-                tile[7][iy][cx] = w2[q];
+                const float* sa = kMode == 0 ? F.a : F.ga;
+                const float* sb = kMode == 0 ? F.b : F.gb;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    v[c] = sa[c * F.n + q];
+                    v[3 + c] = sb[c * F.n + q];
+                }
+                v[6] = (kMode == 0 ? F.va : F.ua)[q];
+                v[7] = (kMode == 0 ? F.vb : F.ub)[q];
             }
+#pragma unroll
+            for (int c = 0; c < 8; ++c) tile[c][iy][cx] = v[c];
         }
         __syncthreads();
         const int lx = threadIdx.x % kRW, ly = threadIdx.x / kRW;
@@ -275,84 +391,34 @@ __global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, de
                 for (int dx = 0; dx < 5; ++dx) {
                     const float w = tap(dx - 2) * tap(dy - 2);
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) acc[c] += w * tile[c][2 * ly + dy][(dx & 1) * kREven + lx + (dx >> 1)];
+                    for (int c = 0; c < 8; ++c)
+                        acc[c] = fmaf(w, tile[c][2 * ly + dy][(dx & 1) * kREven + lx + (dx >> 1)], acc[c]);
                 }
             const size_t p = static_cast<size_t>(y) * C.w + x;
+            float* da = (kMode == 0 || kMode == 2) ? C.a : C.ga;
+            float* db = (kMode == 0 || kMode == 2) ? C.b : C.gb;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                C.a[c * C.n + p] = acc[c];
-                C.b[c * C.n + p] = acc[3 + c];
+                da[c * C.n + p] = acc[c];
+                db[c * C.n + p] = acc[3 + c];
             }
-            if (kind == 0) {
-                C.va[p] = acc[6];
-                C.vb[p] = acc[7];
-            } else {
-                C.ua[p] = acc[6];
-                C.ub[p] = acc[7];
-            }
+            ((kMode == 0 || kMode == 2) ? C.va : C.ua)[p] = acc[6];
+            ((kMode == 0 || kMode == 2) ? C.vb : C.ub)[p] = acc[7];
         }
         __syncthreads();
     }
 }
 
-// ---------------------------------------------------------------- expand
-// Upsampling reads a 3x3 coarse neighbourhood per fine pixel; a 64x4 fine
-// tile needs a 34x4 coarse region, staged in LDS. Taps whose reflected fine
-// position is odd carry weight 0 (adding 0 * v to the running sum leaves it
-// unchanged), so all lanes run the same 25-step loop without divergence.
+// ---------------------------------------------------- push, mix, collapse
+// Fine tiles of 64x4 pixels; their 3x3 coarse neighbourhoods span 34x4
+// coarse pixels, staged in LDS.
 constexpr int kEW = 64, kEH = 4, kCW = kEW / 2 + 2, kCH = kEH / 2 + 2;
 
-struct ExpandTaps {
-    int xi[5], yi[5];      // LDS column/row per offset
-    float wx[5], wy[5];    // tap weight or 0 when the position is odd
-};
-
-__device__ __forceinline__ void expand_setup(int x, int y, int cw, int ch, int cx0, int cy0, ExpandTaps& e) {
-#pragma unroll
-    for (int d = 0; d < 5; ++d) {
-        const int rx = reflect101(x + d - 2, 2 * cw);
-        const int ry = reflect101(y + d - 2, 2 * ch);
-        e.xi[d] = (rx >> 1) - cx0;
-        e.yi[d] = (ry >> 1) - cy0;
-        e.wx[d] = (rx & 1) ? 0.0f : tap(d - 2);
-        e.wy[d] = (ry & 1) ? 0.0f : tap(d - 2);
-    }
-}
-
-// 4 * sum over dy, dx of (tap(dx) * tap(dy)) * coarse, for 3 interleaved channels.
-__device__ __forceinline__ void expand_lds(const float (&src)[3][kCH][kCW], const ExpandTaps& e, float out[3]) {
-    float s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-    for (int dy = 0; dy < 5; ++dy)
-#pragma unroll
-        for (int dx = 0; dx < 5; ++dx) {
-            const float w = e.wx[dx] * e.wy[dy];
-            s0 += w * src[0][e.yi[dy]][e.xi[dx]];
-            s1 += w * src[1][e.yi[dy]][e.xi[dx]];
-            s2 += w * src[2][e.yi[dy]][e.xi[dx]];
-        }
-    out[0] = 4 * s0;
-    out[1] = 4 * s1;
-    out[2] = 4 * s2;
-}
-
-__device__ __forceinline__ void load_coarse(const float* src, int cw, int ch, std::size_t cn, int cx0, int cy0,
-                                            float (&dst)[3][kCH][kCW]) {
-    for (int i = threadIdx.x; i < kCW * kCH; i += kBlock) {
-        const int cx = clampi(cx0 + i % kCW, 0, cw - 1), cy = clampi(cy0 + i / kCW, 0, ch - 1);
-        const size_t q = static_cast<size_t>(cy) * cw + cx;
-        dst[0][i / kCW][i % kCW] = src[q];
-        dst[1][i / kCW][i % kCW] = src[cn + q];
-        dst[2][i / kCW][i % kCW] = src[2 * cn + q];
-    }
-}
-
-// normalize_color + push_missing for level l, top-down (s6.cu:254-270,
-// :399-410). At level 0 the validity is exactly 0 or 1, so valid pixels keep
-// their colour (x / 1 == x) and only missing ones are written.
+// normalize_color + push_missing for level l >= 1, top-down (s6.cu:254-270,
+// :399-410): the coarse level is final when it is expanded.
 __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev::Total work, int l) {
     const long long total = work.get();
-    __shared__ float ca[3][kCH][kCW], cb[3][kCH][kCW];
+    __shared__ float cc[6][kCH][kCW];
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = find_job_by(span, t, [l](const S6Job& j) { return j.pb[l]; });
         const S6Job& j = dev::job_at(span, ji);
@@ -364,44 +430,23 @@ __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev:
         const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 1;
         if (coarse) {
             const S6Level& C = j.lv[l + 1];
-            load_coarse(C.a, C.w, C.h, C.n, cx0, cy0, ca);
-            load_coarse(C.b, C.w, C.h, C.n, cx0, cy0, cb);
+            stage_coarse<kCW>(C.a, 3, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[0][0][0]);
+            stage_coarse<kCW>(C.b, 3, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[3][0][0]);
             __syncthreads();
         }
         const int x = x0 + static_cast<int>(threadIdx.x) % kEW, y = y0 + static_cast<int>(threadIdx.x) / kEW;
         if (x < L.w && y < L.h) {
             const size_t p = static_cast<size_t>(y) * L.w + x;
-            float wa, wb;
-            if (l == 0) {
-                const unsigned f = L.flags[p];
-                wa = (f & kVA) ? 1.0f : 0.0f;
-                wb = (f & kVB) ? 1.0f : 0.0f;
-            } else {
-                wa = L.va[p];
-                wb = L.vb[p];
-            }
-            ExpandTaps e;
-            if (coarse && (!(wa > 0) || !(wb > 0))) expand_setup(x, y, j.lv[l + 1].w, j.lv[l + 1].h, cx0, cy0, e);
-            float v[3];
-            if (wa > 0) {
-                if (l > 0)
+            const float wa = L.va[p], wb = L.vb[p];
+            Expand e;
+            if (coarse && (!(wa > 0) || !(wb > 0)))
+                expand_setup(x, y, j.lv[l + 1].w, j.lv[l + 1].h, cx0, cy0, kCW, kCH, e);
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) L.a[c * L.n + p] = L.a[c * L.n + p] / wa;
-            } else {
-                if (coarse) expand_lds(ca, e, v);
-                else v[0] = v[1] = v[2] = 0;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) L.a[c * L.n + p] = v[c];
-            }
-            if (wb > 0) {
-                if (l > 0)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) L.b[c * L.n + p] = L.b[c * L.n + p] / wb;
-            } else {
-                if (coarse) expand_lds(cb, e, v);
-                else v[0] = v[1] = v[2] = 0;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) L.b[c * L.n + p] = v[c];
+            for (int c = 0; c < 3; ++c) {
+                float& va = L.a[c * L.n + p];
+                va = wa > 0 ? va / wa : (coarse ? expand_one<kCW>(&cc[c][0][0], e) : 0.0f);
+                float& vb = L.b[c * L.n + p];
+                vb = wb > 0 ? vb / wb : (coarse ? expand_one<kCW>(&cc[3 + c][0][0], e) : 0.0f);
             }
         }
         if (coarse) __syncthreads();
@@ -409,11 +454,16 @@ __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev:
 }
 
 // mix_band + reconstruct (s6.cu:271-293) for level l, top-down; at level 0
-// also finish (s6.cu:294-306), where single-source pixels keep their warped
-// colour and only overlap pixels need the band mix.
+// also finish (s6.cu:294-306) and the union mask: single-source pixels keep
+// their warped colour and only overlap pixels need the band mix.
 __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::Total work, int l) {
     const long long total = work.get();
-    __shared__ float ca[3][kCH][kCW], cb[3][kCH][kCW], cz[3][kCH][kCW];
+    __shared__ float cc[9][kCH][kCW];
+    __shared__ float lut[256];
+    if (l == 0) {
+        fill_lut(lut);
+        __syncthreads();
+    }
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         const int ji = find_job_by(span, t, [l](const S6Job& j) { return j.mb[l]; });
         const S6Job& j = dev::job_at(span, ji);
@@ -425,67 +475,67 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
         const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 1;
         if (coarse) {
             const S6Level& C = j.lv[l + 1];
-            load_coarse(C.a, C.w, C.h, C.n, cx0, cy0, ca);
-            load_coarse(C.b, C.w, C.h, C.n, cx0, cy0, cb);
-            load_coarse(C.z, C.w, C.h, C.n, cx0, cy0, cz);
+            stage_coarse<kCW>(C.ga, 3, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[0][0][0]);
+            stage_coarse<kCW>(C.gb, 3, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[3][0][0]);
+            stage_coarse<kCW>(C.z, 3, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[6][0][0]);
             __syncthreads();
         }
         const int x = x0 + static_cast<int>(threadIdx.x) % kEW, y = y0 + static_cast<int>(threadIdx.x) / kEW;
         if (x < L.w && y < L.h) {
             const size_t p = static_cast<size_t>(y) * L.w + x;
-            float ua, ub;
-            bool both = true;
+            float ga[3], gb[3], ua, ub;
+            bool blend = true;
             unsigned f = 0;
             if (l == 0) {
                 f = L.flags[p];
+                j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = f & (kVA | kVB) ? 1 : 0;
+                ga[0] = ga[1] = ga[2] = 0;
+                gb[0] = gb[1] = gb[2] = 0;
+                if (f & kVA) {
+                    const std::uint8_t* s = a_pixel(j, x, y);
+                    ga[0] = lut[s[0]];
+                    ga[1] = lut[s[1]];
+                    ga[2] = lut[s[2]];
+                }
+                if (f & kVB) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) gb[c] = L.b[c * L.n + p];
+                }
                 ua = (f & kUA) ? 1.0f : 0.0f;
                 ub = (f & kUB) ? 1.0f : 0.0f;
-                both = (f & kVA) && (f & kVB);
+                blend = (f & kVA) && (f & kVB);
             } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    ga[c] = L.ga[c * L.n + p];
+                    gb[c] = L.gb[c * L.n + p];
+                }
                 ua = L.ua[p];
                 ub = L.ub[p];
             }
-            if (l == 0 && !both) {
-                std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    out[c] = quantize((f & kVA) ? L.a[c * L.n + p] : ((f & kVB) ? L.b[c * L.n + p] : 0));
-            } else {
-                float la[3], lb[3], zc[3] = {0, 0, 0};
-                if (coarse) {
-                    ExpandTaps e;
-                    expand_setup(x, y, j.lv[l + 1].w, j.lv[l + 1].h, cx0, cy0, e);
-                    float ea[3], eb[3];
-                    expand_lds(ca, e, ea);
-                    expand_lds(cb, e, eb);
-                    expand_lds(cz, e, zc);
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        la[c] = L.a[c * L.n + p] - ea[c];
-                        lb[c] = L.b[c * L.n + p] - eb[c];
-                    }
-                } else {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        la[c] = L.a[c * L.n + p] - 0.0f;
-                        lb[c] = L.b[c * L.n + p] - 0.0f;
-                    }
-                }
+            float z[3];
+            if (blend) {
+                Expand e;
+                if (coarse) expand_setup(x, y, j.lv[l + 1].w, j.lv[l + 1].h, cx0, cy0, kCW, kCH, e);
                 const float tot = ua + ub;
-                float z[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const float m = tot > 0 ? (ua * la[c] + ub * lb[c]) / tot : 0;
-                    z[c] = coarse ? m + zc[c] : m;
+                    const float la = ga[c] - (coarse ? expand_one<kCW>(&cc[c][0][0], e) : 0.0f);
+                    const float lb = gb[c] - (coarse ? expand_one<kCW>(&cc[3 + c][0][0], e) : 0.0f);
+                    const float m = tot > 0 ? fmaf(ua, la, ub * lb) / tot : 0.0f;
+                    z[c] = coarse ? m + expand_one<kCW>(&cc[6 + c][0][0], e) : m;
                 }
-                if (l > 0) {
+            } else {
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) L.z[c * L.n + p] = z[c];
-                } else {
-                    std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
+                for (int c = 0; c < 3; ++c) z[c] = (f & kVA) ? ga[c] : gb[c];
+            }
+            if (l > 0) {
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) out[c] = quantize(z[c]);
-                }
+                for (int c = 0; c < 3; ++c) L.z[c * L.n + p] = z[c];
+            } else {
+                std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) out[c] = quantize(z[c]);
             }
         }
         if (coarse) __syncthreads();
@@ -539,8 +589,10 @@ std::size_t align256(std::size_t v) { return (v + 255) / 256 * 256; }
 namespace jobs {
 
 // Pyramid layout (s6.cu:60-88): level 0 is the canvas, each level halves with
-// ceil down to 1x1; A, B (RGB FP32), VA, VB on every level; UA, UB, Z on the
-// blended levels (Z only above level 0, which is fused into the output).
+// ceil down to 1x1. Level 0: B's warped colour (RGB FP32) and one flag byte;
+// levels >= 1: A, B (RGB FP32), VA, VB, and on the blended levels UA, UB, Z;
+// level 1 also keeps its Gaussian colour planes apart from the pushed ones,
+// which the level-1 Gaussian reads while it writes.
 Status s6_layout(const s6::Geometry& g, s6::BlendMode mode, S6Layout* out) {
     if (!valid_geometry(g) || !valid_mode(mode)) return Status::INVALID_ARGUMENT;
     S6Layout L{};
@@ -561,19 +613,23 @@ Status s6_layout(const s6::Geometry& g, s6::BlendMode mode, S6Layout* out) {
         o.w = w;
         o.h = h;
         const std::size_t n = static_cast<std::size_t>(w) * h;
-        add(n, 3, o.a);
         add(n, 3, o.b);
         if (l == 0) {
             cur = align256(cur);
             o.flags = cur;
             cur += n;
         } else {
+            add(n, 3, o.a);
             add(n, 1, o.va);
             add(n, 1, o.vb);
             if (l <= requested) {
                 add(n, 1, o.ua);
                 add(n, 1, o.ub);
                 add(n, 3, o.z);
+                if (l == 1) {
+                    add(n, 3, o.ga);
+                    add(n, 3, o.gb);
+                }
             }
         }
         L.top = l;
@@ -587,6 +643,51 @@ Status s6_layout(const s6::Geometry& g, s6::BlendMode mode, S6Layout* out) {
     *out = L;
     return Status::OK;
 }
+
+namespace {
+
+// Canvas box of B's footprint: B's corners through the forward map (the
+// inverse of the canvas -> B map, in long double), widened by 2 pixels. A
+// canvas pixel outside it maps more than a pixel outside B, which its
+// validity test (exact to ~1e-10 px) rejects; the whole canvas when the
+// corners cannot be mapped.
+void b_box(const s6::Geometry& g, int box[4]) {
+    box[0] = 0;
+    box[1] = 0;
+    box[2] = g.width - 1;
+    box[3] = g.height - 1;
+    using R = long double;
+    R m[9];
+    for (int i = 0; i < 9; ++i) m[i] = g.inverse[i];
+    const R c0 = m[4] * m[8] - m[5] * m[7], c1 = m[5] * m[6] - m[3] * m[8], c2 = m[3] * m[7] - m[4] * m[6];
+    const R det = m[0] * c0 + m[1] * c1 + m[2] * c2;
+    if (!(std::fabs(det) > 0) || !std::isfinite(det)) return;
+    const R f[9] = {c0, m[2] * m[7] - m[1] * m[8], m[1] * m[5] - m[2] * m[4],
+                    c1, m[0] * m[8] - m[2] * m[6], m[2] * m[3] - m[0] * m[5],
+                    c2, m[1] * m[6] - m[0] * m[7], m[0] * m[4] - m[1] * m[3]};
+    R x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    for (int k = 0; k < 4; ++k) {
+        const R x = (k & 1) ? g.width_b - 1 : 0, y = (k & 2) ? g.height_b - 1 : 0;
+        const R w = f[6] * x + f[7] * y + f[8];
+        if (!(std::fabs(w) > 0) || !std::isfinite(w)) return;
+        const R px = (f[0] * x + f[1] * y + f[2]) / w - g.origin_x;
+        const R py = (f[3] * x + f[4] * y + f[5]) / w - g.origin_y;
+        if (!std::isfinite(px) || !std::isfinite(py)) return;
+        x0 = k ? std::min(x0, px) : px;
+        x1 = k ? std::max(x1, px) : px;
+        y0 = k ? std::min(y0, py) : py;
+        y1 = k ? std::max(y1, py) : py;
+    }
+    const auto clip = [](R v, int lo, int hi) {
+        return static_cast<int>(std::max<R>(lo, std::min<R>(hi, v)));
+    };
+    box[0] = clip(std::floor(x0) - 2, 0, g.width - 1);
+    box[1] = clip(std::floor(y0) - 2, 0, g.height - 1);
+    box[2] = clip(std::ceil(x1) + 2, -1, g.width - 1);
+    box[3] = clip(std::ceil(y1) + 2, -1, g.height - 1);
+}
+
+}  // namespace
 
 Status make_s6_job(const ImageView& a, const ImageView& b, const s6::Geometry& g, s6::BlendMode mode,
                    const ImageView& output, const ImageView& mask, void* ws, std::size_t available,
@@ -639,6 +740,12 @@ Status make_s6_job(const ImageView& a, const ImageView& b, const s6::Geometry& g
     j.denominator_tolerance = g.denominator_tolerance;
     j.snap_tolerance = g.snap_tolerance;
     std::memcpy(j.inverse, g.inverse, sizeof j.inverse);
+    int box[4];
+    b_box(g, box);
+    j.bx0 = box[0];
+    j.by0 = box[1];
+    j.bx1 = box[2];
+    j.by1 = box[3];
     j.out = output.data;
     j.out_pitch = output.pitch_bytes;
     j.mask = mask.data;
@@ -659,17 +766,23 @@ Status make_s6_job(const ImageView& a, const ImageView& b, const s6::Geometry& g
             L.w = o.w;
             L.h = o.h;
             L.n = static_cast<std::size_t>(o.w) * o.h;
-            L.a = f(o.a);
             L.b = f(o.b);
             if (l == 0) {
                 L.flags = base + o.flags;
-            } else {
-                L.va = f(o.va);
-                L.vb = f(o.vb);
-                if (l <= layout.blend_top) {
-                    L.ua = f(o.ua);
-                    L.ub = f(o.ub);
-                    L.z = f(o.z);
+                continue;
+            }
+            L.a = f(o.a);
+            L.va = f(o.va);
+            L.vb = f(o.vb);
+            L.ga = L.a;
+            L.gb = L.b;
+            if (l <= layout.blend_top) {
+                L.ua = f(o.ua);
+                L.ub = f(o.ub);
+                L.z = f(o.z);
+                if (l == 1) {
+                    L.ga = f(o.ga);
+                    L.gb = f(o.gb);
                 }
             }
         }
@@ -695,8 +808,10 @@ S6Totals s6_prefix(S6Job* js, int n) {
             const long long w = j.lv[l].w, h = j.lv[l].h;
             const long long rtiles = ((w + kRW - 1) / kRW) * ((h + kRH - 1) / kRH);
             const long long etiles = ((w + kEW - 1) / kEW) * ((h + kEH - 1) / kEH);
-            if (l >= 1) t.pull[l] += rtiles;
-            t.push[l] += etiles;
+            if (l >= 1) {
+                t.pull[l] += rtiles;
+                t.push[l] += etiles;
+            }
             if (j.blend_top >= l) {
                 if (l >= 1) t.gauss[l] += rtiles;
                 t.mix[l] += etiles;
@@ -712,27 +827,33 @@ hipError_t launch_s6(const S6Job* table, int count, const S6Totals& t, const S6J
                      const dev::Replay<S6Totals>* replay) {
     dev::JobSpan<S6Job> span{table, count, single ? *single : S6Job{}, replay ? replay->count : nullptr};
     const S6Totals* d = replay ? replay->totals : nullptr;
+    const auto total = [&](const long long* host, const long long* dev) { return dev::Total(*host, d ? dev : nullptr); };
     const bool any_stats = single ? single->stats != nullptr : true;
     if (any_stats) s6_clear_stats<<<count, 64, 0, stream>>>(span);
-    if (t.warp > 0) s6_warp<<<persistent_grid(t.warp), kBlock, 0, stream>>>(span, dev::Total(t.warp, d ? &d->warp : nullptr));
-    for (int l = 1; l <= t.top; ++l)
-        if (t.pull[l] > 0)
-            s6_reduce<<<persistent_grid(t.pull[l]), kBlock, 0, stream>>>(
-                span, dev::Total(t.pull[l], d ? &d->pull[l] : nullptr), l, 0);
-    for (int l = t.top; l >= 0; --l)
+    if (t.warp > 0) s6_warp<<<persistent_grid(t.warp), kBlock, 0, stream>>>(span, total(&t.warp, d ? &d->warp : nullptr));
+    for (int l = 1; l <= t.top; ++l) {
+        if (t.pull[l] <= 0) continue;
+        const dev::Total w = total(&t.pull[l], d ? &d->pull[l] : nullptr);
+        if (l == 1) s6_reduce<2><<<persistent_grid(t.pull[l]), kBlock, 0, stream>>>(span, w, l);
+        else s6_reduce<0><<<persistent_grid(t.pull[l]), kBlock, 0, stream>>>(span, w, l);
+    }
+    for (int l = t.top; l >= 1; --l)
         if (t.push[l] > 0)
-            s6_push<<<persistent_grid(t.push[l]), kBlock, 0, stream>>>(span, dev::Total(t.push[l], d ? &d->push[l] : nullptr), l);
-    for (int l = 1; l <= t.blend_top; ++l)
-        if (t.gauss[l] > 0)
-            s6_reduce<<<persistent_grid(t.gauss[l]), kBlock, 0, stream>>>(
-                span, dev::Total(t.gauss[l], d ? &d->gauss[l] : nullptr), l, 1);
+            s6_push<<<persistent_grid(t.push[l]), kBlock, 0, stream>>>(span, total(&t.push[l], d ? &d->push[l] : nullptr), l);
+    for (int l = 1; l <= t.blend_top; ++l) {
+        if (t.gauss[l] <= 0) continue;
+        const dev::Total w = total(&t.gauss[l], d ? &d->gauss[l] : nullptr);
+        if (l == 1) s6_reduce<3><<<persistent_grid(t.gauss[l]), kBlock, 0, stream>>>(span, w, l);
+        else s6_reduce<1><<<persistent_grid(t.gauss[l]), kBlock, 0, stream>>>(span, w, l);
+    }
     for (int l = t.blend_top; l >= 0; --l)
         if (t.mix[l] > 0)
-            s6_mix<<<persistent_grid(t.mix[l]), kBlock, 0, stream>>>(span, dev::Total(t.mix[l], d ? &d->mix[l] : nullptr), l);
+            s6_mix<<<persistent_grid(t.mix[l]), kBlock, 0, stream>>>(span, total(&t.mix[l], d ? &d->mix[l] : nullptr), l);
     return hipGetLastError();
 }
 
 }  // namespace jobs
+
 
 namespace s6 {
 

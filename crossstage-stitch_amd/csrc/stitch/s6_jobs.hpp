@@ -13,7 +13,7 @@ constexpr int kS6MaxLevels = 24;  // canvases up to 2^23 pixels per side
 
 struct S6LevelOffsets {
     int w, h;
-    std::size_t a, b, va, vb, ua, ub, z, flags;
+    std::size_t a, b, va, vb, ua, ub, z, ga, gb, flags;
 };
 
 struct S6Layout {
@@ -22,18 +22,23 @@ struct S6Layout {
     std::size_t bytes;
 };
 
-// Level 0 keeps validity and ownership as bits of one byte per pixel (they
-// are exactly 0 or 1 there); coarser levels hold FP32 planes.
+// Level 0 is never stored as A colours: A sits at an integer offset on the
+// canvas, so its level-0 colour is read from the source bytes through a
+// 256-entry table. Validity and ownership are bits of one byte per pixel
+// (exactly 0 or 1 there) and B's warped colour is stored where B is valid.
+// Coarser levels hold FP32 planes.
 struct S6Level {
     int w, h;
     std::size_t n;  // w * h; colour planes are n floats apart
-    float* a;   // R, G, B planes
-    float* b;   // R, G, B planes
+    float* a;   // R, G, B planes (levels >= 1): pull-push, then Gaussian at levels >= 2
+    float* b;   // R, G, B planes (level 0: warped B where valid)
     float* va;  // levels >= 1
     float* vb;
     float* ua;  // levels 1..blend_top
     float* ub;
     float* z;   // R, G, B planes, levels 1..blend_top
+    float* ga;  // Gaussian colour planes: own planes at level 1, a / b above
+    float* gb;
     std::uint8_t* flags;  // level 0: bit 0 va, 1 vb, 2 ua, 3 ub
 };
 
@@ -52,6 +57,7 @@ struct S6Job {
     int mode;
     int top, blend_top;  // -1 for feather
     S6Level lv[kS6MaxLevels];
+    int bx0, by0, bx1, by1;  // canvas box outside which B is invalid (inclusive)
     long long warp_begin;
     long long rb[kS6MaxLevels];  // pull (reduce into level l) tiles
     long long gb[kS6MaxLevels];  // Gaussian (reduce into level l) tiles

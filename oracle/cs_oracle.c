@@ -977,8 +977,11 @@ int orc_s6_prepare_geometry(int wa, int ha, int wb, int hb, const double* h, dou
  * S6 render (src/s6/s6.cu:122-306, orchestration :370-433): FP64 inverse map,
  * bilinear, feather or Laplacian multiband with FP32 pyramids. Band counts
  * beyond the reference's 3 and 5 use the same depth-generic algorithm.
+ * Filter taps, expansions and the band mix accumulate with fused
+ * multiply-add (one rounding per tap), the contraction nvcc applies to
+ * s6.cu's `value += tap * tap * source` by default.
  * Pinned by: golden vectors from the reference's FP64 NumPy renderer
- * (tests/golden, <= 2 LSB, mask exact), analytic cases of
+ * (tests/golden, <= 1 LSB, mask exact), analytic cases of
  * tests/test_s6_cuda.cu:105-191 and tests/python/test_s6_reference.py:12-79.
  * ====================================================================== */
 
@@ -1003,7 +1006,7 @@ static float expanded(const float* src, int cw, int ch, int channels, int x, int
         for (int dx = -2; dx <= 2; ++dx) {
             const int xx = reflect101_l((long long)x + dx, 2LL * cw);
             if (xx & 1) continue;
-            sum += tap(dx) * tap(dy) * src[((size_t)(yy / 2) * cw + xx / 2) * channels + c];
+            sum = fmaf(tap(dx) * tap(dy), src[((size_t)(yy / 2) * cw + xx / 2) * channels + c], sum);
         }
     }
     return 4 * sum;
@@ -1045,7 +1048,7 @@ static void reduce_level(const float* src, int w, int h, float* dst, int dw, int
                     const int yy = reflect101_l(2LL * y + dy, h);
                     for (int dx = -2; dx <= 2; ++dx) {
                         const int xx = reflect101_l(2LL * x + dx, w);
-                        v += tap(dx) * tap(dy) * src[((size_t)yy * w + xx) * channels + c];
+                        v = fmaf(tap(dx) * tap(dy), src[((size_t)yy * w + xx) * channels + c], v);
                     }
                 }
                 dst[((size_t)y * dw + x) * channels + c] = v;
@@ -1178,7 +1181,7 @@ int orc_s6_render(const uint8_t* a, int pitch_a, const uint8_t* b, int pitch_b, 
                 for (int c = 0; c < 3; ++c) {
                     const float la = A[l][3 * p + c] - (coarse ? expanded(A[l + 1], lw[l + 1], lh[l + 1], 3, x, y, c) : 0);
                     const float lb = B[l][3 * p + c] - (coarse ? expanded(B[l + 1], lw[l + 1], lh[l + 1], 3, x, y, c) : 0);
-                    Z[l][3 * p + c] = total > 0 ? (UA[l][p] * la + UB[l][p] * lb) / total : 0;
+                    Z[l][3 * p + c] = total > 0 ? fmaf(UA[l][p], la, UB[l][p] * lb) / total : 0;
                 }
             }
     }
