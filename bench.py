@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU oracle sample")
     ap.add_argument("--tune-cache", default=None)
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes (profiling runs)")
     return ap.parse_args()
 
 
@@ -228,8 +229,9 @@ def main():
     torch.cuda.synchronize()
     specs = specs_for(dev, work)
     br = B.BatchRunner(max_pairs=args.pairs, pool_bytes=POOL_BYTES)
-    tune = br.autotune(specs, args.tune_cache)
-    log("tuning", tune.s4_segment, tune.workgroups_per_cu, tune.graphs)
+    if not args.no_tune:
+        tune = br.autotune(specs, args.tune_cache)
+        log("tuning", tune.s4_segment, tune.workgroups_per_cu, tune.graphs)
     total, times, outcomes, tm = time_batched(br, specs, args.steps, args.warmup)
     ok = sum(o.status == 0 for o in outcomes)
     batched = dict(pairs_per_s=args.pairs * args.steps / total, ms_per_step=1e3 * total / args.steps,

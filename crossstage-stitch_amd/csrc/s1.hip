@@ -57,17 +57,33 @@ __device__ std::uint8_t s1_pixel(const S1Job& p, int xd, int yd) {
     return static_cast<std::uint8_t>(static_cast<int>(rintf(total * p.scale_area)));
 }
 
-// Persistent walk over 64 x 4 output tiles of every job; one pixel per thread.
+// Persistent walk over 64 x 16 output tiles of every job; four rows per
+// thread, so each thread has four pixels' loads in flight.
+constexpr int kRowsPerThread = kS1TileH / (256 / kS1TileW);
+static_assert(kRowsPerThread * (256 / kS1TileW) == kS1TileH, "S1 tile");
+
 __global__ void __launch_bounds__(256) s1_kernel(dev::JobSpan<S1Job> span, dev::Total work) {
     const long long total = work.get();
+    int j = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int j = dev::find_job(span, t, &S1Job::tile_begin);
+        j = dev::next_job(span, t, &S1Job::tile_begin, j, njobs);
         const S1Job& p = dev::job_at(span, j);
-        const long long local = t - p.tile_begin;
-        const int tx = static_cast<int>(local % p.tiles_x), ty = static_cast<int>(local / p.tiles_x);
+        const unsigned local = static_cast<unsigned>(t - p.tile_begin);
+        const unsigned tiles_x = static_cast<unsigned>(p.tiles_x);
+        const int tx = static_cast<int>(local % tiles_x), ty = static_cast<int>(local / tiles_x);
         const int xd = tx * kS1TileW + static_cast<int>(threadIdx.x % kS1TileW);
-        const int yd = ty * kS1TileH + static_cast<int>(threadIdx.x / kS1TileW);
-        if (xd < p.work_w && yd < p.work_h) p.dst[static_cast<size_t>(yd) * p.dst_pitch + xd] = s1_pixel(p, xd, yd);
+        std::uint8_t v[kRowsPerThread];
+#pragma unroll
+        for (int k = 0; k < kRowsPerThread; ++k) {
+            const int yd = ty * kS1TileH + static_cast<int>(threadIdx.x / kS1TileW) + k * (256 / kS1TileW);
+            v[k] = xd < p.work_w && yd < p.work_h ? s1_pixel(p, xd, yd) : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < kRowsPerThread; ++k) {
+            const int yd = ty * kS1TileH + static_cast<int>(threadIdx.x / kS1TileW) + k * (256 / kS1TileW);
+            if (xd < p.work_w && yd < p.work_h) p.dst[static_cast<size_t>(yd) * p.dst_pitch + xd] = v[k];
+        }
     }
 }
 
@@ -76,18 +92,25 @@ using jobs::GradJob;
 // Sum of |I(x+1,y)-I(x,y)| + |I(x,y+1)-I(x,y)|: wave sums, one atomic per wave.
 __global__ void __launch_bounds__(256) gradient_kernel(dev::JobSpan<GradJob> span, dev::Total work) {
     const long long total = work.get();
+    int j = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int j = dev::find_job(span, t, &GradJob::tile_begin);
+        j = dev::next_job(span, t, &GradJob::tile_begin, j, njobs);
         const GradJob& g = dev::job_at(span, j);
-        const long long local = t - g.tile_begin;
-        const int x = static_cast<int>(local % g.tiles_x) * kS1TileW + static_cast<int>(threadIdx.x % kS1TileW);
-        const int y = static_cast<int>(local / g.tiles_x) * kS1TileH + static_cast<int>(threadIdx.x / kS1TileW);
+        const unsigned local = static_cast<unsigned>(t - g.tile_begin);
+        const unsigned tiles_x = static_cast<unsigned>(g.tiles_x);
+        const int x = static_cast<int>(local % tiles_x) * kS1TileW + static_cast<int>(threadIdx.x % kS1TileW);
         unsigned diff = 0;
-        if (x < g.w && y < g.h) {
-            const std::uint8_t* row = g.src + static_cast<size_t>(y) * g.pitch;
-            const int c = row[x];
-            if (x + 1 < g.w) diff += static_cast<unsigned>(abs(static_cast<int>(row[x + 1]) - c));
-            if (y + 1 < g.h) diff += static_cast<unsigned>(abs(static_cast<int>(row[x + g.pitch]) - c));
+#pragma unroll
+        for (int k = 0; k < kRowsPerThread; ++k) {
+            const int y = static_cast<int>(local / tiles_x) * kS1TileH + static_cast<int>(threadIdx.x / kS1TileW) +
+                          k * (256 / kS1TileW);
+            if (x < g.w && y < g.h) {
+                const std::uint8_t* row = g.src + static_cast<size_t>(y) * g.pitch;
+                const int c = row[x];
+                if (x + 1 < g.w) diff += static_cast<unsigned>(abs(static_cast<int>(row[x + 1]) - c));
+                if (y + 1 < g.h) diff += static_cast<unsigned>(abs(static_cast<int>(row[x + g.pitch]) - c));
+            }
         }
         diff = dev::wave_sum(diff);
         if (dev::lane_id() == 0 && diff) atomicAdd(g.accum, static_cast<unsigned long long>(diff));

@@ -88,8 +88,10 @@ __global__ void __launch_bounds__(256) s2_fast_nms(dev::JobSpan<S2Job> span, dev
     __shared__ std::uint8_t score[kScoreH * kScoreW];
     __shared__ std::uint32_t hist[256];
     const int tid = threadIdx.x;
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = dev::find_job(span, t, &S2Job::fast_begin);
+        ji = dev::next_job(span, t, &S2Job::fast_begin, ji, njobs);
         const S2Job& j = dev::job_at(span, ji);
         const long long local = t - j.fast_begin;
         const int x0 = static_cast<int>(local % j.tiles_x) * kS2TileW;
@@ -235,8 +237,10 @@ __device__ Flags block_scan(Flags v, Flags* excl) {
 
 __global__ void __launch_bounds__(256) s2_count(dev::JobSpan<S2Job> span, dev::Total work) {
     const long long total = work.get();
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = dev::find_job(span, t, &S2Job::chunk_begin);
+        ji = dev::next_job(span, t, &S2Job::chunk_begin, ji, njobs);
         const S2Job& j = dev::job_at(span, ji);
         const long long c = t - j.chunk_begin;
         const long long pixels = static_cast<long long>(j.w) * j.h;
@@ -284,8 +288,10 @@ __global__ void __launch_bounds__(256) s2_scan(dev::JobSpan<S2Job> span) {
 // all points above the boundary score, then the first `ties` points at it.
 __global__ void __launch_bounds__(256) s2_scatter(dev::JobSpan<S2Job> span, dev::Total work) {
     const long long total = work.get();
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = dev::find_job(span, t, &S2Job::chunk_begin);
+        ji = dev::next_job(span, t, &S2Job::chunk_begin, ji, njobs);
         const S2Job& j = dev::job_at(span, ji);
         const long long c = t - j.chunk_begin;
         const long long pixels = static_cast<long long>(j.w) * j.h;
@@ -344,8 +350,10 @@ __device__ float harris_response(const std::uint8_t* image, int pitch, int x0, i
 
 __global__ void __launch_bounds__(256) s2_harris(dev::JobSpan<S2Job> span, dev::Total work) {
     const long long total = work.get();
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = dev::find_job(span, t, &S2Job::cand_begin);
+        ji = dev::next_job(span, t, &S2Job::cand_begin, ji, njobs);
         const S2Job& j = dev::job_at(span, ji);
         const int slot = static_cast<int>(t - j.cand_begin) * 256 + threadIdx.x;
         if (slot >= j.candidates) continue;
@@ -368,8 +376,10 @@ __global__ void __launch_bounds__(256) s2_rank(dev::JobSpan<S2Job> span, dev::To
     __shared__ std::uint64_t seg[kS2RankSegment];
     __shared__ std::uint32_t part[4][kS2RankKeys];
     const int lane = dev::lane_id(), wave = threadIdx.x / 64;
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = dev::find_job(span, t, &S2Job::rank_begin);
+        ji = dev::next_job(span, t, &S2Job::rank_begin, ji, njobs);
         const S2Job& j = dev::job_at(span, ji);
         const long long local = t - j.rank_begin;
         const int segs = (j.candidates + kS2RankSegment - 1) / kS2RankSegment;
@@ -398,8 +408,10 @@ __global__ void __launch_bounds__(256) s2_rank(dev::JobSpan<S2Job> span, dev::To
 
 __global__ void __launch_bounds__(256) s2_write(dev::JobSpan<S2Job> span, dev::Total work) {
     const long long total = work.get();
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = dev::find_job(span, t, &S2Job::cand_begin);
+        ji = dev::next_job(span, t, &S2Job::cand_begin, ji, njobs);
         const S2Job& j = dev::job_at(span, ji);
         const int slot = static_cast<int>(t - j.cand_begin) * 256 + threadIdx.x;
         if (slot >= j.candidates || static_cast<std::uint32_t>(slot) >= j.params->selected) continue;

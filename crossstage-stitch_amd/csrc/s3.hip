@@ -43,8 +43,10 @@ __global__ void __launch_bounds__(256) s3_blur(dev::JobSpan<S3Job> span, dev::To
     __shared__ float in[kInH * kInW];
     __shared__ float rows[kInH * kS3TileW];
     const int tid = threadIdx.x;
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = dev::find_job(span, t, &S3Job::blur_begin);
+        ji = dev::next_job(span, t, &S3Job::blur_begin, ji, njobs);
         const S3Job& j = dev::job_at(span, ji);
         const long long local = t - j.blur_begin;
         const int x0 = static_cast<int>(local % j.tiles_x) * kS3TileW;
@@ -107,8 +109,10 @@ __global__ void __launch_bounds__(64 * kS3KeypointsPerBlock) s3_describe(dev::Jo
     const long long total = work.get();
     const int lane = dev::lane_id();
     const int wave = threadIdx.x / 64;
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = dev::find_job(span, t, &S3Job::kp_begin);
+        ji = dev::next_job(span, t, &S3Job::kp_begin, ji, njobs);
         const S3Job& j = dev::job_at(span, ji);
         const int index = static_cast<int>(t - j.kp_begin) * kS3KeypointsPerBlock + wave;
         if (index >= j.capacity || index >= *j.d_count) continue;

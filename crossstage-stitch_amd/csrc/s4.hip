@@ -66,8 +66,10 @@ __global__ void __launch_bounds__(256) s4_dist(dev::JobSpan<S4Job> span, dev::To
     __shared__ uint4 train[2 * kMaxSegment];
     __shared__ unsigned colmin[kMaxSegment];
     const int lane = dev::lane_id(), wave = threadIdx.x / 64;
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = dev::find_job(span, t, &S4Job::dist_begin);
+        ji = dev::next_job(span, t, &S4Job::dist_begin, ji, njobs);
         const S4Job& j = dev::job_at(span, ji);
         const long long local = t - j.dist_begin;
         const int qb = static_cast<int>(local / j.segs), sg = static_cast<int>(local % j.segs);
@@ -142,8 +144,10 @@ __device__ Top2 empty_top2() { return {INFINITY, INFINITY, -1, -1}; }
 // Forward merge over train segments, one thread per query; writes Top2.
 __global__ void __launch_bounds__(256) s4_qmerge(dev::JobSpan<S4Job> span, dev::Total work) {
     const long long total = work.get();
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = dev::find_job(span, t, &S4Job::qmerge_begin);
+        ji = dev::next_job(span, t, &S4Job::qmerge_begin, ji, njobs);
         const S4Job& j = dev::job_at(span, ji);
         const int q = static_cast<int>(t - j.qmerge_begin) * 256 + threadIdx.x;
         if (q >= j.cap_a) continue;
@@ -173,8 +177,10 @@ __global__ void __launch_bounds__(256) s4_qmerge(dev::JobSpan<S4Job> span, dev::
 // Reverse top-1 per train row: minimum key over query blocks.
 __global__ void __launch_bounds__(256) s4_cmerge(dev::JobSpan<S4Job> span, dev::Total work) {
     const long long total = work.get();
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = dev::find_job(span, t, &S4Job::cmerge_begin);
+        ji = dev::next_job(span, t, &S4Job::cmerge_begin, ji, njobs);
         const S4Job& j = dev::job_at(span, ji);
         const int r = static_cast<int>(t - j.cmerge_begin) * 256 + threadIdx.x;
         if (r >= j.cap_b) continue;
@@ -213,8 +219,10 @@ __device__ void insert_top2(Top2& p, double d, int i) {
 __global__ void __launch_bounds__(256) s4_generic(dev::JobSpan<S4Job> span, dev::Total work, int reverse,
                                                   Top2* external) {
     const long long total = work.get();
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = dev::find_job(span, t, reverse ? &S4Job::grev_begin : &S4Job::gfwd_begin);
+        ji = dev::next_job(span, t, reverse ? &S4Job::grev_begin : &S4Job::gfwd_begin, ji, njobs);
         const S4Job& j = dev::job_at(span, ji);
         const long long first_tile = reverse ? j.grev_begin : j.gfwd_begin;
         const int q = static_cast<int>(t - first_tile) * 256 + threadIdx.x;

@@ -314,8 +314,10 @@ __global__ void __launch_bounds__(256) s5_prepare(dev::JobSpan<S5Job> span) {
 // One lane per hypothesis: sample, degeneracy test, 4-point solve, domain.
 __global__ void __launch_bounds__(256) s5_generate(dev::JobSpan<S5Job> span, dev::Total work) {
     const long long total = work.get();
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = dev::find_job(span, t, &S5Job::gen_begin);
+        ji = dev::next_job(span, t, &S5Job::gen_begin, ji, njobs);
         const S5Job& j = dev::job_at(span, ji);
         const int i = static_cast<int>(t - j.gen_begin) * 256 + threadIdx.x;
         if (i >= j.count) continue;
@@ -339,8 +341,10 @@ __global__ void __launch_bounds__(256) s5_generate(dev::JobSpan<S5Job> span, dev
 __global__ void __launch_bounds__(256) s5_score(dev::JobSpan<S5Job> span, dev::Total work) {
     const long long total = work.get();
     const int lane = dev::lane_id(), wave = threadIdx.x / 64;
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = dev::find_job(span, t, &S5Job::score_begin);
+        ji = dev::next_job(span, t, &S5Job::score_begin, ji, njobs);
         const S5Job& j = dev::job_at(span, ji);
         const int i = static_cast<int>(t - j.score_begin) * 4 + wave;
         if (i >= j.count) continue;

@@ -84,28 +84,13 @@ __device__ __forceinline__ std::uint8_t quantize(float x) {
     return static_cast<std::uint8_t>(floorf(fminf(255.0f, fmaxf(0.0f, 255.0f * x)) + 0.5f));
 }
 
+// Incremental form for tiles visited in increasing order (see dev::next_job).
 template <class Begin>
-__device__ int find_job_by(const dev::JobSpan<S6Job>& span, long long t, Begin begin) {
+__device__ __forceinline__ int next_job_by(const dev::JobSpan<S6Job>& span, long long t, Begin begin, int ji,
+                                           int jobs) {
     if (!span.table) return 0;
-    int lo = 0, hi = span.jobs() - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (begin(span.table[mid]) <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-// Row-segment tiles: tile = (row, 256-pixel segment); no per-thread division.
-__device__ __forceinline__ unsigned tiles_x(int w) { return (static_cast<unsigned>(w) + kBlock - 1) / kBlock; }
-
-__device__ __forceinline__ bool tile_pixel(long long local, int w, int& x, int& y) {
-    const unsigned lt = static_cast<unsigned>(local);
-    const unsigned tx = tiles_x(w);
-    const unsigned row = lt / tx;
-    x = static_cast<int>((lt - row * tx) * kBlock + threadIdx.x);
-    y = static_cast<int>(row);
-    return x < w;
+    while (ji + 1 < jobs && dev::ld_const(&begin(span.table[ji + 1])) <= t) ++ji;
+    return ji;
 }
 
 constexpr unsigned kVA = 1u, kVB = 2u, kUA = 4u, kUB = 8u;
@@ -161,78 +146,93 @@ __device__ void flush_stats(const S6Job& j, unsigned long long u, unsigned long 
 // box that can hold B). Multiband: level-0 flags and B's colour where valid.
 // Feather: the output and the union mask. Union/overlap counters for both.
 // The border scores are only needed where both images are valid.
+__device__ __forceinline__ void warp_pixel(const S6Job& j, int x, int y, const float* lut, unsigned long long& uni,
+                                           unsigned long long& ovl) {
+    const size_t p = static_cast<size_t>(y) * j.width + x;
+    const int ax = x + j.origin_x, ay = y + j.origin_y;
+    const double xa = static_cast<double>(ax), ya = static_cast<double>(ay);
+    float ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
+    const bool ma = ax >= 0 && ax <= j.wa - 1 && ay >= 0 && ay <= j.ha - 1;
+    bool mb = false;
+    double xb = 0, yb = 0;
+    if (x >= j.bx0 && x <= j.bx1 && y >= j.by0 && y <= j.by1) {
+        const double* inv = j.inverse;
+        const double den = inv[6] * xa + inv[7] * ya + inv[8];
+        const double dsc = fabs(inv[6] * xa) + fabs(inv[7] * ya) + fabs(inv[8]);
+        if (isfinite(den) && dsc > 0 && fabs(den) > j.denominator_tolerance * dsc) {
+            xb = (inv[0] * xa + inv[1] * ya + inv[2]) / den;
+            yb = (inv[3] * xa + inv[4] * ya + inv[5]) / den;
+            mb = sample(j.b, j.wb, j.hb, j.pitch_b, xb, yb, rb, nullptr, j.snap_tolerance);
+        }
+    }
+    double sa = 0, sb = 0;
+    if (ma && mb) {
+        sa = fmin(fmin(xa + 0.5, j.wa - 0.5 - xa) / j.wa, fmin(ya + 0.5, j.ha - 0.5 - ya) / j.ha);
+        center_coordinate(xb, j.wb, j.snap_tolerance);
+        center_coordinate(yb, j.hb, j.snap_tolerance);
+        sb = fmin(fmin(xb + 0.5, j.wb - 0.5 - xb) / j.wb, fmin(yb + 0.5, j.hb - 0.5 - yb) / j.hb);
+    }
+    if (j.top >= 0) {
+        const S6Level& L = j.lv[0];
+        unsigned f = 0;
+        if (ma) f |= kVA;
+        if (mb) f |= kVB;
+        if (ma && (!mb || sa >= sb)) f |= kUA;
+        if (mb && (!ma || sb > sa)) f |= kUB;
+        L.flags[p] = static_cast<std::uint8_t>(f);
+        if (mb) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) L.b[c * L.n + p] = rb[c];
+        }
+    } else {
+        if (ma) {
+            const std::uint8_t* q = a_pixel(j, x, y);
+            ra[0] = lut[q[0]];
+            ra[1] = lut[q[1]];
+            ra[2] = lut[q[2]];
+        }
+        std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float color =
+                ma && mb ? static_cast<float>((sa * ra[c] + sb * rb[c]) / (sa + sb)) : (ma ? ra[c] : rb[c]);
+            out[c] = quantize(color);
+        }
+        j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = ma || mb ? 1 : 0;
+    }
+    uni += ma || mb;
+    ovl += ma && mb;
+}
+
+// Tiles are (row, 1024-pixel segment); each thread takes four pixels 256 apart.
+constexpr int kWarpSeg = 4 * kBlock;
+
 __global__ void __launch_bounds__(kBlock) s6_warp(dev::JobSpan<S6Job> span, dev::Total work) {
     const long long total = work.get();
     __shared__ unsigned long long red[2][kBlock / 64];
     __shared__ float lut[256];
     fill_lut(lut);
     __syncthreads();
-    int cur = -1;
+    int cur = -1, ji = 0;
+    const int njobs = span.jobs();
     unsigned long long uni = 0, ovl = 0;
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = find_job_by(span, t, [](const S6Job& j) { return j.warp_begin; });
+        ji = next_job_by(span, t, [](const S6Job& j) -> const long long& { return j.warp_begin; }, ji, njobs);
         if (ji != cur) {
             if (cur >= 0 && dev::job_at(span, cur).stats) flush_stats(dev::job_at(span, cur), uni, ovl, red);
             cur = ji;
             uni = ovl = 0;
         }
         const S6Job& j = dev::job_at(span, ji);
-        int x, y;
-        if (!tile_pixel(t - j.warp_begin, j.width, x, y)) continue;
-        const size_t p = static_cast<size_t>(y) * j.width + x;
-        const int ax = x + j.origin_x, ay = y + j.origin_y;
-        const double xa = static_cast<double>(ax), ya = static_cast<double>(ay);
-        float ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
-        const bool ma = ax >= 0 && ax <= j.wa - 1 && ay >= 0 && ay <= j.ha - 1;
-        bool mb = false;
-        double xb = 0, yb = 0;
-        if (x >= j.bx0 && x <= j.bx1 && y >= j.by0 && y <= j.by1) {
-            const double* inv = j.inverse;
-            const double den = inv[6] * xa + inv[7] * ya + inv[8];
-            const double dsc = fabs(inv[6] * xa) + fabs(inv[7] * ya) + fabs(inv[8]);
-            if (isfinite(den) && dsc > 0 && fabs(den) > j.denominator_tolerance * dsc) {
-                xb = (inv[0] * xa + inv[1] * ya + inv[2]) / den;
-                yb = (inv[3] * xa + inv[4] * ya + inv[5]) / den;
-                mb = sample(j.b, j.wb, j.hb, j.pitch_b, xb, yb, rb, nullptr, j.snap_tolerance);
-            }
-        }
-        double sa = 0, sb = 0;
-        if (ma && mb) {
-            sa = fmin(fmin(xa + 0.5, j.wa - 0.5 - xa) / j.wa, fmin(ya + 0.5, j.ha - 0.5 - ya) / j.ha);
-            center_coordinate(xb, j.wb, j.snap_tolerance);
-            center_coordinate(yb, j.hb, j.snap_tolerance);
-            sb = fmin(fmin(xb + 0.5, j.wb - 0.5 - xb) / j.wb, fmin(yb + 0.5, j.hb - 0.5 - yb) / j.hb);
-        }
-        if (j.top >= 0) {
-            const S6Level& L = j.lv[0];
-            unsigned f = 0;
-            if (ma) f |= kVA;
-            if (mb) f |= kVB;
-            if (ma && (!mb || sa >= sb)) f |= kUA;
-            if (mb && (!ma || sb > sa)) f |= kUB;
-            L.flags[p] = static_cast<std::uint8_t>(f);
-            if (mb) {
+        const unsigned local = static_cast<unsigned>(t - j.warp_begin);
+        const unsigned segs = (static_cast<unsigned>(j.width) + kWarpSeg - 1) / kWarpSeg;
+        const unsigned row = local / segs;
+        const int x0 = static_cast<int>((local - row * segs) * kWarpSeg + threadIdx.x);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) L.b[c * L.n + p] = rb[c];
-            }
-        } else {
-            if (ma) {
-                const std::uint8_t* q = a_pixel(j, x, y);
-                ra[0] = lut[q[0]];
-                ra[1] = lut[q[1]];
-                ra[2] = lut[q[2]];
-            }
-            std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float color =
-                    ma && mb ? static_cast<float>((sa * ra[c] + sb * rb[c]) / (sa + sb)) : (ma ? ra[c] : rb[c]);
-                out[c] = quantize(color);
-            }
-            j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = ma || mb ? 1 : 0;
+        for (int k = 0; k < 4; ++k) {
+            const int x = x0 + k * kBlock;
+            if (x < j.width) warp_pixel(j, x, static_cast<int>(row), lut, uni, ovl);
         }
-        uni += ma || mb;
-        ovl += ma && mb;
     }
     if (cur >= 0 && dev::job_at(span, cur).stats) flush_stats(dev::job_at(span, cur), uni, ovl, red);
 }
@@ -316,9 +316,11 @@ __global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, de
         fill_lut(lut);
         __syncthreads();
     }
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = (kMode == 0 || kMode == 2) ? find_job_by(span, t, [l](const S6Job& j) { return j.rb[l]; })
-                                                  : find_job_by(span, t, [l](const S6Job& j) { return j.gb[l]; });
+        ji = (kMode == 0 || kMode == 2) ? next_job_by(span, t, [l](const S6Job& j) -> const long long& { return j.rb[l]; }, ji, njobs)
+                                        : next_job_by(span, t, [l](const S6Job& j) -> const long long& { return j.gb[l]; }, ji, njobs);
         const S6Job& j = dev::job_at(span, ji);
         const S6Level& C = j.lv[l];
         const S6Level& F = j.lv[l - 1];
@@ -327,55 +329,87 @@ __global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, de
         const int x0 = static_cast<int>(local % tx) * kRW, y0 = static_cast<int>(local / tx) * kRH;
         const int ix0 = 2 * x0 - 2, iy0 = 2 * y0 - 2;
         const int fx0 = (ix0 - 2) >> 1, fy0 = (iy0 - 2) >> 1;  // first staged coarse column / row (mode 3)
-        if (kMode == 3) {
-            stage_coarse<kFCols>(C.a, 3, C.n, C.w, C.h, fx0, fy0, kFCols, kFRows, &fill[0][0][0]);
-            stage_coarse<kFCols>(C.b, 3, C.n, C.w, C.h, fx0, fy0, kFCols, kFRows, &fill[3][0][0]);
-            __syncthreads();
-        }
-        for (int i = threadIdx.x; i < kRIW * kRIH; i += kBlock) {
-            const int ix = i % kRIW, iy = i / kRIW;
-            const int sx = clampi(reflect101(ix0 + ix, F.w), 0, F.w - 1);
-            const int sy = clampi(reflect101(iy0 + iy, F.h), 0, F.h - 1);
-            const size_t q = static_cast<size_t>(sy) * F.w + sx;
-            const int cx = (ix & 1) * kREven + (ix >> 1);
-            float v[8];
+        // Pass 1: every load of this thread's tile elements, issued before any
+        // is used (positions are clamped into the level, so all are in
+        // bounds; level-0 values of invalid pixels are discarded below).
+        constexpr int kElems = kRIW * kRIH, kPer = (kElems + kBlock - 1) / kBlock;
+        int sxs[kPer], sys[kPer];
+        unsigned fls[kPer], abytes[kPer][3];
+        float raw[kPer][8];
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const int i = static_cast<int>(threadIdx.x) + e * kBlock;
+            sxs[e] = clampi(reflect101(ix0 + i % kRIW, F.w), 0, F.w - 1);
+            sys[e] = clampi(reflect101(iy0 + i / kRIW, F.h), 0, F.h - 1);
+            const size_t q = static_cast<size_t>(sys[e]) * F.w + sxs[e];
             if (kFromZero) {
-                const unsigned f = F.flags[q];
-                Expand e;
-                if (kMode == 3 && (f & (kVA | kVB)) != (kVA | kVB))
-                    expand_setup(sx, sy, C.w, C.h, fx0, fy0, kFCols, kFRows, e);
-                if (f & kVA) {
-                    const std::uint8_t* s = a_pixel(j, sx, sy);
-                    v[0] = lut[s[0]];
-                    v[1] = lut[s[1]];
-                    v[2] = lut[s[2]];
-                } else if (kMode == 3) {
+                fls[e] = F.flags[q];
+                const int ax = clampi(sxs[e] + j.origin_x, 0, j.wa - 1);
+                const int ay = clampi(sys[e] + j.origin_y, 0, j.ha - 1);
+                const std::uint8_t* src = j.a + static_cast<size_t>(ay) * j.pitch_a + 3 * ax;
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) v[c] = expand_one<kFCols>(&fill[c][0][0], e);
-                } else {
-                    v[0] = v[1] = v[2] = 0;
+                for (int c = 0; c < 3; ++c) {
+                    abytes[e][c] = src[c];
+                    raw[e][3 + c] = F.b[c * F.n + q];
                 }
-                if (f & kVB) {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) v[3 + c] = F.b[c * F.n + q];
-                } else if (kMode == 3) {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) v[3 + c] = expand_one<kFCols>(&fill[3 + c][0][0], e);
-                } else {
-                    v[3] = v[4] = v[5] = 0;
-                }
-                v[6] = (f & (kMode == 2 ? kVA : kUA)) ? 1.0f : 0.0f;
-                v[7] = (f & (kMode == 2 ? kVB : kUB)) ? 1.0f : 0.0f;
             } else {
                 const float* sa = kMode == 0 ? F.a : F.ga;
                 const float* sb = kMode == 0 ? F.b : F.gb;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    v[c] = sa[c * F.n + q];
-                    v[3 + c] = sb[c * F.n + q];
+                    raw[e][c] = sa[c * F.n + q];
+                    raw[e][3 + c] = sb[c * F.n + q];
                 }
-                v[6] = (kMode == 0 ? F.va : F.ua)[q];
-                v[7] = (kMode == 0 ? F.vb : F.ub)[q];
+                raw[e][6] = (kMode == 0 ? F.va : F.ua)[q];
+                raw[e][7] = (kMode == 0 ? F.vb : F.ub)[q];
+            }
+        }
+        // The pushed level 1 behind the tile, only for the images that miss
+        // pixels in it (mode 3).
+        if (kMode == 3) {
+            bool miss_a = false, miss_b = false;
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) {
+                const bool real = static_cast<int>(threadIdx.x) + e * kBlock < kElems;
+                miss_a |= real && !(fls[e] & kVA);
+                miss_b |= real && !(fls[e] & kVB);
+            }
+            const bool need_a = __syncthreads_or(miss_a), need_b = __syncthreads_or(miss_b);
+            if (need_a) stage_coarse<kFCols>(C.a, 3, C.n, C.w, C.h, fx0, fy0, kFCols, kFRows, &fill[0][0][0]);
+            if (need_b) stage_coarse<kFCols>(C.b, 3, C.n, C.w, C.h, fx0, fy0, kFCols, kFRows, &fill[3][0][0]);
+            if (need_a || need_b) __syncthreads();
+        }
+        // Pass 2: level-0 selection and fill, then the LDS tile.
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const int i = static_cast<int>(threadIdx.x) + e * kBlock;
+            if (i >= kElems) continue;
+            const int ix = i % kRIW, iy = i / kRIW;
+            const int cx = (ix & 1) * kREven + (ix >> 1);
+            float v[8];
+            if (kFromZero) {
+                const unsigned f = fls[e];
+                const bool has_a = f & kVA, has_b = f & kVB;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    v[c] = has_a ? lut[abytes[e][c]] : 0.0f;
+                    v[3 + c] = has_b ? raw[e][3 + c] : 0.0f;
+                }
+                if (kMode == 3 && !(has_a && has_b)) {
+                    Expand ex;
+                    expand_setup(sxs[e], sys[e], C.w, C.h, fx0, fy0, kFCols, kFRows, ex);
+                    if (!has_a)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) v[c] = expand_one<kFCols>(&fill[c][0][0], ex);
+                    if (!has_b)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) v[3 + c] = expand_one<kFCols>(&fill[3 + c][0][0], ex);
+                }
+                v[6] = (f & (kMode == 2 ? kVA : kUA)) ? 1.0f : 0.0f;
+                v[7] = (f & (kMode == 2 ? kVB : kUB)) ? 1.0f : 0.0f;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) v[c] = raw[e][c];
             }
 #pragma unroll
             for (int c = 0; c < 8; ++c) tile[c][iy][cx] = v[c];
@@ -410,17 +444,21 @@ __global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, de
 }
 
 // ---------------------------------------------------- push, mix, collapse
-// Fine tiles of 64x4 pixels; their 3x3 coarse neighbourhoods span 34x4
-// coarse pixels, staged in LDS.
-constexpr int kEW = 64, kEH = 4, kCW = kEW / 2 + 2, kCH = kEH / 2 + 2;
+// Fine tiles of 64x16 pixels, four rows per thread (rows ty, ty + 4, ...),
+// so every thread has four independent pixels' loads in flight; their 3x3
+// coarse neighbourhoods span 34x10 coarse pixels, staged in LDS.
+constexpr int kEW = 64, kEH = 16, kEPer = kEH / (kBlock / kEW), kCW = kEW / 2 + 2, kCH = kEH / 2 + 2;
+static_assert(kEPer * (kBlock / kEW) == kEH, "expand tile");
 
 // normalize_color + push_missing for level l >= 1, top-down (s6.cu:254-270,
 // :399-410): the coarse level is final when it is expanded.
 __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev::Total work, int l) {
     const long long total = work.get();
     __shared__ float cc[6][kCH][kCW];
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = find_job_by(span, t, [l](const S6Job& j) { return j.pb[l]; });
+        ji = next_job_by(span, t, [l](const S6Job& j) -> const long long& { return j.pb[l]; }, ji, njobs);
         const S6Job& j = dev::job_at(span, ji);
         const S6Level& L = j.lv[l];
         const unsigned local = static_cast<unsigned>(t - j.pb[l]);
@@ -428,34 +466,55 @@ __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev:
         const int x0 = static_cast<int>(local % tx) * kEW, y0 = static_cast<int>(local / tx) * kEH;
         const bool coarse = l < j.top;
         const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 1;
-        if (coarse) {
+        const int x = x0 + static_cast<int>(threadIdx.x) % kEW;
+        float wa[kEPer], wb[kEPer], va[kEPer][3], vb[kEPer][3];
+        bool missing = false;
+#pragma unroll
+        for (int k = 0; k < kEPer; ++k) {
+            const int y = y0 + static_cast<int>(threadIdx.x) / kEW + k * (kBlock / kEW);
+            const bool inside = x < L.w && y < L.h;
+            const size_t p = inside ? static_cast<size_t>(y) * L.w + x : 0;
+            wa[k] = inside ? L.va[p] : 1.0f;
+            wb[k] = inside ? L.vb[p] : 1.0f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                va[k][c] = L.a[c * L.n + p];
+                vb[k][c] = L.b[c * L.n + p];
+            }
+            missing |= !(wa[k] > 0) || !(wb[k] > 0);
+        }
+        // The coarse level is staged only when a pixel of the tile misses A or B.
+        const bool staged = coarse && __syncthreads_or(missing);
+        if (staged) {
             const S6Level& C = j.lv[l + 1];
             stage_coarse<kCW>(C.a, 3, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[0][0][0]);
             stage_coarse<kCW>(C.b, 3, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[3][0][0]);
             __syncthreads();
         }
-        const int x = x0 + static_cast<int>(threadIdx.x) % kEW, y = y0 + static_cast<int>(threadIdx.x) / kEW;
-        if (x < L.w && y < L.h) {
+#pragma unroll
+        for (int k = 0; k < kEPer; ++k) {
+            const int y = y0 + static_cast<int>(threadIdx.x) / kEW + k * (kBlock / kEW);
+            if (!(x < L.w && y < L.h)) continue;
             const size_t p = static_cast<size_t>(y) * L.w + x;
-            const float wa = L.va[p], wb = L.vb[p];
             Expand e;
-            if (coarse && (!(wa > 0) || !(wb > 0)))
+            if (coarse && (!(wa[k] > 0) || !(wb[k] > 0)))
                 expand_setup(x, y, j.lv[l + 1].w, j.lv[l + 1].h, cx0, cy0, kCW, kCH, e);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                float& va = L.a[c * L.n + p];
-                va = wa > 0 ? va / wa : (coarse ? expand_one<kCW>(&cc[c][0][0], e) : 0.0f);
-                float& vb = L.b[c * L.n + p];
-                vb = wb > 0 ? vb / wb : (coarse ? expand_one<kCW>(&cc[3 + c][0][0], e) : 0.0f);
+                L.a[c * L.n + p] =
+                    wa[k] > 0 ? va[k][c] / wa[k] : (coarse ? expand_one<kCW>(&cc[c][0][0], e) : 0.0f);
+                L.b[c * L.n + p] =
+                    wb[k] > 0 ? vb[k][c] / wb[k] : (coarse ? expand_one<kCW>(&cc[3 + c][0][0], e) : 0.0f);
             }
         }
-        if (coarse) __syncthreads();
+        if (staged) __syncthreads();
     }
 }
 
 // mix_band + reconstruct (s6.cu:271-293) for level l, top-down; at level 0
 // also finish (s6.cu:294-306) and the union mask: single-source pixels keep
-// their warped colour and only overlap pixels need the band mix.
+// their warped colour and only overlap pixels need the band mix, so a tile
+// without overlap pixels stages nothing.
 __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::Total work, int l) {
     const long long total = work.get();
     __shared__ float cc[9][kCH][kCW];
@@ -464,8 +523,10 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
         fill_lut(lut);
         __syncthreads();
     }
+    int ji = 0;
+    const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        const int ji = find_job_by(span, t, [l](const S6Job& j) { return j.mb[l]; });
+        ji = next_job_by(span, t, [l](const S6Job& j) -> const long long& { return j.mb[l]; }, ji, njobs);
         const S6Job& j = dev::job_at(span, ji);
         const S6Level& L = j.lv[l];
         const unsigned local = static_cast<unsigned>(t - j.mb[l]);
@@ -473,72 +534,102 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
         const int x0 = static_cast<int>(local % tx) * kEW, y0 = static_cast<int>(local / tx) * kEH;
         const bool coarse = l < j.blend_top;
         const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 1;
-        if (coarse) {
+        const int x = x0 + static_cast<int>(threadIdx.x) % kEW;
+        // Pass 1: every pixel's inputs, loads issued before any is used; at
+        // level 0 A's source byte comes from an in-bounds position and is
+        // kept only where A is valid (B likewise).
+        float ga[kEPer][3], gb[kEPer][3], ua[kEPer], ub[kEPer];
+        unsigned f[kEPer];
+        bool any_blend = false;
+        if (l == 0) {
+            unsigned fr[kEPer], ab[kEPer][3];
+            float bv[kEPer][3];
+#pragma unroll
+            for (int k = 0; k < kEPer; ++k) {
+                const int y = y0 + static_cast<int>(threadIdx.x) / kEW + k * (kBlock / kEW);
+                const size_t p = x < L.w && y < L.h ? static_cast<size_t>(y) * L.w + x : 0;
+                fr[k] = L.flags[p];
+                const int ax = clampi(x + j.origin_x, 0, j.wa - 1), ay = clampi(y + j.origin_y, 0, j.ha - 1);
+                const std::uint8_t* src = j.a + static_cast<size_t>(ay) * j.pitch_a + 3 * ax;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    ab[k][c] = src[c];
+                    bv[k][c] = L.b[c * L.n + p];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kEPer; ++k) {
+                const int y = y0 + static_cast<int>(threadIdx.x) / kEW + k * (kBlock / kEW);
+                f[k] = x < L.w && y < L.h ? fr[k] : 0u;
+                const bool has_a = f[k] & kVA, has_b = f[k] & kVB;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    ga[k][c] = has_a ? lut[ab[k][c]] : 0.0f;
+                    gb[k][c] = has_b ? bv[k][c] : 0.0f;
+                }
+                ua[k] = (f[k] & kUA) ? 1.0f : 0.0f;
+                ub[k] = (f[k] & kUB) ? 1.0f : 0.0f;
+                any_blend |= has_a && has_b;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kEPer && l > 0; ++k) {
+            const int y = y0 + static_cast<int>(threadIdx.x) / kEW + k * (kBlock / kEW);
+            const bool inside = x < L.w && y < L.h;
+            const size_t p = inside ? static_cast<size_t>(y) * L.w + x : 0;
+            {
+                f[k] = inside ? (kVA | kVB) : 0u;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    ga[k][c] = L.ga[c * L.n + p];
+                    gb[k][c] = L.gb[c * L.n + p];
+                }
+                ua[k] = L.ua[p];
+                ub[k] = L.ub[p];
+                any_blend |= inside;
+            }
+        }
+        const bool staged = coarse && __syncthreads_or(any_blend);
+        if (staged) {
             const S6Level& C = j.lv[l + 1];
             stage_coarse<kCW>(C.ga, 3, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[0][0][0]);
             stage_coarse<kCW>(C.gb, 3, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[3][0][0]);
             stage_coarse<kCW>(C.z, 3, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[6][0][0]);
             __syncthreads();
         }
-        const int x = x0 + static_cast<int>(threadIdx.x) % kEW, y = y0 + static_cast<int>(threadIdx.x) / kEW;
-        if (x < L.w && y < L.h) {
+        // Pass 2: band mix and collapse where both images are valid.
+#pragma unroll
+        for (int k = 0; k < kEPer; ++k) {
+            const int y = y0 + static_cast<int>(threadIdx.x) / kEW + k * (kBlock / kEW);
+            if (!(x < L.w && y < L.h)) continue;
             const size_t p = static_cast<size_t>(y) * L.w + x;
-            float ga[3], gb[3], ua, ub;
-            bool blend = true;
-            unsigned f = 0;
-            if (l == 0) {
-                f = L.flags[p];
-                j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = f & (kVA | kVB) ? 1 : 0;
-                ga[0] = ga[1] = ga[2] = 0;
-                gb[0] = gb[1] = gb[2] = 0;
-                if (f & kVA) {
-                    const std::uint8_t* s = a_pixel(j, x, y);
-                    ga[0] = lut[s[0]];
-                    ga[1] = lut[s[1]];
-                    ga[2] = lut[s[2]];
-                }
-                if (f & kVB) {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) gb[c] = L.b[c * L.n + p];
-                }
-                ua = (f & kUA) ? 1.0f : 0.0f;
-                ub = (f & kUB) ? 1.0f : 0.0f;
-                blend = (f & kVA) && (f & kVB);
-            } else {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    ga[c] = L.ga[c * L.n + p];
-                    gb[c] = L.gb[c * L.n + p];
-                }
-                ua = L.ua[p];
-                ub = L.ub[p];
-            }
             float z[3];
-            if (blend) {
+            if ((f[k] & (kVA | kVB)) == (kVA | kVB)) {
                 Expand e;
                 if (coarse) expand_setup(x, y, j.lv[l + 1].w, j.lv[l + 1].h, cx0, cy0, kCW, kCH, e);
-                const float tot = ua + ub;
+                const float tot = ua[k] + ub[k];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const float la = ga[c] - (coarse ? expand_one<kCW>(&cc[c][0][0], e) : 0.0f);
-                    const float lb = gb[c] - (coarse ? expand_one<kCW>(&cc[3 + c][0][0], e) : 0.0f);
-                    const float m = tot > 0 ? fmaf(ua, la, ub * lb) / tot : 0.0f;
+                    const float la = ga[k][c] - (coarse ? expand_one<kCW>(&cc[c][0][0], e) : 0.0f);
+                    const float lb = gb[k][c] - (coarse ? expand_one<kCW>(&cc[3 + c][0][0], e) : 0.0f);
+                    const float m = tot > 0 ? fmaf(ua[k], la, ub[k] * lb) / tot : 0.0f;
                     z[c] = coarse ? m + expand_one<kCW>(&cc[6 + c][0][0], e) : m;
                 }
             } else {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) z[c] = (f & kVA) ? ga[c] : gb[c];
+                for (int c = 0; c < 3; ++c) z[c] = (f[k] & kVA) ? ga[k][c] : gb[k][c];
             }
             if (l > 0) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) L.z[c * L.n + p] = z[c];
             } else {
+                j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = f[k] & (kVA | kVB) ? 1 : 0;
                 std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) out[c] = quantize(z[c]);
             }
         }
-        if (coarse) __syncthreads();
+        if (staged) __syncthreads();
     }
 }
 
@@ -798,7 +889,7 @@ S6Totals s6_prefix(S6Job* js, int n) {
     for (int i = 0; i < n; ++i) {
         S6Job& j = js[i];
         j.warp_begin = t.warp;
-        t.warp += static_cast<long long>((j.width + kBlock - 1) / kBlock) * j.height;
+        t.warp += static_cast<long long>((j.width + kWarpSeg - 1) / kWarpSeg) * j.height;
         for (int l = 0; l < kS6MaxLevels; ++l) {
             j.rb[l] = t.pull[l];
             j.gb[l] = t.gauss[l];

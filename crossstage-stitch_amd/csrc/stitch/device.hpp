@@ -12,6 +12,16 @@ namespace stitch::dev {
 
 constexpr int kWave = 64;
 
+// Load of launch-constant data (job tables, counts, totals) through the
+// constant address space: the compiler then knows the value is the same for
+// every lane of a wave when the address is, and uses scalar loads. A load
+// through a plain (generic) pointer could reach per-lane scratch memory, so
+// everything computed from it would be treated as divergent.
+template <class T>
+__device__ __forceinline__ T ld_const(const T* p) {
+    return *(const __attribute__((address_space(4))) T*)p;
+}
+
 // A launch covers `count` jobs (images, pairs or canvases). One job may be
 // passed by value (single-image stage calls, no upload needed); batches pass a
 // device table. Each job records, per kernel, the first tile it owns in that
@@ -22,7 +32,7 @@ struct JobSpan {
     int count;
     Job single;
     const int* count_dev = nullptr;  // graph replay: job count read on the device
-    __device__ int jobs() const { return count_dev ? *count_dev : count; }
+    __device__ int jobs() const { return count_dev ? ld_const(count_dev) : count; }
 };
 
 // Work size (tiles) of one kernel launch. Direct launches pass the value;
@@ -33,7 +43,7 @@ struct Total {
     long long value;
     const long long* dev;
     __host__ __device__ Total(long long v, const long long* d = nullptr) : value(v), dev(d) {}
-    __device__ long long get() const { return dev ? *dev : value; }
+    __device__ long long get() const { return dev ? ld_const(dev) : value; }
 };
 
 // Device copies of a launch's job count and per-kernel totals (graph replay).
@@ -45,7 +55,13 @@ struct Replay {
 
 template <class Job>
 __device__ inline const Job& job_at(const JobSpan<Job>& span, int i) {
-    return span.table ? span.table[i] : span.single;
+    // Job tables and the by-value job in the kernel arguments are read-only
+    // for the whole launch: address both through the constant address space,
+    // so the (wave-uniform) job fields are fetched with scalar loads instead
+    // of one vector load per lane through a generic pointer.
+    using CJob = const __attribute__((address_space(4))) Job;
+    CJob* p = span.table ? (CJob*)span.table + i : (CJob*)&span.single;
+    return *(const Job*)p;
 }
 
 // Index of the job owning flattened tile `t` for the kernel whose prefix is
@@ -60,6 +76,16 @@ __device__ inline int find_job(const JobSpan<Job>& span, long long t, long long 
         else hi = mid - 1;
     }
     return lo;
+}
+
+// Job owning tile t when a workgroup visits its tiles in increasing order:
+// advances from the previous answer `ji` instead of searching the table
+// again (same result as find_job: the last job whose prefix is <= t).
+template <class Job>
+__device__ inline int next_job(const JobSpan<Job>& span, long long t, long long Job::*begin, int ji, int jobs) {
+    if (!span.table) return 0;
+    while (ji + 1 < jobs && ld_const(&(span.table[ji + 1].*begin)) <= t) ++ji;
+    return ji;
 }
 
 __device__ inline int lane_id() { return __lane_id(); }

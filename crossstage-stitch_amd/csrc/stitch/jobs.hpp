@@ -25,7 +25,7 @@ struct S1Job {
     int tiles_x;   // tiles of kS1TileW x kS1TileH output pixels
     long long tile_begin;
 };
-constexpr int kS1TileW = 64, kS1TileH = 4;
+constexpr int kS1TileW = 64, kS1TileH = 16;
 
 Status make_s1_job(const ImageView& in, const ImageView& out, const ScaleInfo& si, S1Job* job);
 inline long long s1_tiles(const S1Job& j) {
