@@ -184,6 +184,20 @@ __device__ __forceinline__ void warp_pixel(const S6Job& j, int x, int y, const f
 #pragma unroll
             for (int c = 0; c < 3; ++c) L.b[c * L.n + p] = rb[c];
         }
+        // Single-source pixels keep their warped colour (s6.cu:294-306):
+        // written here; the band mix writes the overlap.
+        j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = ma || mb ? 1 : 0;
+        if (!(ma && mb)) {
+            if (ma) {
+                const std::uint8_t* q = a_pixel(j, x, y);
+                ra[0] = lut[q[0]];
+                ra[1] = lut[q[1]];
+                ra[2] = lut[q[2]];
+            }
+            std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) out[c] = quantize(ma ? ra[c] : rb[c]);
+        }
     } else {
         if (ma) {
             const std::uint8_t* q = a_pixel(j, x, y);
@@ -266,26 +280,39 @@ __device__ __forceinline__ void expand_setup(int x, int y, int cw, int ch, int c
     expand_axis(y, ch, cy0, rows - 1, e.cy, e.wy);
 }
 
-// src: LDS plane with row pitch P.
+// RGB of one coarse pixel per LDS slot (row pitch P slots), so a tap is one
+// 16-byte LDS read for the three channels; each channel keeps its own sum
+// in the reference's order.
 template <int P>
-__device__ __forceinline__ float expand_one(const float* src, const Expand& e) {
-    float s = 0;
+__device__ __forceinline__ void expand_rgb(const float4* src, const Expand& e, float out[3]) {
+    float s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
     for (int j = 0; j < 3; ++j)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) s = fmaf(e.wx[k] * e.wy[j], src[e.cy[j] * P + e.cx[k]], s);
-    return 4 * s;
+        for (int k = 0; k < 3; ++k) {
+            const float w = e.wx[k] * e.wy[j];
+            const float4 v = src[e.cy[j] * P + e.cx[k]];
+            s0 = fmaf(w, v.x, s0);
+            s1 = fmaf(w, v.y, s1);
+            s2 = fmaf(w, v.z, s2);
+        }
+    out[0] = 4 * s0;
+    out[1] = 4 * s1;
+    out[2] = 4 * s2;
 }
 
-// Stages rows [cy0, cy0 + rows) x cols [cx0, cx0 + cols) of `planes` planes
-// (n floats apart) of a coarse level, clamped to its extent.
+// Stages rows [cy0, cy0 + rows) x cols [cx0, cx0 + cols) of the R, G, B
+// planes (n floats apart) of a coarse level, clamped to its extent.
 template <int P>
-__device__ __forceinline__ void stage_coarse(const float* src, int planes, std::size_t n, int cw, int ch, int cx0,
-                                             int cy0, int cols, int rows, float* dst) {
+__device__ __forceinline__ void stage_rgb(const float* src, std::size_t n, int cw, int ch, int cx0, int cy0, int cols,
+                                          int rows, float4* dst) {
+    const float* r = src;
+    const float* g = src + n;
+    const float* b = src + 2 * n;
     for (int i = threadIdx.x; i < cols * rows; i += blockDim.x) {
         const int cx = clampi(cx0 + i % cols, 0, cw - 1), cy = clampi(cy0 + i / cols, 0, ch - 1);
-        const size_t q = static_cast<size_t>(cy) * cw + cx;
-        for (int c = 0; c < planes; ++c) dst[(c * rows + i / cols) * P + i % cols] = src[c * n + q];
+        const unsigned q = static_cast<unsigned>(cy) * static_cast<unsigned>(cw) + static_cast<unsigned>(cx);
+        dst[(i / cols) * P + i % cols] = make_float4(r[q], g[q], b[q], 0.0f);
     }
 }
 
@@ -306,12 +333,12 @@ constexpr int kFCols = (kRIW - 1 + 4) / 2 + 1, kFRows = (kRIH - 1 + 4) / 2 + 1;
 // Modes: 0 pull at l >= 2, 1 Gaussian at l >= 2, 2 pull at l = 1 (from the
 // virtual level 0), 3 Gaussian at l = 1 (level 0 filled on the fly).
 template <int kMode>
-__global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, dev::Total work, int l) {
+__global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span, dev::Total work, int l) {
     const long long total = work.get();
     constexpr bool kFromZero = kMode >= 2;
     __shared__ float tile[8][kRIH][kRPitch];
     __shared__ float lut[kFromZero ? 256 : 1];
-    __shared__ float fill[kMode == 3 ? 6 : 1][kMode == 3 ? kFRows : 1][kMode == 3 ? kFCols : 1];
+    __shared__ float4 fill[kMode == 3 ? 2 : 1][kMode == 3 ? kFRows : 1][kMode == 3 ? kFCols : 1];
     if (kFromZero) {
         fill_lut(lut);
         __syncthreads();
@@ -329,6 +356,19 @@ __global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, de
         const int x0 = static_cast<int>(local % tx) * kRW, y0 = static_cast<int>(local / tx) * kRH;
         const int ix0 = 2 * x0 - 2, iy0 = 2 * y0 - 2;
         const int fx0 = (ix0 - 2) >> 1, fy0 = (iy0 - 2) >> 1;  // first staged coarse column / row (mode 3)
+        // Source planes: A, B colour and the two weights (level 0: B only).
+        const float* planes[8];
+        {
+            const float* sa = kMode == 0 ? F.a : F.ga;
+            const float* sb = kMode == 0 || kFromZero ? F.b : F.gb;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                planes[c] = kFromZero ? nullptr : sa + c * F.n;
+                planes[3 + c] = sb + c * F.n;
+            }
+            planes[6] = kFromZero ? nullptr : (kMode == 0 ? F.va : F.ua);
+            planes[7] = kFromZero ? nullptr : (kMode == 0 ? F.vb : F.ub);
+        }
         // Pass 1: every load of this thread's tile elements, issued before any
         // is used (positions are clamped into the level, so all are in
         // bounds; level-0 values of invalid pixels are discarded below).
@@ -341,7 +381,7 @@ __global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, de
             const int i = static_cast<int>(threadIdx.x) + e * kBlock;
             sxs[e] = clampi(reflect101(ix0 + i % kRIW, F.w), 0, F.w - 1);
             sys[e] = clampi(reflect101(iy0 + i / kRIW, F.h), 0, F.h - 1);
-            const size_t q = static_cast<size_t>(sys[e]) * F.w + sxs[e];
+            const unsigned q = static_cast<unsigned>(sys[e]) * static_cast<unsigned>(F.w) + static_cast<unsigned>(sxs[e]);
             if (kFromZero) {
                 fls[e] = F.flags[q];
                 const int ax = clampi(sxs[e] + j.origin_x, 0, j.wa - 1);
@@ -350,18 +390,11 @@ __global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, de
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     abytes[e][c] = src[c];
-                    raw[e][3 + c] = F.b[c * F.n + q];
+                    raw[e][3 + c] = planes[3 + c][q];
                 }
             } else {
-                const float* sa = kMode == 0 ? F.a : F.ga;
-                const float* sb = kMode == 0 ? F.b : F.gb;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    raw[e][c] = sa[c * F.n + q];
-                    raw[e][3 + c] = sb[c * F.n + q];
-                }
-                raw[e][6] = (kMode == 0 ? F.va : F.ua)[q];
-                raw[e][7] = (kMode == 0 ? F.vb : F.ub)[q];
+                for (int c = 0; c < 8; ++c) raw[e][c] = planes[c][q];
             }
         }
         // The pushed level 1 behind the tile, only for the images that miss
@@ -375,8 +408,8 @@ __global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, de
                 miss_b |= real && !(fls[e] & kVB);
             }
             const bool need_a = __syncthreads_or(miss_a), need_b = __syncthreads_or(miss_b);
-            if (need_a) stage_coarse<kFCols>(C.a, 3, C.n, C.w, C.h, fx0, fy0, kFCols, kFRows, &fill[0][0][0]);
-            if (need_b) stage_coarse<kFCols>(C.b, 3, C.n, C.w, C.h, fx0, fy0, kFCols, kFRows, &fill[3][0][0]);
+            if (need_a) stage_rgb<kFCols>(C.a, C.n, C.w, C.h, fx0, fy0, kFCols, kFRows, &fill[0][0][0]);
+            if (need_b) stage_rgb<kFCols>(C.b, C.n, C.w, C.h, fx0, fy0, kFCols, kFRows, &fill[kMode == 3 ? 1 : 0][0][0]);
             if (need_a || need_b) __syncthreads();
         }
         // Pass 2: level-0 selection and fill, then the LDS tile.
@@ -398,12 +431,8 @@ __global__ void __launch_bounds__(kBlock) s6_reduce(dev::JobSpan<S6Job> span, de
                 if (kMode == 3 && !(has_a && has_b)) {
                     Expand ex;
                     expand_setup(sxs[e], sys[e], C.w, C.h, fx0, fy0, kFCols, kFRows, ex);
-                    if (!has_a)
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) v[c] = expand_one<kFCols>(&fill[c][0][0], ex);
-                    if (!has_b)
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) v[3 + c] = expand_one<kFCols>(&fill[3 + c][0][0], ex);
+                    if (!has_a) expand_rgb<kFCols>(&fill[0][0][0], ex, &v[0]);
+                    if (!has_b) expand_rgb<kFCols>(&fill[kMode == 3 ? 1 : 0][0][0], ex, &v[3]);
                 }
                 v[6] = (f & (kMode == 2 ? kVA : kUA)) ? 1.0f : 0.0f;
                 v[7] = (f & (kMode == 2 ? kVB : kUB)) ? 1.0f : 0.0f;
@@ -454,7 +483,7 @@ static_assert(kEPer * (kBlock / kEW) == kEH, "expand tile");
 // :399-410): the coarse level is final when it is expanded.
 __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev::Total work, int l) {
     const long long total = work.get();
-    __shared__ float cc[6][kCH][kCW];
+    __shared__ float4 cc[2][kCH][kCW];
     int ji = 0;
     const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
@@ -487,8 +516,8 @@ __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev:
         const bool staged = coarse && __syncthreads_or(missing);
         if (staged) {
             const S6Level& C = j.lv[l + 1];
-            stage_coarse<kCW>(C.a, 3, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[0][0][0]);
-            stage_coarse<kCW>(C.b, 3, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[3][0][0]);
+            stage_rgb<kCW>(C.a, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[0][0][0]);
+            stage_rgb<kCW>(C.b, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[1][0][0]);
             __syncthreads();
         }
 #pragma unroll
@@ -496,15 +525,17 @@ __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev:
             const int y = y0 + static_cast<int>(threadIdx.x) / kEW + k * (kBlock / kEW);
             if (!(x < L.w && y < L.h)) continue;
             const size_t p = static_cast<size_t>(y) * L.w + x;
-            Expand e;
-            if (coarse && (!(wa[k] > 0) || !(wb[k] > 0)))
+            float ea[3] = {0, 0, 0}, eb[3] = {0, 0, 0};
+            if (coarse && (!(wa[k] > 0) || !(wb[k] > 0))) {
+                Expand e;
                 expand_setup(x, y, j.lv[l + 1].w, j.lv[l + 1].h, cx0, cy0, kCW, kCH, e);
+                if (!(wa[k] > 0)) expand_rgb<kCW>(&cc[0][0][0], e, ea);
+                if (!(wb[k] > 0)) expand_rgb<kCW>(&cc[1][0][0], e, eb);
+            }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                L.a[c * L.n + p] =
-                    wa[k] > 0 ? va[k][c] / wa[k] : (coarse ? expand_one<kCW>(&cc[c][0][0], e) : 0.0f);
-                L.b[c * L.n + p] =
-                    wb[k] > 0 ? vb[k][c] / wb[k] : (coarse ? expand_one<kCW>(&cc[3 + c][0][0], e) : 0.0f);
+                L.a[c * L.n + p] = wa[k] > 0 ? va[k][c] / wa[k] : ea[c];
+                L.b[c * L.n + p] = wb[k] > 0 ? vb[k][c] / wb[k] : eb[c];
             }
         }
         if (staged) __syncthreads();
@@ -517,7 +548,7 @@ __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev:
 // without overlap pixels stages nothing.
 __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::Total work, int l) {
     const long long total = work.get();
-    __shared__ float cc[9][kCH][kCW];
+    __shared__ float4 cc[3][kCH][kCW];
     __shared__ float lut[256];
     if (l == 0) {
         fill_lut(lut);
@@ -530,8 +561,10 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
         const S6Job& j = dev::job_at(span, ji);
         const S6Level& L = j.lv[l];
         const unsigned local = static_cast<unsigned>(t - j.mb[l]);
-        const unsigned tx = (static_cast<unsigned>(L.w) + kEW - 1) / kEW;
-        const int x0 = static_cast<int>(local % tx) * kEW, y0 = static_cast<int>(local / tx) * kEH;
+        const int bx = l == 0 ? j.ox0 : 0, by = l == 0 ? j.oy0 : 0;
+        const int bw = l == 0 ? j.ox1 - j.ox0 + 1 : L.w;
+        const unsigned tx = (static_cast<unsigned>(bw) + kEW - 1) / kEW;
+        const int x0 = bx + static_cast<int>(local % tx) * kEW, y0 = by + static_cast<int>(local / tx) * kEH;
         const bool coarse = l < j.blend_top;
         const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 1;
         const int x = x0 + static_cast<int>(threadIdx.x) % kEW;
@@ -542,34 +575,36 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
         unsigned f[kEPer];
         bool any_blend = false;
         if (l == 0) {
-            unsigned fr[kEPer], ab[kEPer][3];
-            float bv[kEPer][3];
+            // Flags first; colours only where both images are valid (the
+            // other pixels' output came from the warp).
+            unsigned ab[kEPer][3];
 #pragma unroll
             for (int k = 0; k < kEPer; ++k) {
                 const int y = y0 + static_cast<int>(threadIdx.x) / kEW + k * (kBlock / kEW);
-                const size_t p = x < L.w && y < L.h ? static_cast<size_t>(y) * L.w + x : 0;
-                fr[k] = L.flags[p];
-                const int ax = clampi(x + j.origin_x, 0, j.wa - 1), ay = clampi(y + j.origin_y, 0, j.ha - 1);
-                const std::uint8_t* src = j.a + static_cast<size_t>(ay) * j.pitch_a + 3 * ax;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    ab[k][c] = src[c];
-                    bv[k][c] = L.b[c * L.n + p];
-                }
+                f[k] = x < L.w && y < L.h ? L.flags[static_cast<unsigned>(y) * static_cast<unsigned>(L.w) + x] : 0u;
             }
 #pragma unroll
             for (int k = 0; k < kEPer; ++k) {
                 const int y = y0 + static_cast<int>(threadIdx.x) / kEW + k * (kBlock / kEW);
-                f[k] = x < L.w && y < L.h ? fr[k] : 0u;
-                const bool has_a = f[k] & kVA, has_b = f[k] & kVB;
+                ab[k][0] = ab[k][1] = ab[k][2] = 0;
+                gb[k][0] = gb[k][1] = gb[k][2] = 0;
+                if ((f[k] & (kVA | kVB)) == (kVA | kVB)) {
+                    const unsigned p = static_cast<unsigned>(y) * static_cast<unsigned>(L.w) + x;
+                    const std::uint8_t* src = a_pixel(j, x, y);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    ga[k][c] = has_a ? lut[ab[k][c]] : 0.0f;
-                    gb[k][c] = has_b ? bv[k][c] : 0.0f;
+                    for (int c = 0; c < 3; ++c) {
+                        ab[k][c] = src[c];
+                        gb[k][c] = L.b[c * L.n + p];
+                    }
                 }
+            }
+#pragma unroll
+            for (int k = 0; k < kEPer; ++k) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) ga[k][c] = lut[ab[k][c]];
                 ua[k] = (f[k] & kUA) ? 1.0f : 0.0f;
                 ub[k] = (f[k] & kUB) ? 1.0f : 0.0f;
-                any_blend |= has_a && has_b;
+                any_blend |= (f[k] & (kVA | kVB)) == (kVA | kVB);
             }
         }
 #pragma unroll
@@ -592,9 +627,9 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
         const bool staged = coarse && __syncthreads_or(any_blend);
         if (staged) {
             const S6Level& C = j.lv[l + 1];
-            stage_coarse<kCW>(C.ga, 3, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[0][0][0]);
-            stage_coarse<kCW>(C.gb, 3, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[3][0][0]);
-            stage_coarse<kCW>(C.z, 3, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[6][0][0]);
+            stage_rgb<kCW>(C.ga, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[0][0][0]);
+            stage_rgb<kCW>(C.gb, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[1][0][0]);
+            stage_rgb<kCW>(C.z, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[2][0][0]);
             __syncthreads();
         }
         // Pass 2: band mix and collapse where both images are valid.
@@ -605,25 +640,29 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
             const size_t p = static_cast<size_t>(y) * L.w + x;
             float z[3];
             if ((f[k] & (kVA | kVB)) == (kVA | kVB)) {
-                Expand e;
-                if (coarse) expand_setup(x, y, j.lv[l + 1].w, j.lv[l + 1].h, cx0, cy0, kCW, kCH, e);
+                float ea[3] = {0, 0, 0}, eb[3] = {0, 0, 0}, ez[3] = {0, 0, 0};
+                if (coarse) {
+                    Expand e;
+                    expand_setup(x, y, j.lv[l + 1].w, j.lv[l + 1].h, cx0, cy0, kCW, kCH, e);
+                    expand_rgb<kCW>(&cc[0][0][0], e, ea);
+                    expand_rgb<kCW>(&cc[1][0][0], e, eb);
+                    expand_rgb<kCW>(&cc[2][0][0], e, ez);
+                }
                 const float tot = ua[k] + ub[k];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const float la = ga[k][c] - (coarse ? expand_one<kCW>(&cc[c][0][0], e) : 0.0f);
-                    const float lb = gb[k][c] - (coarse ? expand_one<kCW>(&cc[3 + c][0][0], e) : 0.0f);
+                    const float la = ga[k][c] - (coarse ? ea[c] : 0.0f);
+                    const float lb = gb[k][c] - (coarse ? eb[c] : 0.0f);
                     const float m = tot > 0 ? fmaf(ua[k], la, ub[k] * lb) / tot : 0.0f;
-                    z[c] = coarse ? m + expand_one<kCW>(&cc[6 + c][0][0], e) : m;
+                    z[c] = coarse ? m + ez[c] : m;
                 }
             } else {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) z[c] = (f[k] & kVA) ? ga[k][c] : gb[k][c];
+                continue;  // level 0 outside the overlap: written by the warp
             }
             if (l > 0) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) L.z[c * L.n + p] = z[c];
             } else {
-                j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = f[k] & (kVA | kVB) ? 1 : 0;
                 std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) out[c] = quantize(z[c]);
@@ -837,6 +876,18 @@ Status make_s6_job(const ImageView& a, const ImageView& b, const s6::Geometry& g
     j.by0 = box[1];
     j.bx1 = box[2];
     j.by1 = box[3];
+    // Overlap box: A's canvas rectangle within B's box.
+    j.ox0 = std::max(j.bx0, -g.origin_x);
+    j.oy0 = std::max(j.by0, -g.origin_y);
+    j.ox1 = std::min(j.bx1, -g.origin_x + g.width_a - 1);
+    j.oy1 = std::min(j.by1, -g.origin_y + g.height_a - 1);
+    if (j.ox1 < j.ox0 || j.oy1 < j.oy0) {
+        j.ox0 = j.oy0 = 0;
+        j.ox1 = j.oy1 = -1;
+    } else {
+        j.ox0 -= j.ox0 % kEW;
+        j.oy0 -= j.oy0 % kEH;
+    }
     j.out = output.data;
     j.out_pitch = output.pitch_bytes;
     j.mask = mask.data;
@@ -899,13 +950,16 @@ S6Totals s6_prefix(S6Job* js, int n) {
             const long long w = j.lv[l].w, h = j.lv[l].h;
             const long long rtiles = ((w + kRW - 1) / kRW) * ((h + kRH - 1) / kRH);
             const long long etiles = ((w + kEW - 1) / kEW) * ((h + kEH - 1) / kEH);
+            // Level 0 of the band mix covers only the overlap box.
+            const long long otiles = j.ox1 < j.ox0 ? 0
+                                                   : ((j.ox1 - j.ox0 + kEW) / kEW) * ((j.oy1 - j.oy0 + kEH) / kEH);
             if (l >= 1) {
                 t.pull[l] += rtiles;
                 t.push[l] += etiles;
             }
             if (j.blend_top >= l) {
                 if (l >= 1) t.gauss[l] += rtiles;
-                t.mix[l] += etiles;
+                t.mix[l] += l == 0 ? otiles : etiles;
             }
         }
         t.top = std::max(t.top, j.top);

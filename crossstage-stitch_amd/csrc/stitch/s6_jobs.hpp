@@ -58,6 +58,7 @@ struct S6Job {
     int top, blend_top;  // -1 for feather
     S6Level lv[kS6MaxLevels];
     int bx0, by0, bx1, by1;  // canvas box outside which B is invalid (inclusive)
+    int ox0, oy0, ox1, oy1;  // box holding every overlap pixel, start aligned to the mix tile
     long long warp_begin;
     long long rb[kS6MaxLevels];  // pull (reduce into level l) tiles
     long long gb[kS6MaxLevels];  // Gaussian (reduce into level l) tiles
