@@ -103,6 +103,23 @@ def time_batched(br, specs, steps, warmup):
     return total, times, last, tm
 
 
+def time_stream(br, specs, steps, warmup):
+    """Stream of batches: each submit runs the batch's S1-S5 on the feature
+    stream while the previous batch's S6 runs on the render stream. The timed
+    region starts with nothing pending and ends once the last batch is done."""
+    import torch
+    for _ in range(warmup):
+        br.submit(specs)
+    br.finish()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        br.submit(specs)
+    last = br.finish()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, last
+
+
 def plain_runners(dev, work, outcomes):
     """One plain runner per canvas box, so every pair the batch stitched fits a
     runner (a box may hold at most 16,777,216 pixels, like the reference)."""
@@ -238,14 +255,21 @@ def main():
                    p50_ms=1e3 * float(np.percentile(times, 50)), p99_ms=1e3 * float(np.percentile(times, 99)),
                    feature_ms=tm.feature_ms, render_ms=tm.render_ms, host_ms=tm.host_ms, ok=ok)
     log("batched", batched)
+    s_total, s_last = time_stream(br, specs, args.steps, args.warmup)
+    assert [o.status for o in s_last] == [o.status for o in outcomes]
+    stream = dict(pairs_per_s=args.pairs * args.steps / s_total, ms_per_step=1e3 * s_total / args.steps,
+                  ok=sum(o.status == 0 for o in s_last))
+    log("stream", stream)
     # 1920x1080 subset on its own.
     idx = [i for i, (w, h, _, _) in enumerate(work) if (w, h) == (1920, 1080)]
     sub_specs = [specs[i] for i in idx]
     br.run(sub_specs)
     t_sub, times_sub, _, _ = time_batched(br, sub_specs, args.steps, 0)
-    c1 = dict(pairs=len(idx), batched_pairs_per_s=len(idx) * args.steps / t_sub,
+    t_ss, _ = time_stream(br, sub_specs, args.steps, 1)
+    c1 = dict(pairs=len(idx), stream_pairs_per_s=len(idx) * args.steps / t_ss,
+              batched_pairs_per_s=len(idx) * args.steps / t_sub,
               batched_p50_ms=1e3 * float(np.percentile(times_sub, 50)))
-    result = dict(batched=batched, c1_1080p=c1)
+    result = dict(stream=stream, batched=batched, c1_1080p=c1)
     if not args.no_plain:
         runners, assign = plain_runners(dev, work, outcomes)
         plain = time_plain(dev, work, runners, assign)
@@ -253,11 +277,12 @@ def main():
         plain_sub = time_plain([dev[i] for i in idx], [work[i] for i in idx], runners, [assign[i] for i in idx])
         c1["plain_pairs_per_s"] = plain_sub["pairs_per_s"]
         c1["plain_p50_ms"] = plain_sub["p50_ms"]
-        c1["speedup"] = c1["batched_pairs_per_s"] / plain_sub["pairs_per_s"]
+        c1["speedup"] = c1["stream_pairs_per_s"] / plain_sub["pairs_per_s"]
         for r in runners.values():
             r.close()
         result["plain"] = plain
-        result["speedup_vs_plain"] = batched["pairs_per_s"] / plain["pairs_per_s"]
+        result["speedup_vs_plain"] = stream["pairs_per_s"] / plain["pairs_per_s"]
+        result["batched_speedup_vs_plain"] = batched["pairs_per_s"] / plain["pairs_per_s"]
     br.close()
     if not args.no_serve:
         pool = serve_pool(args.pairs)
@@ -267,18 +292,18 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(pairs, work, args.cpu_seconds)
     line = {
         "metric": "stitched pairs/s + p50/p99 latency, batched/served vs plain per-pair, 1 GPU",
-        "value": round(batched["pairs_per_s"], 3),
+        "value": round(stream["pairs_per_s"], 3),
         "unit": "pairs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(batched["ms_per_step"], 3),
+        "ms_per_step": round(stream["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8/int32/f32/f64",
         "data": "synthetic",
-        "config": {"workload": f"batched form, {args.pairs} pairs per step: sizes from "
+        "config": {"workload": f"batched form as a stream of batches, {args.pairs} pairs per step: sizes from "
                                f"{{640x480,1280x720,1920x1080,3840x2160}}, K caps U[500,8000], overlap U[0.2,0.8], "
                                f"cross-checked Hamming, RANSAC {R}, {BLEND}", "pairs_per_step": args.pairs},
         **result,

@@ -212,9 +212,13 @@ SIGNATURES = {
     "cs_batch_create": (C.c_int, [C.c_int, C.c_uint64, C.c_int, P(vp)]),
     "cs_batch_destroy": (None, [vp]),
     "cs_batch_run": (C.c_int, [vp, P(PairSpec), C.c_int, P(PairOutcome), P(BatchTimings)]),
+    "cs_batch_submit": (C.c_int, [vp, P(PairSpec), C.c_int, P(PairOutcome), C.c_int, P(BatchTimings)]),
+    "cs_batch_finish": (C.c_int, [vp, P(PairOutcome), C.c_int]),
     "cs_batch_autotune": (C.c_int, [vp, P(PairSpec), C.c_int, C.c_char_p, P(Tuning)]),
     "cs_batch_apply": (C.c_int, [vp, P(Tuning)]),
     "cs_batch_pool": (C.c_int, [vp, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)]),
+    "cs_batch_pair_bytes": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_uint64)]),
+    "cs_batch_render_bytes": (C.c_int, [C.c_int, C.c_int, C.c_int, P(C.c_uint64)]),
     "cs_server_create": (C.c_int, [P(ServerConfig), P(vp)]),
     "cs_server_destroy": (None, [vp]),
     "cs_server_run_trace": (C.c_int, [vp, P(Request), C.c_int, P(Completion), P(ServeStats)]),

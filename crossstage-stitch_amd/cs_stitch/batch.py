@@ -31,6 +31,7 @@ class BatchRunner:
         self.handle = C.c_void_p()
         _check(F.load().cs_batch_create(max_pairs, pool_bytes, int(graphs), C.byref(self.handle)), "batch_create")
         self.max_pairs = max_pairs
+        self._pending = None  # size of the submitted, not yet completed batch
 
     def run(self, specs):
         """Runs one batch; returns (outcomes, timings). Outcome device views stay valid until the next run."""
@@ -39,7 +40,32 @@ class BatchRunner:
         out = (F.PairOutcome * max(n, 1))()
         t = F.BatchTimings()
         _check(F.load().cs_batch_run(self.handle, arr, n, out, C.byref(t)), "batch_run")
+        self._pending = None
         return list(out[:n]), t
+
+    def submit(self, specs):
+        """Stream form: enqueues the batch, whose S1-S5 overlap the previous batch's S6.
+
+        Returns (outcomes of the previous batch, or None when none was pending; timings).
+        A batch's device views stay valid until the second submit after its own.
+        """
+        n = len(specs)
+        arr = (F.PairSpec * max(n, 1))(*specs)
+        t = F.BatchTimings()
+        had = self._pending is not None
+        prev_n = self._pending or 0
+        prev = (F.PairOutcome * max(prev_n, 1))()
+        _check(F.load().cs_batch_submit(self.handle, arr, n, prev, prev_n, C.byref(t)), "batch_submit")
+        self._pending = n
+        return (list(prev[:prev_n]) if had else None), t
+
+    def finish(self):
+        """Completes the last submitted batch; returns its outcomes."""
+        n = self._pending or 0
+        out = (F.PairOutcome * max(n, 1))()
+        _check(F.load().cs_batch_finish(self.handle, out, n), "batch_finish")
+        self._pending = None
+        return list(out[:n])
 
     def autotune(self, specs, cache_path: str | None = None) -> F.Tuning:
         arr = (F.PairSpec * len(specs))(*specs)
@@ -64,6 +90,21 @@ class BatchRunner:
 
     def __del__(self):
         self.close()
+
+
+def pair_bytes(width, height, K, R) -> int:
+    """Pool bytes one pair takes for S1-S5 (width/height: max over A and B)."""
+    v = C.c_uint64()
+    _check(F.load().cs_batch_pair_bytes(width, height, K, R, C.byref(v)), "batch_pair_bytes")
+    return v.value
+
+
+def render_bytes(canvas_width, canvas_height, blend) -> int:
+    """Pool bytes one canvas with its S6 workspace takes."""
+    mode = F.BLEND_NAMES[blend] if isinstance(blend, str) else int(blend)
+    v = C.c_uint64()
+    _check(F.load().cs_batch_render_bytes(canvas_width, canvas_height, mode, C.byref(v)), "batch_render_bytes")
+    return v.value
 
 
 def request(id, a, b, arrival_ms, seed=0, fixed: F.PairSpec | None = None, keep_output=False) -> F.Request:

@@ -759,10 +759,16 @@ FeatureTask fixed_task(const PairSpec& p, PairBuffers* buffers) {
     return t;
 }
 
-void BatchRunner::initialize(int max_pairs, std::size_t pool_bytes, bool use_graphs) {
+void BatchRunner::initialize(int max_pairs, std::size_t pool_bytes, bool use_graphs, double pair_share) {
+    if (!(pair_share > 0) || !(pair_share < 1)) throw admission::Error("ERR_CAPACITY");
     max_pairs_ = max_pairs;
     tuning_.graphs = use_graphs;
-    pool_.initialize(pool_bytes);
+    const std::size_t slot_bytes = pool_bytes / 2;
+    const std::size_t pair_bytes_total = align_up(static_cast<std::size_t>(slot_bytes * pair_share));
+    for (Slot& s : slots_) {
+        s.pair_pool.initialize(pair_bytes_total);
+        s.render_pool.initialize(slot_bytes > pair_bytes_total ? slot_bytes - pair_bytes_total : kAlign);
+    }
     features_.initialize(max_pairs, use_graphs);
     render_.initialize(max_pairs, use_graphs);
 }
@@ -774,13 +780,46 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 }
 }  // namespace
 
-void BatchRunner::run(const PairSpec* pairs, int n, std::vector<PairOutcome>* out, BatchTimings* timings) {
+void BatchRunner::complete(Slot& s) {
+    if (!s.pending) return;
+    render_.wait();
+    last_render_ms_ = s.owner.empty() ? 0.0 : render_.last_ms();
+    for (std::size_t k = 0; k < s.owner.size(); ++k) {
+        PairOutcome& o = s.out[s.owner[k]];
+        o.render = render_.results()[k];
+        if (o.render.status != Status::OK) {
+            o.status = o.render.status;
+            o.fail_stage = "s6";
+        }
+    }
+    s.pending = false;
+}
+
+void BatchRunner::pool_use(std::size_t* capacity, std::size_t* in_use, std::size_t* peak) const {
+    std::size_t c = 0, u = 0, p = 0;
+    for (const Slot& s : slots_) {
+        c += s.pair_pool.capacity() + s.render_pool.capacity();
+        u += s.pair_pool.in_use() + s.render_pool.in_use();
+        p += s.pair_pool.peak() + s.render_pool.peak();
+    }
+    if (capacity) *capacity = c;
+    if (in_use) *in_use = u;
+    if (peak) *peak = p;
+}
+
+void BatchRunner::submit(const PairSpec* pairs, int n, std::vector<PairOutcome>* previous, BatchTimings* timings) {
     if (n < 0 || n > max_pairs_) throw admission::Error("ERR_CAPACITY");
     const auto t0 = Clock::now();
-    pool_.reset();
-    pair_buffers_.assign(n, PairBuffers{});
-    render_buffers_.assign(n, RenderBuffers{});
-    out->assign(n, PairOutcome{});
+    Slot& prev = slots_[cur_ ^ 1];
+    Slot& s = slots_[cur_];
+    complete(s);  // two batches back: long done
+    if (previous) previous->clear();
+    const bool had_prev = prev.pending;
+    s.pair_pool.reset();
+    s.pairs.assign(n, PairBuffers{});
+    s.renders.assign(n, RenderBuffers{});
+    s.out.assign(n, PairOutcome{});
+    s.owner.clear();
     std::vector<FeatureTask> tasks(n);
     for (int i = 0; i < n; ++i) {
         const PairSpec& p = pairs[i];
@@ -788,18 +827,23 @@ void BatchRunner::run(const PairSpec* pairs, int n, std::vector<PairOutcome>* ou
                                std::max(p.R, 1)};
         tasks[i] = fixed_task(p, nullptr);
         try {
-            if (allocate_pair(pool_, cap, &pair_buffers_[i])) tasks[i].buffers = &pair_buffers_[i];
+            if (allocate_pair(s.pair_pool, cap, &s.pairs[i])) tasks[i].buffers = &s.pairs[i];
         } catch (const admission::Error&) {
         }
-        (*out)[i].buffers = &pair_buffers_[i];
+        s.out[i].buffers = &s.pairs[i];
     }
+    // S1-S5 on the feature stream, alongside the previous batch's S6.
     features_.submit(tasks.data(), n);
     features_.wait();
     const auto t1 = Clock::now();
+    // The previous batch's render must finish before its results are read
+    // and before the render tables are rebuilt.
+    complete(prev);
+    if (previous && had_prev) *previous = prev.out;
+    s.render_pool.reset();
     std::vector<RenderTask> renders;
-    std::vector<int> owner;
     for (int i = 0; i < n; ++i) {
-        PairOutcome& o = (*out)[i];
+        PairOutcome& o = s.out[i];
         o.features = features_.results()[i];
         if (!tasks[i].buffers) {
             o.status = Status::INSUFFICIENT_WORKSPACE;
@@ -826,7 +870,7 @@ void BatchRunner::run(const PairSpec* pairs, int n, std::vector<PairOutcome>* ou
         }
         bool ok = false;
         try {
-            ok = allocate_render(pool_, o.geometry, p.blend, &render_buffers_[i]);
+            ok = allocate_render(s.render_pool, o.geometry, p.blend, &s.renders[i]);
         } catch (const admission::Error&) {
         }
         if (!ok) {
@@ -834,27 +878,38 @@ void BatchRunner::run(const PairSpec* pairs, int n, std::vector<PairOutcome>* ou
             o.fail_stage = "s6";
             continue;
         }
-        o.render_buffers = &render_buffers_[i];
-        renders.push_back({p.a, p.b, o.geometry, p.blend, &render_buffers_[i]});
-        owner.push_back(i);
+        o.render_buffers = &s.renders[i];
+        renders.push_back({p.a, p.b, o.geometry, p.blend, &s.renders[i]});
+        s.owner.push_back(i);
     }
     const auto t2 = Clock::now();
     render_.submit(renders.data(), static_cast<int>(renders.size()));
-    render_.wait();
-    for (std::size_t k = 0; k < owner.size(); ++k) {
-        PairOutcome& o = (*out)[owner[k]];
-        o.render = render_.results()[k];
-        if (o.render.status != Status::OK) {
-            o.status = o.render.status;
-            o.fail_stage = "s6";
-        }
-    }
-    const auto t3 = Clock::now();
+    s.pending = true;
+    cur_ ^= 1;
     if (timings) {
         timings->feature_ms = features_.last_ms();
-        timings->render_ms = renders.empty() ? 0.0 : render_.last_ms();
+        timings->render_ms = last_render_ms_;  // the previous batch's S6
         timings->host_ms = ms_between(t1, t2);
-        timings->total_ms = ms_between(t0, t3);
+        timings->total_ms = ms_between(t0, Clock::now());
+    }
+}
+
+void BatchRunner::finish(std::vector<PairOutcome>* out, BatchTimings* timings) {
+    Slot& last = slots_[cur_ ^ 1];
+    complete(last);
+    if (out) *out = last.out;
+    if (timings) timings->render_ms = last_render_ms_;
+}
+
+void BatchRunner::run(const PairSpec* pairs, int n, std::vector<PairOutcome>* out, BatchTimings* timings) {
+    const auto t0 = Clock::now();
+    BatchTimings t{};
+    if (pending()) finish(nullptr, nullptr);
+    submit(pairs, n, nullptr, &t);
+    finish(out, &t);
+    if (timings) {
+        *timings = t;
+        timings->total_ms = ms_between(t0, Clock::now());
     }
 }
 

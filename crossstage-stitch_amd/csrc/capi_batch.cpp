@@ -12,6 +12,7 @@ using namespace stitch;
 struct cs_batch {
     batch::BatchRunner impl;
     std::vector<batch::PairOutcome> last;
+    std::vector<cs_pair_spec> specs, previous;  // of the last two submitted batches
 };
 
 struct cs_server {
@@ -113,15 +114,11 @@ int cs_batch_create(int max_pairs, uint64_t pool_bytes, int use_graphs, cs_batch
 
 void cs_batch_destroy(cs_batch* b) { delete b; }
 
-int cs_batch_run(cs_batch* b, const cs_pair_spec* pairs, int n, cs_pair_outcome* outcomes, cs_batch_timings* t) {
-    if (!b || (n > 0 && (!pairs || !outcomes)) || n < 0) return CS_INVALID_ARGUMENT;
-    try {
-        std::vector<batch::PairSpec> specs(n);
-        for (int i = 0; i < n; ++i) specs[i] = spec(pairs[i]);
-        batch::BatchTimings tm{};
-        b->impl.run(specs.data(), n, &b->last, &tm);
+namespace {
+void put_outcomes(const std::vector<batch::PairOutcome>& last, const cs_pair_spec* pairs, int n,
+                  cs_pair_outcome* outcomes) {
         for (int i = 0; i < n; ++i) {
-            const batch::PairOutcome& o = b->last[i];
+            const batch::PairOutcome& o = last[i];
             cs_pair_outcome& r = outcomes[i];
             std::memset(&r, 0, sizeof r);
             r.status = static_cast<int>(o.status);
@@ -149,7 +146,52 @@ int cs_batch_run(cs_batch* b, const cs_pair_spec* pairs, int n, cs_pair_outcome*
                 r.inlier_mask = B.inlier_mask;
             }
         }
+}
+}  // namespace
+
+int cs_batch_run(cs_batch* b, const cs_pair_spec* pairs, int n, cs_pair_outcome* outcomes, cs_batch_timings* t) {
+    if (!b || (n > 0 && (!pairs || !outcomes)) || n < 0) return CS_INVALID_ARGUMENT;
+    try {
+        std::vector<batch::PairSpec> specs(n);
+        for (int i = 0; i < n; ++i) specs[i] = spec(pairs[i]);
+        batch::BatchTimings tm{};
+        b->impl.run(specs.data(), n, &b->last, &tm);
+        b->specs.assign(pairs, pairs + n);
+        b->previous.clear();
+        put_outcomes(b->last, pairs, n, outcomes);
         if (t) *t = {tm.feature_ms, tm.render_ms, tm.host_ms, tm.total_ms};
+        return CS_OK;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+int cs_batch_submit(cs_batch* b, const cs_pair_spec* pairs, int n, cs_pair_outcome* previous, int previous_n,
+                    cs_batch_timings* t) {
+    if (!b || (n > 0 && !pairs) || n < 0 || previous_n < 0 || (previous_n > 0 && !previous)) return CS_INVALID_ARGUMENT;
+    try {
+        const int have = b->impl.pending() ? static_cast<int>(b->specs.size()) : 0;
+        if (previous && previous_n != have) return CS_INVALID_ARGUMENT;
+        std::vector<batch::PairSpec> specs(n);
+        for (int i = 0; i < n; ++i) specs[i] = spec(pairs[i]);
+        batch::BatchTimings tm{};
+        b->impl.submit(specs.data(), n, &b->last, &tm);
+        b->previous.swap(b->specs);
+        b->specs.assign(pairs, pairs + n);
+        if (previous && have > 0) put_outcomes(b->last, b->previous.data(), have, previous);
+        if (t) *t = {tm.feature_ms, tm.render_ms, tm.host_ms, tm.total_ms};
+        return CS_OK;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+int cs_batch_finish(cs_batch* b, cs_pair_outcome* outcomes, int n) {
+    if (!b || n < 0 || (n > 0 && !outcomes)) return CS_INVALID_ARGUMENT;
+    try {
+        b->impl.finish(&b->last, nullptr);
+        if (n != static_cast<int>(b->last.size()) || n != static_cast<int>(b->specs.size())) return CS_INVALID_ARGUMENT;
+        put_outcomes(b->last, b->specs.data(), n, outcomes);
         return CS_OK;
     } catch (...) {
         return map_exception();
@@ -181,10 +223,37 @@ int cs_batch_apply(cs_batch* b, const cs_tuning* t) {
 
 int cs_batch_pool(cs_batch* b, uint64_t* capacity, uint64_t* in_use, uint64_t* peak) {
     if (!b) return CS_INVALID_ARGUMENT;
-    if (capacity) *capacity = b->impl.pool().capacity();
-    if (in_use) *in_use = b->impl.pool().in_use();
-    if (peak) *peak = b->impl.pool().peak();
+    std::size_t c = 0, u = 0, p = 0;
+    b->impl.pool_use(&c, &u, &p);
+    if (capacity) *capacity = c;
+    if (in_use) *in_use = u;
+    if (peak) *peak = p;
     return CS_OK;
+}
+
+int cs_batch_pair_bytes(int width, int height, int K, int R, uint64_t* bytes) {
+    if (!bytes || width <= 0 || height <= 0 || K <= 0 || R <= 0) return CS_INVALID_ARGUMENT;
+    try {
+        *bytes = batch::pair_bytes({width, height, K, R});
+        return CS_OK;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+int cs_batch_render_bytes(int canvas_width, int canvas_height, int mode, uint64_t* bytes) {
+    if (!bytes || canvas_width <= 0 || canvas_height <= 0) return CS_INVALID_ARGUMENT;
+    try {
+        s6::Geometry g{};
+        g.width = canvas_width;
+        g.height = canvas_height;
+        g.width_a = g.height_a = g.width_b = g.height_b = 1;  // sizes depend on the canvas only
+        g.inverse[0] = g.inverse[4] = g.inverse[8] = 1.0;
+        *bytes = batch::render_bytes(g, static_cast<s6::BlendMode>(mode));
+        return CS_OK;
+    } catch (...) {
+        return map_exception();
+    }
 }
 
 int cs_server_create(const cs_server_config* c, cs_server** out) {

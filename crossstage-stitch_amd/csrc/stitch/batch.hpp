@@ -260,22 +260,47 @@ struct Tuning {
 class BatchRunner {
 public:
     BatchRunner() = default;
-    void initialize(int max_pairs, std::size_t pool_bytes, bool use_graphs = true);
+    // The pool is split into two slots (consecutive batches alternate between
+    // them), each split between pair buffers (pair_share of it) and canvases
+    // with their pyramids, so a batch's S1-S5 never touch memory that the
+    // previous batch's S6 or its caller may still be using.
+    void initialize(int max_pairs, std::size_t pool_bytes, bool use_graphs = true, double pair_share = 0.25);
     // One batch: features for every pair, one host step (geometry), render.
     void run(const PairSpec* pairs, int n, std::vector<PairOutcome>* out, BatchTimings* timings = nullptr);
+    // Stream of batches: returns once this batch's S6 is enqueued; its S1-S5
+    // ran on the feature stream while the previous batch's S6 was still on
+    // the render stream. The previous batch completes inside the call and
+    // its outcomes go to *previous (when given). A batch's device outputs
+    // stay valid until the second submit after its own.
+    void submit(const PairSpec* pairs, int n, std::vector<PairOutcome>* previous = nullptr,
+                BatchTimings* timings = nullptr);
+    // Completes the last submitted batch and returns its outcomes.
+    void finish(std::vector<PairOutcome>* out, BatchTimings* timings = nullptr);
+    // Pending batch count (0..1 between calls).
+    bool pending() const { return slots_[cur_ ^ 1].pending; }
     // Times the tunable launch parameters on this batch and keeps the fastest
     // (results never depend on them). Reads / writes a JSON cache when a path is given.
     Tuning autotune(const PairSpec* pairs, int n, const std::string& cache_path = "");
     void apply(const Tuning& t);
     const Tuning& tuning() const { return tuning_; }
-    DevicePool& pool() { return pool_; }
+    // Summed over both slots.
+    void pool_use(std::size_t* capacity, std::size_t* in_use, std::size_t* peak) const;
 
 private:
-    DevicePool pool_;
+    struct Slot {
+        DevicePool pair_pool, render_pool;
+        std::vector<PairBuffers> pairs;
+        std::vector<RenderBuffers> renders;
+        std::vector<PairOutcome> out;
+        std::vector<int> owner;
+        bool pending{false};
+    };
+    void complete(Slot& s);
     FeatureEngine features_;
     RenderEngine render_;
-    std::vector<PairBuffers> pair_buffers_;
-    std::vector<RenderBuffers> render_buffers_;
+    Slot slots_[2];
+    int cur_{0};
+    double last_render_ms_{0};
     Tuning tuning_{};
     int max_pairs_{0};
 };

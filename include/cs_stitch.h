@@ -220,11 +220,29 @@ int cs_batch_create(int max_pairs, uint64_t pool_bytes, int use_graphs, cs_batch
 void cs_batch_destroy(cs_batch* batch);
 int cs_batch_run(cs_batch* batch, const cs_pair_spec* pairs, int n, cs_pair_outcome* outcomes,
                  cs_batch_timings* timings);
+/* Stream of batches. cs_batch_submit returns once the batch's S6 is enqueued:
+ * its S1-S5 ran on the feature stream while the previous batch's S6 was still
+ * on the render stream. The previous batch completes inside the call; its
+ * outcomes go to previous[0..previous_n) (previous_n must equal its size;
+ * previous may be NULL to drop them). cs_batch_finish completes the last
+ * submitted batch and writes its n outcomes. A batch's device outputs stay
+ * valid until the second submit after its own. timings->render_ms of a
+ * submit is the previous batch's S6. Beyond cs_batch_run's contract, no
+ * counterpart in the reference (stream form of the batched runner). */
+int cs_batch_submit(cs_batch* batch, const cs_pair_spec* pairs, int n, cs_pair_outcome* previous, int previous_n,
+                    cs_batch_timings* timings);
+int cs_batch_finish(cs_batch* batch, cs_pair_outcome* outcomes, int n);
 /* Times the launch-shape knobs on this batch, keeps the fastest and returns
  * it; cache_path (may be NULL) is a JSON file of earlier choices. */
 int cs_batch_autotune(cs_batch* batch, const cs_pair_spec* pairs, int n, const char* cache_path, cs_tuning* chosen);
 int cs_batch_apply(cs_batch* batch, const cs_tuning* tuning);
 int cs_batch_pool(cs_batch* batch, uint64_t* capacity, uint64_t* in_use, uint64_t* peak);
+/* Pool bytes one pair takes for S1-S5 (width/height: max over A and B) and
+ * one canvas with its S6 workspace: a batch slot holds pool_bytes / 2, a
+ * quarter of it for pairs and the rest for canvases. Sizing queries with no
+ * counterpart in the reference. */
+int cs_batch_pair_bytes(int width, int height, int K, int R, uint64_t* bytes);
+int cs_batch_render_bytes(int canvas_width, int canvas_height, int mode, uint64_t* bytes);
 
 /* Served form (no counterpart in the reference): a request loop replaying an
  * arrival trace with continuous batching, HBM-budget admission and the

@@ -169,13 +169,52 @@ def test_batch_rejects_what_does_not_fit():
     from cs_stitch import batch as B
     pairs = make_pairs(2, seed0=8100, sizes=[(640, 480)])
     cfgs = [dict(knobs(1), blend="multiband_7"), dict(knobs(1), blend="feather")]
-    br = B.BatchRunner(max_pairs=2, pool_bytes=20 << 20)
+    pool = 48 << 20
+    br = B.BatchRunner(max_pairs=2, pool_bytes=pool)
     specs = [B.pair_spec(ta, tb, **c) for (ta, tb), c in zip(pairs, cfgs)]
     outs, _ = br.run(specs)
+    # A slot holds half the pool: a quarter of it for pairs, the rest for canvases.
+    slot = pool // 2
+    assert 2 * B.pair_bytes(640, 480, cfgs[0]["K"], cfgs[0]["R"]) <= slot // 4
+    g0, g1 = outs[0].geometry, outs[1].geometry
+    assert B.render_bytes(g0.width, g0.height, "multiband_7") > slot - slot // 4
+    assert B.render_bytes(g1.width, g1.height, "feather") <= slot - slot // 4
     assert outs[0].status == 2 and outs[0].fail_stage == b"s6"
     assert outs[1].status == 0 and outs[1].canvas.data
     cap, used, peak = br.pool()
-    assert cap >= 20 << 20 and peak <= cap
+    assert cap >= pool - 4096 and peak <= cap
+    br.close()
+
+
+def test_batch_stream_equals_batch_by_batch():
+    """submit() overlaps a batch's S1-S5 with the previous batch's S6 on two
+    streams and hands back the previous batch's outcomes: bit for bit what
+    run() gives for each batch on its own."""
+    from cs_stitch import batch as B
+    groups = [make_pairs(4, seed0=9100 + 10 * g, sizes=SIZES[g % 2:] + SIZES[:g % 2]) for g in range(3)]
+    cfgs = [[knobs(i + g) for i in range(4)] for g in range(3)]
+    br = B.BatchRunner(max_pairs=4, pool_bytes=4 << 30)
+    ref = [run_batch(br, groups[g], cfgs[g])[0] for g in range(3)]
+    specs = [[B.pair_spec(ta, tb, **c) for (ta, tb), c in zip(groups[g], cfgs[g])] for g in range(3)]
+    got = []
+    prev, _ = br.submit(specs[0])
+    assert prev is None
+    for g in (1, 2):
+        prev, t = br.submit(specs[g])
+        assert len(prev) == 4 and t.render_ms > 0
+        got.append([(o.status, download_outcome(outcome_dict(o), c["K"])) for o, c in zip(prev, cfgs[g - 1])])
+    last = br.finish()
+    got.append([(o.status, download_outcome(outcome_dict(o), c["K"])) for o, c in zip(last, cfgs[2])])
+    for g in range(3):
+        for i in range(4):
+            assert got[g][i][0] == ref[g][i][0], (g, i)
+            same(ref[g][i][1], got[g][i][1], f"stream batch {g} pair {i}")
+    assert sum(s == 0 for grp in ref for s, _ in grp) >= 9
+    # run() after a stream completes whatever is pending and still agrees.
+    br.submit(specs[0])
+    again, _ = run_batch(br, groups[1], cfgs[1])
+    for i in range(4):
+        same(ref[1][i][1], again[i][1], f"run after submit pair {i}")
     br.close()
 
 
