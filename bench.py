@@ -7,12 +7,21 @@ Workload (value): the batched form on a ragged batch of 64 synthetic pairs
 - resolutions drawn from {640x480, 1280x720, 1920x1080, 3840x2160}, keypoint
 caps from U[500, 8000], overlap U[20 %, 80 %], cross-checked Hamming matching,
 RANSAC 4096, 6-band multiband blend. A step is that batch through S1-S6, with
-the inputs resident in HBM before the timed region. The same 64 pairs run
+the inputs resident in HBM before the timed region; `value` is the stream of
+such batches (each batch's S1-S5 overlap the previous batch's S6 on a second
+stream), and "batched" times one batch at a time. The same 64 pairs run
 through the plain per-pair runner (the reference's execute_run flow: one
 pair at a time, per-stage launches, two host synchronizations) for the ratio;
 the 1920x1080 subset is reported on its own. The served form replays Poisson
 arrival traces (adaptive legacy-v1 controller, 64 GB HBM budget, textured,
 low-texture, low-overlap and repetitive scenes) and reports p50/p99.
+
+"roofline": after the timed regions, K more batches run with direct launches
+and HIP events around every S6 kernel on the render stream; the kernel with
+the largest summed time is reported with its compulsory HBM bytes per launch
+(computed from each canvas's pixel counts) over its mean launch duration,
+against the 8 TB/s HBM peak. "traffic" is the measured HBM bytes per launch
+of that kernel from the committed rocprofv3 PMC summary, when it covers it.
 """
 from __future__ import annotations
 
@@ -33,6 +42,11 @@ SIZES = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
 R, BLEND = 4096, "multiband_6"
 SEED0 = 20261022
 POOL_BYTES = 48 << 30
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+# Measured HBM bytes per launch of the dominant kernel on this workload
+# (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration).
+TRAFFIC_PROFILE = ROOT / "profiles" / "s6_reduce_gauss_l1_mixed64_traffic.md"
+COPY_CALIBRATION_BYTES = 1 << 30
 SERVE_BUDGET = 64 << 30
 
 
@@ -40,15 +54,18 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--pairs", type=int, default=64, help="pairs per batch")
     ap.add_argument("--serve-seconds", type=float, default=1.5, help="length of each arrival trace")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-subset", action="store_true", help="skip the 1920x1080 subset (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU oracle sample")
     ap.add_argument("--tune-cache", default=None)
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes (profiling runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one plain 1 GiB device copy (the reference kernel for PMC byte counters)")
     return ap.parse_args()
 
 
@@ -118,6 +135,45 @@ def time_stream(br, specs, steps, warmup):
     last = br.finish()
     torch.cuda.synchronize()
     return time.perf_counter() - t0, last
+
+
+def probe_kernels(br, specs, steps):
+    """Per-kernel S6 timing over `steps` batches: direct launches with HIP
+    events around every kernel on the render stream, plus each launch's
+    compulsory HBM bytes computed from the canvases and their pixel counts."""
+    import torch
+    br.set_probe(True)
+    for _ in range(steps):
+        br.run(specs)
+    torch.cuda.synchronize()
+    stats = br.probe_read()
+    br.set_probe(False)
+    return stats
+
+
+def measured_traffic():
+    """Bytes per launch from the committed PMC summary, or None."""
+    import re
+    if not TRAFFIC_PROFILE.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", TRAFFIC_PROFILE.read_text(), re.M)
+    return int(m.group(1)) if m else None
+
+
+def roofline(stats):
+    dom = max(stats, key=lambda e: e["total_ms"])
+    launch_ms = dom["total_ms"] / dom["launches"]
+    per_launch = dom["algorithmic_bytes"] / dom["launches"]
+    achieved = per_launch / (launch_ms * 1e-3) / 1e9
+    name = f"{dom['kernel']} level {dom['level']}"
+    traffic = measured_traffic() if name == "s6_reduce_gauss level 1" else None
+    top = sorted(stats, key=lambda e: -e["total_ms"])[:10]
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=name,
+                launch_ms=round(launch_ms, 4), algorithmic_bytes=int(per_launch),
+                kernels=[dict(kernel=f"{e['kernel']} level {e['level']}", launches=e["launches"],
+                              ms_per_launch=round(e["total_ms"] / e["launches"], 4),
+                              gbs=round(e["algorithmic_bytes"] / (e["total_ms"] * 1e-3) / 1e9, 1)) for e in top])
 
 
 def plain_runners(dev, work, outcomes):
@@ -239,6 +295,12 @@ def main():
     from cs_stitch import _ffi
     from cs_stitch import batch as B
     _ffi.load()
+    if args.calibrate_copy:
+        src = torch.empty(COPY_CALIBRATION_BYTES, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     work = batch_workload(args.pairs)
     log("synthesizing", len(work), "pairs")
     pairs = [synth(SEED0 + i, w, h, ov) for i, (w, h, _, ov) in enumerate(work)]
@@ -260,15 +322,18 @@ def main():
     stream = dict(pairs_per_s=args.pairs * args.steps / s_total, ms_per_step=1e3 * s_total / args.steps,
                   ok=sum(o.status == 0 for o in s_last))
     log("stream", stream)
+    roof = roofline(probe_kernels(br, specs, args.steps))
+    log("roofline", {k: v for k, v in roof.items() if k != "kernels"})
     # 1920x1080 subset on its own.
     idx = [i for i, (w, h, _, _) in enumerate(work) if (w, h) == (1920, 1080)]
-    sub_specs = [specs[i] for i in idx]
-    br.run(sub_specs)
-    t_sub, times_sub, _, _ = time_batched(br, sub_specs, args.steps, 0)
-    t_ss, _ = time_stream(br, sub_specs, args.steps, 1)
-    c1 = dict(pairs=len(idx), stream_pairs_per_s=len(idx) * args.steps / t_ss,
-              batched_pairs_per_s=len(idx) * args.steps / t_sub,
-              batched_p50_ms=1e3 * float(np.percentile(times_sub, 50)))
+    c1 = dict(pairs=len(idx))
+    if not args.no_subset:
+        sub_specs = [specs[i] for i in idx]
+        br.run(sub_specs)
+        t_sub, times_sub, _, _ = time_batched(br, sub_specs, args.steps, 0)
+        t_ss, _ = time_stream(br, sub_specs, args.steps, 1)
+        c1.update(stream_pairs_per_s=len(idx) * args.steps / t_ss, batched_pairs_per_s=len(idx) * args.steps / t_sub,
+                  batched_p50_ms=1e3 * float(np.percentile(times_sub, 50)))
     result = dict(stream=stream, batched=batched, c1_1080p=c1)
     if not args.no_plain:
         runners, assign = plain_runners(dev, work, outcomes)
@@ -277,7 +342,8 @@ def main():
         plain_sub = time_plain([dev[i] for i in idx], [work[i] for i in idx], runners, [assign[i] for i in idx])
         c1["plain_pairs_per_s"] = plain_sub["pairs_per_s"]
         c1["plain_p50_ms"] = plain_sub["p50_ms"]
-        c1["speedup"] = c1["stream_pairs_per_s"] / plain_sub["pairs_per_s"]
+        if "stream_pairs_per_s" in c1:
+            c1["speedup"] = c1["stream_pairs_per_s"] / plain_sub["pairs_per_s"]
         for r in runners.values():
             r.close()
         result["plain"] = plain
@@ -307,6 +373,7 @@ def main():
                                f"{{640x480,1280x720,1920x1080,3840x2160}}, K caps U[500,8000], overlap U[0.2,0.8], "
                                f"cross-checked Hamming, RANSAC {R}, {BLEND}", "pairs_per_step": args.pairs},
         **result,
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

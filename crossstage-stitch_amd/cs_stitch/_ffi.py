@@ -124,6 +124,11 @@ class BatchTimings(C.Structure):
                 ("total_ms", C.c_double)]
 
 
+class KernelProbe(C.Structure):
+    _fields_ = [("kernel", C.c_char * 32), ("level", C.c_int), ("pad", C.c_int), ("launches", C.c_int64),
+                ("total_ms", C.c_double), ("algorithmic_bytes", C.c_double)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("s4_segment", C.c_int), ("workgroups_per_cu", C.c_int), ("graphs", C.c_int)]
 
@@ -217,6 +222,8 @@ SIGNATURES = {
     "cs_batch_autotune": (C.c_int, [vp, P(PairSpec), C.c_int, C.c_char_p, P(Tuning)]),
     "cs_batch_apply": (C.c_int, [vp, P(Tuning)]),
     "cs_batch_pool": (C.c_int, [vp, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)]),
+    "cs_batch_set_probe": (C.c_int, [vp, C.c_int]),
+    "cs_batch_probe_read": (C.c_int, [vp, P(KernelProbe), C.c_int, P(C.c_int)]),
     "cs_batch_pair_bytes": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_uint64)]),
     "cs_batch_render_bytes": (C.c_int, [C.c_int, C.c_int, C.c_int, P(C.c_uint64)]),
     "cs_server_create": (C.c_int, [P(ServerConfig), P(vp)]),

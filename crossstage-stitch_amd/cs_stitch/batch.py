@@ -60,7 +60,7 @@ class BatchRunner:
         return (list(prev[:prev_n]) if had else None), t
 
     def finish(self):
-        """Completes the last submitted batch; returns its outcomes."""
+        """Completes the last submitted batch; returns its outcomes ([] when none is pending)."""
         n = self._pending or 0
         out = (F.PairOutcome * max(n, 1))()
         _check(F.load().cs_batch_finish(self.handle, out, n), "batch_finish")
@@ -77,6 +77,19 @@ class BatchRunner:
     def apply(self, s4_segment=128, workgroups_per_cu=8, graphs=True):
         _check(F.load().cs_batch_apply(self.handle, C.byref(F.Tuning(s4_segment, workgroups_per_cu, int(graphs)))),
                "batch_apply")
+
+    def set_probe(self, on: bool):
+        """Per-kernel S6 timing: direct launches with HIP events around each kernel."""
+        _check(F.load().cs_batch_set_probe(self.handle, int(on)), "batch_set_probe")
+
+    def probe_read(self):
+        """[{kernel, level, launches, total_ms, algorithmic_bytes}] summed over the probed renders."""
+        n = C.c_int()
+        _check(F.load().cs_batch_probe_read(self.handle, None, 0, C.byref(n)), "batch_probe_read")
+        out = (F.KernelProbe * max(n.value, 1))()
+        _check(F.load().cs_batch_probe_read(self.handle, out, n.value, C.byref(n)), "batch_probe_read")
+        return [dict(kernel=e.kernel.decode(), level=e.level, launches=e.launches, total_ms=e.total_ms,
+                     algorithmic_bytes=e.algorithmic_bytes) for e in out[:n.value]]
 
     def pool(self):
         cap, use, peak = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -599,7 +599,53 @@ RenderEngine::~RenderEngine() {
     if (out_) hipHostFree(out_);
     if (ev_begin_) hipEventDestroy(ev_begin_);
     if (ev_end_) hipEventDestroy(ev_end_);
+    if (probe_) {
+        for (int i = 0; i < jobs::LaunchProbe::kMax; ++i) {
+            if (probe_->begin[i]) hipEventDestroy(probe_->begin[i]);
+            if (probe_->end[i]) hipEventDestroy(probe_->end[i]);
+        }
+        delete probe_;
+    }
     if (stream_) hipStreamDestroy(stream_);
+}
+
+void RenderEngine::set_probe(bool on) {
+    if (pending_) wait();
+    if (on && !probe_) {
+        probe_ = new jobs::LaunchProbe{};
+        for (int i = 0; i < jobs::LaunchProbe::kMax; ++i) {
+            check(hipEventCreate(&probe_->begin[i]), "probe event");
+            check(hipEventCreate(&probe_->end[i]), "probe event");
+        }
+    }
+    if (on) probe_stats_.clear();
+    probe_on_ = on;
+}
+
+void RenderEngine::read_probe() {
+    if (!probe_ || probe_->count == 0) return;
+    const Header& h = *reinterpret_cast<const Header*>(staging_);
+    const auto* js = reinterpret_cast<const jobs::S6Job*>(staging_ + off_s6_);
+    const auto* st = reinterpret_cast<const s6::RenderStatistics*>(out_);
+    for (int i = 0; i < probe_->count; ++i) {
+        float ms = 0;
+        check(hipEventElapsedTime(&ms, probe_->begin[i], probe_->end[i]), "probe elapsed");
+        double bytes = 0;
+        for (int k = 0; k < h.n_s6; ++k)
+            bytes += jobs::s6_algorithmic_bytes(probe_->kernel[i], probe_->level[i], js[k], st[k]);
+        ProbeStat* s = nullptr;
+        for (ProbeStat& e : probe_stats_)
+            if (e.kernel == probe_->kernel[i] && e.level == probe_->level[i]) s = &e;
+        if (!s) {
+            probe_stats_.push_back(
+                {jobs::probe_kernel_name(probe_->kernel[i]), probe_->kernel[i], probe_->level[i], 0, 0.0, 0.0});
+            s = &probe_stats_.back();
+        }
+        s->launches += 1;
+        s->ms += ms;
+        s->bytes += bytes;
+    }
+    probe_->count = 0;
 }
 
 void RenderEngine::initialize(int max_tasks, bool use_graph) {
@@ -708,10 +754,13 @@ void RenderEngine::submit(const RenderTask* tasks, int n) {
     check(hipMemcpyAsync(dev_, staging_, table_bytes_, hipMemcpyHostToDevice, stream_), "upload tables");
     check(hipEventRecord(ev_begin_, stream_), "event");
     if (h.n_s6 > 0) {
-        if (use_graph_) {
+        if (use_graph_ && !probe_on_) {
             check(hipGraphLaunch(graph_for(h.t6.top, h.t6.blend_top), stream_), "graph launch");
         } else {
-            check(jobs::launch_s6(at<jobs::S6Job>(dev_, off_s6_), h.n_s6, h.t6, nullptr, stream_), "render launch");
+            jobs::LaunchProbe* probe = probe_on_ ? probe_ : nullptr;
+            if (probe) probe->count = 0;
+            check(jobs::launch_s6(at<jobs::S6Job>(dev_, off_s6_), h.n_s6, h.t6, nullptr, stream_, nullptr, probe),
+                  "render launch");
             check(launch_copy_list(at<CopyOp>(dev_, off_copy_), nullptr, h.n_copy, stream_), "copy list");
         }
         check(hipMemcpyAsync(out_, dev_ + off_out_, v6.size() * sizeof(s6::RenderStatistics), hipMemcpyDeviceToHost,
@@ -738,6 +787,7 @@ void RenderEngine::wait() {
     const auto* outs = reinterpret_cast<const s6::RenderStatistics*>(out_);
     for (std::size_t i = 0; i < results_.size(); ++i)
         if (out_slot_[i] >= 0) results_[i].stats = outs[out_slot_[i]];
+    if (probe_on_) read_probe();
 }
 
 float RenderEngine::last_ms() const {

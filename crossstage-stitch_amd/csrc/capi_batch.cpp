@@ -188,6 +188,7 @@ int cs_batch_submit(cs_batch* b, const cs_pair_spec* pairs, int n, cs_pair_outco
 
 int cs_batch_finish(cs_batch* b, cs_pair_outcome* outcomes, int n) {
     if (!b || n < 0 || (n > 0 && !outcomes)) return CS_INVALID_ARGUMENT;
+    if (!b->impl.pending()) return n == 0 ? CS_OK : CS_INVALID_ARGUMENT;  // nothing submitted
     try {
         b->impl.finish(&b->last, nullptr);
         if (n != static_cast<int>(b->last.size()) || n != static_cast<int>(b->specs.size())) return CS_INVALID_ARGUMENT;
@@ -239,6 +240,32 @@ int cs_batch_pair_bytes(int width, int height, int K, int R, uint64_t* bytes) {
     } catch (...) {
         return map_exception();
     }
+}
+
+int cs_batch_set_probe(cs_batch* b, int on) {
+    if (!b) return CS_INVALID_ARGUMENT;
+    try {
+        b->impl.set_probe(on != 0);
+        return CS_OK;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+int cs_batch_probe_read(cs_batch* b, cs_kernel_probe* out, int capacity, int* count) {
+    if (!b || capacity < 0 || (capacity > 0 && !out)) return CS_INVALID_ARGUMENT;
+    const auto& st = b->impl.probe_stats();
+    if (count) *count = static_cast<int>(st.size());
+    for (int i = 0; i < capacity && i < static_cast<int>(st.size()); ++i) {
+        cs_kernel_probe& o = out[i];
+        std::memset(&o, 0, sizeof o);
+        copy_text(o.kernel, sizeof o.kernel, st[i].name);
+        o.level = st[i].level;
+        o.launches = st[i].launches;
+        o.total_ms = st[i].ms;
+        o.algorithmic_bytes = st[i].bytes;
+    }
+    return CS_OK;
 }
 
 int cs_batch_render_bytes(int canvas_width, int canvas_height, int mode, uint64_t* bytes) {

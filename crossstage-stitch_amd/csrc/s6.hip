@@ -968,32 +968,100 @@ S6Totals s6_prefix(S6Job* js, int n) {
     return t;
 }
 
+const char* probe_kernel_name(int kernel) {
+    static const char* names[] = {"s6_clear_stats", "s6_warp", "s6_reduce_pull", "s6_push", "s6_reduce_gauss", "s6_mix"};
+    return kernel >= 0 && kernel < LaunchProbe::kKernels ? names[kernel] : "?";
+}
+
+double s6_algorithmic_bytes(int kernel, int l, const S6Job& j, const s6::RenderStatistics& st) {
+    const auto n = [&](int lv) { return static_cast<double>(j.lv[lv].w) * j.lv[lv].h; };
+    const double P = static_cast<double>(j.width) * j.height;
+    const double nA = static_cast<double>(j.wa) * j.ha, SB = static_cast<double>(j.wb) * j.hb;
+    const double U = static_cast<double>(st.union_pixels), O = static_cast<double>(st.overlap_pixels);
+    const double nB = U - nA + O;
+    switch (kernel) {
+        case LaunchProbe::kClear:
+            return sizeof(s6::RenderStatistics);
+        case LaunchProbe::kWarp:
+            // A and B sources once; flags and mask (multiband) or the mask
+            // (feather); B's colour where valid; every output byte the warp owns.
+            if (j.top < 0) return 3 * nA + 3 * SB + P + 3 * P;
+            return 3 * (nA - O) + 3 * SB + 2 * P + 12 * nB + 3 * (P - O);
+        case LaunchProbe::kPull:
+        case LaunchProbe::kGauss: {
+            if (l > (kernel == LaunchProbe::kPull ? j.top : j.blend_top)) return 0;
+            const double out = 32 * n(l);  // two colours and two weights
+            if (l >= 2) return 32 * n(l - 1) + out;
+            // From level 0: flags, A's bytes, B's colour; the Gaussian also
+            // reads the pushed level-1 colours that fill missing pixels.
+            return P + 3 * nA + 12 * nB + out + (kernel == LaunchProbe::kGauss ? 24 * n(1) : 0);
+        }
+        case LaunchProbe::kPush:
+            if (l > j.top) return 0;
+            return 56 * n(l) + (l < j.top ? 24 * n(l + 1) : 0);
+        case LaunchProbe::kMix:
+            if (l > j.blend_top) return 0;
+            if (l >= 1) return 44 * n(l) + (l < j.blend_top ? 36 * n(l + 1) : 0);
+            {
+                const double box = j.ox1 >= j.ox0 && j.oy1 >= j.oy0
+                                       ? static_cast<double>(j.ox1 - j.ox0 + 1) * (j.oy1 - j.oy0 + 1) : 0.0;
+                return box + 15 * O + 3 * O + (j.blend_top > 0 ? 9 * O : 0);
+            }
+        default:
+            return 0;
+    }
+}
+
 hipError_t launch_s6(const S6Job* table, int count, const S6Totals& t, const S6Job* single, hipStream_t stream,
-                     const dev::Replay<S6Totals>* replay) {
+                     const dev::Replay<S6Totals>* replay, LaunchProbe* probe) {
     dev::JobSpan<S6Job> span{table, count, single ? *single : S6Job{}, replay ? replay->count : nullptr};
     const S6Totals* d = replay ? replay->totals : nullptr;
     const auto total = [&](const long long* host, const long long* dev) { return dev::Total(*host, d ? dev : nullptr); };
+    // Events around each launch when probed.
+    const auto timed = [&](int kernel, int level, auto&& launch) {
+        const bool on = probe && probe->count < LaunchProbe::kMax;
+        if (on) {
+            probe->kernel[probe->count] = kernel;
+            probe->level[probe->count] = level;
+            (void)hipEventRecord(probe->begin[probe->count], stream);
+        }
+        launch();
+        if (on) (void)hipEventRecord(probe->end[probe->count++], stream);
+    };
     const bool any_stats = single ? single->stats != nullptr : true;
-    if (any_stats) s6_clear_stats<<<count, 64, 0, stream>>>(span);
-    if (t.warp > 0) s6_warp<<<persistent_grid(t.warp), kBlock, 0, stream>>>(span, total(&t.warp, d ? &d->warp : nullptr));
+    if (any_stats) timed(LaunchProbe::kClear, 0, [&] { s6_clear_stats<<<count, 64, 0, stream>>>(span); });
+    if (t.warp > 0)
+        timed(LaunchProbe::kWarp, 0, [&] {
+            s6_warp<<<persistent_grid(t.warp), kBlock, 0, stream>>>(span, total(&t.warp, d ? &d->warp : nullptr));
+        });
     for (int l = 1; l <= t.top; ++l) {
         if (t.pull[l] <= 0) continue;
         const dev::Total w = total(&t.pull[l], d ? &d->pull[l] : nullptr);
-        if (l == 1) s6_reduce<2><<<persistent_grid(t.pull[l]), kBlock, 0, stream>>>(span, w, l);
-        else s6_reduce<0><<<persistent_grid(t.pull[l]), kBlock, 0, stream>>>(span, w, l);
+        timed(LaunchProbe::kPull, l, [&] {
+            if (l == 1) s6_reduce<2><<<persistent_grid(t.pull[l]), kBlock, 0, stream>>>(span, w, l);
+            else s6_reduce<0><<<persistent_grid(t.pull[l]), kBlock, 0, stream>>>(span, w, l);
+        });
     }
     for (int l = t.top; l >= 1; --l)
         if (t.push[l] > 0)
-            s6_push<<<persistent_grid(t.push[l]), kBlock, 0, stream>>>(span, total(&t.push[l], d ? &d->push[l] : nullptr), l);
+            timed(LaunchProbe::kPush, l, [&] {
+                s6_push<<<persistent_grid(t.push[l]), kBlock, 0, stream>>>(
+                    span, total(&t.push[l], d ? &d->push[l] : nullptr), l);
+            });
     for (int l = 1; l <= t.blend_top; ++l) {
         if (t.gauss[l] <= 0) continue;
         const dev::Total w = total(&t.gauss[l], d ? &d->gauss[l] : nullptr);
-        if (l == 1) s6_reduce<3><<<persistent_grid(t.gauss[l]), kBlock, 0, stream>>>(span, w, l);
-        else s6_reduce<1><<<persistent_grid(t.gauss[l]), kBlock, 0, stream>>>(span, w, l);
+        timed(LaunchProbe::kGauss, l, [&] {
+            if (l == 1) s6_reduce<3><<<persistent_grid(t.gauss[l]), kBlock, 0, stream>>>(span, w, l);
+            else s6_reduce<1><<<persistent_grid(t.gauss[l]), kBlock, 0, stream>>>(span, w, l);
+        });
     }
     for (int l = t.blend_top; l >= 0; --l)
         if (t.mix[l] > 0)
-            s6_mix<<<persistent_grid(t.mix[l]), kBlock, 0, stream>>>(span, total(&t.mix[l], d ? &d->mix[l] : nullptr), l);
+            timed(LaunchProbe::kMix, l, [&] {
+                s6_mix<<<persistent_grid(t.mix[l]), kBlock, 0, stream>>>(
+                    span, total(&t.mix[l], d ? &d->mix[l] : nullptr), l);
+            });
     return hipGetLastError();
 }
 

@@ -29,6 +29,10 @@
 
 #include "stitch/stages.hpp"
 
+namespace stitch::jobs {
+struct LaunchProbe;
+}
+
 namespace stitch::batch {
 
 // ------------------------------------------------------------ device pool
@@ -201,9 +205,24 @@ public:
     float last_ms() const;
     void set_use_graph(bool on) { use_graph_ = on; }
     void invalidate();
+    // Per-kernel timing: while on, renders launch directly (no graph) with
+    // events around every kernel; each completed render adds its launches'
+    // durations and compulsory bytes. Turning it on clears the sums.
+    struct ProbeStat {
+        const char* name;
+        int kernel, level;
+        long long launches;
+        double ms, bytes;
+    };
+    void set_probe(bool on);
+    const std::vector<ProbeStat>& probe_stats() const { return probe_stats_; }
 
 private:
     struct Header;
+    void read_probe();
+    bool probe_on_{false};
+    jobs::LaunchProbe* probe_{nullptr};  // owned; created on first use
+    std::vector<ProbeStat> probe_stats_;
     hipGraphExec_t graph_for(int top, int blend_top);
     int max_tasks_{0};
     bool use_graph_{true};
@@ -283,6 +302,8 @@ public:
     Tuning autotune(const PairSpec* pairs, int n, const std::string& cache_path = "");
     void apply(const Tuning& t);
     const Tuning& tuning() const { return tuning_; }
+    void set_probe(bool on) { render_.set_probe(on); }
+    const std::vector<RenderEngine::ProbeStat>& probe_stats() const { return render_.probe_stats(); }
     // Summed over both slots.
     void pool_use(std::size_t* capacity, std::size_t* in_use, std::size_t* peak) const;
 

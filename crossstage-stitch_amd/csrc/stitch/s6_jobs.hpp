@@ -80,7 +80,25 @@ Status make_s6_job(const ImageView& a, const ImageView& b, const s6::Geometry& g
                    const ImageView& output, const ImageView& mask, void* ws, std::size_t available,
                    s6::RenderStatistics* stats, S6Job* job);
 S6Totals s6_prefix(S6Job* jobs, int n);
+// Events around every kernel launch of a render (direct launches only), for
+// per-kernel timing; kernel[i] / level[i] name launch i.
+struct LaunchProbe {
+    enum Kernel { kClear, kWarp, kPull, kPush, kGauss, kMix, kKernels };
+    static constexpr int kMax = 4 * kS6MaxLevels + 4;
+    hipEvent_t begin[kMax]{}, end[kMax]{};
+    int kernel[kMax]{}, level[kMax]{};
+    int count{0};
+};
+const char* probe_kernel_name(int kernel);
+
+// Compulsory HBM bytes of one job's share of a launch of `kernel` at `level`
+// (stats: the job's union / overlap pixel counts after the render). Each
+// input the kernel needs is counted once and each output written once:
+// level-0 flags 1 B/px, A's source bytes 3 B/px, B's warped colour 12 B/px
+// where valid, FP32 planes 4 B per pixel and channel.
+double s6_algorithmic_bytes(int kernel, int level, const S6Job& job, const s6::RenderStatistics& stats);
+
 hipError_t launch_s6(const S6Job* table, int count, const S6Totals& totals, const S6Job* single,
-                     hipStream_t stream, const dev::Replay<S6Totals>* replay = nullptr);
+                     hipStream_t stream, const dev::Replay<S6Totals>* replay = nullptr, LaunchProbe* probe = nullptr);
 
 }  // namespace stitch::jobs

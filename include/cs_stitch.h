@@ -214,6 +214,16 @@ typedef struct {
     uint8_t* inlier_mask;
 } cs_pair_outcome;
 typedef struct { double feature_ms, render_ms, host_ms, total_ms; } cs_batch_timings;
+/* Per-kernel sums over the renders completed while probing: launches,
+ * summed event-timed duration, summed compulsory HBM bytes. */
+typedef struct {
+    char kernel[32];
+    int level;
+    int pad;
+    int64_t launches;
+    double total_ms;
+    double algorithmic_bytes;
+} cs_kernel_probe;
 typedef struct { int s4_segment, workgroups_per_cu, graphs; } cs_tuning;
 
 int cs_batch_create(int max_pairs, uint64_t pool_bytes, int use_graphs, cs_batch** out);
@@ -225,7 +235,8 @@ int cs_batch_run(cs_batch* batch, const cs_pair_spec* pairs, int n, cs_pair_outc
  * on the render stream. The previous batch completes inside the call; its
  * outcomes go to previous[0..previous_n) (previous_n must equal its size;
  * previous may be NULL to drop them). cs_batch_finish completes the last
- * submitted batch and writes its n outcomes. A batch's device outputs stay
+ * submitted batch and writes its n outcomes (n = 0 and nothing written when
+ * no batch is pending). A batch's device outputs stay
  * valid until the second submit after its own. timings->render_ms of a
  * submit is the previous batch's S6. Beyond cs_batch_run's contract, no
  * counterpart in the reference (stream form of the batched runner). */
@@ -242,6 +253,12 @@ int cs_batch_pool(cs_batch* batch, uint64_t* capacity, uint64_t* in_use, uint64_
  * quarter of it for pairs and the rest for canvases. Sizing queries with no
  * counterpart in the reference. */
 int cs_batch_pair_bytes(int width, int height, int K, int R, uint64_t* bytes);
+/* Per-kernel timing of the S6 launches (measurement; no counterpart in the
+ * reference): while on, renders launch directly with HIP events around every
+ * kernel on the render stream; enabling clears the sums. probe_read writes up
+ * to capacity entries and the number available to *count. */
+int cs_batch_set_probe(cs_batch* batch, int on);
+int cs_batch_probe_read(cs_batch* batch, cs_kernel_probe* out, int capacity, int* count);
 int cs_batch_render_bytes(int canvas_width, int canvas_height, int mode, uint64_t* bytes);
 
 /* Served form (no counterpart in the reference): a request loop replaying an

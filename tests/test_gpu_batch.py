@@ -218,6 +218,37 @@ def test_batch_stream_equals_batch_by_batch():
     br.close()
 
 
+def test_probe_times_every_s6_kernel_without_changing_results():
+    """The per-kernel probe (direct launches, events around each kernel)
+    leaves every output bit unchanged and accounts for each launch: the warp
+    once per render, one pull and one push per pyramid level, the level-0 mix;
+    compulsory bytes follow the canvases' pixel counts."""
+    from cs_stitch import batch as B
+    pairs = make_pairs(4, seed0=9500)
+    cfgs = [dict(knobs(i), blend=b) for i, b in enumerate(["multiband_6", "feather", "multiband_3", "multiband_6"])]
+    br = B.BatchRunner(max_pairs=4, pool_bytes=4 << 30)
+    ref, _ = run_batch(br, pairs, cfgs)
+    br.set_probe(True)
+    got, _ = run_batch(br, pairs, cfgs)
+    got2, _ = run_batch(br, pairs, cfgs)
+    stats = {(e["kernel"], e["level"]): e for e in br.probe_read()}
+    br.set_probe(False)
+    for i in range(4):
+        same(ref[i][1], got[i][1], f"probed pair {i}")
+        same(ref[i][1], got2[i][1], f"probed twice pair {i}")
+    assert stats[("s6_warp", 0)]["launches"] == 2
+    assert stats[("s6_reduce_gauss", 1)]["launches"] == 2
+    assert stats[("s6_mix", 0)]["launches"] == 2
+    for e in stats.values():
+        assert e["total_ms"] > 0 and e["algorithmic_bytes"] > 0, e
+    # The feather canvas alone makes the warp read A and B (3 B per pixel each)
+    # and write the panorama and mask (4 B per canvas pixel).
+    outs, _ = br.run([B.pair_spec(ta, tb, **c) for (ta, tb), c in zip(pairs, cfgs)])
+    g = outs[1].geometry
+    assert stats[("s6_warp", 0)]["algorithmic_bytes"] / 2 > 3 * 2 * 1280 * 720 + 4 * g.width * g.height
+    br.close()
+
+
 def serve_requests(pairs, arrivals, seeds, keep):
     from cs_stitch import batch as B
     return [B.request(i, ta, tb, t, seed=s, keep_output=k)
