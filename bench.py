@@ -151,13 +151,19 @@ def probe_kernels(br, specs, steps):
     return stats
 
 
-def measured_traffic():
-    """Bytes per launch from the committed PMC summary, or None."""
+def measured_traffic(kernel, algorithmic_bytes):
+    """Bytes per launch from the committed PMC summary, or None. The summary
+    names the kernel and the compulsory bytes per launch it was measured at;
+    a different kernel version or workload gives other bytes, and then the
+    measurement does not apply."""
     import re
     if not TRAFFIC_PROFILE.exists():
         return None
-    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", TRAFFIC_PROFILE.read_text(), re.M)
-    return int(m.group(1)) if m else None
+    text = TRAFFIC_PROFILE.read_text()
+    fields = dict(re.findall(r"^(\w+):\s*(.+?)\s*$", text, re.M))
+    if fields.get("kernel") != kernel or fields.get("algorithmic_bytes_per_launch") != str(int(algorithmic_bytes)):
+        return None
+    return int(fields["traffic_bytes_per_launch"])
 
 
 def roofline(stats):
@@ -166,7 +172,7 @@ def roofline(stats):
     per_launch = dom["algorithmic_bytes"] / dom["launches"]
     achieved = per_launch / (launch_ms * 1e-3) / 1e9
     name = f"{dom['kernel']} level {dom['level']}"
-    traffic = measured_traffic() if name == "s6_reduce_gauss level 1" else None
+    traffic = measured_traffic(name, per_launch)
     top = sorted(stats, key=lambda e: -e["total_ms"])[:10]
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=name,
@@ -315,7 +321,8 @@ def main():
     ok = sum(o.status == 0 for o in outcomes)
     batched = dict(pairs_per_s=args.pairs * args.steps / total, ms_per_step=1e3 * total / args.steps,
                    p50_ms=1e3 * float(np.percentile(times, 50)), p99_ms=1e3 * float(np.percentile(times, 99)),
-                   feature_ms=tm.feature_ms, render_ms=tm.render_ms, host_ms=tm.host_ms, ok=ok)
+                   feature_ms=tm.feature_ms, render_ms=tm.render_ms, host_ms=tm.host_ms, ok=ok,
+                   steps_ms=[round(1e3 * t, 3) for t in times])
     log("batched", batched)
     s_total, s_last = time_stream(br, specs, args.steps, args.warmup)
     assert [o.status for o in s_last] == [o.status for o in outcomes]

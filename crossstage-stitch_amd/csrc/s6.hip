@@ -9,11 +9,12 @@
 //
 // MI355X layout: the full-resolution level is the expensive one, so it is
 // never materialised as A colours (A is read from its source bytes through
-// a 256-entry table) and never filled in memory: the Gaussian reduce that
-// builds level 1 fills missing level-0 pixels on the fly from the pushed
-// level 1 staged in LDS. Level 0 keeps one flag byte per pixel and B's warped
-// colour; the final band mix writes the panorama and the union mask. Levels
-// >= 1 are FP32 planes. One launch per level covers every canvas of a batch;
+// a 256-entry table) and never filled in memory: the Gaussian into level 1
+// is the pull's raw sums plus the filter of the fill, which is a stencil of
+// the pushed level 1 (s6_gauss1). Level 0 keeps one flag byte per pixel and
+// B's warped colour; the final band mix writes the panorama and the union
+// mask. Levels >= 1 are FP32 planes. One launch per level covers every
+// canvas of a batch;
 // normalisation is fused into the push pass and band mixing into the
 // collapse. Expansions visit only the taps whose reflected fine position has
 // the pixel's parity (reflect-101 keeps parity), 9 instead of 25.
@@ -327,18 +328,16 @@ __device__ __forceinline__ void stage_rgb(const float* src, std::size_t n, int c
 constexpr int kRW = 32, kRH = 8, kRIW = 2 * kRW + 3, kRIH = 2 * kRH + 3;
 constexpr int kREven = (kRIW + 1) / 2, kRPitch = 80;
 static_assert(kRIW <= kRPitch && (2 * kRPitch) % 64 == 32, "reduce tile pitch");
-// Pushed level 1 behind a tile of level 0 (Gaussian of level 1, mode 3).
-constexpr int kFCols = (kRIW - 1 + 4) / 2 + 1, kFRows = (kRIH - 1 + 4) / 2 + 1;
 
 // Modes: 0 pull at l >= 2, 1 Gaussian at l >= 2, 2 pull at l = 1 (from the
-// virtual level 0), 3 Gaussian at l = 1 (level 0 filled on the fly).
+// virtual level 0). The Gaussian at l = 1 is s6_gauss1.
 template <int kMode>
 __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span, dev::Total work, int l) {
+    static_assert(kMode >= 0 && kMode <= 2, "reduce mode");
     const long long total = work.get();
-    constexpr bool kFromZero = kMode >= 2;
+    constexpr bool kFromZero = kMode == 2;
     __shared__ float tile[8][kRIH][kRPitch];
     __shared__ float lut[kFromZero ? 256 : 1];
-    __shared__ float4 fill[kMode == 3 ? 2 : 1][kMode == 3 ? kFRows : 1][kMode == 3 ? kFCols : 1];
     if (kFromZero) {
         fill_lut(lut);
         __syncthreads();
@@ -346,21 +345,22 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
     int ji = 0;
     const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        ji = (kMode == 0 || kMode == 2) ? next_job_by(span, t, [l](const S6Job& j) -> const long long& { return j.rb[l]; }, ji, njobs)
-                                        : next_job_by(span, t, [l](const S6Job& j) -> const long long& { return j.gb[l]; }, ji, njobs);
+        ji = kMode != 1 ? next_job_by(span, t, [l](const S6Job& j) -> const long long& { return j.rb[l]; }, ji, njobs)
+                        : next_job_by(span, t, [l](const S6Job& j) -> const long long& { return j.gb[l]; }, ji, njobs);
         const S6Job& j = dev::job_at(span, ji);
         const S6Level& C = j.lv[l];
         const S6Level& F = j.lv[l - 1];
-        const unsigned local = static_cast<unsigned>(t - ((kMode == 0 || kMode == 2) ? j.rb[l] : j.gb[l]));
+        const unsigned local = static_cast<unsigned>(t - (kMode != 1 ? j.rb[l] : j.gb[l]));
         const unsigned tx = (static_cast<unsigned>(C.w) + kRW - 1) / kRW;
         const int x0 = static_cast<int>(local % tx) * kRW, y0 = static_cast<int>(local / tx) * kRH;
         const int ix0 = 2 * x0 - 2, iy0 = 2 * y0 - 2;
-        const int fx0 = (ix0 - 2) >> 1, fy0 = (iy0 - 2) >> 1;  // first staged coarse column / row (mode 3)
         // Source planes: A, B colour and the two weights (level 0: B only).
         const float* planes[8];
         {
-            const float* sa = kMode == 0 ? F.a : F.ga;
-            const float* sb = kMode == 0 || kFromZero ? F.b : F.gb;
+            // Pull reads the finer level's raw sums, which level 1 keeps in
+            // ga / gb (they alias a / b above level 1).
+            const float* sa = F.ga;
+            const float* sb = kFromZero ? F.b : F.gb;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 planes[c] = kFromZero ? nullptr : sa + c * F.n;
@@ -397,22 +397,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 for (int c = 0; c < 8; ++c) raw[e][c] = planes[c][q];
             }
         }
-        // The pushed level 1 behind the tile, only for the images that miss
-        // pixels in it (mode 3).
-        if (kMode == 3) {
-            bool miss_a = false, miss_b = false;
-#pragma unroll
-            for (int e = 0; e < kPer; ++e) {
-                const bool real = static_cast<int>(threadIdx.x) + e * kBlock < kElems;
-                miss_a |= real && !(fls[e] & kVA);
-                miss_b |= real && !(fls[e] & kVB);
-            }
-            const bool need_a = __syncthreads_or(miss_a), need_b = __syncthreads_or(miss_b);
-            if (need_a) stage_rgb<kFCols>(C.a, C.n, C.w, C.h, fx0, fy0, kFCols, kFRows, &fill[0][0][0]);
-            if (need_b) stage_rgb<kFCols>(C.b, C.n, C.w, C.h, fx0, fy0, kFCols, kFRows, &fill[kMode == 3 ? 1 : 0][0][0]);
-            if (need_a || need_b) __syncthreads();
-        }
-        // Pass 2: level-0 selection and fill, then the LDS tile.
+        // Pass 2: level-0 selection, then the LDS tile.
 #pragma unroll
         for (int e = 0; e < kPer; ++e) {
             const int i = static_cast<int>(threadIdx.x) + e * kBlock;
@@ -428,14 +413,8 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                     v[c] = has_a ? lut[abytes[e][c]] : 0.0f;
                     v[3 + c] = has_b ? raw[e][3 + c] : 0.0f;
                 }
-                if (kMode == 3 && !(has_a && has_b)) {
-                    Expand ex;
-                    expand_setup(sxs[e], sys[e], C.w, C.h, fx0, fy0, kFCols, kFRows, ex);
-                    if (!has_a) expand_rgb<kFCols>(&fill[0][0][0], ex, &v[0]);
-                    if (!has_b) expand_rgb<kFCols>(&fill[kMode == 3 ? 1 : 0][0][0], ex, &v[3]);
-                }
-                v[6] = (f & (kMode == 2 ? kVA : kUA)) ? 1.0f : 0.0f;
-                v[7] = (f & (kMode == 2 ? kVB : kUB)) ? 1.0f : 0.0f;
+                v[6] = has_a ? 1.0f : 0.0f;
+                v[7] = has_b ? 1.0f : 0.0f;
             } else {
 #pragma unroll
                 for (int c = 0; c < 8; ++c) v[c] = raw[e][c];
@@ -458,15 +437,175 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                         acc[c] = fmaf(w, tile[c][2 * ly + dy][(dx & 1) * kREven + lx + (dx >> 1)], acc[c]);
                 }
             const size_t p = static_cast<size_t>(y) * C.w + x;
-            float* da = (kMode == 0 || kMode == 2) ? C.a : C.ga;
-            float* db = (kMode == 0 || kMode == 2) ? C.b : C.gb;
+            float* da = kMode == 0 ? C.a : C.ga;  // pull into level 1: raw sums kept in ga / gb
+            float* db = kMode == 0 ? C.b : C.gb;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 da[c * C.n + p] = acc[c];
                 db[c * C.n + p] = acc[3 + c];
             }
-            ((kMode == 0 || kMode == 2) ? C.va : C.ua)[p] = acc[6];
-            ((kMode == 0 || kMode == 2) ? C.vb : C.ub)[p] = acc[7];
+            (kMode != 1 ? C.va : C.ua)[p] = acc[6];
+            (kMode != 1 ? C.vb : C.ub)[p] = acc[7];
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------ Gaussian into level 1
+// The Gaussian level 1 is the filter of the filled level 0, F = m V + (1 - m) E
+// with m the validity, V the warped colour and E the expansion of the pushed
+// level 1 (s6.cu:236-270). It is linear in F, so
+//   G = sum w w m V  +  sum w w (1 - m) E.
+// The first sum is exactly the pull into level 1, whose raw sums sit in
+// ga / gb; the second vanishes where the 5x5 footprint is all valid, is a
+// fixed separable stencil of the pushed level 1 where it is all missing, and
+// is summed tap by tap where the footprint straddles the image's border. So
+// no level-0 colour is read here. The sum is the reference's up to float
+// rounding order. Ownership weights are integer sums of the flag bits, exact
+// (every partial sum of tap products is a multiple of 1/256).
+constexpr int kGW = 32, kGH = 8;                         // level-1 outputs per tile
+constexpr int kGFW = 2 * kGW + 3, kGFH = 2 * kGH + 3;    // level-0 footprint region
+constexpr int kGPW = kGW + 4, kGPH = kGH + 4;            // staged pushed level 1 (2 on each side)
+constexpr int kGElems = kGFW * kGFH, kGPer = (kGElems + kBlock - 1) / kBlock;
+static_assert(kGW == kRW && kGH == kRH && kGW * kGH == kBlock, "level-1 Gaussian tiles are the reduce tiles");
+
+__device__ __forceinline__ int itap(int d) { return d == 0 ? 6 : (d == 1 || d == -1 ? 4 : 1); }
+
+// Coefficients of the coarse indices i-2..i+2 in sum_dx w(dx) E(X(dx)),
+// E the 1D expansion (factor 2 per axis) at the reflected fine position X.
+// Every coefficient is a multiple of 1/128, so the float sums are exact.
+__device__ __forceinline__ void stencil_axis(int i, int n0, int n1, float (&c)[5]) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) c[k] = 0.0f;
+#pragma unroll
+    for (int dx = -2; dx <= 2; ++dx) {
+        const int X = clampi(reflect101(2 * i + dx, n0), 0, n0 - 1);
+        const bool odd = X & 1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (odd && k == 2) continue;
+            const int d = odd ? (k == 0 ? -1 : 1) : 2 * k - 2;
+            const int slot = clampi((reflect101(X + d, 2 * n1) >> 1) - i + 2, 0, 4);
+            c[slot] += tap(dx) * 2.0f * tap(d);
+        }
+    }
+}
+
+// sum over the footprint of w (1 - m) E for one image: stencil when the
+// footprint misses the image everywhere, tap by tap otherwise.
+__device__ __forceinline__ void gauss1_fill(const float4* pp, bool none, unsigned bit, const std::uint8_t (*fl)[kGFW + 1],
+                                            int lx, int ly, int x, int y, int x0, int y0, const S6Level& F,
+                                            const S6Level& C, float out[3]) {
+    float s0 = 0, s1 = 0, s2 = 0;
+    if (none) {
+        float cx[5], cy[5];
+        stencil_axis(x, F.w, C.w, cx);
+        stencil_axis(y, F.h, C.h, cy);
+#pragma unroll
+        for (int b = 0; b < 5; ++b) {
+            float r0 = 0, r1 = 0, r2 = 0;
+#pragma unroll
+            for (int a = 0; a < 5; ++a) {
+                const float4 v = pp[(ly + b) * kGPW + lx + a];
+                r0 = fmaf(cx[a], v.x, r0);
+                r1 = fmaf(cx[a], v.y, r1);
+                r2 = fmaf(cx[a], v.z, r2);
+            }
+            s0 = fmaf(cy[b], r0, s0);
+            s1 = fmaf(cy[b], r1, s1);
+            s2 = fmaf(cy[b], r2, s2);
+        }
+    } else {
+        for (int dy = 0; dy < 5; ++dy)
+            for (int dx = 0; dx < 5; ++dx) {
+                if (fl[2 * ly + dy][2 * lx + dx] & bit) continue;
+                const int X = clampi(reflect101(2 * x + dx - 2, F.w), 0, F.w - 1);
+                const int Y = clampi(reflect101(2 * y + dy - 2, F.h), 0, F.h - 1);
+                Expand e;
+                expand_setup(X, Y, C.w, C.h, x0 - 2, y0 - 2, kGPW, kGPH, e);
+                float E[3];
+                expand_rgb<kGPW>(pp, e, E);
+                const float w = tap(dx - 2) * tap(dy - 2);
+                s0 = fmaf(w, E[0], s0);
+                s1 = fmaf(w, E[1], s1);
+                s2 = fmaf(w, E[2], s2);
+            }
+    }
+    out[0] = s0;
+    out[1] = s1;
+    out[2] = s2;
+}
+
+__global__ void __launch_bounds__(kBlock) s6_gauss1(dev::JobSpan<S6Job> span, dev::Total work) {
+    const long long total = work.get();
+    __shared__ std::uint8_t fl[kGFH][kGFW + 1];
+    __shared__ float4 pp[2][kGPH][kGPW];
+    int ji = 0;
+    const int njobs = span.jobs();
+    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+        ji = next_job_by(span, t, [](const S6Job& j) -> const long long& { return j.gb[1]; }, ji, njobs);
+        const S6Job& j = dev::job_at(span, ji);
+        const S6Level& C = j.lv[1];
+        const S6Level& F = j.lv[0];
+        const unsigned local = static_cast<unsigned>(t - j.gb[1]);
+        const unsigned tx = (static_cast<unsigned>(C.w) + kGW - 1) / kGW;
+        const int x0 = static_cast<int>(local % tx) * kGW, y0 = static_cast<int>(local / tx) * kGH;
+        const int lx = threadIdx.x % kGW, ly = threadIdx.x / kGW;
+        const int x = x0 + lx, y = y0 + ly;
+        const bool inside = x < C.w && y < C.h;
+        const size_t p = inside ? static_cast<size_t>(y) * C.w + x : 0;
+        // The raw pull sums, issued first.
+        float ra[3], rb[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            ra[c] = C.ga[c * C.n + p];
+            rb[c] = C.gb[c * C.n + p];
+        }
+        // Level-0 flags of the tile's footprint (reflect-101 applied here).
+        unsigned fls[kGPer];
+#pragma unroll
+        for (int e = 0; e < kGPer; ++e) {
+            const int i = static_cast<int>(threadIdx.x) + e * kBlock;
+            const int sx = clampi(reflect101(2 * x0 - 2 + i % kGFW, F.w), 0, F.w - 1);
+            const int sy = clampi(reflect101(2 * y0 - 2 + i / kGFW, F.h), 0, F.h - 1);
+            fls[e] = i < kGElems ? F.flags[static_cast<unsigned>(sy) * static_cast<unsigned>(F.w) + sx] : (kVA | kVB);
+        }
+        bool miss_a = false, miss_b = false;
+#pragma unroll
+        for (int e = 0; e < kGPer; ++e) {
+            const int i = static_cast<int>(threadIdx.x) + e * kBlock;
+            if (i < kGElems) fl[i / kGFW][i % kGFW] = static_cast<std::uint8_t>(fls[e]);
+            miss_a |= !(fls[e] & kVA);
+            miss_b |= !(fls[e] & kVB);
+        }
+        const bool need_a = __syncthreads_or(miss_a), need_b = __syncthreads_or(miss_b);
+        if (need_a) stage_rgb<kGPW>(C.a, C.n, C.w, C.h, x0 - 2, y0 - 2, kGPW, kGPH, &pp[0][0][0]);
+        if (need_b) stage_rgb<kGPW>(C.b, C.n, C.w, C.h, x0 - 2, y0 - 2, kGPW, kGPH, &pp[1][0][0]);
+        if (need_a || need_b) __syncthreads();
+        if (inside) {
+            int ua = 0, ub = 0;
+            unsigned all = kVA | kVB, any = 0;
+#pragma unroll
+            for (int dy = 0; dy < 5; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 5; ++dx) {
+                    const unsigned f = fl[2 * ly + dy][2 * lx + dx];
+                    const int w = itap(dx - 2) * itap(dy - 2);
+                    ua += (f & kUA) ? w : 0;
+                    ub += (f & kUB) ? w : 0;
+                    all &= f;
+                    any |= f;
+                }
+            float fa[3] = {0, 0, 0}, fb[3] = {0, 0, 0};
+            if (!(all & kVA)) gauss1_fill(&pp[0][0][0], !(any & kVA), kVA, fl, lx, ly, x, y, x0, y0, F, C, fa);
+            if (!(all & kVB)) gauss1_fill(&pp[1][0][0], !(any & kVB), kVB, fl, lx, ly, x, y, x0, y0, F, C, fb);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                C.ga[c * C.n + p] = ra[c] + fa[c];
+                C.gb[c * C.n + p] = rb[c] + fb[c];
+            }
+            C.ua[p] = static_cast<float>(ua) * (1.0f / 256.0f);
+            C.ub[p] = static_cast<float>(ub) * (1.0f / 256.0f);
         }
         __syncthreads();
     }
@@ -507,8 +646,8 @@ __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev:
             wb[k] = inside ? L.vb[p] : 1.0f;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                va[k][c] = L.a[c * L.n + p];
-                vb[k][c] = L.b[c * L.n + p];
+                va[k][c] = L.ga[c * L.n + p];  // raw pull sums (the same planes as a / b above level 1)
+                vb[k][c] = L.gb[c * L.n + p];
             }
             missing |= !(wa[k] > 0) || !(wb[k] > 0);
         }
@@ -992,9 +1131,9 @@ double s6_algorithmic_bytes(int kernel, int l, const S6Job& j, const s6::RenderS
             if (l > (kernel == LaunchProbe::kPull ? j.top : j.blend_top)) return 0;
             const double out = 32 * n(l);  // two colours and two weights
             if (l >= 2) return 32 * n(l - 1) + out;
-            // From level 0: flags, A's bytes, B's colour; the Gaussian also
-            // reads the pushed level-1 colours that fill missing pixels.
-            return P + 3 * nA + 12 * nB + out + (kernel == LaunchProbe::kGauss ? 24 * n(1) : 0);
+            // Pull from level 0: flags, A's bytes, B's colour. Gaussian into
+            // level 1: flags, the raw pull sums and the pushed level 1.
+            return kernel == LaunchProbe::kPull ? P + 3 * nA + 12 * nB + out : P + 48 * n(1) + out;
         }
         case LaunchProbe::kPush:
             if (l > j.top) return 0;
@@ -1052,7 +1191,7 @@ hipError_t launch_s6(const S6Job* table, int count, const S6Totals& t, const S6J
         if (t.gauss[l] <= 0) continue;
         const dev::Total w = total(&t.gauss[l], d ? &d->gauss[l] : nullptr);
         timed(LaunchProbe::kGauss, l, [&] {
-            if (l == 1) s6_reduce<3><<<persistent_grid(t.gauss[l]), kBlock, 0, stream>>>(span, w, l);
+            if (l == 1) s6_gauss1<<<persistent_grid(t.gauss[l]), kBlock, 0, stream>>>(span, w);
             else s6_reduce<1><<<persistent_grid(t.gauss[l]), kBlock, 0, stream>>>(span, w, l);
         });
     }

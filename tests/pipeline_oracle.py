@@ -24,3 +24,45 @@ def run(a, b, S, K, R, blend, cross_check, fast=20, ratio=0.8, refinements=1, th
         pano, pmask, stats = O.s6_render(a, b, geo, O.MODES[blend] if isinstance(blend, str) else blend)
         out.update(panorama=pano, pano_mask=pmask, stats=stats, geometry=geo)
     return out
+
+
+def check_runner(runner, a, b, S, K, R, blend, cross):
+    """Uploads the pair, runs the plain runner once and checks every output
+    against the oracle chain: bit-exact keypoints, descriptors and matches,
+    identical S5 result and inlier mask, panorama within 1 LSB with the union
+    mask exact. Returns (run record, oracle outputs)."""
+    from cs_stitch import pipeline as P
+    runner.upload_pair(a, b)
+    rec = runner.execute_run(P.harness_config(S=S, K=K, R=R, blend=blend, cross_check=cross))
+    ref = run(a, b, S, K, R, blend, cross)
+    out = runner.outputs()
+    na, nb = rec.num_kp_a, rec.num_kp_b
+    assert na == ref["kp_a"].shape[0] and nb == ref["kp_b"].shape[0]
+    for key, kp, n in (("kp_a", out["ka"], na), ("kp_b", out["kb"], nb)):
+        xs = P.download(kp.x, 4 * n).view(np.float32)
+        ys = P.download(kp.y, 4 * n).view(np.float32)
+        rs = P.download(kp.response, 4 * n).view(np.float32)
+        assert np.array_equal(np.stack([xs, ys, rs], 1), ref[key]), key
+    assert np.array_equal(P.download(out["da"].data, 32 * na).reshape(-1, 32), ref["desc_a"])
+    assert np.array_equal(P.download(out["db"].data, 32 * nb).reshape(-1, 32), ref["desc_b"])
+    res = ref["result"]
+    assert rec.num_matches == res.num_matches == len(ref["idx_a"])
+    m = out["matches"]
+    nm = rec.num_matches
+    assert np.array_equal(P.download(m.idx_a, 4 * nm).view(np.int32), ref["idx_a"])
+    assert np.array_equal(P.download(m.idx_b, 4 * nm).view(np.int32), ref["idx_b"])
+    assert np.array_equal(P.download(m.distance, 4 * nm).view(np.float32), ref["dist"])
+    h = out["h"]
+    assert h.status == res.status and h.num_inliers == res.num_inliers
+    assert np.array_equal(np.array(h.H[:]), np.array(res.H[:])) or res.status != 0
+    assert np.array_equal(P.download(out["inlier_mask"], nm), ref["mask"][:nm])
+    if res.status != 0:
+        return rec, ref
+    assert rec.status.decode() == "OK", rec.status
+    pano = P.download_image(out["canvas"], 3)
+    pmask = P.download_image(out["mask"], 1)
+    assert pano.shape == ref["panorama"].shape
+    assert np.array_equal(pmask, ref["pano_mask"])
+    assert np.abs(pano.astype(int) - ref["panorama"].astype(int)).max() <= 1
+    assert rec.overlap_pixels == ref["stats"][1]
+    return rec, ref

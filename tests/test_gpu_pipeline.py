@@ -11,44 +11,6 @@ import pipeline_oracle
 pytestmark = pytest.mark.gpu
 
 
-def check_runner_against_oracle(runner, a, b, S, K, R, blend, cross):
-    from cs_stitch import pipeline as P
-    runner.upload_pair(a, b)
-    rec = runner.execute_run(P.harness_config(S=S, K=K, R=R, blend=blend, cross_check=cross))
-    ref = pipeline_oracle.run(a, b, S, K, R, blend, cross)
-    out = runner.outputs()
-    na, nb = rec.num_kp_a, rec.num_kp_b
-    assert na == ref["kp_a"].shape[0] and nb == ref["kp_b"].shape[0]
-    for key, kp, n in (("kp_a", out["ka"], na), ("kp_b", out["kb"], nb)):
-        xs = P.download(kp.x, 4 * n).view(np.float32)
-        ys = P.download(kp.y, 4 * n).view(np.float32)
-        rs = P.download(kp.response, 4 * n).view(np.float32)
-        assert np.array_equal(np.stack([xs, ys, rs], 1), ref[key]), key
-    assert np.array_equal(P.download(out["da"].data, 32 * na).reshape(-1, 32), ref["desc_a"])
-    assert np.array_equal(P.download(out["db"].data, 32 * nb).reshape(-1, 32), ref["desc_b"])
-    res = ref["result"]
-    assert rec.num_matches == res.num_matches == len(ref["idx_a"])
-    m = out["matches"]
-    nm = rec.num_matches
-    assert np.array_equal(P.download(m.idx_a, 4 * nm).view(np.int32), ref["idx_a"])
-    assert np.array_equal(P.download(m.idx_b, 4 * nm).view(np.int32), ref["idx_b"])
-    assert np.array_equal(P.download(m.distance, 4 * nm).view(np.float32), ref["dist"])
-    h = out["h"]
-    assert h.status == res.status and h.num_inliers == res.num_inliers
-    assert np.array_equal(np.array(h.H[:]), np.array(res.H[:])) or res.status != 0
-    assert np.array_equal(P.download(out["inlier_mask"], nm), ref["mask"][:nm])
-    if res.status != 0:
-        return rec, ref
-    assert rec.status.decode() == "OK", rec.status
-    pano = P.download_image(out["canvas"], 3)
-    pmask = P.download_image(out["mask"], 1)
-    assert pano.shape == ref["panorama"].shape
-    assert np.array_equal(pmask, ref["pano_mask"])
-    assert np.abs(pano.astype(int) - ref["panorama"].astype(int)).max() <= 1
-    assert rec.overlap_pixels == ref["stats"][1]
-    return rec, ref
-
-
 @pytest.mark.parametrize("seed,w,h,S,K,R,blend,cross", [
     (20261022, 512, 512, 1.0, 1000, 2000, "multiband_5", False),
     (20261023, 640, 480, 0.5, 800, 512, "feather", False),
@@ -60,7 +22,7 @@ def test_plain_runner_matches_oracle(seed, w, h, S, K, R, blend, cross):
     a, b, H = P.synth_pair(seed, w, h, overlap=(0.3, 0.5) if w > 1000 else (0.6, 0.8))
     runner = P.PipelineRunner(w, h, max_k=K, max_r=R, canvas_width=4 * w, canvas_height=2 * h,
                               workspace_mode=P.F.MULTIBAND7)
-    rec, ref = check_runner_against_oracle(runner, a, b, S, K, R, blend, cross)
+    rec, ref = pipeline_oracle.check_runner(runner, a, b, S, K, R, blend, cross)
     assert ref["result"].status == 0
     # The estimate recovers the generator's homography: B pixels that land
     # inside A reproject within 1.5 px of the true mapping (median).
