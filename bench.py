@@ -26,6 +26,7 @@ of that kernel from the committed rocprofv3 PMC summary, when it covers it.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -317,6 +318,11 @@ def main():
     if not args.no_tune:
         tune = br.autotune(specs, args.tune_cache)
         log("tuning", tune.s4_segment, tune.workgroups_per_cu, tune.graphs)
+    # Python's cyclic collector can pause the host thread for tens of ms in
+    # the middle of a timed step (many tracked objects once torch is loaded);
+    # it stays off while batches are timed.
+    gc.collect()
+    gc.disable()
     total, times, outcomes, tm = time_batched(br, specs, args.steps, args.warmup)
     ok = sum(o.status == 0 for o in outcomes)
     batched = dict(pairs_per_s=args.pairs * args.steps / total, ms_per_step=1e3 * total / args.steps,
@@ -357,6 +363,7 @@ def main():
         result["speedup_vs_plain"] = stream["pairs_per_s"] / plain["pairs_per_s"]
         result["batched_speedup_vs_plain"] = batched["pairs_per_s"] / plain["pairs_per_s"]
     br.close()
+    gc.enable()
     if not args.no_serve:
         pool = serve_pool(args.pairs)
         sdev = [upload(*synth(SEED0 + 1000 + i, w, h, ov, kind)[:2]) for i, (w, h, kind, ov) in enumerate(pool)]

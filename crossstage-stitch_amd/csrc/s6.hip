@@ -61,6 +61,8 @@ __device__ bool center_coordinate(double& x, int n, double tol) {
 }
 
 // Bilinear B sample (FP64, then FP32 / 255); the border score only when asked.
+// The division by 255 is a multiplication by its FP64 reciprocal: the two
+// differ by at most one FP64 ulp, far below the FP32 colour's resolution.
 __device__ bool sample(const std::uint8_t* img, int w, int h, int pitch, double x, double y, float* rgb, double* score,
                        double tol) {
     if (!center_coordinate(x, w, tol) || !center_coordinate(y, h, tol)) return false;
@@ -73,7 +75,7 @@ __device__ bool sample(const std::uint8_t* img, int w, int h, int pitch, double 
     for (int c = 0; c < 3; ++c) {
         const double v0 = (1 - ax) * r0[3 * x0 + c] + ax * r0[3 * x1 + c];
         const double v1 = (1 - ax) * r1[3 * x0 + c] + ax * r1[3 * x1 + c];
-        rgb[c] = static_cast<float>(((1 - ay) * v0 + ay * v1) / 255.0);
+        rgb[c] = static_cast<float>(((1 - ay) * v0 + ay * v1) * (1.0 / 255.0));
     }
     // min((x+.5)/w, (w-.5-x)/w) == min(x+.5, w-.5-x)/w exactly: a correctly
     // rounded division by a positive w is monotonic, so it commutes with min.
@@ -317,6 +319,35 @@ __device__ __forceinline__ void stage_rgb(const float* src, std::size_t n, int c
     }
 }
 
+// The same staging split in two: load() issues this thread's loads into
+// registers, store() writes them to LDS, so a kernel can issue every load of
+// a tile before its first barrier (one memory round trip per tile).
+template <int P, int Cols, int Rows>
+struct StageRGB {
+    static constexpr int kN = Cols * Rows, kIt = (kN + kBlock - 1) / kBlock;
+    float v[kIt][3];
+    __device__ __forceinline__ void load(const float* src, std::size_t n, int cw, int ch, int cx0, int cy0) {
+#pragma unroll
+        for (int it = 0; it < kIt; ++it) {
+            const int i = static_cast<int>(threadIdx.x) + it * kBlock;
+            if (i >= kN) continue;
+            const int cx = clampi(cx0 + i % Cols, 0, cw - 1), cy = clampi(cy0 + i / Cols, 0, ch - 1);
+            const std::size_t q = static_cast<std::size_t>(static_cast<unsigned>(cy) * static_cast<unsigned>(cw) +
+                                                           static_cast<unsigned>(cx));
+            v[it][0] = src[q];
+            v[it][1] = src[n + q];
+            v[it][2] = src[2 * n + q];
+        }
+    }
+    __device__ __forceinline__ void store(float4* dst) const {
+#pragma unroll
+        for (int it = 0; it < kIt; ++it) {
+            const int i = static_cast<int>(threadIdx.x) + it * kBlock;
+            if (i < kN) dst[(i / Cols) * P + i % Cols] = make_float4(v[it][0], v[it][1], v[it][2], 0.0f);
+        }
+    }
+};
+
 // ---------------------------------------------------------------- reduce
 // 5x5 [1 4 6 4 1]/16 filter + 2x decimation (s6.cu:236-253) on LDS tiles:
 // a 32x8 tile of level-l outputs reads a 67x19 region of level l-1, loaded
@@ -471,72 +502,89 @@ static_assert(kGW == kRW && kGH == kRH && kGW * kGH == kBlock, "level-1 Gaussian
 
 __device__ __forceinline__ int itap(int d) { return d == 0 ? 6 : (d == 1 || d == -1 ? 4 : 1); }
 
-// Coefficients of the coarse indices i-2..i+2 in sum_dx w(dx) E(X(dx)),
-// E the 1D expansion (factor 2 per axis) at the reflected fine position X.
-// Every coefficient is a multiple of 1/128, so the float sums are exact.
-__device__ __forceinline__ void stencil_axis(int i, int n0, int n1, float (&c)[5]) {
+// Coefficients of the coarse indices i-2..i+2 in sum_dx w(dx) E(X(dx)) and in
+// the same sum restricted to fine positions X in [lo, hi] (A's columns or
+// rows): E is the 1D expansion (factor 2 per axis) at the reflected fine
+// position X. Every coefficient is a multiple of 1/128, so the float sums
+// are exact. Away from the borders nothing reflects and the full sum is
+// [1 28 70 28 1] / 128.
+__device__ __forceinline__ void stencil_axis(int i, int n0, int n1, int lo, int hi, float (&c)[5], float (&d)[5]) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) d[k] = 0.0f;
+    if (i >= 2 && i <= n1 - 3) {
+        c[0] = c[4] = 1.0f / 128;
+        c[1] = c[3] = 28.0f / 128;
+        c[2] = 70.0f / 128;
+        // Fine position 2i + dx reaches slots (dx + e) / 2 + 2 for the taps e
+        // of its parity: even dx -> e in {-2, 0, 2}, odd dx -> e in {-1, 1}.
+#pragma unroll
+        for (int dx = -2; dx <= 2; ++dx) {
+            const int X = 2 * i + dx;
+            if (X < lo || X > hi) continue;
+            const float w = tap(dx);
+            if (dx & 1) {
+                d[(dx - 1) / 2 + 2] = fmaf(w, 0.5f, d[(dx - 1) / 2 + 2]);
+                d[(dx + 1) / 2 + 2] = fmaf(w, 0.5f, d[(dx + 1) / 2 + 2]);
+            } else {
+                d[(dx - 2) / 2 + 2] = fmaf(w, 0.125f, d[(dx - 2) / 2 + 2]);
+                d[dx / 2 + 2] = fmaf(w, 0.75f, d[dx / 2 + 2]);
+                d[(dx + 2) / 2 + 2] = fmaf(w, 0.125f, d[(dx + 2) / 2 + 2]);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 5; ++k) c[k] = 0.0f;
 #pragma unroll
     for (int dx = -2; dx <= 2; ++dx) {
         const int X = clampi(reflect101(2 * i + dx, n0), 0, n0 - 1);
-        const bool odd = X & 1;
+        const bool odd = X & 1, in = X >= lo && X <= hi;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             if (odd && k == 2) continue;
-            const int d = odd ? (k == 0 ? -1 : 1) : 2 * k - 2;
-            const int slot = clampi((reflect101(X + d, 2 * n1) >> 1) - i + 2, 0, 4);
-            c[slot] += tap(dx) * 2.0f * tap(d);
+            const int e = odd ? (k == 0 ? -1 : 1) : 2 * k - 2;
+            const int slot = clampi((reflect101(X + e, 2 * n1) >> 1) - i + 2, 0, 4);
+            const float v = tap(dx) * 2.0f * tap(e);
+            c[slot] += v;
+            if (in) d[slot] += v;
         }
     }
 }
 
-// sum over the footprint of w (1 - m) E for one image: stencil when the
-// footprint misses the image everywhere, tap by tap otherwise.
-__device__ __forceinline__ void gauss1_fill(const float4* pp, bool none, unsigned bit, const std::uint8_t (*fl)[kGFW + 1],
-                                            int lx, int ly, int x, int y, int x0, int y0, const S6Level& F,
-                                            const S6Level& C, float out[3]) {
-    float s0 = 0, s1 = 0, s2 = 0;
-    if (none) {
-        float cx[5], cy[5];
-        stencil_axis(x, F.w, C.w, cx);
-        stencil_axis(y, F.h, C.h, cy);
+// sum over the staged 5x5 coarse neighbourhood of (cy cx^T - dy dx^T) P.
+__device__ __forceinline__ void stencil_apply(const float4* pp, int lx, int ly, const float (&cx)[5], const float (&cy)[5],
+                                              const float (&dx)[5], const float (&dy)[5], bool with_d, float out[3]) {
+    float s0 = 0, s1 = 0, s2 = 0, t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll 1
+    for (int b = 0; b < 5; ++b) {
+        float r0 = 0, r1 = 0, r2 = 0, q0 = 0, q1 = 0, q2 = 0;
 #pragma unroll
-        for (int b = 0; b < 5; ++b) {
-            float r0 = 0, r1 = 0, r2 = 0;
-#pragma unroll
-            for (int a = 0; a < 5; ++a) {
-                const float4 v = pp[(ly + b) * kGPW + lx + a];
-                r0 = fmaf(cx[a], v.x, r0);
-                r1 = fmaf(cx[a], v.y, r1);
-                r2 = fmaf(cx[a], v.z, r2);
+        for (int a = 0; a < 5; ++a) {
+            const float4 v = pp[(ly + b) * kGPW + lx + a];
+            r0 = fmaf(cx[a], v.x, r0);
+            r1 = fmaf(cx[a], v.y, r1);
+            r2 = fmaf(cx[a], v.z, r2);
+            if (with_d) {
+                q0 = fmaf(dx[a], v.x, q0);
+                q1 = fmaf(dx[a], v.y, q1);
+                q2 = fmaf(dx[a], v.z, q2);
             }
-            s0 = fmaf(cy[b], r0, s0);
-            s1 = fmaf(cy[b], r1, s1);
-            s2 = fmaf(cy[b], r2, s2);
         }
-    } else {
-        for (int dy = 0; dy < 5; ++dy)
-            for (int dx = 0; dx < 5; ++dx) {
-                if (fl[2 * ly + dy][2 * lx + dx] & bit) continue;
-                const int X = clampi(reflect101(2 * x + dx - 2, F.w), 0, F.w - 1);
-                const int Y = clampi(reflect101(2 * y + dy - 2, F.h), 0, F.h - 1);
-                Expand e;
-                expand_setup(X, Y, C.w, C.h, x0 - 2, y0 - 2, kGPW, kGPH, e);
-                float E[3];
-                expand_rgb<kGPW>(pp, e, E);
-                const float w = tap(dx - 2) * tap(dy - 2);
-                s0 = fmaf(w, E[0], s0);
-                s1 = fmaf(w, E[1], s1);
-                s2 = fmaf(w, E[2], s2);
-            }
+        s0 = fmaf(cy[b], r0, s0);
+        s1 = fmaf(cy[b], r1, s1);
+        s2 = fmaf(cy[b], r2, s2);
+        if (with_d) {
+            t0 = fmaf(dy[b], q0, t0);
+            t1 = fmaf(dy[b], q1, t1);
+            t2 = fmaf(dy[b], q2, t2);
+        }
     }
-    out[0] = s0;
-    out[1] = s1;
-    out[2] = s2;
+    out[0] = s0 - t0;
+    out[1] = s1 - t1;
+    out[2] = s2 - t2;
 }
 
-__global__ void __launch_bounds__(kBlock) s6_gauss1(dev::JobSpan<S6Job> span, dev::Total work) {
+__global__ void __launch_bounds__(kBlock, 4) s6_gauss1(dev::JobSpan<S6Job> span, dev::Total work) {
     const long long total = work.get();
     __shared__ std::uint8_t fl[kGFH][kGFW + 1];
     __shared__ float4 pp[2][kGPH][kGPW];
@@ -554,14 +602,17 @@ __global__ void __launch_bounds__(kBlock) s6_gauss1(dev::JobSpan<S6Job> span, de
         const int x = x0 + lx, y = y0 + ly;
         const bool inside = x < C.w && y < C.h;
         const size_t p = inside ? static_cast<size_t>(y) * C.w + x : 0;
-        // The raw pull sums, issued first.
+        // Every load of the tile is issued before the barrier: the raw pull
+        // sums, the level-0 flags of the footprint (reflect-101 applied
+        // here) and the pushed level 1 behind it - A's only when the
+        // footprint leaves A's rectangle, B's always (its border is known
+        // only from the flags).
         float ra[3], rb[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             ra[c] = C.ga[c * C.n + p];
             rb[c] = C.gb[c * C.n + p];
         }
-        // Level-0 flags of the tile's footprint (reflect-101 applied here).
         unsigned fls[kGPer];
 #pragma unroll
         for (int e = 0; e < kGPer; ++e) {
@@ -570,22 +621,25 @@ __global__ void __launch_bounds__(kBlock) s6_gauss1(dev::JobSpan<S6Job> span, de
             const int sy = clampi(reflect101(2 * y0 - 2 + i / kGFW, F.h), 0, F.h - 1);
             fls[e] = i < kGElems ? F.flags[static_cast<unsigned>(sy) * static_cast<unsigned>(F.w) + sx] : (kVA | kVB);
         }
-        bool miss_a = false, miss_b = false;
+        // The footprint of the tile lies in level-0 columns 2 x0 - 2 .. 2 x0 + 2 kGW
+        // (reflected into the level), rows likewise.
+        const int ax0 = -j.origin_x, ay0 = -j.origin_y, ax1 = ax0 + j.wa - 1, ay1 = ay0 + j.ha - 1;
+        const bool need_a = !(2 * x0 - 2 >= ax0 && 2 * x0 + 2 * kGW <= ax1 && 2 * y0 - 2 >= ay0 &&
+                              2 * y0 + 2 * kGH <= ay1);
+        StageRGB<kGPW, kGPW, kGPH> sa, sb;
+        if (need_a) sa.load(C.a, C.n, C.w, C.h, x0 - 2, y0 - 2);
+        sb.load(C.b, C.n, C.w, C.h, x0 - 2, y0 - 2);
 #pragma unroll
         for (int e = 0; e < kGPer; ++e) {
             const int i = static_cast<int>(threadIdx.x) + e * kBlock;
             if (i < kGElems) fl[i / kGFW][i % kGFW] = static_cast<std::uint8_t>(fls[e]);
-            miss_a |= !(fls[e] & kVA);
-            miss_b |= !(fls[e] & kVB);
         }
-        const bool need_a = __syncthreads_or(miss_a), need_b = __syncthreads_or(miss_b);
-        if (need_a) stage_rgb<kGPW>(C.a, C.n, C.w, C.h, x0 - 2, y0 - 2, kGPW, kGPH, &pp[0][0][0]);
-        if (need_b) stage_rgb<kGPW>(C.b, C.n, C.w, C.h, x0 - 2, y0 - 2, kGPW, kGPH, &pp[1][0][0]);
-        if (need_a || need_b) __syncthreads();
+        if (need_a) sa.store(&pp[0][0][0]);
+        sb.store(&pp[1][0][0]);
+        __syncthreads();
         if (inside) {
-            int ua = 0, ub = 0;
-            unsigned all = kVA | kVB, any = 0;
-#pragma unroll
+            int ua = 0, ub = 0, nb = 0;
+#pragma unroll 1
             for (int dy = 0; dy < 5; ++dy)
 #pragma unroll
                 for (int dx = 0; dx < 5; ++dx) {
@@ -593,12 +647,54 @@ __global__ void __launch_bounds__(kBlock) s6_gauss1(dev::JobSpan<S6Job> span, de
                     const int w = itap(dx - 2) * itap(dy - 2);
                     ua += (f & kUA) ? w : 0;
                     ub += (f & kUB) ? w : 0;
-                    all &= f;
-                    any |= f;
+                    nb += (f & kVB) ? 1 : 0;
                 }
             float fa[3] = {0, 0, 0}, fb[3] = {0, 0, 0};
-            if (!(all & kVA)) gauss1_fill(&pp[0][0][0], !(any & kVA), kVA, fl, lx, ly, x, y, x0, y0, F, C, fa);
-            if (!(all & kVB)) gauss1_fill(&pp[1][0][0], !(any & kVB), kVB, fl, lx, ly, x, y, x0, y0, F, C, fb);
+            float cx[5], cy[5], dx[5], dy[5];
+            if (need_a || nb < 25) {
+                // A is valid on a rectangle of the canvas, so its fill term
+                // is (cy cx^T - dy dx^T) P with dx, dy restricted to it.
+                stencil_axis(x, F.w, C.w, ax0, ax1, cx, dx);
+                stencil_axis(y, F.h, C.h, ay0, ay1, cy, dy);
+            }
+            if (need_a) {
+                bool full = true, empty_x = true, empty_y = true;
+#pragma unroll
+                for (int k = 0; k < 5; ++k) {
+                    full &= dx[k] == cx[k] && dy[k] == cy[k];
+                    empty_x &= dx[k] == 0.0f;
+                    empty_y &= dy[k] == 0.0f;
+                }
+                if (!full) stencil_apply(&pp[0][0][0], lx, ly, cx, cy, dx, dy, !(empty_x || empty_y), fa);
+            }
+            if (nb == 0) {
+                stencil_apply(&pp[1][0][0], lx, ly, cx, cy, dx, dy, false, fb);
+            } else if (nb < 25) {
+                // B's border crosses the footprint: the taps where B is
+                // missing, or the full stencil less the taps where it is
+                // valid, whichever visits fewer taps.
+                const bool complement = nb < 13;
+                if (complement) stencil_apply(&pp[1][0][0], lx, ly, cx, cy, dx, dy, false, fb);
+                float s0 = 0, s1 = 0, s2 = 0;
+                for (int ty = 0; ty < 5; ++ty)
+                    for (int tx2 = 0; tx2 < 5; ++tx2) {
+                        const bool valid = fl[2 * ly + ty][2 * lx + tx2] & kVB;
+                        if (valid != complement) continue;
+                        const int X = clampi(reflect101(2 * x + tx2 - 2, F.w), 0, F.w - 1);
+                        const int Y = clampi(reflect101(2 * y + ty - 2, F.h), 0, F.h - 1);
+                        Expand e;
+                        expand_setup(X, Y, C.w, C.h, x0 - 2, y0 - 2, kGPW, kGPH, e);
+                        float E[3];
+                        expand_rgb<kGPW>(&pp[1][0][0], e, E);
+                        const float w = tap(tx2 - 2) * tap(ty - 2);
+                        s0 = fmaf(w, E[0], s0);
+                        s1 = fmaf(w, E[1], s1);
+                        s2 = fmaf(w, E[2], s2);
+                    }
+                fb[0] = complement ? fb[0] - s0 : s0;
+                fb[1] = complement ? fb[1] - s1 : s1;
+                fb[2] = complement ? fb[2] - s2 : s2;
+            }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 C.ga[c * C.n + p] = ra[c] + fa[c];
@@ -763,13 +859,23 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
                 any_blend |= inside;
             }
         }
-        const bool staged = coarse && __syncthreads_or(any_blend);
-        if (staged) {
+        // The coarse level behind the tile: at level 0 only for tiles with
+        // an overlap pixel (known after a first barrier); above it every
+        // tile needs it, so its loads go out with the pixels' own.
+        bool staged = false;
+        if (coarse) {
             const S6Level& C = j.lv[l + 1];
-            stage_rgb<kCW>(C.ga, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[0][0][0]);
-            stage_rgb<kCW>(C.gb, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[1][0][0]);
-            stage_rgb<kCW>(C.z, C.n, C.w, C.h, cx0, cy0, kCW, kCH, &cc[2][0][0]);
-            __syncthreads();
+            StageRGB<kCW, kCW, kCH> s0, s1, s2;
+            staged = l > 0 || __syncthreads_or(any_blend);
+            if (staged) {
+                s0.load(C.ga, C.n, C.w, C.h, cx0, cy0);
+                s1.load(C.gb, C.n, C.w, C.h, cx0, cy0);
+                s2.load(C.z, C.n, C.w, C.h, cx0, cy0);
+                s0.store(&cc[0][0][0]);
+                s1.store(&cc[1][0][0]);
+                s2.store(&cc[2][0][0]);
+                __syncthreads();
+            }
         }
         // Pass 2: band mix and collapse where both images are valid.
 #pragma unroll
