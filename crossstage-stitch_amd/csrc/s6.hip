@@ -360,15 +360,24 @@ constexpr int kRW = 32, kRH = 8, kRIW = 2 * kRW + 3, kRIH = 2 * kRH + 3;
 constexpr int kREven = (kRIW + 1) / 2, kRPitch = 80;
 static_assert(kRIW <= kRPitch && (2 * kRPitch) % 64 == 32, "reduce tile pitch");
 
+// Integer [1 4 6 4 1] taps.
+__device__ __forceinline__ int itap(int d) { return d == 0 ? 6 : (d == 1 || d == -1 ? 4 : 1); }
+
 // Modes: 0 pull at l >= 2, 1 Gaussian at l >= 2, 2 pull at l = 1 (from the
-// virtual level 0). The Gaussian at l = 1 is s6_gauss1.
+// virtual level 0). The Gaussian at l = 1 is s6_gauss1. From level 0 the
+// tile holds the six colour channels; the four level-1 weights (validity and
+// ownership of A and B) are integer sums of the flag bits, exact (every
+// partial sum of tap products is a multiple of 1/256), and the ownership
+// weights are the Gaussian's own at level 1.
 template <int kMode>
 __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span, dev::Total work, int l) {
     static_assert(kMode >= 0 && kMode <= 2, "reduce mode");
     const long long total = work.get();
     constexpr bool kFromZero = kMode == 2;
-    __shared__ float tile[8][kRIH][kRPitch];
+    constexpr int kCh = kFromZero ? 6 : 8;
+    __shared__ float tile[kCh][kRIH][kRPitch];
     __shared__ float lut[kFromZero ? 256 : 1];
+    __shared__ std::uint8_t fl[kFromZero ? kRIH : 1][kRIW + 1];
     if (kFromZero) {
         fill_lut(lut);
         __syncthreads();
@@ -386,19 +395,21 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
         const int x0 = static_cast<int>(local % tx) * kRW, y0 = static_cast<int>(local / tx) * kRH;
         const int ix0 = 2 * x0 - 2, iy0 = 2 * y0 - 2;
         // Source planes: A, B colour and the two weights (level 0: B only).
-        const float* planes[8];
+        const float* planes[kCh];
         {
-            // Pull reads the finer level's raw sums, which level 1 keeps in
-            // ga / gb (they alias a / b above level 1).
-            const float* sa = F.ga;
-            const float* sb = kFromZero ? F.b : F.gb;
+            // Pull reads the finer level's raw sums (a / b), the Gaussian its
+            // Gaussian (ga / gb, which alias a / b above level 1).
+            const float* sa = kMode == 1 ? F.ga : F.a;
+            const float* sb = kMode == 1 ? F.gb : F.b;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 planes[c] = kFromZero ? nullptr : sa + c * F.n;
                 planes[3 + c] = sb + c * F.n;
             }
-            planes[6] = kFromZero ? nullptr : (kMode == 0 ? F.va : F.ua);
-            planes[7] = kFromZero ? nullptr : (kMode == 0 ? F.vb : F.ub);
+            if (!kFromZero) {
+                planes[kCh - 2] = kMode == 0 ? F.va : F.ua;
+                planes[kCh - 1] = kMode == 0 ? F.vb : F.ub;
+            }
         }
         // Pass 1: every load of this thread's tile elements, issued before any
         // is used (positions are clamped into the level, so all are in
@@ -406,7 +417,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
         constexpr int kElems = kRIW * kRIH, kPer = (kElems + kBlock - 1) / kBlock;
         int sxs[kPer], sys[kPer];
         unsigned fls[kPer], abytes[kPer][3];
-        float raw[kPer][8];
+        float raw[kPer][kCh];
 #pragma unroll
         for (int e = 0; e < kPer; ++e) {
             const int i = static_cast<int>(threadIdx.x) + e * kBlock;
@@ -425,7 +436,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 }
             } else {
 #pragma unroll
-                for (int c = 0; c < 8; ++c) raw[e][c] = planes[c][q];
+                for (int c = 0; c < kCh; ++c) raw[e][c] = planes[c][q];
             }
         }
         // Pass 2: level-0 selection, then the LDS tile.
@@ -435,7 +446,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
             if (i >= kElems) continue;
             const int ix = i % kRIW, iy = i / kRIW;
             const int cx = (ix & 1) * kREven + (ix >> 1);
-            float v[8];
+            float v[kCh];
             if (kFromZero) {
                 const unsigned f = fls[e];
                 const bool has_a = f & kVA, has_b = f & kVB;
@@ -444,39 +455,59 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                     v[c] = has_a ? lut[abytes[e][c]] : 0.0f;
                     v[3 + c] = has_b ? raw[e][3 + c] : 0.0f;
                 }
-                v[6] = has_a ? 1.0f : 0.0f;
-                v[7] = has_b ? 1.0f : 0.0f;
+                fl[iy][ix] = static_cast<std::uint8_t>(f);
             } else {
 #pragma unroll
-                for (int c = 0; c < 8; ++c) v[c] = raw[e][c];
+                for (int c = 0; c < kCh; ++c) v[c] = raw[e][c];
             }
 #pragma unroll
-            for (int c = 0; c < 8; ++c) tile[c][iy][cx] = v[c];
+            for (int c = 0; c < kCh; ++c) tile[c][iy][cx] = v[c];
         }
         __syncthreads();
         const int lx = threadIdx.x % kRW, ly = threadIdx.x / kRW;
         const int x = x0 + lx, y = y0 + ly;
         if (x < C.w && y < C.h) {
-            float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            float acc[kCh];
+#pragma unroll
+            for (int c = 0; c < kCh; ++c) acc[c] = 0.0f;
 #pragma unroll
             for (int dy = 0; dy < 5; ++dy)
 #pragma unroll
                 for (int dx = 0; dx < 5; ++dx) {
                     const float w = tap(dx - 2) * tap(dy - 2);
 #pragma unroll
-                    for (int c = 0; c < 8; ++c)
+                    for (int c = 0; c < kCh; ++c)
                         acc[c] = fmaf(w, tile[c][2 * ly + dy][(dx & 1) * kREven + lx + (dx >> 1)], acc[c]);
                 }
             const size_t p = static_cast<size_t>(y) * C.w + x;
-            float* da = kMode == 0 ? C.a : C.ga;  // pull into level 1: raw sums kept in ga / gb
-            float* db = kMode == 0 ? C.b : C.gb;
+            float* da = kMode == 1 ? C.ga : C.a;
+            float* db = kMode == 1 ? C.gb : C.b;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 da[c * C.n + p] = acc[c];
                 db[c * C.n + p] = acc[3 + c];
             }
-            (kMode != 1 ? C.va : C.ua)[p] = acc[6];
-            (kMode != 1 ? C.vb : C.ub)[p] = acc[7];
+            if (kFromZero) {
+                int va = 0, vb = 0, ua = 0, ub = 0;
+#pragma unroll 1
+                for (int dy = 0; dy < 5; ++dy)
+#pragma unroll
+                    for (int dx = 0; dx < 5; ++dx) {
+                        const unsigned f = fl[2 * ly + dy][2 * lx + dx];
+                        const int w = itap(dx - 2) * itap(dy - 2);
+                        va += (f & kVA) ? w : 0;
+                        vb += (f & kVB) ? w : 0;
+                        ua += (f & kUA) ? w : 0;
+                        ub += (f & kUB) ? w : 0;
+                    }
+                C.va[p] = static_cast<float>(va) * (1.0f / 256.0f);
+                C.vb[p] = static_cast<float>(vb) * (1.0f / 256.0f);
+                C.ua[p] = static_cast<float>(ua) * (1.0f / 256.0f);
+                C.ub[p] = static_cast<float>(ub) * (1.0f / 256.0f);
+            } else {
+                (kMode == 0 ? C.va : C.ua)[p] = acc[6];
+                (kMode == 0 ? C.vb : C.ub)[p] = acc[7];
+            }
         }
         __syncthreads();
     }
@@ -488,19 +519,17 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
 // level 1 (s6.cu:236-270). It is linear in F, so
 //   G = sum w w m V  +  sum w w (1 - m) E.
 // The first sum is exactly the pull into level 1, whose raw sums sit in
-// ga / gb; the second vanishes where the 5x5 footprint is all valid, is a
-// fixed separable stencil of the pushed level 1 where it is all missing, and
-// is summed tap by tap where the footprint straddles the image's border. So
-// no level-0 colour is read here. The sum is the reference's up to float
-// rounding order. Ownership weights are integer sums of the flag bits, exact
-// (every partial sum of tap products is a multiple of 1/256).
+// a / b; the second vanishes where the 5x5 footprint is all valid (the
+// pulled weight is 1), is a fixed separable stencil of the pushed level 1
+// where it is all missing (weight 0), and for B is summed tap by tap where
+// its border crosses the footprint (A's validity is a rectangle, so its term
+// has a closed form everywhere). So no level-0 colour is read here, and
+// level-0 flags only along B's border. The sum is the reference's up to
+// float rounding order. The ownership weights come from the pull.
 constexpr int kGW = 32, kGH = 8;                         // level-1 outputs per tile
-constexpr int kGFW = 2 * kGW + 3, kGFH = 2 * kGH + 3;    // level-0 footprint region
 constexpr int kGPW = kGW + 4, kGPH = kGH + 4;            // staged pushed level 1 (2 on each side)
-constexpr int kGElems = kGFW * kGFH, kGPer = (kGElems + kBlock - 1) / kBlock;
 static_assert(kGW == kRW && kGH == kRH && kGW * kGH == kBlock, "level-1 Gaussian tiles are the reduce tiles");
 
-__device__ __forceinline__ int itap(int d) { return d == 0 ? 6 : (d == 1 || d == -1 ? 4 : 1); }
 
 // Coefficients of the coarse indices i-2..i+2 in sum_dx w(dx) E(X(dx)) and in
 // the same sum restricted to fine positions X in [lo, hi] (A's columns or
@@ -584,10 +613,18 @@ __device__ __forceinline__ void stencil_apply(const float4* pp, int lx, int ly, 
     out[2] = s2 - t2;
 }
 
-__global__ void __launch_bounds__(kBlock, 4) s6_gauss1(dev::JobSpan<S6Job> span, dev::Total work) {
+// The level-1 region a Gaussian tile stages (the tile and two pixels around)
+// and the pushed level 2 behind it, from which missing level-1 pixels are
+// filled: push_missing for level 1 (s6.cu:399-410) happens here, on the
+// staged pixels, instead of in a pass of its own.
+constexpr int kGSN = kGPW * kGPH, kGSIt = (kGSN + kBlock - 1) / kBlock;
+constexpr int kG2W = kGPW / 2 + 4, kG2H = kGPH / 2 + 4;
+
+__global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span, dev::Total work) {
     const long long total = work.get();
-    __shared__ std::uint8_t fl[kGFH][kGFW + 1];
-    __shared__ float4 pp[2][kGPH][kGPW];
+    __shared__ float4 pp[2][kGPH][kGPW];  // pushed level 1, A and B
+    __shared__ float4 p2[2][kG2H][kG2W];  // pushed level 2, A and B
+    __shared__ float vv[2][kGPH][kGPW];   // level-1 validity weights
     int ji = 0;
     const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
@@ -602,84 +639,114 @@ __global__ void __launch_bounds__(kBlock, 4) s6_gauss1(dev::JobSpan<S6Job> span,
         const int x = x0 + lx, y = y0 + ly;
         const bool inside = x < C.w && y < C.h;
         const size_t p = inside ? static_cast<size_t>(y) * C.w + x : 0;
-        // Every load of the tile is issued before the barrier: the raw pull
-        // sums, the level-0 flags of the footprint (reflect-101 applied
-        // here) and the pushed level 1 behind it - A's only when the
-        // footprint leaves A's rectangle, B's always (its border is known
-        // only from the flags).
+        const bool coarse = j.top >= 2;
+        // Every load of the tile before the first barrier: this pixel's raw
+        // pull sums, the staged level-1 raw sums and weights, the level 2.
+        // The raw sums (a / b) are never written here: the result goes to
+        // ga / gb, so neighbouring tiles read unchanged halos.
         float ra[3], rb[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            ra[c] = C.ga[c * C.n + p];
-            rb[c] = C.gb[c * C.n + p];
+            ra[c] = C.a[c * C.n + p];
+            rb[c] = C.b[c * C.n + p];
         }
-        unsigned fls[kGPer];
+        float sa[kGSIt][4], sb[kGSIt][4];
 #pragma unroll
-        for (int e = 0; e < kGPer; ++e) {
-            const int i = static_cast<int>(threadIdx.x) + e * kBlock;
-            const int sx = clampi(reflect101(2 * x0 - 2 + i % kGFW, F.w), 0, F.w - 1);
-            const int sy = clampi(reflect101(2 * y0 - 2 + i / kGFW, F.h), 0, F.h - 1);
-            fls[e] = i < kGElems ? F.flags[static_cast<unsigned>(sy) * static_cast<unsigned>(F.w) + sx] : (kVA | kVB);
-        }
-        // The footprint of the tile lies in level-0 columns 2 x0 - 2 .. 2 x0 + 2 kGW
-        // (reflected into the level), rows likewise.
-        const int ax0 = -j.origin_x, ay0 = -j.origin_y, ax1 = ax0 + j.wa - 1, ay1 = ay0 + j.ha - 1;
-        const bool need_a = !(2 * x0 - 2 >= ax0 && 2 * x0 + 2 * kGW <= ax1 && 2 * y0 - 2 >= ay0 &&
-                              2 * y0 + 2 * kGH <= ay1);
-        StageRGB<kGPW, kGPW, kGPH> sa, sb;
-        if (need_a) sa.load(C.a, C.n, C.w, C.h, x0 - 2, y0 - 2);
-        sb.load(C.b, C.n, C.w, C.h, x0 - 2, y0 - 2);
+        for (int it = 0; it < kGSIt; ++it) {
+            const int i = static_cast<int>(threadIdx.x) + it * kBlock;
+            if (i >= kGSN) continue;
+            const int cx = clampi(x0 - 2 + i % kGPW, 0, C.w - 1), cy = clampi(y0 - 2 + i / kGPW, 0, C.h - 1);
+            const size_t q = static_cast<size_t>(static_cast<unsigned>(cy) * static_cast<unsigned>(C.w) +
+                                                 static_cast<unsigned>(cx));
 #pragma unroll
-        for (int e = 0; e < kGPer; ++e) {
-            const int i = static_cast<int>(threadIdx.x) + e * kBlock;
-            if (i < kGElems) fl[i / kGFW][i % kGFW] = static_cast<std::uint8_t>(fls[e]);
+            for (int c = 0; c < 3; ++c) {
+                sa[it][c] = C.a[c * C.n + q];
+                sb[it][c] = C.b[c * C.n + q];
+            }
+            sa[it][3] = C.va[q];
+            sb[it][3] = C.vb[q];
         }
-        if (need_a) sa.store(&pp[0][0][0]);
-        sb.store(&pp[1][0][0]);
+        StageRGB<kG2W, kG2W, kG2H> la, lb;
+        if (coarse) {
+            const S6Level& D = j.lv[2];
+            la.load(D.a, D.n, D.w, D.h, x0 / 2 - 2, y0 / 2 - 2);
+            lb.load(D.b, D.n, D.w, D.h, x0 / 2 - 2, y0 / 2 - 2);
+            la.store(&p2[0][0][0]);
+            lb.store(&p2[1][0][0]);
+        }
+        __syncthreads();
+        // Normalise the staged pixels, or fill them from the expanded level 2.
+#pragma unroll
+        for (int it = 0; it < kGSIt; ++it) {
+            const int i = static_cast<int>(threadIdx.x) + it * kBlock;
+            if (i >= kGSN) continue;
+            const int cx = clampi(x0 - 2 + i % kGPW, 0, C.w - 1), cy = clampi(y0 - 2 + i / kGPW, 0, C.h - 1);
+            float ea[3] = {0, 0, 0}, eb[3] = {0, 0, 0};
+            if (coarse && (!(sa[it][3] > 0) || !(sb[it][3] > 0))) {
+                Expand e;
+                expand_setup(cx, cy, j.lv[2].w, j.lv[2].h, x0 / 2 - 2, y0 / 2 - 2, kG2W, kG2H, e);
+                if (!(sa[it][3] > 0)) expand_rgb<kG2W>(&p2[0][0][0], e, ea);
+                if (!(sb[it][3] > 0)) expand_rgb<kG2W>(&p2[1][0][0], e, eb);
+            }
+            float4 va, vb;
+            va.x = sa[it][3] > 0 ? sa[it][0] / sa[it][3] : ea[0];
+            va.y = sa[it][3] > 0 ? sa[it][1] / sa[it][3] : ea[1];
+            va.z = sa[it][3] > 0 ? sa[it][2] / sa[it][3] : ea[2];
+            vb.x = sb[it][3] > 0 ? sb[it][0] / sb[it][3] : eb[0];
+            vb.y = sb[it][3] > 0 ? sb[it][1] / sb[it][3] : eb[1];
+            vb.z = sb[it][3] > 0 ? sb[it][2] / sb[it][3] : eb[2];
+            va.w = vb.w = 0.0f;
+            pp[0][i / kGPW][i % kGPW] = va;
+            pp[1][i / kGPW][i % kGPW] = vb;
+            vv[0][i / kGPW][i % kGPW] = sa[it][3];
+            vv[1][i / kGPW][i % kGPW] = sb[it][3];
+        }
         __syncthreads();
         if (inside) {
-            int ua = 0, ub = 0, nb = 0;
-#pragma unroll 1
-            for (int dy = 0; dy < 5; ++dy)
-#pragma unroll
-                for (int dx = 0; dx < 5; ++dx) {
-                    const unsigned f = fl[2 * ly + dy][2 * lx + dx];
-                    const int w = itap(dx - 2) * itap(dy - 2);
-                    ua += (f & kUA) ? w : 0;
-                    ub += (f & kUB) ? w : 0;
-                    nb += (f & kVB) ? 1 : 0;
-                }
+            // The footprint is all valid where the weight is 1 and all
+            // missing where it is 0 (the weights are exact sums of taps).
+            const float wa = vv[0][ly + 2][lx + 2], wb = vv[1][ly + 2][lx + 2];
             float fa[3] = {0, 0, 0}, fb[3] = {0, 0, 0};
+            const int ax0 = -j.origin_x, ay0 = -j.origin_y, ax1 = ax0 + j.wa - 1, ay1 = ay0 + j.ha - 1;
             float cx[5], cy[5], dx[5], dy[5];
-            if (need_a || nb < 25) {
-                // A is valid on a rectangle of the canvas, so its fill term
-                // is (cy cx^T - dy dx^T) P with dx, dy restricted to it.
+            if (wa != 1.0f || wb != 1.0f) {
                 stencil_axis(x, F.w, C.w, ax0, ax1, cx, dx);
                 stencil_axis(y, F.h, C.h, ay0, ay1, cy, dy);
             }
-            if (need_a) {
-                bool full = true, empty_x = true, empty_y = true;
+            if (wa != 1.0f) {
+                // A is valid on a rectangle of the canvas, so its fill term
+                // is (cy cx^T - dy dx^T) P with dx, dy restricted to it.
+                bool empty_x = true, empty_y = true;
 #pragma unroll
                 for (int k = 0; k < 5; ++k) {
-                    full &= dx[k] == cx[k] && dy[k] == cy[k];
                     empty_x &= dx[k] == 0.0f;
                     empty_y &= dy[k] == 0.0f;
                 }
-                if (!full) stencil_apply(&pp[0][0][0], lx, ly, cx, cy, dx, dy, !(empty_x || empty_y), fa);
+                stencil_apply(&pp[0][0][0], lx, ly, cx, cy, dx, dy, !(empty_x || empty_y), fa);
             }
-            if (nb == 0) {
+            if (wb == 0.0f) {
                 stencil_apply(&pp[1][0][0], lx, ly, cx, cy, dx, dy, false, fb);
-            } else if (nb < 25) {
+            } else if (wb != 1.0f) {
                 // B's border crosses the footprint: the taps where B is
                 // missing, or the full stencil less the taps where it is
-                // valid, whichever visits fewer taps.
+                // valid, whichever visits fewer taps (flags read directly:
+                // such pixels line B's border only).
+                unsigned valid = 0;
+                int nb = 0;
+                for (int ty = 0; ty < 5; ++ty)
+                    for (int tx2 = 0; tx2 < 5; ++tx2) {
+                        const int X = clampi(reflect101(2 * x + tx2 - 2, F.w), 0, F.w - 1);
+                        const int Y = clampi(reflect101(2 * y + ty - 2, F.h), 0, F.h - 1);
+                        const bool v = F.flags[static_cast<unsigned>(Y) * static_cast<unsigned>(F.w) + X] & kVB;
+                        valid |= (v ? 1u : 0u) << (ty * 5 + tx2);
+                        nb += v;
+                    }
                 const bool complement = nb < 13;
                 if (complement) stencil_apply(&pp[1][0][0], lx, ly, cx, cy, dx, dy, false, fb);
                 float s0 = 0, s1 = 0, s2 = 0;
                 for (int ty = 0; ty < 5; ++ty)
                     for (int tx2 = 0; tx2 < 5; ++tx2) {
-                        const bool valid = fl[2 * ly + ty][2 * lx + tx2] & kVB;
-                        if (valid != complement) continue;
+                        if ((((valid >> (ty * 5 + tx2)) & 1u) != 0) != complement) continue;
                         const int X = clampi(reflect101(2 * x + tx2 - 2, F.w), 0, F.w - 1);
                         const int Y = clampi(reflect101(2 * y + ty - 2, F.h), 0, F.h - 1);
                         Expand e;
@@ -700,8 +767,6 @@ __global__ void __launch_bounds__(kBlock, 4) s6_gauss1(dev::JobSpan<S6Job> span,
                 C.ga[c * C.n + p] = ra[c] + fa[c];
                 C.gb[c * C.n + p] = rb[c] + fb[c];
             }
-            C.ua[p] = static_cast<float>(ua) * (1.0f / 256.0f);
-            C.ub[p] = static_cast<float>(ub) * (1.0f / 256.0f);
         }
         __syncthreads();
     }
@@ -742,8 +807,8 @@ __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev:
             wb[k] = inside ? L.vb[p] : 1.0f;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                va[k][c] = L.ga[c * L.n + p];  // raw pull sums (the same planes as a / b above level 1)
-                vb[k][c] = L.gb[c * L.n + p];
+                va[k][c] = L.a[c * L.n + p];
+                vb[k][c] = L.b[c * L.n + p];
             }
             missing |= !(wa[k] > 0) || !(wb[k] > 0);
         }
@@ -778,9 +843,10 @@ __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev:
 }
 
 // mix_band + reconstruct (s6.cu:271-293) for level l, top-down; at level 0
-// also finish (s6.cu:294-306) and the union mask: single-source pixels keep
-// their warped colour and only overlap pixels need the band mix, so a tile
-// without overlap pixels stages nothing.
+// also finish (s6.cu:294-306): single-source pixels keep their warped colour
+// (written by the warp) and only overlap pixels need the band mix, so a tile
+// without overlap pixels stages nothing. Level 1 stores z - ga and z - gb, the
+// two planes level 0 expands (one per overlap pixel, its owner's).
 __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::Total work, int l) {
     const long long total = work.get();
     __shared__ float4 cc[3][kCH][kCW];
@@ -867,13 +933,20 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
             const S6Level& C = j.lv[l + 1];
             StageRGB<kCW, kCW, kCH> s0, s1, s2;
             staged = l > 0 || __syncthreads_or(any_blend);
-            if (staged) {
+            if (staged && l > 0) {
                 s0.load(C.ga, C.n, C.w, C.h, cx0, cy0);
                 s1.load(C.gb, C.n, C.w, C.h, cx0, cy0);
                 s2.load(C.z, C.n, C.w, C.h, cx0, cy0);
                 s0.store(&cc[0][0][0]);
                 s1.store(&cc[1][0][0]);
                 s2.store(&cc[2][0][0]);
+                __syncthreads();
+            } else if (staged) {
+                // Level 0 expands z1 - ga1 (kept in z) and z1 - gb1 (kept in a).
+                s0.load(C.z, C.n, C.w, C.h, cx0, cy0);
+                s1.load(C.a, C.n, C.w, C.h, cx0, cy0);
+                s0.store(&cc[0][0][0]);
+                s1.store(&cc[1][0][0]);
                 __syncthreads();
             }
         }
@@ -884,7 +957,26 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
             if (!(x < L.w && y < L.h)) continue;
             const size_t p = static_cast<size_t>(y) * L.w + x;
             float z[3];
-            if ((f[k] & (kVA | kVB)) == (kVA | kVB)) {
+            if (l == 0) {
+                // An overlap pixel's ownership weights are 0 and 1, so its
+                // band is the owner's level-0 colour less the owner's
+                // expanded level-1 Gaussian, and the collapse adds the
+                // expanded level-1 result: one expansion of
+                // d = z1 - g1_owner, which the level-1 mix stored.
+                if ((f[k] & (kVA | kVB)) != (kVA | kVB)) continue;  // written by the warp
+                const bool own_a = f[k] & kUA;
+                float e[3] = {0, 0, 0};
+                if (coarse) {
+                    Expand ex;
+                    expand_setup(x, y, j.lv[1].w, j.lv[1].h, cx0, cy0, kCW, kCH, ex);
+                    expand_rgb<kCW>(&cc[own_a ? 0 : 1][0][0], ex, e);
+                }
+                std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) out[c] = quantize((own_a ? ga[k][c] : gb[k][c]) + e[c]);
+                continue;
+            }
+            {
                 float ea[3] = {0, 0, 0}, eb[3] = {0, 0, 0}, ez[3] = {0, 0, 0};
                 if (coarse) {
                     Expand e;
@@ -901,16 +993,18 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
                     const float m = tot > 0 ? fmaf(ua[k], la, ub[k] * lb) / tot : 0.0f;
                     z[c] = coarse ? m + ez[c] : m;
                 }
-            } else {
-                continue;  // level 0 outside the overlap: written by the warp
             }
-            if (l > 0) {
+            if (l > 1) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) L.z[c * L.n + p] = z[c];
             } else {
-                std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
+                // Level 1 keeps what level 0 expands: z - ga in z, z - gb in a
+                // (the pushed colour there is no longer read).
 #pragma unroll
-                for (int c = 0; c < 3; ++c) out[c] = quantize(z[c]);
+                for (int c = 0; c < 3; ++c) {
+                    L.z[c * L.n + p] = z[c] - ga[k][c];
+                    L.a[c * L.n + p] = z[c] - gb[k][c];
+                }
             }
         }
         if (staged) __syncthreads();
@@ -1235,22 +1329,28 @@ double s6_algorithmic_bytes(int kernel, int l, const S6Job& j, const s6::RenderS
         case LaunchProbe::kPull:
         case LaunchProbe::kGauss: {
             if (l > (kernel == LaunchProbe::kPull ? j.top : j.blend_top)) return 0;
-            const double out = 32 * n(l);  // two colours and two weights
-            if (l >= 2) return 32 * n(l - 1) + out;
-            // Pull from level 0: flags, A's bytes, B's colour. Gaussian into
-            // level 1: flags, the raw pull sums and the pushed level 1.
-            return kernel == LaunchProbe::kPull ? P + 3 * nA + 12 * nB + out : P + 48 * n(1) + out;
+            if (l >= 2) return 32 * n(l - 1) + 32 * n(l);  // two colours and two weights
+            // Pull from level 0: flags, A's bytes and B's colour in; two raw
+            // colours and four weights out. Gaussian into level 1: the raw
+            // sums and validity weights (pushed on the fly) and the pushed
+            // level 2 in, two colours out.
+            if (kernel == LaunchProbe::kPull) return P + 3 * nA + 12 * nB + 40 * n(1);
+            return 32 * n(1) + (j.top >= 2 ? 24 * n(2) : 0) + 24 * n(1);
         }
         case LaunchProbe::kPush:
             if (l > j.top) return 0;
             return 56 * n(l) + (l < j.top ? 24 * n(l + 1) : 0);
         case LaunchProbe::kMix:
             if (l > j.blend_top) return 0;
-            if (l >= 1) return 44 * n(l) + (l < j.blend_top ? 36 * n(l + 1) : 0);
+            // Gaussians and weights in, the result out (two planes at level 1),
+            // the coarser Gaussians and result staged.
+            if (l >= 1) return (l == 1 ? 56 : 44) * n(l) + (l < j.blend_top ? 36 * n(l + 1) : 0);
             {
+                // Flags over the overlap box; A's bytes, B's colour, the two
+                // level-1 difference planes under the overlap and the output.
                 const double box = j.ox1 >= j.ox0 && j.oy1 >= j.oy0
                                        ? static_cast<double>(j.ox1 - j.ox0 + 1) * (j.oy1 - j.oy0 + 1) : 0.0;
-                return box + 15 * O + 3 * O + (j.blend_top > 0 ? 9 * O : 0);
+                return box + 15 * O + 3 * O + (j.blend_top > 0 ? 6 * O : 0);
             }
         default:
             return 0;
@@ -1287,7 +1387,7 @@ hipError_t launch_s6(const S6Job* table, int count, const S6Totals& t, const S6J
             else s6_reduce<0><<<persistent_grid(t.pull[l]), kBlock, 0, stream>>>(span, w, l);
         });
     }
-    for (int l = t.top; l >= 1; --l)
+    for (int l = t.top; l >= 2; --l)  // level 1 is pushed inside s6_gauss1
         if (t.push[l] > 0)
             timed(LaunchProbe::kPush, l, [&] {
                 s6_push<<<persistent_grid(t.push[l]), kBlock, 0, stream>>>(
