@@ -62,6 +62,45 @@ __device__ std::uint8_t s1_pixel(const S1Job& p, int xd, int yd) {
 constexpr int kRowsPerThread = kS1TileH / (256 / kS1TileW);
 static_assert(kRowsPerThread * (256 / kS1TileW) == kS1TileH, "S1 tile");
 
+// Four horizontally adjacent pixels of a full-size (mode 0) job from three
+// dword loads and one dword store, for rows whose addresses are 4-aligned.
+__device__ __forceinline__ unsigned gray4(unsigned w0, unsigned w1, unsigned w2) {
+    const auto b = [](unsigned w, int k) { return static_cast<int>((w >> (8 * k)) & 0xffu); };
+    const unsigned g0 = static_cast<unsigned>(rgb_to_gray(b(w0, 0), b(w0, 1), b(w0, 2)));
+    const unsigned g1 = static_cast<unsigned>(rgb_to_gray(b(w0, 3), b(w1, 0), b(w1, 1)));
+    const unsigned g2 = static_cast<unsigned>(rgb_to_gray(b(w1, 2), b(w1, 3), b(w2, 0)));
+    const unsigned g3 = static_cast<unsigned>(rgb_to_gray(b(w2, 1), b(w2, 2), b(w2, 3)));
+    return g0 | (g1 << 8) | (g2 << 16) | (g3 << 24);
+}
+
+__device__ __forceinline__ void s1_tile_vec4(const S1Job& p, int tx, int ty) {
+    constexpr int kQuads = kS1TileW / 4, kRows = 256 / kQuads, kIt = kS1TileH / kRows;
+    const int xd = tx * kS1TileW + 4 * static_cast<int>(threadIdx.x % kQuads);
+    unsigned w[kIt][3];
+#pragma unroll
+    for (int k = 0; k < kIt; ++k) {
+        const int yd = ty * kS1TileH + static_cast<int>(threadIdx.x / kQuads) + k * kRows;
+        if (yd < p.work_h && xd + 3 < p.work_w) {
+            const auto* s = reinterpret_cast<const unsigned*>(p.src + static_cast<size_t>(yd) * p.src_pitch + 3 * xd);
+            w[k][0] = s[0];
+            w[k][1] = s[1];
+            w[k][2] = s[2];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kIt; ++k) {
+        const int yd = ty * kS1TileH + static_cast<int>(threadIdx.x / kQuads) + k * kRows;
+        if (yd >= p.work_h || xd >= p.work_w) continue;
+        std::uint8_t* d = p.dst + static_cast<size_t>(yd) * p.dst_pitch + xd;
+        if (xd + 3 < p.work_w) {
+            *reinterpret_cast<unsigned*>(d) = gray4(w[k][0], w[k][1], w[k][2]);
+        } else {
+            for (int i = 0; xd + i < p.work_w; ++i)
+                d[i] = static_cast<std::uint8_t>(gray_at(p.src + static_cast<size_t>(yd) * p.src_pitch + 3 * (xd + i)));
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) s1_kernel(dev::JobSpan<S1Job> span, dev::Total work) {
     const long long total = work.get();
     int j = 0;
@@ -72,6 +111,12 @@ __global__ void __launch_bounds__(256) s1_kernel(dev::JobSpan<S1Job> span, dev::
         const unsigned local = static_cast<unsigned>(t - p.tile_begin);
         const unsigned tiles_x = static_cast<unsigned>(p.tiles_x);
         const int tx = static_cast<int>(local % tiles_x), ty = static_cast<int>(local / tiles_x);
+        if (p.mode == 0 && ((reinterpret_cast<std::uintptr_t>(p.src) | reinterpret_cast<std::uintptr_t>(p.dst) |
+                             static_cast<std::uintptr_t>(p.src_pitch) | static_cast<std::uintptr_t>(p.dst_pitch)) &
+                            3u) == 0) {
+            s1_tile_vec4(p, tx, ty);
+            continue;
+        }
         const int xd = tx * kS1TileW + static_cast<int>(threadIdx.x % kS1TileW);
         std::uint8_t v[kRowsPerThread];
 #pragma unroll

@@ -64,18 +64,40 @@ __device__ int fast_score_lds(const std::uint8_t* g, int threshold) {
     }
     if (brighter < 2 && darker < 2) return 0;
     int d[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) d[k] = v - g[kCircleY[k] * kGrayW + kCircleX[k]];
-    int best = INT_MIN;
+    unsigned dark = 0, bright = 0;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-        int lo = INT_MAX, hi = INT_MIN;
+        d[k] = v - g[kCircleY[k] * kGrayW + kCircleX[k]];
+        dark |= static_cast<unsigned>(d[k] > threshold) << k;
+        bright |= static_cast<unsigned>(d[k] < -threshold) << k;
+    }
+    // The score exceeds the threshold exactly when 9 contiguous circle
+    // pixels are all darker or all brighter; otherwise it is 0.
+    const auto arc9 = [](unsigned m) {
+        const unsigned x = m | (m << 16);
+        unsigned y = x & (x >> 1);
+        y &= y >> 2;
+        y &= y >> 4;
+        return (y & (x >> 8)) != 0;
+    };
+    if (!arc9(dark) && !arc9(bright)) return 0;
+    // cornerScore: the best over the 16 arcs of 9 of min(d) and of -max(d),
+    // by doubling windows (2, 4, 8, then the ninth pixel).
+    int lo2[16], hi2[16], lo4[16], hi4[16], best = INT_MIN;
 #pragma unroll
-        for (int j = 0; j < 9; ++j) {
-            const int e = d[(k + j) & 15];
-            lo = min(lo, e);
-            hi = max(hi, e);
-        }
+    for (int k = 0; k < 16; ++k) {
+        lo2[k] = min(d[k], d[(k + 1) & 15]);
+        hi2[k] = max(d[k], d[(k + 1) & 15]);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        lo4[k] = min(lo2[k], lo2[(k + 2) & 15]);
+        hi4[k] = max(hi2[k], hi2[(k + 2) & 15]);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int lo = min(min(lo4[k], lo4[(k + 4) & 15]), d[(k + 8) & 15]);
+        const int hi = max(max(hi4[k], hi4[(k + 4) & 15]), d[(k + 8) & 15]);
         best = max(best, max(lo, -hi));
     }
     return best > threshold ? best - 1 : 0;
@@ -84,7 +106,7 @@ __device__ int fast_score_lds(const std::uint8_t* g, int threshold) {
 // FAST + NMS + histogram on 64x16 tiles with a 4-pixel gray halo in LDS.
 __global__ void __launch_bounds__(256) s2_fast_nms(dev::JobSpan<S2Job> span, dev::Total work) {
     const long long total = work.get();
-    __shared__ std::uint8_t gray[kGrayH * kGrayW];
+    __shared__ __align__(4) std::uint8_t gray[kGrayH * kGrayW];
     __shared__ std::uint8_t score[kScoreH * kScoreW];
     __shared__ std::uint32_t hist[256];
     const int tid = threadIdx.x;
@@ -97,9 +119,30 @@ __global__ void __launch_bounds__(256) s2_fast_nms(dev::JobSpan<S2Job> span, dev
         const int x0 = static_cast<int>(local % j.tiles_x) * kS2TileW;
         const int y0 = static_cast<int>(local / j.tiles_x) * kS2TileH;
         hist[tid] = 0;
-        for (int i = tid; i < kGrayH * kGrayW; i += 256) {
-            const int gx = x0 - kHalo + i % kGrayW, gy = y0 - kHalo + i / kGrayW;
-            gray[i] = (gx >= 0 && gx < j.w && gy >= 0 && gy < j.h) ? j.img[static_cast<size_t>(gy) * j.pitch + gx] : 0;
+        if (((reinterpret_cast<std::uintptr_t>(j.img) | static_cast<std::uintptr_t>(j.pitch)) & 3u) == 0) {
+            // Four pixels per load (x0 - kHalo is a multiple of 4); pixels
+            // outside the image read 0.
+            constexpr int kWords = kGrayW / 4;
+            static_assert(kGrayW % 4 == 0 && kHalo % 4 == 0 && kS2TileW % 4 == 0, "gray tile words");
+            for (int i = tid; i < kGrayH * kWords; i += 256) {
+                const int gx = x0 - kHalo + 4 * (i % kWords), gy = y0 - kHalo + i / kWords;
+                unsigned word = 0;
+                if (gy >= 0 && gy < j.h) {
+                    const std::uint8_t* row = j.img + static_cast<size_t>(gy) * j.pitch;
+                    if (gx >= 0 && gx + 3 < j.w) {
+                        word = *reinterpret_cast<const unsigned*>(row + gx);
+                    } else {
+                        for (int b = 0; b < 4; ++b)
+                            if (gx + b >= 0 && gx + b < j.w) word |= static_cast<unsigned>(row[gx + b]) << (8 * b);
+                    }
+                }
+                reinterpret_cast<unsigned*>(gray)[(i / kWords) * kWords + i % kWords] = word;
+            }
+        } else {
+            for (int i = tid; i < kGrayH * kGrayW; i += 256) {
+                const int gx = x0 - kHalo + i % kGrayW, gy = y0 - kHalo + i / kGrayW;
+                gray[i] = (gx >= 0 && gx < j.w && gy >= 0 && gy < j.h) ? j.img[static_cast<size_t>(gy) * j.pitch + gx] : 0;
+            }
         }
         __syncthreads();
         for (int i = tid; i < kScoreH * kScoreW; i += 256) {

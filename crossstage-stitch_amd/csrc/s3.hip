@@ -51,10 +51,34 @@ __global__ void __launch_bounds__(256) s3_blur(dev::JobSpan<S3Job> span, dev::To
         const long long local = t - j.blur_begin;
         const int x0 = static_cast<int>(local % j.tiles_x) * kS3TileW;
         const int y0 = static_cast<int>(local / j.tiles_x) * kS3TileH;
-        for (int i = tid; i < kInH * kInW; i += 256) {
-            const int gx = reflect101(x0 - kR + i % kInW, j.w);
-            const int gy = reflect101(y0 - kR + i / kInW, j.h);
-            in[i] = static_cast<float>(j.img[static_cast<size_t>(gy) * j.pitch + gx]);
+        if (((reinterpret_cast<std::uintptr_t>(j.img) | static_cast<std::uintptr_t>(j.pitch)) & 3u) == 0) {
+            // Aligned words over columns x0 - 4 .. x0 + kS3TileW + 3; words
+            // that would need a column reflected go byte by byte.
+            constexpr int kWords = (kS3TileW + 8) / 4;
+            static_assert(kS3TileW % 4 == 0 && kR == 3, "blur tile words");
+            for (int i = tid; i < kInH * kWords; i += 256) {
+                const int r = i / kWords, k = i % kWords;
+                const int gx = x0 - 4 + 4 * k;
+                const int gy = reflect101(y0 - kR + r, j.h);
+                const std::uint8_t* row = j.img + static_cast<size_t>(gy) * j.pitch;
+                unsigned word = 0;
+                if (gx >= 0 && gx + 3 < j.w) {
+                    word = *reinterpret_cast<const unsigned*>(row + gx);
+                } else {
+                    for (int b = 0; b < 4; ++b) word |= static_cast<unsigned>(row[reflect101(gx + b, j.w)]) << (8 * b);
+                }
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int col = 4 * k - 1 + b;  // column in the tile (x0 - kR = column 0)
+                    if (col >= 0 && col < kInW) in[r * kInW + col] = static_cast<float>((word >> (8 * b)) & 0xffu);
+                }
+            }
+        } else {
+            for (int i = tid; i < kInH * kInW; i += 256) {
+                const int gx = reflect101(x0 - kR + i % kInW, j.w);
+                const int gy = reflect101(y0 - kR + i / kInW, j.h);
+                in[i] = static_cast<float>(j.img[static_cast<size_t>(gy) * j.pitch + gx]);
+            }
         }
         __syncthreads();
         for (int i = tid; i < kInH * kS3TileW; i += 256) {
