@@ -90,6 +90,16 @@ class HarnessConfig(C.Structure):
                 ("msac_threshold", C.c_double), ("seed", C.c_uint64)]
 
 
+class AttemptRecord(C.Structure):
+    _fields_ = [("S", C.c_double), ("K", C.c_int), ("R", C.c_int), ("blend_mode", C.c_int), ("action", C.c_int),
+                ("reason", C.c_int), ("next_S", C.c_double), ("next_K", C.c_int), ("next_R", C.c_int),
+                ("next_blend_mode", C.c_int), ("previous_R", C.c_int), ("generation", C.c_uint64),
+                ("homography", HomographyResult), ("t_s1", C.c_double), ("t_s2", C.c_double), ("t_s3", C.c_double),
+                ("t_s4", C.c_double), ("t_s5", C.c_double), ("t_attempt", C.c_double),
+                ("restart_from", C.c_char * 8), ("fail_stage", C.c_char * 8), ("execution_status", C.c_int),
+                ("has_decision", C.c_int), ("has_homography", C.c_int), ("pad", C.c_int)]
+
+
 class RunRecord(C.Structure):
     _fields_ = [("S", C.c_double), ("K", C.c_int), ("R", C.c_int), ("blend_mode", C.c_int),
                 ("t_s1", C.c_double), ("t_s2", C.c_double), ("t_s3", C.c_double), ("t_s4", C.c_double),
@@ -206,6 +216,7 @@ SIGNATURES = {
     "cs_runner_outputs": (C.c_int, [vp, P(Image), P(Image), P(Keypoints), P(Keypoints), P(Descriptors),
                                     P(Descriptors), P(Matches), P(vp), P(HomographyResult)]),
     "cs_runner_stream": (vp, [vp]),
+    "cs_runner_attempt_trace": (C.c_int, [vp, P(AttemptRecord), C.c_int, P(C.c_int)]),
     "cs_controller_create_legacy": (vp, [C.c_int]),
     "cs_controller_destroy": (None, [vp]),
     "cs_plan_retry": (C.c_int, [C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,

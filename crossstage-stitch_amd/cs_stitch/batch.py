@@ -102,7 +102,10 @@ class BatchRunner:
             self.handle = C.c_void_p()
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except (AttributeError, TypeError):  # interpreter teardown: the library module is already gone
+            pass
 
 
 def pair_bytes(width, height, K, R) -> int:
@@ -158,7 +161,10 @@ class Server:
             self.handle = C.c_void_p()
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except (AttributeError, TypeError):  # interpreter teardown: the library module is already gone
+            pass
 
 
 def percentile(values, q):

@@ -89,6 +89,13 @@ class PipelineRunner:
                "runner.execute_adaptive_run")
         return rec
 
+    def attempt_trace(self) -> list:
+        """Per-attempt records of the last adaptive run (knobs, decision, S5 result, stage times)."""
+        n = C.c_int()
+        out = (F.AttemptRecord * 16)()
+        _check(F.load().cs_runner_attempt_trace(self.handle, out, 16, C.byref(n)), "runner.attempt_trace")
+        return list(out[: n.value])
+
     def set_flags(self, reuse=True, probe=True):
         F.load().cs_runner_set_flags(self.handle, int(reuse), int(probe))
 
@@ -110,7 +117,10 @@ class PipelineRunner:
             self.handle = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except (AttributeError, TypeError):  # interpreter teardown: the library module is already gone
+            pass
 
 
 _hip = None

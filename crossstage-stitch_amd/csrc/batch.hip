@@ -1068,7 +1068,7 @@ Tuning BatchRunner::autotune(const PairSpec* pairs, int n, const std::string& ca
     };
     // Occupancy first (every stage), then the S4 train segment (S4 only).
     double best_ms = 1e30;
-    for (int wpc : {4, 8, 16}) {
+    for (int wpc : {8, 16, 24, 32}) {
         Tuning t = best;
         t.workgroups_per_cu = wpc;
         const double ms = measure(t, false);

@@ -182,6 +182,42 @@ int cs_runner_execute_adaptive_run(cs_runner* r, cs_controller* ctl, uint64_t se
     }
 }
 
+int cs_runner_attempt_trace(cs_runner* r, cs_attempt_record* out, int capacity, int* count) {
+    if (!r || capacity < 0 || (capacity > 0 && !out)) return CS_INVALID_ARGUMENT;
+    const harness::AttemptTrace& t = r->impl.attempt_trace();
+    if (count) *count = t.count;
+    for (int i = 0; i < capacity && i < t.count; ++i) {
+        const harness::AttemptRecord& a = t.records[i];
+        cs_attempt_record& o = out[i];
+        std::memset(&o, 0, sizeof o);
+        o.S = a.knobs.S;
+        o.K = a.knobs.K;
+        o.R = a.knobs.R;
+        o.blend_mode = static_cast<int>(a.knobs.B);
+        o.action = static_cast<int>(a.decision.action);
+        o.reason = static_cast<int>(a.decision.reason);
+        o.next_S = a.decision.next.S;
+        o.next_K = a.decision.next.K;
+        o.next_R = a.decision.next.R;
+        o.next_blend_mode = static_cast<int>(a.decision.next.B);
+        o.previous_R = a.previous_R;
+        o.generation = a.generation;
+        std::memcpy(&o.homography, &a.homography, sizeof o.homography);
+        o.t_s1 = a.t_s1;
+        o.t_s2 = a.t_s2;
+        o.t_s3 = a.t_s3;
+        o.t_s4 = a.t_s4;
+        o.t_s5 = a.t_s5;
+        o.t_attempt = a.t_attempt;
+        std::strncpy(o.restart_from, a.restart_from, sizeof o.restart_from - 1);
+        std::strncpy(o.fail_stage, a.fail_stage, sizeof o.fail_stage - 1);
+        o.execution_status = static_cast<int>(a.execution_status);
+        o.has_decision = a.has_decision ? 1 : 0;
+        o.has_homography = a.has_homography ? 1 : 0;
+    }
+    return CS_OK;
+}
+
 int cs_runner_set_flags(cs_runner* r, int reuse_enabled, int probe_enabled) {
     if (!r) return CS_INVALID_ARGUMENT;
     r->impl.set_reuse_enabled(reuse_enabled != 0);

@@ -322,6 +322,7 @@ void PipelineRunner::execute_adaptive_run(controller::Controller& ctl, std::uint
     *rec = RunRecord{};
     rec->seed = seed;
     rec->config_id = "adaptive_unstarted";
+    attempt_trace_.count = 0;
     const auto t0 = Clock::now();
     auto timed = [&](double& total, auto&& submit) {
         check(hipEventRecord(ev_[0], stream_), "event");
@@ -378,6 +379,20 @@ void PipelineRunner::execute_adaptive_run(controller::Controller& ctl, std::uint
     for (int attempt = 0;; ++attempt) {
         if (attempt >= 16) return finish("ATTEMPT_CAPACITY_EXCEEDED", "controller");
         rec->attempt_count = attempt + 1;
+        AttemptRecord& tr = attempt_trace_.records[attempt];
+        tr = AttemptRecord{};
+        tr.knobs = knobs;
+        attempt_trace_.count = attempt + 1;
+        const auto a0 = Clock::now();
+        const double s1_0 = rec->t_s1, s2_0 = rec->t_s2, s3_0 = rec->t_s3, s4_0 = rec->t_s4, s5_0 = rec->t_s5;
+        const auto close_attempt = [&] {
+            tr.t_s1 = rec->t_s1 - s1_0;
+            tr.t_s2 = rec->t_s2 - s2_0;
+            tr.t_s3 = rec->t_s3 - s3_0;
+            tr.t_s4 = rec->t_s4 - s4_0;
+            tr.t_s5 = rec->t_s5 - s5_0;
+            tr.t_attempt = ms_since(a0);
+        };
         rec->S = knobs.S;
         rec->K = knobs.K;
         rec->R = knobs.R;
@@ -389,6 +404,9 @@ void PipelineRunner::execute_adaptive_run(controller::Controller& ctl, std::uint
             if (controller::plan_retry(previous, knobs, &plan) != Status::OK)
                 return finish("INVALID_RETRY_PLAN", "controller");
             rerun_s1 = plan.restart_from == controller::RestartFrom::S1;
+            tr.restart_from = plan.restart_from == controller::RestartFrom::S1   ? "S1"
+                              : plan.restart_from == controller::RestartFrom::S2 ? "S2"
+                                                                                 : "S5";
             rerun_features = rerun_s1 || plan.restart_from == controller::RestartFrom::S2;
             if (plan.change_generation) {
                 ++generation;
@@ -411,7 +429,12 @@ void PipelineRunner::execute_adaptive_run(controller::Controller& ctl, std::uint
             const Status s = timed(rec->t_s1, [&] {
                 return s1::s1_preprocess(use_a_, use_b_, knobs.S, &gray_a_, &gray_b_, &sa, &sb, stream_);
             });
-            if (s != Status::OK) return finish(status_name(s), "s1");
+            if (s != Status::OK) {
+                tr.execution_status = s;
+                tr.fail_stage = "s1";
+                close_attempt();
+                return finish(status_name(s), "s1");
+            }
         }
         Keypoints ka = kp_a_, kb = kp_b_;
         ka.capacity = kb.capacity = knobs.K;
@@ -421,18 +444,33 @@ void PipelineRunner::execute_adaptive_run(controller::Controller& ctl, std::uint
                 const Status x = s2::s2_detect(gray_a_, c, ka, ws_s2_a_, bytes_s2_, stream_);
                 return x == Status::OK ? s2::s2_detect(gray_b_, c, kb, ws_s2_b_, bytes_s2_, stream_) : x;
             });
-            if (s != Status::OK) return finish(status_name(s), "s2");
+            if (s != Status::OK) {
+                tr.execution_status = s;
+                tr.fail_stage = "s2";
+                close_attempt();
+                return finish(status_name(s), "s2");
+            }
             s = timed(rec->t_s3, [&] {
                 const Status x = s3::s3_describe(gray_a_, ka, desc_a_, ws_s3_, bytes_s3_, stream_);
                 return x == Status::OK ? s3::s3_describe(gray_b_, kb, desc_b_, ws_s3_, bytes_s3_, stream_) : x;
             });
-            if (s != Status::OK) return finish(status_name(s), "s3");
+            if (s != Status::OK) {
+                tr.execution_status = s;
+                tr.fail_stage = "s3";
+                close_attempt();
+                return finish(status_name(s), "s3");
+            }
         }
         if (rerun_s4) {
             const Status s = timed(rec->t_s4, [&] {
                 return s4::s4_match(ka, desc_a_, kb, desc_b_, {0.8, false}, matches_, ws_s4_, bytes_s4_, stream_);
             });
-            if (s != Status::OK) return finish(status_name(s), "s4");
+            if (s != Status::OK) {
+                tr.execution_status = s;
+                tr.fail_stage = "s4";
+                close_attempt();
+                return finish(status_name(s), "s4");
+            }
         }
         *pinned_h_ = HomographyResult{};
         pinned_h_->status = Status::INVALID_ARGUMENT;
@@ -441,12 +479,23 @@ void PipelineRunner::execute_adaptive_run(controller::Controller& ctl, std::uint
             return s5::estimate(ka, kb, matches_, sa, sb, {knobs.R, previous_r, 64, 1, 3.0, seed, generation},
                                 inlier_mask_, pinned_h_, ws_s5_, bytes_s5_, stream_);
         });
+        tr.previous_R = previous_r;
+        tr.generation = generation;
+        tr.execution_status = s5s;
+        if (s5s != Status::OK) tr.fail_stage = "s5";
+        tr.homography = *pinned_h_;
+        tr.has_homography = true;
         if (s5s == Status::OK) completed_r = knobs.R;
         const controller::Feedback fb{knobs, s5s, *pinned_h_};
         controller::Decision d{};
         const auto c0 = Clock::now();
         const Status observed = ctl.observe(fb, &d);
         rec->t_controller += ms_since(c0);
+        close_attempt();
+        if (observed == Status::OK) {
+            tr.decision = d;
+            tr.has_decision = true;
+        }
         if (observed != Status::OK) return finish(status_name(observed), s5s == Status::OK ? "controller" : "s5");
         if (d.action == controller::Action::RETRY) {
             previous = knobs;

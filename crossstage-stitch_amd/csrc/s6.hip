@@ -63,7 +63,8 @@ __device__ bool center_coordinate(double& x, int n, double tol) {
 // Bilinear B sample (FP64, then FP32 / 255); the border score only when asked.
 // The division by 255 is a multiplication by its FP64 reciprocal: the two
 // differ by at most one FP64 ulp, far below the FP32 colour's resolution.
-__device__ bool sample(const std::uint8_t* img, int w, int h, int pitch, double x, double y, float* rgb, double* score,
+// x and y come back snapped, as the border score needs them.
+__device__ bool sample(const std::uint8_t* img, int w, int h, int pitch, double& x, double& y, float* rgb, double* score,
                        double tol) {
     if (!center_coordinate(x, w, tol) || !center_coordinate(y, h, tol)) return false;
     const int x0 = static_cast<int>(floor(x)), y0 = static_cast<int>(floor(y));
@@ -171,8 +172,6 @@ __device__ __forceinline__ void warp_pixel(const S6Job& j, int x, int y, const f
     double sa = 0, sb = 0;
     if (ma && mb) {
         sa = fmin(fmin(xa + 0.5, j.wa - 0.5 - xa) / j.wa, fmin(ya + 0.5, j.ha - 0.5 - ya) / j.ha);
-        center_coordinate(xb, j.wb, j.snap_tolerance);
-        center_coordinate(yb, j.hb, j.snap_tolerance);
         sb = fmin(fmin(xb + 0.5, j.wb - 0.5 - xb) / j.wb, fmin(yb + 0.5, j.hb - 0.5 - yb) / j.hb);
     }
     if (j.top >= 0) {
@@ -624,7 +623,6 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
     const long long total = work.get();
     __shared__ float4 pp[2][kGPH][kGPW];  // pushed level 1, A and B
     __shared__ float4 p2[2][kG2H][kG2W];  // pushed level 2, A and B
-    __shared__ float vv[2][kGPH][kGPW];   // level-1 validity weights
     int ji = 0;
     const int njobs = span.jobs();
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
@@ -650,6 +648,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
             ra[c] = C.a[c * C.n + p];
             rb[c] = C.b[c * C.n + p];
         }
+        const float wa = C.va[p], wb = C.vb[p];
         float sa[kGSIt][4], sb[kGSIt][4];
 #pragma unroll
         for (int it = 0; it < kGSIt; ++it) {
@@ -674,38 +673,45 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
             la.store(&p2[0][0][0]);
             lb.store(&p2[1][0][0]);
         }
-        __syncthreads();
+        // An image's pushed level 1 is needed only if some output of the
+        // tile misses it somewhere in its footprint (weight below 1).
+        const bool need_a = __syncthreads_or(inside && wa != 1.0f);
+        const bool need_b = __syncthreads_or(inside && wb != 1.0f);
         // Normalise the staged pixels, or fill them from the expanded level 2.
 #pragma unroll
         for (int it = 0; it < kGSIt; ++it) {
             const int i = static_cast<int>(threadIdx.x) + it * kBlock;
             if (i >= kGSN) continue;
             const int cx = clampi(x0 - 2 + i % kGPW, 0, C.w - 1), cy = clampi(y0 - 2 + i / kGPW, 0, C.h - 1);
+            const bool fill_a = need_a && !(sa[it][3] > 0), fill_b = need_b && !(sb[it][3] > 0);
             float ea[3] = {0, 0, 0}, eb[3] = {0, 0, 0};
-            if (coarse && (!(sa[it][3] > 0) || !(sb[it][3] > 0))) {
+            if (coarse && (fill_a || fill_b)) {
                 Expand e;
                 expand_setup(cx, cy, j.lv[2].w, j.lv[2].h, x0 / 2 - 2, y0 / 2 - 2, kG2W, kG2H, e);
-                if (!(sa[it][3] > 0)) expand_rgb<kG2W>(&p2[0][0][0], e, ea);
-                if (!(sb[it][3] > 0)) expand_rgb<kG2W>(&p2[1][0][0], e, eb);
+                if (fill_a) expand_rgb<kG2W>(&p2[0][0][0], e, ea);
+                if (fill_b) expand_rgb<kG2W>(&p2[1][0][0], e, eb);
             }
-            float4 va, vb;
-            va.x = sa[it][3] > 0 ? sa[it][0] / sa[it][3] : ea[0];
-            va.y = sa[it][3] > 0 ? sa[it][1] / sa[it][3] : ea[1];
-            va.z = sa[it][3] > 0 ? sa[it][2] / sa[it][3] : ea[2];
-            vb.x = sb[it][3] > 0 ? sb[it][0] / sb[it][3] : eb[0];
-            vb.y = sb[it][3] > 0 ? sb[it][1] / sb[it][3] : eb[1];
-            vb.z = sb[it][3] > 0 ? sb[it][2] / sb[it][3] : eb[2];
-            va.w = vb.w = 0.0f;
-            pp[0][i / kGPW][i % kGPW] = va;
-            pp[1][i / kGPW][i % kGPW] = vb;
-            vv[0][i / kGPW][i % kGPW] = sa[it][3];
-            vv[1][i / kGPW][i % kGPW] = sb[it][3];
+            if (need_a) {
+                float4 v;
+                v.x = sa[it][3] > 0 ? sa[it][0] / sa[it][3] : ea[0];
+                v.y = sa[it][3] > 0 ? sa[it][1] / sa[it][3] : ea[1];
+                v.z = sa[it][3] > 0 ? sa[it][2] / sa[it][3] : ea[2];
+                v.w = 0.0f;
+                pp[0][i / kGPW][i % kGPW] = v;
+            }
+            if (need_b) {
+                float4 v;
+                v.x = sb[it][3] > 0 ? sb[it][0] / sb[it][3] : eb[0];
+                v.y = sb[it][3] > 0 ? sb[it][1] / sb[it][3] : eb[1];
+                v.z = sb[it][3] > 0 ? sb[it][2] / sb[it][3] : eb[2];
+                v.w = 0.0f;
+                pp[1][i / kGPW][i % kGPW] = v;
+            }
         }
-        __syncthreads();
+        if (need_a || need_b) __syncthreads();
         if (inside) {
             // The footprint is all valid where the weight is 1 and all
             // missing where it is 0 (the weights are exact sums of taps).
-            const float wa = vv[0][ly + 2][lx + 2], wb = vv[1][ly + 2][lx + 2];
             float fa[3] = {0, 0, 0}, fb[3] = {0, 0, 0};
             const int ax0 = -j.origin_x, ay0 = -j.origin_y, ax1 = ax0 + j.wa - 1, ay1 = ay0 + j.ha - 1;
             float cx[5], cy[5], dx[5], dy[5];

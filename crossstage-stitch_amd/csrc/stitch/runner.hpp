@@ -55,6 +55,29 @@ struct RunRecord {
     std::string fail_stage{};
 };
 
+// Bounded per-attempt trace of the adaptive run (harness/attempt_trace.hpp:10-31):
+// knobs, the controller's decision, the S5 result and the stage times of each
+// attempt, without allocation in the timed run.
+struct AttemptRecord {
+    controller::Knobs knobs{};
+    controller::Decision decision{};
+    HomographyResult homography{};
+    int previous_R{0};
+    std::uint64_t generation{0};
+    double t_s1{0}, t_s2{0}, t_s3{0}, t_s4{0}, t_s5{0}, t_attempt{0};
+    const char* restart_from{"none"};
+    const char* fail_stage{""};
+    Status execution_status{Status::OK};
+    bool has_decision{false};
+    bool has_homography{false};
+};
+
+struct AttemptTrace {
+    static constexpr int capacity = 16;
+    AttemptRecord records[capacity]{};
+    int count{0};
+};
+
 struct PipelineMemoryPlan {
     std::size_t device_bytes = 0;
     std::size_t workspace_bytes = 0;
@@ -95,8 +118,11 @@ public:
     const std::uint8_t* inlier_mask() const { return inlier_mask_; }
     const HomographyResult& homography() const { return *pinned_h_; }
     double last_gpu_span_ms() const { return last_gpu_span_ms_; }
+    double last_fixed_gpu_span_ms() const { return last_gpu_span_ms_; }
+    const AttemptTrace& attempt_trace() const noexcept { return attempt_trace_; }
 
 private:
+    AttemptTrace attempt_trace_{};
     void check(hipError_t e, const char* what);
     bool reuse_enabled_{true};
     bool probe_enabled_{true};

@@ -170,6 +170,24 @@ int cs_runner_outputs(cs_runner* runner, cs_image* canvas, cs_image* mask, cs_ke
                       cs_keypoints* keypoints_b, cs_descriptors* descriptors_a, cs_descriptors* descriptors_b,
                       cs_matches* matches, uint8_t** inlier_mask, cs_homography_result* homography);
 void* cs_runner_stream(cs_runner* runner);
+/* Per-attempt trace of the last adaptive run (PipelineRunner::attempt_trace,
+ * harness/pipeline_runner.hpp:78 and harness/attempt_trace.hpp:10-31): up to
+ * 16 records; *count = attempts of that run. */
+typedef struct {
+    double S;
+    int K, R, blend_mode;
+    int action, reason;           /* controller decision (ACCEPT 0 / RETRY 1 / REJECT 2) */
+    double next_S;
+    int next_K, next_R, next_blend_mode;
+    int previous_R;               /* hypotheses reused from the previous attempt */
+    uint64_t generation;
+    cs_homography_result homography;
+    double t_s1, t_s2, t_s3, t_s4, t_s5, t_attempt;
+    char restart_from[8];         /* "none", "S1", "S2", "S5" */
+    char fail_stage[8];
+    int execution_status, has_decision, has_homography, pad;
+} cs_attempt_record;
+int cs_runner_attempt_trace(cs_runner* runner, cs_attempt_record* out, int capacity, int* count);
 
 /* Rule-based controller with the built-in legacy-v1 ladder (blind = 1: the
  * probe-free S=0.5 variant), and plan_retry. cs_controller_walk runs

@@ -16,7 +16,7 @@ STRUCTS = {
     "cs_harness_config": "HarnessConfig", "cs_run_record": "RunRecord", "cs_pair_spec": "PairSpec",
     "cs_pair_outcome": "PairOutcome", "cs_batch_timings": "BatchTimings", "cs_tuning": "Tuning",
     "cs_server_config": "ServerConfig", "cs_request": "Request", "cs_completion": "Completion",
-    "cs_serve_stats": "ServeStats", "cs_kernel_probe": "KernelProbe",
+    "cs_serve_stats": "ServeStats", "cs_kernel_probe": "KernelProbe", "cs_attempt_record": "AttemptRecord",
 }
 
 

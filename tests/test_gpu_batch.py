@@ -152,7 +152,7 @@ def test_autotune_picks_and_caches(tmp_path):
     br = B.BatchRunner(max_pairs=4, pool_bytes=4 << 30)
     cache = tmp_path / "tune.json"
     t1 = br.autotune(specs, str(cache))
-    assert t1.s4_segment in (64, 128, 256, 512) and t1.workgroups_per_cu in (4, 8, 16)
+    assert t1.s4_segment in (64, 128, 256, 512) and t1.workgroups_per_cu in (8, 16, 24, 32)
     entries = json.loads(cache.read_text())
     assert len(entries) == 1
     (entry,) = entries.values()

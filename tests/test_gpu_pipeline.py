@@ -57,6 +57,23 @@ def test_adaptive_runner_legacy_policy():
         assert (rec.S, rec.K, rec.R) in ladder
         if status == "OK":
             assert rec.canvas_pixels > 0
+        # The attempt trace walks the same ladder: each attempt's knobs are
+        # the previous decision's next knobs, every attempt has an S5 result
+        # and a decision, only the last one ends the run, and the first
+        # attempt starts from scratch.
+        trace = runner.attempt_trace()
+        assert len(trace) == rec.attempt_count
+        for i, a in enumerate(trace):
+            assert (a.S, a.K, a.R) in ladder and a.has_homography and a.has_decision
+            assert a.t_s5 > 0 and a.t_attempt >= a.t_s5
+            if i + 1 < len(trace):
+                assert a.action == 1 and (a.next_S, a.next_K, a.next_R) == (trace[i + 1].S, trace[i + 1].K,
+                                                                              trace[i + 1].R)
+        assert trace[0].restart_from == b"none" and trace[0].previous_R == 0
+        assert trace[-1].action == (0 if status == "OK" else 2)
+        assert (trace[-1].S, trace[-1].K, trace[-1].R) == (rec.S, rec.K, rec.R)
+        if status == "OK":
+            assert trace[-1].homography.num_inliers == rec.num_inliers
         rec2 = runner.execute_adaptive_run(ctl, seed=0)
         assert rec2.status == rec.status and rec2.attempt_count == rec.attempt_count
         assert list(rec2.H) == list(rec.H) or status != "OK"
