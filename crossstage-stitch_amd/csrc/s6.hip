@@ -60,23 +60,25 @@ __device__ bool center_coordinate(double& x, int n, double tol) {
     return x >= 0 && x <= n - 1.0;
 }
 
-// Bilinear B sample (FP64, then FP32 / 255); the border score only when asked.
-// The division by 255 is a multiplication by its FP64 reciprocal: the two
-// differ by at most one FP64 ulp, far below the FP32 colour's resolution.
+// Bilinear B sample; the border score only when asked. The position, its
+// snapping and the validity are FP64 as in the reference; the blend of the
+// four bytes runs in FP32 with the FP64 fractions rounded to FP32, then
+// scales by 1/255: the colour (FP32 either way) moves by a few ulps, far
+// below what the 8-bit output can show.
 // x and y come back snapped, as the border score needs them.
 __device__ bool sample(const std::uint8_t* img, int w, int h, int pitch, double& x, double& y, float* rgb, double* score,
                        double tol) {
     if (!center_coordinate(x, w, tol) || !center_coordinate(y, h, tol)) return false;
     const int x0 = static_cast<int>(floor(x)), y0 = static_cast<int>(floor(y));
     const int x1 = x0 < w - 1 ? x0 + 1 : x0, y1 = y0 < h - 1 ? y0 + 1 : y0;
-    const double ax = x - x0, ay = y - y0;
+    const float ax = static_cast<float>(x - x0), ay = static_cast<float>(y - y0);
     const std::uint8_t* r0 = img + static_cast<size_t>(y0) * pitch;
     const std::uint8_t* r1 = img + static_cast<size_t>(y1) * pitch;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const double v0 = (1 - ax) * r0[3 * x0 + c] + ax * r0[3 * x1 + c];
-        const double v1 = (1 - ax) * r1[3 * x0 + c] + ax * r1[3 * x1 + c];
-        rgb[c] = static_cast<float>(((1 - ay) * v0 + ay * v1) * (1.0 / 255.0));
+        const float v0 = fmaf(ax, static_cast<float>(r0[3 * x1 + c]), (1 - ax) * static_cast<float>(r0[3 * x0 + c]));
+        const float v1 = fmaf(ax, static_cast<float>(r1[3 * x1 + c]), (1 - ax) * static_cast<float>(r1[3 * x0 + c]));
+        rgb[c] = fmaf(ay, v1, (1 - ay) * v0) * (1.0f / 255.0f);
     }
     // min((x+.5)/w, (w-.5-x)/w) == min(x+.5, w-.5-x)/w exactly: a correctly
     // rounded division by a positive w is monotonic, so it commutes with min.
@@ -612,6 +614,48 @@ __device__ __forceinline__ void stencil_apply(const float4* pp, int lx, int ly, 
     out[2] = s2 - t2;
 }
 
+// B's border crosses the footprint: the taps where B is missing, or the full
+// stencil less the taps where it is valid, whichever visits fewer taps (flags
+// read directly: such pixels only line B's border).
+__device__ __forceinline__ void gauss1_b_border(const float4* pb, int lx, int ly, int x, int y, int x0, int y0,
+                                             const S6Level& F, const S6Level& C, const float (&cx)[5],
+                                             const float (&cy)[5], float fb[3]) {
+    unsigned valid = 0;
+    int nb = 0;
+    for (int ty = 0; ty < 5; ++ty)
+        for (int tx2 = 0; tx2 < 5; ++tx2) {
+            const int X = clampi(reflect101(2 * x + tx2 - 2, F.w), 0, F.w - 1);
+            const int Y = clampi(reflect101(2 * y + ty - 2, F.h), 0, F.h - 1);
+            const bool v = F.flags[static_cast<unsigned>(Y) * static_cast<unsigned>(F.w) + X] & kVB;
+            valid |= (v ? 1u : 0u) << (ty * 5 + tx2);
+            nb += v;
+        }
+    const bool complement = nb < 13;
+    float full[3] = {0, 0, 0};
+    if (complement) {
+        const float none[5] = {0, 0, 0, 0, 0};
+        stencil_apply(pb, lx, ly, cx, cy, none, none, false, full);
+    }
+    float s0 = 0, s1 = 0, s2 = 0;
+    for (int ty = 0; ty < 5; ++ty)
+        for (int tx2 = 0; tx2 < 5; ++tx2) {
+            if ((((valid >> (ty * 5 + tx2)) & 1u) != 0) != complement) continue;
+            const int X = clampi(reflect101(2 * x + tx2 - 2, F.w), 0, F.w - 1);
+            const int Y = clampi(reflect101(2 * y + ty - 2, F.h), 0, F.h - 1);
+            Expand e;
+            expand_setup(X, Y, C.w, C.h, x0 - 2, y0 - 2, kGPW, kGPH, e);
+            float E[3];
+            expand_rgb<kGPW>(pb, e, E);
+            const float w = tap(tx2 - 2) * tap(ty - 2);
+            s0 = fmaf(w, E[0], s0);
+            s1 = fmaf(w, E[1], s1);
+            s2 = fmaf(w, E[2], s2);
+        }
+    fb[0] = complement ? full[0] - s0 : s0;
+    fb[1] = complement ? full[1] - s1 : s1;
+    fb[2] = complement ? full[2] - s2 : s2;
+}
+
 // The level-1 region a Gaussian tile stages (the tile and two pixels around)
 // and the pushed level 2 behind it, from which missing level-1 pixels are
 // filled: push_missing for level 1 (s6.cu:399-410) happens here, on the
@@ -691,19 +735,24 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
                 if (fill_a) expand_rgb<kG2W>(&p2[0][0][0], e, ea);
                 if (fill_b) expand_rgb<kG2W>(&p2[1][0][0], e, eb);
             }
+            // normalize_color divides each channel by the weight; one
+            // correctly rounded reciprocal and three products differ from it
+            // by at most an ulp, far below what the blend can show.
             if (need_a) {
+                const float r = sa[it][3] > 0 ? 1.0f / sa[it][3] : 0.0f;
                 float4 v;
-                v.x = sa[it][3] > 0 ? sa[it][0] / sa[it][3] : ea[0];
-                v.y = sa[it][3] > 0 ? sa[it][1] / sa[it][3] : ea[1];
-                v.z = sa[it][3] > 0 ? sa[it][2] / sa[it][3] : ea[2];
+                v.x = sa[it][3] > 0 ? sa[it][0] * r : ea[0];
+                v.y = sa[it][3] > 0 ? sa[it][1] * r : ea[1];
+                v.z = sa[it][3] > 0 ? sa[it][2] * r : ea[2];
                 v.w = 0.0f;
                 pp[0][i / kGPW][i % kGPW] = v;
             }
             if (need_b) {
+                const float r = sb[it][3] > 0 ? 1.0f / sb[it][3] : 0.0f;
                 float4 v;
-                v.x = sb[it][3] > 0 ? sb[it][0] / sb[it][3] : eb[0];
-                v.y = sb[it][3] > 0 ? sb[it][1] / sb[it][3] : eb[1];
-                v.z = sb[it][3] > 0 ? sb[it][2] / sb[it][3] : eb[2];
+                v.x = sb[it][3] > 0 ? sb[it][0] * r : eb[0];
+                v.y = sb[it][3] > 0 ? sb[it][1] * r : eb[1];
+                v.z = sb[it][3] > 0 ? sb[it][2] * r : eb[2];
                 v.w = 0.0f;
                 pp[1][i / kGPW][i % kGPW] = v;
             }
@@ -733,40 +782,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
             if (wb == 0.0f) {
                 stencil_apply(&pp[1][0][0], lx, ly, cx, cy, dx, dy, false, fb);
             } else if (wb != 1.0f) {
-                // B's border crosses the footprint: the taps where B is
-                // missing, or the full stencil less the taps where it is
-                // valid, whichever visits fewer taps (flags read directly:
-                // such pixels line B's border only).
-                unsigned valid = 0;
-                int nb = 0;
-                for (int ty = 0; ty < 5; ++ty)
-                    for (int tx2 = 0; tx2 < 5; ++tx2) {
-                        const int X = clampi(reflect101(2 * x + tx2 - 2, F.w), 0, F.w - 1);
-                        const int Y = clampi(reflect101(2 * y + ty - 2, F.h), 0, F.h - 1);
-                        const bool v = F.flags[static_cast<unsigned>(Y) * static_cast<unsigned>(F.w) + X] & kVB;
-                        valid |= (v ? 1u : 0u) << (ty * 5 + tx2);
-                        nb += v;
-                    }
-                const bool complement = nb < 13;
-                if (complement) stencil_apply(&pp[1][0][0], lx, ly, cx, cy, dx, dy, false, fb);
-                float s0 = 0, s1 = 0, s2 = 0;
-                for (int ty = 0; ty < 5; ++ty)
-                    for (int tx2 = 0; tx2 < 5; ++tx2) {
-                        if ((((valid >> (ty * 5 + tx2)) & 1u) != 0) != complement) continue;
-                        const int X = clampi(reflect101(2 * x + tx2 - 2, F.w), 0, F.w - 1);
-                        const int Y = clampi(reflect101(2 * y + ty - 2, F.h), 0, F.h - 1);
-                        Expand e;
-                        expand_setup(X, Y, C.w, C.h, x0 - 2, y0 - 2, kGPW, kGPH, e);
-                        float E[3];
-                        expand_rgb<kGPW>(&pp[1][0][0], e, E);
-                        const float w = tap(tx2 - 2) * tap(ty - 2);
-                        s0 = fmaf(w, E[0], s0);
-                        s1 = fmaf(w, E[1], s1);
-                        s2 = fmaf(w, E[2], s2);
-                    }
-                fb[0] = complement ? fb[0] - s0 : s0;
-                fb[1] = complement ? fb[1] - s1 : s1;
-                fb[2] = complement ? fb[2] - s2 : s2;
+                gauss1_b_border(&pp[1][0][0], lx, ly, x, y, x0, y0, F, C, cx, cy, fb);
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
