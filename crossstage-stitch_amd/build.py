@@ -74,7 +74,27 @@ def build(jobs: int = 0, verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_harness(verbose)
     return OUT
+
+
+HARNESS_SRC = PKG / "harness" / "stitch_harness.cpp"
+HARNESS = PKG / "bin" / "stitch_harness"
+
+
+def build_harness(verbose: bool = False) -> Path:
+    """The stitch_harness command-line driver, linked against libcsstitch.so
+    (found next to it through the run path)."""
+    HARNESS.parent.mkdir(exist_ok=True)
+    cmd = [HIPCC, *COMMON, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", str(HARNESS_SRC),
+           "-o", str(HARNESS), f"-L{OUT.parent}", "-lcsstitch", "-Wl,-rpath,$ORIGIN/../cs_stitch",
+           "-L/opt/rocm/lib", "-lamdhip64", "-lpthread"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"harness build failed\n{r.stdout}\n{r.stderr}")
+    return HARNESS
 
 
 if __name__ == "__main__":
