@@ -11,8 +11,11 @@ the inputs resident in HBM before the timed region; `value` is the stream of
 such batches (each batch's S1-S5 overlap the previous batch's S6 on a second
 stream), and "batched" times one batch at a time. The same 64 pairs run
 through the plain per-pair runner (the reference's execute_run flow: one
-pair at a time, per-stage launches, two host synchronizations) for the ratio;
-the 1920x1080 subset is reported on its own. The served form replays Poisson
+pair at a time, per-stage launches, two host synchronizations) for the ratio.
+"c1_1080p" runs the 1080p load the same way: 64 pairs shaped like the
+single-pair 1080p config (1920x1080, 30-50 % overlap, K 5000, cross-check,
+RANSAC 4096, multiband_6), batched and streamed, against the plain runner on
+the same pairs. The served form replays Poisson
 arrival traces (adaptive legacy-v1 controller, 64 GB HBM budget, textured,
 low-texture, low-overlap and repetitive scenes) and reports p50/p99.
 
@@ -61,7 +64,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-subset", action="store_true", help="skip the 1920x1080 subset (profiling runs)")
+    ap.add_argument("--no-subset", action="store_true", help="skip the 64-pair 1920x1080 batch (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU oracle sample")
     ap.add_argument("--tune-cache", default=None)
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes (profiling runs)")
@@ -337,28 +340,41 @@ def main():
     log("stream", stream)
     roof = roofline(probe_kernels(br, specs, args.steps))
     log("roofline", {k: v for k, v in roof.items() if k != "kernels"})
-    # 1920x1080 subset on its own.
-    idx = [i for i, (w, h, _, _) in enumerate(work) if (w, h) == (1920, 1080)]
-    c1 = dict(pairs=len(idx))
+    # The 1080p load: a batch of 64 pairs shaped like the single-pair 1080p
+    # config (1920x1080, 30-50 % overlap, K 5000, cross-check, RANSAC 4096,
+    # multiband_6), against the plain runner on the same pairs.
+    c1 = None
     if not args.no_subset:
-        sub_specs = [specs[i] for i in idx]
-        br.run(sub_specs)
-        t_sub, times_sub, _, _ = time_batched(br, sub_specs, args.steps, 0)
-        t_ss, _ = time_stream(br, sub_specs, args.steps, 1)
-        c1.update(stream_pairs_per_s=len(idx) * args.steps / t_ss, batched_pairs_per_s=len(idx) * args.steps / t_sub,
-                  batched_p50_ms=1e3 * float(np.percentile(times_sub, 50)))
+        rng = np.random.default_rng(SEED0 + 2000)
+        work1 = [(1920, 1080, 5000, float(rng.uniform(0.3, 0.5))) for _ in range(args.pairs)]
+        pairs1 = [synth(SEED0 + 2000 + i, w, h, ov) for i, (w, h, _, ov) in enumerate(work1)]
+        dev1 = [upload(a, b) for a, b, _ in pairs1]
+        del pairs1
+        torch.cuda.synchronize()
+        specs1 = specs_for(dev1, work1)
+        out1 = br.run(specs1)[0]
+        t_b1, times_b1, _, _ = time_batched(br, specs1, args.steps, args.warmup)
+        t_s1, last1 = time_stream(br, specs1, args.steps, args.warmup)
+        c1 = dict(workload=f"{args.pairs} pairs of 1920x1080, overlap U[0.3,0.5], K 5000, cross-check, RANSAC {R}, "
+                           f"{BLEND}", stream_pairs_per_s=args.pairs * args.steps / t_s1,
+                  batched_pairs_per_s=args.pairs * args.steps / t_b1,
+                  batched_p50_ms=1e3 * float(np.percentile(times_b1, 50)), ok=sum(o.status == 0 for o in last1))
+        log("c1_1080p", c1)
     result = dict(stream=stream, batched=batched, c1_1080p=c1)
     if not args.no_plain:
         runners, assign = plain_runners(dev, work, outcomes)
         plain = time_plain(dev, work, runners, assign)
         log("plain", plain)
-        plain_sub = time_plain([dev[i] for i in idx], [work[i] for i in idx], runners, [assign[i] for i in idx])
-        c1["plain_pairs_per_s"] = plain_sub["pairs_per_s"]
-        c1["plain_p50_ms"] = plain_sub["p50_ms"]
-        if "stream_pairs_per_s" in c1:
-            c1["speedup"] = c1["stream_pairs_per_s"] / plain_sub["pairs_per_s"]
         for r in runners.values():
             r.close()
+        if c1 is not None:
+            runners1, assign1 = plain_runners(dev1, work1, out1)
+            plain1 = time_plain(dev1, work1, runners1, assign1)
+            log("plain 1080p", plain1)
+            for r in runners1.values():
+                r.close()
+            c1.update(plain_pairs_per_s=plain1["pairs_per_s"], plain_p50_ms=plain1["p50_ms"],
+                      plain_p99_ms=plain1["p99_ms"], speedup=c1["stream_pairs_per_s"] / plain1["pairs_per_s"])
         result["plain"] = plain
         result["speedup_vs_plain"] = stream["pairs_per_s"] / plain["pairs_per_s"]
         result["batched_speedup_vs_plain"] = batched["pairs_per_s"] / plain["pairs_per_s"]
