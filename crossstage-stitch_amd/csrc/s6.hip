@@ -392,8 +392,11 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
         const S6Level& C = j.lv[l];
         const S6Level& F = j.lv[l - 1];
         const unsigned local = static_cast<unsigned>(t - (kMode != 1 ? j.rb[l] : j.gb[l]));
-        const unsigned tx = (static_cast<unsigned>(C.w) + kRW - 1) / kRW;
-        const int x0 = static_cast<int>(local % tx) * kRW, y0 = static_cast<int>(local / tx) * kRH;
+        // Pulls cover the level; the Gaussian its dependency box.
+        const int bx = kMode == 1 ? j.gbox[l][0] : 0, by = kMode == 1 ? j.gbox[l][1] : 0;
+        const int bw = kMode == 1 ? j.gbox[l][2] - bx + 1 : C.w;
+        const unsigned tx = (static_cast<unsigned>(bw) + kRW - 1) / kRW;
+        const int x0 = bx + static_cast<int>(local % tx) * kRW, y0 = by + static_cast<int>(local / tx) * kRH;
         const int ix0 = 2 * x0 - 2, iy0 = 2 * y0 - 2;
         // Source planes: A, B colour and the two weights (level 0: B only).
         const float* planes[kCh];
@@ -675,8 +678,9 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
         const S6Level& C = j.lv[1];
         const S6Level& F = j.lv[0];
         const unsigned local = static_cast<unsigned>(t - j.gb[1]);
-        const unsigned tx = (static_cast<unsigned>(C.w) + kGW - 1) / kGW;
-        const int x0 = static_cast<int>(local % tx) * kGW, y0 = static_cast<int>(local / tx) * kGH;
+        const int bx = j.gbox[1][0], by = j.gbox[1][1];
+        const unsigned tx = (static_cast<unsigned>(j.gbox[1][2] - bx + 1) + kGW - 1) / kGW;
+        const int x0 = bx + static_cast<int>(local % tx) * kGW, y0 = by + static_cast<int>(local / tx) * kGH;
         const int lx = threadIdx.x % kGW, ly = threadIdx.x / kGW;
         const int x = x0 + lx, y = y0 + ly;
         const bool inside = x < C.w && y < C.h;
@@ -813,8 +817,9 @@ __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev:
         const S6Job& j = dev::job_at(span, ji);
         const S6Level& L = j.lv[l];
         const unsigned local = static_cast<unsigned>(t - j.pb[l]);
-        const unsigned tx = (static_cast<unsigned>(L.w) + kEW - 1) / kEW;
-        const int x0 = static_cast<int>(local % tx) * kEW, y0 = static_cast<int>(local / tx) * kEH;
+        const int bx = j.pbox[l][0], by = j.pbox[l][1];
+        const unsigned tx = (static_cast<unsigned>(j.pbox[l][2] - bx + 1) + kEW - 1) / kEW;
+        const int x0 = bx + static_cast<int>(local % tx) * kEW, y0 = by + static_cast<int>(local / tx) * kEH;
         const bool coarse = l < j.top;
         const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 1;
         const int x = x0 + static_cast<int>(threadIdx.x) % kEW;
@@ -884,8 +889,8 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
         const S6Job& j = dev::job_at(span, ji);
         const S6Level& L = j.lv[l];
         const unsigned local = static_cast<unsigned>(t - j.mb[l]);
-        const int bx = l == 0 ? j.ox0 : 0, by = l == 0 ? j.oy0 : 0;
-        const int bw = l == 0 ? j.ox1 - j.ox0 + 1 : L.w;
+        const int bx = l == 0 ? j.ox0 : j.zbox[l][0], by = l == 0 ? j.oy0 : j.zbox[l][1];
+        const int bw = l == 0 ? j.ox1 - j.ox0 + 1 : j.zbox[l][2] - j.zbox[l][0] + 1;
         const unsigned tx = (static_cast<unsigned>(bw) + kEW - 1) / kEW;
         const int x0 = bx + static_cast<int>(local % tx) * kEW, y0 = by + static_cast<int>(local / tx) * kEH;
         const bool coarse = l < j.blend_top;
@@ -1180,6 +1185,100 @@ void b_box(const s6::Geometry& g, int box[4]) {
 
 }  // namespace
 
+namespace {
+
+struct Box {
+    int x0, y0, x1, y1;
+    bool empty() const { return x1 < x0 || y1 < y0; }
+};
+
+constexpr Box kEmpty{0, 0, -1, -1};
+
+Box clip(Box b, int w, int h) {
+    if (b.empty()) return kEmpty;
+    b.x0 = std::max(b.x0, 0);
+    b.y0 = std::max(b.y0, 0);
+    b.x1 = std::min(b.x1, w - 1);
+    b.y1 = std::min(b.y1, h - 1);
+    return b.empty() ? kEmpty : b;
+}
+
+Box unite(const Box& a, const Box& b) {
+    if (a.empty()) return b;
+    if (b.empty()) return a;
+    return {std::min(a.x0, b.x0), std::min(a.y0, b.y0), std::max(a.x1, b.x1), std::max(a.y1, b.y1)};
+}
+
+Box grow(const Box& b, int m, int w, int h) {
+    return b.empty() ? kEmpty : clip({b.x0 - m, b.y0 - m, b.x1 + m, b.y1 + m}, w, h);
+}
+
+// Coarse pixels an expansion reads for fine pixels in b: (q + d) >> 1 for
+// |d| <= 2; reflected indices land inside the same span at the borders.
+Box expand_need(const Box& b, int cw, int ch) {
+    if (b.empty()) return kEmpty;
+    return clip({(b.x0 - 2) >> 1, (b.y0 - 2) >> 1, (b.x1 + 2) >> 1, (b.y1 + 2) >> 1}, cw, ch);
+}
+
+// Fine pixels a 5x5 reduce reads for coarse pixels in b: 2q + d, |d| <= 2.
+Box reduce_need(const Box& b, int fw, int fh) {
+    if (b.empty()) return kEmpty;
+    return clip({2 * b.x0 - 2, 2 * b.y0 - 2, 2 * b.x1 + 2, 2 * b.y1 + 2}, fw, fh);
+}
+
+void put(int (&dst)[4], const Box& b, int tw, int th) {
+    if (b.empty()) {
+        dst[0] = dst[1] = 0;
+        dst[2] = dst[3] = -1;
+        return;
+    }
+    dst[0] = b.x0 - b.x0 % tw;
+    dst[1] = b.y0 - b.y0 % th;
+    dst[2] = b.x1;
+    dst[3] = b.y1;
+}
+
+}  // namespace
+
+// Only overlap pixels take the band mix at level 0; every other pixel keeps
+// its warped colour. Working back from the overlap box: the band mix at
+// level l reads the Gaussians and the result of level l + 1 through an
+// expansion (z regions), the Gaussian at level l + 1 reads level l through a
+// reduce (g regions grow by the reduce footprint), the Gaussian into level 1
+// fills its footprint from the pushed level 1 two pixels around it, which it
+// pushes itself from the pushed level 2, and the push at each level reads the
+// next one through an expansion (p regions). Everything outside these boxes
+// is never read by anything that reaches the output, so those kernels skip
+// it; the pulls and the warp still cover whole levels.
+static void s6_regions(S6Job* job) {
+    S6Job& j = *job;
+    for (int l = 0; l < kS6MaxLevels; ++l) {
+        put(j.zbox[l], kEmpty, 1, 1);
+        put(j.gbox[l], kEmpty, 1, 1);
+        put(j.pbox[l], kEmpty, 1, 1);
+    }
+    if (j.top < 1 || j.blend_top < 1 || j.ox1 < j.ox0) return;
+    const auto W = [&](int l) { return j.lv[l].w; };
+    const auto Hh = [&](int l) { return j.lv[l].h; };
+    Box z[kS6MaxLevels], g[kS6MaxLevels], p[kS6MaxLevels];
+    z[0] = clip({j.ox0, j.oy0, j.ox1, j.oy1}, W(0), Hh(0));
+    for (int l = 1; l <= j.blend_top; ++l) z[l] = expand_need(z[l - 1], W(l), Hh(l));
+    g[j.blend_top] = z[j.blend_top];
+    for (int l = j.blend_top - 1; l >= 1; --l) g[l] = unite(z[l], reduce_need(g[l + 1], W(l), Hh(l)));
+    for (int l = 0; l <= j.top; ++l) p[l] = kEmpty;
+    if (j.top >= 2) {
+        p[2] = expand_need(grow(g[1], 2, W(1), Hh(1)), W(2), Hh(2));
+        for (int l = 3; l <= j.top; ++l) p[l] = expand_need(p[l - 1], W(l), Hh(l));
+    }
+    for (int l = 1; l <= j.top; ++l) {
+        if (l <= j.blend_top) {
+            put(j.zbox[l], z[l], kEW, kEH);
+            put(j.gbox[l], g[l], kRW, kRH);
+        }
+        if (l >= 2) put(j.pbox[l], p[l], kEW, kEH);
+    }
+}
+
 Status make_s6_job(const ImageView& a, const ImageView& b, const s6::Geometry& g, s6::BlendMode mode,
                    const ImageView& output, const ImageView& mask, void* ws, std::size_t available,
                    s6::RenderStatistics* stats, S6Job* job) {
@@ -1290,6 +1389,7 @@ Status make_s6_job(const ImageView& a, const ImageView& b, const s6::Geometry& g
             }
         }
     }
+    s6_regions(&j);
     *job = j;
     return Status::OK;
 }
@@ -1310,17 +1410,21 @@ S6Totals s6_prefix(S6Job* js, int n) {
             if (j.top < l) continue;
             const long long w = j.lv[l].w, h = j.lv[l].h;
             const long long rtiles = ((w + kRW - 1) / kRW) * ((h + kRH - 1) / kRH);
-            const long long etiles = ((w + kEW - 1) / kEW) * ((h + kEH - 1) / kEH);
-            // Level 0 of the band mix covers only the overlap box.
+            const auto in_box = [](const int (&b)[4], int tw, int th) -> long long {
+                return b[2] < b[0] || b[3] < b[1] ? 0
+                                                  : static_cast<long long>((b[2] - b[0] + tw) / tw) * ((b[3] - b[1] + th) / th);
+            };
+            // Level 0 of the band mix covers only the overlap box; the Gaussian,
+            // the push and the band mix above it cover their dependency regions.
             const long long otiles = j.ox1 < j.ox0 ? 0
                                                    : ((j.ox1 - j.ox0 + kEW) / kEW) * ((j.oy1 - j.oy0 + kEH) / kEH);
             if (l >= 1) {
                 t.pull[l] += rtiles;
-                t.push[l] += etiles;
+                if (l >= 2) t.push[l] += in_box(j.pbox[l], kEW, kEH);
             }
             if (j.blend_top >= l) {
-                if (l >= 1) t.gauss[l] += rtiles;
-                t.mix[l] += l == 0 ? otiles : etiles;
+                if (l >= 1) t.gauss[l] += in_box(j.gbox[l], kRW, kRH);
+                t.mix[l] += l == 0 ? otiles : in_box(j.zbox[l], kEW, kEH);
             }
         }
         t.top = std::max(t.top, j.top);
@@ -1340,6 +1444,11 @@ double s6_algorithmic_bytes(int kernel, int l, const S6Job& j, const s6::RenderS
     const double nA = static_cast<double>(j.wa) * j.ha, SB = static_cast<double>(j.wb) * j.hb;
     const double U = static_cast<double>(st.union_pixels), O = static_cast<double>(st.overlap_pixels);
     const double nB = U - nA + O;
+    // Pixels of a region box within its level.
+    const auto area = [&](const int (&b)[4], int lv) {
+        const int x1 = std::min(b[2], j.lv[lv].w - 1), y1 = std::min(b[3], j.lv[lv].h - 1);
+        return x1 < b[0] || y1 < b[1] ? 0.0 : static_cast<double>(x1 - b[0] + 1) * (y1 - b[1] + 1);
+    };
     switch (kernel) {
         case LaunchProbe::kClear:
             return sizeof(s6::RenderStatistics);
@@ -1351,22 +1460,25 @@ double s6_algorithmic_bytes(int kernel, int l, const S6Job& j, const s6::RenderS
         case LaunchProbe::kPull:
         case LaunchProbe::kGauss: {
             if (l > (kernel == LaunchProbe::kPull ? j.top : j.blend_top)) return 0;
-            if (l >= 2) return 32 * n(l - 1) + 32 * n(l);  // two colours and two weights
+            // Two colours and two weights per pixel, in and out; the pulls
+            // cover whole levels, the Gaussians their regions (reading four
+            // finer pixels per pixel).
+            if (l >= 2) return kernel == LaunchProbe::kPull ? 32 * n(l - 1) + 32 * n(l) : 5 * 32 * area(j.gbox[l], l);
             // Pull from level 0: flags, A's bytes and B's colour in; two raw
             // colours and four weights out. Gaussian into level 1: the raw
             // sums and validity weights (pushed on the fly) and the pushed
-            // level 2 in, two colours out.
+            // level 2 in, two colours out, over its region.
             if (kernel == LaunchProbe::kPull) return P + 3 * nA + 12 * nB + 40 * n(1);
-            return 32 * n(1) + (j.top >= 2 ? 24 * n(2) : 0) + 24 * n(1);
+            return (32 + (j.top >= 2 ? 6 : 0) + 24) * area(j.gbox[1], 1);
         }
         case LaunchProbe::kPush:
-            if (l > j.top) return 0;
-            return 56 * n(l) + (l < j.top ? 24 * n(l + 1) : 0);
+            if (l > j.top || l < 2) return 0;
+            return (56 + (l < j.top ? 6 : 0)) * area(j.pbox[l], l);
         case LaunchProbe::kMix:
             if (l > j.blend_top) return 0;
             // Gaussians and weights in, the result out (two planes at level 1),
-            // the coarser Gaussians and result staged.
-            if (l >= 1) return (l == 1 ? 56 : 44) * n(l) + (l < j.blend_top ? 36 * n(l + 1) : 0);
+            // the coarser Gaussians and result staged, over the region.
+            if (l >= 1) return ((l == 1 ? 56 : 44) + (l < j.blend_top ? 9 : 0)) * area(j.zbox[l], l);
             {
                 // Flags over the overlap box; A's bytes, B's colour, the two
                 // level-1 difference planes under the overlap and the output.
