@@ -429,14 +429,18 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
             sys[e] = clampi(reflect101(iy0 + i / kRIW, F.h), 0, F.h - 1);
             const unsigned q = static_cast<unsigned>(sys[e]) * static_cast<unsigned>(F.w) + static_cast<unsigned>(sxs[e]);
             if (kFromZero) {
+                // A's bytes only on A's rectangle and B's colour only inside
+                // B's box: both are known from the position alone, so the
+                // loads still go out together; elsewhere the image is invalid.
                 fls[e] = F.flags[q];
-                const int ax = clampi(sxs[e] + j.origin_x, 0, j.wa - 1);
-                const int ay = clampi(sys[e] + j.origin_y, 0, j.ha - 1);
-                const std::uint8_t* src = j.a + static_cast<size_t>(ay) * j.pitch_a + 3 * ax;
+                const int ax = sxs[e] + j.origin_x, ay = sys[e] + j.origin_y;
+                const bool in_a = ax >= 0 && ax < j.wa && ay >= 0 && ay < j.ha;
+                const bool in_b = sxs[e] >= j.bx0 && sxs[e] <= j.bx1 && sys[e] >= j.by0 && sys[e] <= j.by1;
+                const std::uint8_t* src = j.a + static_cast<size_t>(in_a ? ay : 0) * j.pitch_a + 3 * (in_a ? ax : 0);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    abytes[e][c] = src[c];
-                    raw[e][3 + c] = planes[3 + c][q];
+                    abytes[e][c] = in_a ? src[c] : 0u;
+                    raw[e][3 + c] = in_b ? planes[3 + c][q] : 0.0f;
                 }
             } else {
 #pragma unroll
@@ -506,8 +510,12 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                     }
                 C.va[p] = static_cast<float>(va) * (1.0f / 256.0f);
                 C.vb[p] = static_cast<float>(vb) * (1.0f / 256.0f);
-                C.ua[p] = static_cast<float>(ua) * (1.0f / 256.0f);
-                C.ub[p] = static_cast<float>(ub) * (1.0f / 256.0f);
+                // Ownership weights are read only inside the level-1 Gaussian
+                // region (by the band mix and the Gaussian into level 2).
+                if (x >= j.gbox[1][0] && x <= j.gbox[1][2] && y >= j.gbox[1][1] && y <= j.gbox[1][3]) {
+                    C.ua[p] = static_cast<float>(ua) * (1.0f / 256.0f);
+                    C.ub[p] = static_cast<float>(ub) * (1.0f / 256.0f);
+                }
             } else {
                 (kMode == 0 ? C.va : C.ua)[p] = acc[6];
                 (kMode == 0 ? C.vb : C.ub)[p] = acc[7];
