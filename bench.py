@@ -49,7 +49,7 @@ POOL_BYTES = 48 << 30
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 # Measured HBM bytes per launch of the dominant kernel on this workload
 # (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration).
-TRAFFIC_PROFILE = ROOT / "profiles" / "s6_reduce_gauss_l1_mixed64_traffic.md"
+TRAFFIC_PROFILE = ROOT / "profiles" / "s6_warp_mixed64_traffic.md"
 COPY_CALIBRATION_BYTES = 1 << 30
 SERVE_BUDGET = 64 << 30
 
