@@ -343,7 +343,7 @@ void FeatureEngine::launch(bool replay) {
         cap.n_copy = 4 * max_tasks_;
         cap.t_s1 = cap.t_grad = kReplayTiles;
         cap.t2 = {kReplayTiles, kReplayTiles, kReplayTiles, kReplayTiles, kReplayTiles};
-        cap.t3 = {kReplayTiles, kReplayTiles};
+        cap.t3 = {kReplayTiles, kReplayTiles, kReplayTiles};
         cap.t4 = {kReplayTiles, kReplayTiles, kReplayTiles, kReplayTiles, kReplayTiles, kReplayTiles};
         cap.t5 = {kReplayTiles, kReplayTiles};
     }

@@ -6,6 +6,7 @@
 // Semantics: src/s2_s3/s3.cu:26-196 and tools/s2_s3/reference.py:152-284.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <climits>
 
 #include "orb_pairs.h"
@@ -16,6 +17,7 @@ namespace {
 
 using jobs::S3Job;
 using jobs::kS3KeypointsPerBlock;
+using jobs::kS3MarkBlock;
 using jobs::kS3TileH;
 using jobs::kS3TileW;
 
@@ -36,8 +38,43 @@ __device__ int reflect101(int i, int n) {
     return i;
 }
 
+// Rounded keypoint position; false when the keypoint lies within kEdge of a
+// border, where its descriptor row is zero and nothing is sampled.
+__device__ __forceinline__ bool keypoint_centre(const S3Job& j, int index, int& x, int& y) {
+    x = static_cast<int>(rintf(j.kx[index]));
+    y = static_cast<int>(rintf(j.ky[index]));
+    return x >= kEdge && x < j.w - kEdge && y >= kEdge && y < j.h - kEdge;
+}
+
+// Farthest blurred pixel a descriptor reads from its centre, per axis: the
+// pattern's points lie within radius 13*sqrt(2) = 18.38 (the corner
+// (-13, -13)), so a rotated offset rint(p*cos - q*sin) is at most 18.
+constexpr int kSampleReach = 18;
+static_assert(kEdge > kSampleReach, "samples stay inside the image");
+
+// Marks every blur tile that some descriptor of the job samples: the tiles
+// meeting the (2*18+1)^2 square around each keypoint kept by describe.
+__global__ void __launch_bounds__(kS3MarkBlock) s3_mark(dev::JobSpan<S3Job> span, dev::Total work) {
+    const long long total = work.get();
+    int ji = 0;
+    const int njobs = span.jobs();
+    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+        ji = dev::next_job(span, t, &S3Job::mark_begin, ji, njobs);
+        const S3Job& j = dev::job_at(span, ji);
+        const int index = static_cast<int>(t - j.mark_begin) * kS3MarkBlock + static_cast<int>(threadIdx.x);
+        int x, y;
+        if (index >= j.capacity || index >= *j.d_count || !keypoint_centre(j, index, x, y)) continue;
+        const int tx0 = (x - kSampleReach) / kS3TileW, tx1 = (x + kSampleReach) / kS3TileW;
+        const int ty0 = (y - kSampleReach) / kS3TileH, ty1 = (y + kSampleReach) / kS3TileH;
+        for (int ty = ty0; ty <= ty1; ++ty)
+            for (int tx = tx0; tx <= tx1; ++tx) j.marks[ty * j.tiles_x + tx] = j.epoch;
+    }
+}
+
 // Row pass then column pass, taps accumulated 0..6, each product and sum
-// rounded separately; round half to even, clamp to [0, 255].
+// rounded separately; round half to even, clamp to [0, 255]. Only the tiles
+// marked for this job are computed: the rest of the blurred image is never
+// read.
 __global__ void __launch_bounds__(256) s3_blur(dev::JobSpan<S3Job> span, dev::Total work) {
     const long long total = work.get();
     __shared__ float in[kInH * kInW];
@@ -49,6 +86,7 @@ __global__ void __launch_bounds__(256) s3_blur(dev::JobSpan<S3Job> span, dev::To
         ji = dev::next_job(span, t, &S3Job::blur_begin, ji, njobs);
         const S3Job& j = dev::job_at(span, ji);
         const long long local = t - j.blur_begin;
+        if (__builtin_amdgcn_readfirstlane(j.marks[local]) != j.epoch) continue;
         const int x0 = static_cast<int>(local % j.tiles_x) * kS3TileW;
         const int y0 = static_cast<int>(local / j.tiles_x) * kS3TileH;
         if (((reinterpret_cast<std::uintptr_t>(j.img) | static_cast<std::uintptr_t>(j.pitch)) & 3u) == 0) {
@@ -141,9 +179,8 @@ __global__ void __launch_bounds__(64 * kS3KeypointsPerBlock) s3_describe(dev::Jo
         const int index = static_cast<int>(t - j.kp_begin) * kS3KeypointsPerBlock + wave;
         if (index >= j.capacity || index >= *j.d_count) continue;
         std::uint8_t* row = j.desc + static_cast<size_t>(index) * j.desc_stride;
-        const int x = static_cast<int>(rintf(j.kx[index]));
-        const int y = static_cast<int>(rintf(j.ky[index]));
-        if (x < kEdge || x >= j.w - kEdge || y < kEdge || y >= j.h - kEdge) {
+        int x, y;
+        if (!keypoint_centre(j, index, x, y)) {
             if (lane < 32) row[lane] = 0;
             continue;
         }
@@ -184,6 +221,13 @@ __global__ void __launch_bounds__(64 * kS3KeypointsPerBlock) s3_describe(dev::Jo
 
 std::size_t align256(std::size_t v) { return (v + 255) / 256 * 256; }
 
+// Workspace: the blurred image, then one mark word per blur tile.
+std::size_t blurred_bytes(int w, int h) { return align256(static_cast<std::size_t>(w) * h); }
+std::size_t mark_bytes(int w, int h) {
+    return sizeof(unsigned) * static_cast<std::size_t>((w + kS3TileW - 1) / kS3TileW) *
+           static_cast<std::size_t>((h + kS3TileH - 1) / kS3TileH);
+}
+
 }  // namespace
 
 namespace jobs {
@@ -214,15 +258,20 @@ Status make_s3_job(const ImageView& image, const Keypoints& kp, const Descriptor
     j.desc_stride = out.row_stride_bytes;
     j.tiles_x = (j.w + kS3TileW - 1) / kS3TileW;
     j.tiles = kp.capacity > 0 ? j.tiles_x * ((j.h + kS3TileH - 1) / kS3TileH) : 0;
+    j.marks = reinterpret_cast<unsigned*>(static_cast<std::uint8_t*>(ws) + blurred_bytes(j.w, j.h));
+    static std::atomic<unsigned> next_epoch{1};
+    j.epoch = next_epoch.fetch_add(1, std::memory_order_relaxed);
     *job = j;
     return Status::OK;
 }
 
 S3Totals s3_prefix(S3Job* js, int n) {
-    S3Totals t{0, 0};
+    S3Totals t{0, 0, 0};
     for (int i = 0; i < n; ++i) {
+        js[i].mark_begin = t.mark;
         js[i].blur_begin = t.blur;
         js[i].kp_begin = t.kp;
+        t.mark += (js[i].capacity + kS3MarkBlock - 1) / kS3MarkBlock;
         t.blur += js[i].tiles;
         t.kp += (js[i].capacity + kS3KeypointsPerBlock - 1) / kS3KeypointsPerBlock;
     }
@@ -235,6 +284,8 @@ hipError_t launch_s3(const S3Job* table, int count, const S3Totals& t, const S3J
     const auto total = [&](long long S3Totals::*m) {
         return dev::Total(t.*m, replay ? &(replay->totals->*m) : nullptr);
     };
+    if (t.mark > 0)
+        s3_mark<<<persistent_grid(t.mark), kS3MarkBlock, 0, stream>>>(span, total(&S3Totals::mark));
     if (t.blur > 0) s3_blur<<<persistent_grid(t.blur), 256, 0, stream>>>(span, total(&S3Totals::blur));
     if (t.kp > 0)
         s3_describe<<<persistent_grid(t.kp, 16), 64 * kS3KeypointsPerBlock, 0, stream>>>(span, total(&S3Totals::kp));
@@ -249,7 +300,7 @@ Status s3_workspace_bytes(int width, int height, std::size_t* bytes) {
     if (!bytes) return Status::INVALID_ARGUMENT;
     *bytes = 0;
     if (width <= 0 || height <= 0 || width > INT_MAX / height) return Status::INVALID_ARGUMENT;
-    *bytes = align256(static_cast<std::size_t>(width) * height);
+    *bytes = blurred_bytes(width, height) + align256(mark_bytes(width, height));
     return Status::OK;
 }
 

@@ -234,7 +234,8 @@ __global__ void __launch_bounds__(kBlock) s6_warp(dev::JobSpan<S6Job> span, dev:
     int cur = -1, ji = 0;
     const int njobs = span.jobs();
     unsigned long long uni = 0, ovl = 0;
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    for (dev::TileWalk w = dev::tile_walk(total); w.t < w.end; w.t += w.step) {
+        const long long t = w.t;
         ji = next_job_by(span, t, [](const S6Job& j) -> const long long& { return j.warp_begin; }, ji, njobs);
         if (ji != cur) {
             if (cur >= 0 && dev::job_at(span, cur).stats) flush_stats(dev::job_at(span, cur), uni, ovl, red);
@@ -385,7 +386,8 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
     }
     int ji = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    for (dev::TileWalk w = dev::tile_walk(total); w.t < w.end; w.t += w.step) {
+        const long long t = w.t;
         ji = kMode != 1 ? next_job_by(span, t, [l](const S6Job& j) -> const long long& { return j.rb[l]; }, ji, njobs)
                         : next_job_by(span, t, [l](const S6Job& j) -> const long long& { return j.gb[l]; }, ji, njobs);
         const S6Job& j = dev::job_at(span, ji);
@@ -680,7 +682,8 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
     __shared__ float4 p2[2][kG2H][kG2W];  // pushed level 2, A and B
     int ji = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    for (dev::TileWalk w = dev::tile_walk(total); w.t < w.end; w.t += w.step) {
+        const long long t = w.t;
         ji = next_job_by(span, t, [](const S6Job& j) -> const long long& { return j.gb[1]; }, ji, njobs);
         const S6Job& j = dev::job_at(span, ji);
         const S6Level& C = j.lv[1];
@@ -820,7 +823,8 @@ __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev:
     __shared__ float4 cc[2][kCH][kCW];
     int ji = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    for (dev::TileWalk w = dev::tile_walk(total); w.t < w.end; w.t += w.step) {
+        const long long t = w.t;
         ji = next_job_by(span, t, [l](const S6Job& j) -> const long long& { return j.pb[l]; }, ji, njobs);
         const S6Job& j = dev::job_at(span, ji);
         const S6Level& L = j.lv[l];
@@ -892,7 +896,8 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
     }
     int ji = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    for (dev::TileWalk w = dev::tile_walk(total); w.t < w.end; w.t += w.step) {
+        const long long t = w.t;
         ji = next_job_by(span, t, [l](const S6Job& j) -> const long long& { return j.mb[l]; }, ji, njobs);
         const S6Job& j = dev::job_at(span, ji);
         const S6Level& L = j.lv[l];

@@ -88,6 +88,29 @@ __device__ inline int next_job(const JobSpan<Job>& span, long long t, long long 
     return ji;
 }
 
+// Tile walk of a persistent grid, XCD by XCD. The dispatcher hands
+// workgroups to the chip's 8 XCDs in turn (workgroup g to XCD (g + c) % 8),
+// and each XCD has its own L2. A plain grid-stride walk puts neighbouring
+// tiles, which share halo rows and source lines, on different XCDs, so each
+// L2 fetches those lines again. Here the workgroups that share g % 8 take
+// one contiguous eighth of the tile range between them, striding by
+// gridDim.x / 8 through it: neighbours stay in one L2. Every workgroup still
+// visits its tiles in increasing order (next_job relies on that), and the
+// tiles visited are the same set, each once. A grid that is not a multiple
+// of 8 falls back to the plain walk.
+struct TileWalk {
+    long long t, end;
+    int step;
+};
+
+__device__ inline TileWalk tile_walk(long long total) {
+    constexpr int kXcds = 8;
+    const int g = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
+    if (g % kXcds != 0) return {b, total, g};
+    const int x = b % kXcds;
+    return {total * x / kXcds + b / kXcds, total * (x + 1) / kXcds, g / kXcds};
+}
+
 __device__ inline int lane_id() { return __lane_id(); }
 
 template <class T>

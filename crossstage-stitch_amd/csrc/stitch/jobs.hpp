@@ -102,15 +102,21 @@ struct S3Job {
     std::uint8_t* desc;
     int desc_stride;
     int tiles_x, tiles;
-    long long blur_begin, kp_begin;
+    // Blur tiles that a descriptor samples hold `epoch` in `marks` (one word
+    // per tile); the blur skips the others. Each job takes a fresh epoch, so
+    // marks left by earlier launches never match.
+    unsigned* marks;
+    unsigned epoch;
+    long long mark_begin, blur_begin, kp_begin;
 };
 constexpr int kS3TileW = 64, kS3TileH = 16;
 constexpr int kS3KeypointsPerBlock = 4;  // one wave per keypoint
+constexpr int kS3MarkBlock = 256;        // keypoints per marking workgroup
 
 Status make_s3_job(const ImageView& image, const Keypoints& kp, const Descriptors& out, void* ws,
                    std::size_t ws_bytes, S3Job* job);
 struct S3Totals {
-    long long blur, kp;
+    long long mark, blur, kp;
 };
 S3Totals s3_prefix(S3Job* jobs, int n);
 hipError_t launch_s3(const S3Job* table, int count, const S3Totals& totals, const S3Job* single,
