@@ -52,17 +52,24 @@ __global__ void s2_init(dev::JobSpan<S2Job> span) {
     }
 }
 
-// cornerScore<16> on an LDS tile; g points at the centre pixel, row stride kGrayW.
+// OpenCV's FAST-9 early exit on the four compass pixels: a pixel can only be
+// a corner when at least two of them are brighter than v + t or at least two
+// darker than v - t. "At least two of four above a bound" is "the second
+// largest above it": max(med3(a, b, c), min(max3(a, b, c), d)); likewise the
+// second smallest is min(med3(a, b, c), max(min3(a, b, c), d)). Branch-free,
+// about 11 VALU per pixel instead of 8 compare-and-count steps.
+__device__ __forceinline__ bool fast_candidate(const std::uint8_t* g, int threshold) {
+    const int v = g[0];
+    const int a = g[3 * kGrayW], b = g[3], c = g[-3 * kGrayW], d = g[-3];
+    const int hi3 = max(max(a, b), c), lo3 = min(min(a, b), c);
+    const int mid3 = max(min(a, b), min(max(a, b), c));
+    return max(mid3, min(hi3, d)) > v + threshold || min(mid3, max(lo3, d)) < v - threshold;
+}
+
+// cornerScore<16> on an LDS tile for a pixel that passed fast_candidate; g
+// points at the centre pixel, row stride kGrayW.
 __device__ int fast_score_lds(const std::uint8_t* g, int threshold) {
     const int v = g[0];
-    int brighter = 0, darker = 0;
-#pragma unroll
-    for (int k = 0; k < 16; k += 4) {
-        const int p = g[kCircleY[k] * kGrayW + kCircleX[k]];
-        brighter += p > v + threshold;
-        darker += p < v - threshold;
-    }
-    if (brighter < 2 && darker < 2) return 0;
     int d[16];
     unsigned dark = 0, bright = 0;
 #pragma unroll
@@ -115,56 +122,76 @@ __global__ void __launch_bounds__(256) s2_fast_nms(dev::JobSpan<S2Job> span, dev
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         ji = dev::next_job(span, t, &S2Job::fast_begin, ji, njobs);
         const S2Job& j = dev::job_at(span, ji);
-        const long long local = t - j.fast_begin;
-        const int x0 = static_cast<int>(local % j.tiles_x) * kS2TileW;
-        const int y0 = static_cast<int>(local / j.tiles_x) * kS2TileH;
+        // Job fields used per pixel, read once per tile.
+        const int w = j.w, h = j.h, pitch = j.pitch, thr = j.threshold;
+        const std::uint8_t* img = j.img;
+        std::uint8_t* maxima = j.maxima;
+        const unsigned local = static_cast<unsigned>(t - j.fast_begin), tiles_x = static_cast<unsigned>(j.tiles_x);
+        const int x0 = static_cast<int>(local % tiles_x) * kS2TileW;
+        const int y0 = static_cast<int>(local / tiles_x) * kS2TileH;
         hist[tid] = 0;
-        if (((reinterpret_cast<std::uintptr_t>(j.img) | static_cast<std::uintptr_t>(j.pitch)) & 3u) == 0) {
+        constexpr int kWords = kGrayW / 4;
+        static_assert(kGrayW % 4 == 0 && kHalo % 4 == 0 && kS2TileW % 4 == 0, "gray tile words");
+        const bool words = ((reinterpret_cast<std::uintptr_t>(img) | static_cast<std::uintptr_t>(pitch)) & 3u) == 0;
+        if (words && x0 >= kHalo && x0 - kHalo + kGrayW <= w && y0 >= kHalo && y0 - kHalo + kGrayH <= h) {
+            // Interior tile: every word of the window lies in the image.
+            for (int base = 0; base < kGrayH * kWords; base += 256) {
+                const int i = base + tid;
+                if (i < kGrayH * kWords)
+                    reinterpret_cast<unsigned*>(gray)[i] = *reinterpret_cast<const unsigned*>(
+                        img + static_cast<size_t>(y0 - kHalo + i / kWords) * pitch + x0 - kHalo + 4 * (i % kWords));
+            }
+        } else if (words) {
             // Four pixels per load (x0 - kHalo is a multiple of 4); pixels
             // outside the image read 0.
-            constexpr int kWords = kGrayW / 4;
-            static_assert(kGrayW % 4 == 0 && kHalo % 4 == 0 && kS2TileW % 4 == 0, "gray tile words");
             for (int i = tid; i < kGrayH * kWords; i += 256) {
                 const int gx = x0 - kHalo + 4 * (i % kWords), gy = y0 - kHalo + i / kWords;
                 unsigned word = 0;
-                if (gy >= 0 && gy < j.h) {
-                    const std::uint8_t* row = j.img + static_cast<size_t>(gy) * j.pitch;
-                    if (gx >= 0 && gx + 3 < j.w) {
+                if (gy >= 0 && gy < h) {
+                    const std::uint8_t* row = img + static_cast<size_t>(gy) * pitch;
+                    if (gx >= 0 && gx + 3 < w) {
                         word = *reinterpret_cast<const unsigned*>(row + gx);
                     } else {
                         for (int b = 0; b < 4; ++b)
-                            if (gx + b >= 0 && gx + b < j.w) word |= static_cast<unsigned>(row[gx + b]) << (8 * b);
+                            if (gx + b >= 0 && gx + b < w) word |= static_cast<unsigned>(row[gx + b]) << (8 * b);
                     }
                 }
-                reinterpret_cast<unsigned*>(gray)[(i / kWords) * kWords + i % kWords] = word;
+                reinterpret_cast<unsigned*>(gray)[i] = word;
             }
         } else {
             for (int i = tid; i < kGrayH * kGrayW; i += 256) {
                 const int gx = x0 - kHalo + i % kGrayW, gy = y0 - kHalo + i / kGrayW;
-                gray[i] = (gx >= 0 && gx < j.w && gy >= 0 && gy < j.h) ? j.img[static_cast<size_t>(gy) * j.pitch + gx] : 0;
+                gray[i] = (gx >= 0 && gx < w && gy >= 0 && gy < h) ? img[static_cast<size_t>(gy) * pitch + gx] : 0;
             }
         }
         __syncthreads();
-        for (int i = tid; i < kScoreH * kScoreW; i += 256) {
-            const int sx = i % kScoreW, sy = i / kScoreW;
+        // Scores of the tile and its one-pixel ring. Trip counts are uniform;
+        // the candidate test runs on every lane (indexes clamped into the
+        // tile, conditions combined without branches) and the full score only
+        // where it passed.
+        for (int base = 0; base < kScoreH * kScoreW; base += 256) {
+            const int i = base + tid, ic = min(i, kScoreH * kScoreW - 1);
+            const int sx = ic % kScoreW, sy = ic / kScoreW;
             const int gx = x0 - 1 + sx, gy = y0 - 1 + sy;
+            const std::uint8_t* g = &gray[(sy + kHalo - 1) * kGrayW + sx + kHalo - 1];
+            const bool inside = (gx >= 3) & (gx < w - 3) & (gy >= 3) & (gy < h - 3);
             int s = 0;
-            if (gx >= 3 && gx < j.w - 3 && gy >= 3 && gy < j.h - 3)
-                s = fast_score_lds(&gray[(sy + kHalo - 1) * kGrayW + sx + kHalo - 1], j.threshold);
-            score[i] = static_cast<std::uint8_t>(s);
+            if (inside & fast_candidate(g, thr)) s = fast_score_lds(g, thr);
+            if (i < kScoreH * kScoreW) score[i] = static_cast<std::uint8_t>(s);
         }
         __syncthreads();
-        for (int i = tid; i < kS2TileW * kS2TileH; i += 256) {
+        static_assert(kS2TileW * kS2TileH % 256 == 0, "NMS trips");
+        for (int base = 0; base < kS2TileW * kS2TileH; base += 256) {
+            const int i = base + tid;
             const int lx = i % kS2TileW, ly = i / kS2TileW;
             const int gx = x0 + lx, gy = y0 + ly;
-            if (gx >= j.w || gy >= j.h) continue;
             const std::uint8_t* c = &score[(ly + 1) * kScoreW + lx + 1];
             const int s = c[0];
-            bool keep = s > 0 && gx >= kEdge && gx < j.w - kEdge && gy >= kEdge && gy < j.h - kEdge;
+            bool keep = (s > 0) & (gx >= kEdge) & (gx < w - kEdge) & (gy >= kEdge) & (gy < h - kEdge);
             if (keep)
                 keep = s > c[-kScoreW - 1] && s > c[-kScoreW] && s > c[-kScoreW + 1] && s > c[-1] && s > c[1] &&
                        s > c[kScoreW - 1] && s > c[kScoreW] && s > c[kScoreW + 1];
-            j.maxima[static_cast<size_t>(gy) * j.w + gx] = keep ? static_cast<std::uint8_t>(s) : 0;
+            if ((gx < w) & (gy < h)) maxima[static_cast<size_t>(gy) * w + gx] = keep ? static_cast<std::uint8_t>(s) : 0;
             if (keep) atomicAdd(&hist[s], 1u);
         }
         __syncthreads();
@@ -424,7 +451,7 @@ __global__ void __launch_bounds__(256) s2_rank(dev::JobSpan<S2Job> span, dev::To
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         ji = dev::next_job(span, t, &S2Job::rank_begin, ji, njobs);
         const S2Job& j = dev::job_at(span, ji);
-        const long long local = t - j.rank_begin;
+        const unsigned local = static_cast<unsigned>(t - j.rank_begin);
         const int segs = (j.candidates + kS2RankSegment - 1) / kS2RankSegment;
         const int kb = static_cast<int>(local / segs), sg = static_cast<int>(local % segs);
         const int selected = static_cast<int>(j.params->selected);

@@ -85,10 +85,10 @@ __global__ void __launch_bounds__(256) s3_blur(dev::JobSpan<S3Job> span, dev::To
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         ji = dev::next_job(span, t, &S3Job::blur_begin, ji, njobs);
         const S3Job& j = dev::job_at(span, ji);
-        const long long local = t - j.blur_begin;
+        const unsigned local = static_cast<unsigned>(t - j.blur_begin), tiles_x = static_cast<unsigned>(j.tiles_x);
         if (__builtin_amdgcn_readfirstlane(j.marks[local]) != j.epoch) continue;
-        const int x0 = static_cast<int>(local % j.tiles_x) * kS3TileW;
-        const int y0 = static_cast<int>(local / j.tiles_x) * kS3TileH;
+        const int x0 = static_cast<int>(local % tiles_x) * kS3TileW;
+        const int y0 = static_cast<int>(local / tiles_x) * kS3TileH;
         if (((reinterpret_cast<std::uintptr_t>(j.img) | static_cast<std::uintptr_t>(j.pitch)) & 3u) == 0) {
             // Aligned words over columns x0 - 4 .. x0 + kS3TileW + 3; words
             // that would need a column reflected go byte by byte.

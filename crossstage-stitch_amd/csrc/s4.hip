@@ -71,8 +71,8 @@ __global__ void __launch_bounds__(256) s4_dist(dev::JobSpan<S4Job> span, dev::To
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         ji = dev::next_job(span, t, &S4Job::dist_begin, ji, njobs);
         const S4Job& j = dev::job_at(span, ji);
-        const long long local = t - j.dist_begin;
-        const int qb = static_cast<int>(local / j.segs), sg = static_cast<int>(local % j.segs);
+        const unsigned local = static_cast<unsigned>(t - j.dist_begin), segs = static_cast<unsigned>(j.segs);
+        const int qb = static_cast<int>(local / segs), sg = static_cast<int>(local % segs);
         int na, nb;
         if (!counts_valid(j, &na, &nb)) continue;
         const int q0 = qb * kS4QueryBlock, t0 = sg * j.segment;
