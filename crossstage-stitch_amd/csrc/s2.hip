@@ -274,6 +274,7 @@ __device__ Flags count16(const std::uint8_t* maxima, long long pixels, long long
             bytes[w] = word;
         }
     }
+    if ((bytes[0] | bytes[1] | bytes[2] | bytes[3]) == 0) return f;  // no maximum: the common case
     for (int i = 0; i < 16; ++i) {
         const std::uint32_t s = (bytes[i >> 2] >> (8 * (i & 3))) & 0xffu;
         f.above += s > thr;
@@ -305,6 +306,23 @@ __device__ Flags block_scan(Flags v, Flags* excl) {
     return total;
 }
 
+// Block-wide totals of (above, tie).
+__device__ Flags block_total(Flags v) {
+    __shared__ Flags wave_tot[4];
+    const int wave = threadIdx.x / 64;
+    v.above = dev::wave_sum(v.above);
+    v.tie = dev::wave_sum(v.tie);
+    if (dev::lane_id() == 0) wave_tot[wave] = v;
+    __syncthreads();
+    Flags total{0, 0};
+    for (int w = 0; w < 4; ++w) {
+        total.above += wave_tot[w].above;
+        total.tie += wave_tot[w].tie;
+    }
+    __syncthreads();
+    return total;
+}
+
 __global__ void __launch_bounds__(256) s2_count(dev::JobSpan<S2Job> span, dev::Total work) {
     const long long total = work.get();
     int ji = 0;
@@ -317,8 +335,7 @@ __global__ void __launch_bounds__(256) s2_count(dev::JobSpan<S2Job> span, dev::T
         const std::uint32_t thr = j.params->threshold;
         std::uint32_t bytes[4];
         const Flags f = count16(j.maxima, pixels, c * kS2Chunk + threadIdx.x * 16, thr, bytes);
-        Flags excl;
-        const Flags tot = block_scan(f, &excl);
+        const Flags tot = block_total(f);
         if (threadIdx.x == 0) {
             j.chunk_counts[2 * c] = tot.above;
             j.chunk_counts[2 * c + 1] = tot.tie;
@@ -365,6 +382,11 @@ __global__ void __launch_bounds__(256) s2_scatter(dev::JobSpan<S2Job> span, dev:
         const S2Job& j = dev::job_at(span, ji);
         const long long c = t - j.chunk_begin;
         const long long pixels = static_cast<long long>(j.w) * j.h;
+        // A chunk that selects nothing (the count pass saw no maximum at or
+        // above the boundary score) writes nothing.
+        if ((__builtin_amdgcn_readfirstlane(j.chunk_counts[2 * c]) |
+             __builtin_amdgcn_readfirstlane(j.chunk_counts[2 * c + 1])) == 0)
+            continue;
         const SelectParams p = *j.params;
         const long long first = c * kS2Chunk + threadIdx.x * 16;
         std::uint32_t bytes[4];

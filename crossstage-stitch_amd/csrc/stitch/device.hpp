@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 #include "stitch/tuning.hpp"
@@ -59,9 +60,24 @@ __device__ inline const Job& job_at(const JobSpan<Job>& span, int i) {
     // for the whole launch: address both through the constant address space,
     // so the (wave-uniform) job fields are fetched with scalar loads instead
     // of one vector load per lane through a generic pointer.
+    //
+    // Every kernel taking a JobSpan takes it as its first argument, at offset
+    // 0 of the kernel argument segment, so the by-value job's address is the
+    // segment pointer plus offsetof(single). The chosen address goes through
+    // readfirstlane: the compiler then reads each field with one scalar load
+    // from whichever record applies. (Selecting between the table and
+    // &span.single directly lets it keep every field of the by-value copy in
+    // SGPRs and branch on table/single at each field access, which costs
+    // registers - spills in the larger kernels - and scalar instructions.)
     using CJob = const __attribute__((address_space(4))) Job;
-    CJob* p = span.table ? (CJob*)span.table + i : (CJob*)&span.single;
-    return *(const Job*)p;
+    const std::uint64_t single =
+        reinterpret_cast<std::uint64_t>(__builtin_amdgcn_kernarg_segment_ptr()) + offsetof(JobSpan<Job>, single);
+    const std::uint64_t chosen = span.table ? reinterpret_cast<std::uint64_t>(span.table + i) : single;
+    const std::uint64_t u =
+        static_cast<std::uint64_t>(static_cast<std::uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(chosen)))) |
+        static_cast<std::uint64_t>(static_cast<std::uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(chosen >> 32))))
+            << 32;
+    return *(const Job*)reinterpret_cast<CJob*>(u);
 }
 
 // Index of the job owning flattened tile `t` for the kernel whose prefix is
