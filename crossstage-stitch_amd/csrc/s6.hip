@@ -20,6 +20,8 @@
 // the pixel's parity (reflect-101 keeps parity), 9 instead of 25.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -60,6 +62,36 @@ __device__ bool center_coordinate(double& x, int n, double tol) {
     return x >= 0 && x <= n - 1.0;
 }
 
+// Bytes p[0 .. n) of an RGB row, n = 6 (two neighbouring pixels) or 3 (one),
+// through aligned dword loads: lo holds p[0..3], hi p[4..7] (bytes past n
+// undefined). A dword that holds no needed byte is not read: its load reads
+// the first dword again instead, so the loads stay unconditional (no branch
+// between them) and every address read holds a byte of the image - an
+// aligned dword holding any byte of a buffer lies in that buffer's page.
+// kSkip: dwords holding no needed byte are skipped by a branch instead (fewer
+// memory accesses where the loads are not batched with others).
+template <bool kSkip = false>
+__device__ __forceinline__ void load_rgb(const std::uint8_t* p, bool two, unsigned& lo, unsigned& hi) {
+    const std::uintptr_t addr = reinterpret_cast<std::uintptr_t>(p);
+    const unsigned* a = reinterpret_cast<const unsigned*>(addr & ~static_cast<std::uintptr_t>(3));
+    const unsigned sh = static_cast<unsigned>(addr & 3u);
+    const unsigned last = sh + (two ? 5u : 2u);  // offset of the last needed byte from a
+    const unsigned w0 = a[0];
+    unsigned w1, w2;
+    if (kSkip) {
+        w1 = w2 = 0;
+        if (last >= 4) w1 = a[1];
+        if (last >= 8) w2 = a[2];
+    } else {
+        w1 = a[last >= 4 ? 1 : 0];
+        w2 = two ? a[last >= 8 ? 2 : 0] : w1;
+    }
+    lo = __builtin_amdgcn_alignbyte(w1, w0, sh);
+    hi = __builtin_amdgcn_alignbyte(w2, w1, sh);
+}
+
+__device__ __forceinline__ unsigned byte_of(unsigned w, int k) { return (w >> (8 * k)) & 0xffu; }
+
 // Bilinear B sample; the border score only when asked. The position, its
 // snapping and the validity are FP64 as in the reference; the blend of the
 // four bytes runs in FP32 with the FP64 fractions rounded to FP32, then
@@ -72,12 +104,19 @@ __device__ bool sample(const std::uint8_t* img, int w, int h, int pitch, double&
     const int x0 = static_cast<int>(floor(x)), y0 = static_cast<int>(floor(y));
     const int x1 = x0 < w - 1 ? x0 + 1 : x0, y1 = y0 < h - 1 ? y0 + 1 : y0;
     const float ax = static_cast<float>(x - x0), ay = static_cast<float>(y - y0);
-    const std::uint8_t* r0 = img + static_cast<size_t>(y0) * pitch;
-    const std::uint8_t* r1 = img + static_cast<size_t>(y1) * pitch;
+    // Row bytes of pixels x0 and x1 (x1 = x0 at the right border).
+    const bool two = x1 != x0;
+    unsigned lo0, hi0, lo1, hi1;
+    load_rgb<true>(img + static_cast<size_t>(y0) * pitch + 3 * x0, two, lo0, hi0);
+    load_rgb<true>(img + static_cast<size_t>(y1) * pitch + 3 * x0, two, lo1, hi1);
+    const unsigned q00[3] = {byte_of(lo0, 0), byte_of(lo0, 1), byte_of(lo0, 2)};
+    const unsigned q10[3] = {byte_of(lo1, 0), byte_of(lo1, 1), byte_of(lo1, 2)};
+    const unsigned q01[3] = {two ? byte_of(lo0, 3) : q00[0], two ? byte_of(hi0, 0) : q00[1], two ? byte_of(hi0, 1) : q00[2]};
+    const unsigned q11[3] = {two ? byte_of(lo1, 3) : q10[0], two ? byte_of(hi1, 0) : q10[1], two ? byte_of(hi1, 1) : q10[2]};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const float v0 = fmaf(ax, static_cast<float>(r0[3 * x1 + c]), (1 - ax) * static_cast<float>(r0[3 * x0 + c]));
-        const float v1 = fmaf(ax, static_cast<float>(r1[3 * x1 + c]), (1 - ax) * static_cast<float>(r1[3 * x0 + c]));
+        const float v0 = fmaf(ax, static_cast<float>(q01[c]), (1 - ax) * static_cast<float>(q00[c]));
+        const float v1 = fmaf(ax, static_cast<float>(q11[c]), (1 - ax) * static_cast<float>(q10[c]));
         rgb[c] = fmaf(ay, v1, (1 - ay) * v0) * (1.0f / 255.0f);
     }
     // min((x+.5)/w, (w-.5-x)/w) == min(x+.5, w-.5-x)/w exactly: a correctly
@@ -193,10 +232,11 @@ __device__ __forceinline__ void warp_pixel(const S6Job& j, int x, int y, const f
         j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = ma || mb ? 1 : 0;
         if (!(ma && mb)) {
             if (ma) {
-                const std::uint8_t* q = a_pixel(j, x, y);
-                ra[0] = lut[q[0]];
-                ra[1] = lut[q[1]];
-                ra[2] = lut[q[2]];
+                unsigned lo, hi;
+                load_rgb<true>(a_pixel(j, x, y), false, lo, hi);
+                ra[0] = lut[byte_of(lo, 0)];
+                ra[1] = lut[byte_of(lo, 1)];
+                ra[2] = lut[byte_of(lo, 2)];
             }
             std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
 #pragma unroll
@@ -204,10 +244,11 @@ __device__ __forceinline__ void warp_pixel(const S6Job& j, int x, int y, const f
         }
     } else {
         if (ma) {
-            const std::uint8_t* q = a_pixel(j, x, y);
-            ra[0] = lut[q[0]];
-            ra[1] = lut[q[1]];
-            ra[2] = lut[q[2]];
+            unsigned lo, hi;
+            load_rgb<true>(a_pixel(j, x, y), false, lo, hi);
+            ra[0] = lut[byte_of(lo, 0)];
+            ra[1] = lut[byte_of(lo, 1)];
+            ra[2] = lut[byte_of(lo, 2)];
         }
         std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
 #pragma unroll
@@ -380,6 +421,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
     __shared__ float tile[kCh][kRIH][kRPitch];
     __shared__ float lut[kFromZero ? 256 : 1];
     __shared__ std::uint8_t fl[kFromZero ? kRIH : 1][kRIW + 1];
+    __shared__ unsigned wave_flags[2][kBlock / 64];  // per wave: OR and AND of the staged flags
     if (kFromZero) {
         fill_lut(lut);
         __syncthreads();
@@ -438,10 +480,14 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 const int ax = sxs[e] + j.origin_x, ay = sys[e] + j.origin_y;
                 const bool in_a = ax >= 0 && ax < j.wa && ay >= 0 && ay < j.ha;
                 const bool in_b = sxs[e] >= j.bx0 && sxs[e] <= j.bx1 && sys[e] >= j.by0 && sys[e] <= j.by1;
-                const std::uint8_t* src = j.a + static_cast<size_t>(in_a ? ay : 0) * j.pitch_a + 3 * (in_a ? ax : 0);
+                // Outside A's rectangle the load reads A's first pixel (in
+                // bounds) and the bytes are dropped.
+                unsigned lo, hi;
+                load_rgb(j.a + static_cast<size_t>(in_a ? ay : 0) * j.pitch_a + 3 * (in_a ? ax : 0), false, lo, hi);
+                lo = in_a ? lo : 0u;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    abytes[e][c] = in_a ? src[c] : 0u;
+                    abytes[e][c] = byte_of(lo, c);
                     raw[e][3 + c] = in_b ? planes[3 + c][q] : 0.0f;
                 }
             } else {
@@ -450,10 +496,15 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
             }
         }
         // Pass 2: level-0 selection, then the LDS tile.
+        unsigned f_or = 0, f_and = 0xfu;
 #pragma unroll
         for (int e = 0; e < kPer; ++e) {
             const int i = static_cast<int>(threadIdx.x) + e * kBlock;
             if (i >= kElems) continue;
+            if (kFromZero) {
+                f_or |= fls[e];
+                f_and &= fls[e];
+            }
             const int ix = i % kRIW, iy = i / kRIW;
             const int cx = (ix & 1) * kREven + (ix >> 1);
             float v[kCh];
@@ -473,22 +524,58 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
 #pragma unroll
             for (int c = 0; c < kCh; ++c) tile[c][iy][cx] = v[c];
         }
+        if (kFromZero) {
+            // Which flag bits the staged tile holds somewhere (OR) and
+            // everywhere (AND), per wave; combined after the barrier.
+            unsigned w_or = 0, w_and = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (__ballot((f_or >> k) & 1u)) w_or |= 1u << k;
+                if (!__ballot(!((f_and >> k) & 1u))) w_and |= 1u << k;
+            }
+            if (dev::lane_id() == 0) {
+                wave_flags[0][threadIdx.x / 64] = w_or;
+                wave_flags[1][threadIdx.x / 64] = w_and;
+            }
+        }
         __syncthreads();
+        // Level 0: a tile whose staged flags are all equal has integer
+        // weights 256 * bit (or 0) at every output, and one with no valid
+        // pixel has colour sums of exact zeros (every tap adds 0 * 0).
+        unsigned t_or = 0xfu, t_and = 0;
+        if (kFromZero) {
+            t_or = wave_flags[0][0] | wave_flags[0][1] | wave_flags[0][2] | wave_flags[0][3];
+            t_and = wave_flags[1][0] & wave_flags[1][1] & wave_flags[1][2] & wave_flags[1][3];
+        }
+        const bool uniform = kFromZero && t_or == t_and;
         const int lx = threadIdx.x % kRW, ly = threadIdx.x / kRW;
         const int x = x0 + lx, y = y0 + ly;
         if (x < C.w && y < C.h) {
             float acc[kCh];
 #pragma unroll
             for (int c = 0; c < kCh; ++c) acc[c] = 0.0f;
+            const auto filter = [&](auto c0, auto c1) {
 #pragma unroll
-            for (int dy = 0; dy < 5; ++dy)
+                for (int dy = 0; dy < 5; ++dy)
 #pragma unroll
-                for (int dx = 0; dx < 5; ++dx) {
-                    const float w = tap(dx - 2) * tap(dy - 2);
+                    for (int dx = 0; dx < 5; ++dx) {
+                        const float w = tap(dx - 2) * tap(dy - 2);
 #pragma unroll
-                    for (int c = 0; c < kCh; ++c)
-                        acc[c] = fmaf(w, tile[c][2 * ly + dy][(dx & 1) * kREven + lx + (dx >> 1)], acc[c]);
-                }
+                        for (int c = decltype(c0)::value; c < decltype(c1)::value; ++c)
+                            acc[c] = fmaf(w, tile[c][2 * ly + dy][(dx & 1) * kREven + lx + (dx >> 1)], acc[c]);
+                    }
+            };
+            using I0 = std::integral_constant<int, 0>;
+            using I3 = std::integral_constant<int, 3>;
+            using IC = std::integral_constant<int, kCh>;
+            if (!kFromZero) {
+                filter(I0{}, IC{});
+            } else {
+                // A colour sums where some staged pixel has A, B's where some
+                // has B; the others are sums of zeros.
+                if (t_or & kVA) filter(I0{}, I3{});
+                if (t_or & kVB) filter(I3{}, IC{});
+            }
             const size_t p = static_cast<size_t>(y) * C.w + x;
             float* da = kMode == 1 ? C.ga : C.a;
             float* db = kMode == 1 ? C.gb : C.b;
@@ -499,6 +586,12 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
             }
             if (kFromZero) {
                 int va = 0, vb = 0, ua = 0, ub = 0;
+                if (uniform) {
+                    va = (t_or & kVA) ? 256 : 0;
+                    vb = (t_or & kVB) ? 256 : 0;
+                    ua = (t_or & kUA) ? 256 : 0;
+                    ub = (t_or & kUB) ? 256 : 0;
+                } else
 #pragma unroll 1
                 for (int dy = 0; dy < 5; ++dy)
 #pragma unroll
