@@ -72,9 +72,10 @@ __device__ bool center_coordinate(double& x, int n, double tol) {
 // memory accesses where the loads are not batched with others).
 template <bool kSkip = false>
 __device__ __forceinline__ void load_rgb(const std::uint8_t* p, bool two, unsigned& lo, unsigned& hi) {
-    const std::uintptr_t addr = reinterpret_cast<std::uintptr_t>(p);
-    const unsigned* a = reinterpret_cast<const unsigned*>(addr & ~static_cast<std::uintptr_t>(3));
-    const unsigned sh = static_cast<unsigned>(addr & 3u);
+    // Pointer arithmetic (not an integer round trip) keeps the global
+    // address space known, so these are global loads, not flat ones.
+    const unsigned sh = static_cast<unsigned>(reinterpret_cast<std::uintptr_t>(p) & 3u);
+    const unsigned* a = reinterpret_cast<const unsigned*>(p - sh);
     const unsigned last = sh + (two ? 5u : 2u);  // offset of the last needed byte from a
     const unsigned w0 = a[0];
     unsigned w1, w2;
