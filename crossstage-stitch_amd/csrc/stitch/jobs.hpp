@@ -25,7 +25,7 @@ struct S1Job {
     int tiles_x;   // tiles of kS1TileW x kS1TileH output pixels
     long long tile_begin;
 };
-constexpr int kS1TileW = 64, kS1TileH = 16;
+constexpr int kS1TileW = 64, kS1TileH = 64;
 
 Status make_s1_job(const ImageView& in, const ImageView& out, const ScaleInfo& si, S1Job* job);
 inline long long s1_tiles(const S1Job& j) {
@@ -75,7 +75,7 @@ struct S2Job {
     int chunks;                    // compaction chunks of kS2Chunk pixels
     long long fast_begin, chunk_begin, cand_begin, rank_begin;
 };
-constexpr int kS2TileW = 64, kS2TileH = 16, kS2Chunk = 4096;
+constexpr int kS2TileW = 64, kS2TileH = 32, kS2Chunk = 4096;
 constexpr int kS2RankKeys = 64;      // keys ranked per workgroup
 constexpr int kS2RankSegment = 2048; // comparand keys per workgroup
 
