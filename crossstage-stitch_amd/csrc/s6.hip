@@ -437,9 +437,10 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
         const S6Level& C = j.lv[l];
         const S6Level& F = j.lv[l - 1];
         const unsigned local = static_cast<unsigned>(t - (kMode != 1 ? j.rb[l] : j.gb[l]));
-        // Pulls cover the level; the Gaussian its dependency box.
-        const int bx = kMode == 1 ? j.gbox[l][0] : 0, by = kMode == 1 ? j.gbox[l][1] : 0;
-        const int bw = kMode == 1 ? j.gbox[l][2] - bx + 1 : C.w;
+        // Pulls cover the region whose pulled values are read, the Gaussian
+        // its dependency region.
+        const int(&box)[4] = kMode == 1 ? j.gbox[l] : j.ubox[l];
+        const int bx = box[0], by = box[1], bw = box[2] - bx + 1;
         const unsigned tx = (static_cast<unsigned>(bw) + kRW - 1) / kRW;
         const int x0 = bx + static_cast<int>(local % tx) * kRW, y0 = by + static_cast<int>(local / tx) * kRH;
         const int ix0 = 2 * x0 - 2, iy0 = 2 * y0 - 2;
@@ -1356,13 +1357,18 @@ void put(int (&dst)[4], const Box& b, int tw, int th) {
 // pushes itself from the pushed level 2, and the push at each level reads the
 // next one through an expansion (p regions). Everything outside these boxes
 // is never read by anything that reaches the output, so those kernels skip
-// it; the pulls and the warp still cover whole levels.
+// it. Pulled values (raw sums and weights) are read by the push at each level
+// over its p region, by the Gaussian into level 1 over its region and two
+// pixels around (from which it pushes level 1 itself), and by the pull into
+// the next level through a reduce (u regions). Only the warp covers the
+// whole canvas.
 static void s6_regions(S6Job* job) {
     S6Job& j = *job;
     for (int l = 0; l < kS6MaxLevels; ++l) {
         put(j.zbox[l], kEmpty, 1, 1);
         put(j.gbox[l], kEmpty, 1, 1);
         put(j.pbox[l], kEmpty, 1, 1);
+        put(j.ubox[l], kEmpty, 1, 1);
     }
     if (j.top < 1 || j.blend_top < 1 || j.ox1 < j.ox0) return;
     const auto W = [&](int l) { return j.lv[l].w; };
@@ -1377,12 +1383,18 @@ static void s6_regions(S6Job* job) {
         p[2] = expand_need(grow(g[1], 2, W(1), Hh(1)), W(2), Hh(2));
         for (int l = 3; l <= j.top; ++l) p[l] = expand_need(p[l - 1], W(l), Hh(l));
     }
+    Box u[kS6MaxLevels];
+    for (int l = j.top; l >= 1; --l) {
+        u[l] = l >= 2 ? p[l] : grow(g[1], 2, W(1), Hh(1));
+        if (l < j.top) u[l] = unite(u[l], reduce_need(u[l + 1], W(l), Hh(l)));
+    }
     for (int l = 1; l <= j.top; ++l) {
         if (l <= j.blend_top) {
             put(j.zbox[l], z[l], kEW, kEH);
             put(j.gbox[l], g[l], kRW, kRH);
         }
         if (l >= 2) put(j.pbox[l], p[l], kEW, kEH);
+        put(j.ubox[l], u[l], kRW, kRH);
     }
 }
 
@@ -1515,8 +1527,6 @@ S6Totals s6_prefix(S6Job* js, int n) {
             j.pb[l] = t.push[l];
             j.mb[l] = t.mix[l];
             if (j.top < l) continue;
-            const long long w = j.lv[l].w, h = j.lv[l].h;
-            const long long rtiles = ((w + kRW - 1) / kRW) * ((h + kRH - 1) / kRH);
             const auto in_box = [](const int (&b)[4], int tw, int th) -> long long {
                 return b[2] < b[0] || b[3] < b[1] ? 0
                                                   : static_cast<long long>((b[2] - b[0] + tw) / tw) * ((b[3] - b[1] + th) / th);
@@ -1526,7 +1536,7 @@ S6Totals s6_prefix(S6Job* js, int n) {
             const long long otiles = j.ox1 < j.ox0 ? 0
                                                    : ((j.ox1 - j.ox0 + kEW) / kEW) * ((j.oy1 - j.oy0 + kEH) / kEH);
             if (l >= 1) {
-                t.pull[l] += rtiles;
+                t.pull[l] += in_box(j.ubox[l], kRW, kRH);
                 if (l >= 2) t.push[l] += in_box(j.pbox[l], kEW, kEH);
             }
             if (j.blend_top >= l) {
@@ -1567,15 +1577,32 @@ double s6_algorithmic_bytes(int kernel, int l, const S6Job& j, const s6::RenderS
         case LaunchProbe::kPull:
         case LaunchProbe::kGauss: {
             if (l > (kernel == LaunchProbe::kPull ? j.top : j.blend_top)) return 0;
-            // Two colours and two weights per pixel, in and out; the pulls
-            // cover whole levels, the Gaussians their regions (reading four
-            // finer pixels per pixel).
-            if (l >= 2) return kernel == LaunchProbe::kPull ? 32 * n(l - 1) + 32 * n(l) : 5 * 32 * area(j.gbox[l], l);
-            // Pull from level 0: flags, A's bytes and B's colour in; two raw
-            // colours and four weights out. Gaussian into level 1: the raw
-            // sums and validity weights (pushed on the fly) and the pushed
-            // level 2 in, two colours out, over its region.
-            if (kernel == LaunchProbe::kPull) return P + 3 * nA + 12 * nB + 40 * n(1);
+            // Two colours and two weights per pixel, in and out, over the
+            // region (reading four finer pixels per pixel).
+            if (l >= 2) return 5 * 32 * area(kernel == LaunchProbe::kPull ? j.ubox[l] : j.gbox[l], l);
+            // Pull from level 0: flags, A's bytes and B's colour in (B's
+            // colour at the share of its valid pixels that its box holds
+            // inside the footprint); two raw colours and two weights out,
+            // and two ownership weights in the Gaussian region. Gaussian
+            // into level 1: the raw sums and validity weights (pushed on the
+            // fly) and the pushed level 2 in, two colours out, over its region.
+            if (kernel == LaunchProbe::kPull) {
+                const double u1 = area(j.ubox[1], 1);
+                if (u1 == 0) return 0;
+                const int fx0 = std::max(2 * j.ubox[1][0] - 2, 0), fy0 = std::max(2 * j.ubox[1][1] - 2, 0);
+                const int fx1 = std::min(2 * j.ubox[1][2] + 2, j.width - 1), fy1 = std::min(2 * j.ubox[1][3] + 2, j.height - 1);
+                const auto span = [](int a0, int a1, int b0, int b1) {
+                    return std::max(0, std::min(a1, b1) - std::max(a0, b0) + 1);
+                };
+                const double foot = static_cast<double>(fx1 - fx0 + 1) * (fy1 - fy0 + 1);
+                const double fa = static_cast<double>(span(fx0, fx1, -j.origin_x, j.wa - 1 - j.origin_x)) *
+                                  span(fy0, fy1, -j.origin_y, j.ha - 1 - j.origin_y);
+                const double bbox = static_cast<double>(j.bx1 - j.bx0 + 1) * (j.by1 - j.by0 + 1);
+                const double fb = bbox > 0 ? static_cast<double>(span(fx0, fx1, j.bx0, j.bx1)) *
+                                                 span(fy0, fy1, j.by0, j.by1) / bbox * nB
+                                           : 0.0;
+                return foot + 3 * fa + 12 * fb + 32 * u1 + 8 * area(j.gbox[1], 1);
+            }
             return (32 + (j.top >= 2 ? 6 : 0) + 24) * area(j.gbox[1], 1);
         }
         case LaunchProbe::kPush:

@@ -61,11 +61,13 @@ struct S6Job {
     int ox0, oy0, ox1, oy1;  // box holding every overlap pixel, start aligned to the mix tile
     // Regions (x0, y0, x1, y1 inclusive, in the level's pixels; empty when
     // x1 < x0) that the level-0 band mix depends on: where the band mix runs
-    // at levels >= 1, where the Gaussian is needed, and where pushed (filled)
-    // values are read. Starts are aligned to the tiles of the kernel using them.
+    // at levels >= 1, where the Gaussian is needed, where pushed (filled)
+    // values are read, and where pulled values (raw sums and weights) are
+    // read. Starts are aligned to the tiles of the kernel using them.
     int zbox[kS6MaxLevels][4];
     int gbox[kS6MaxLevels][4];
     int pbox[kS6MaxLevels][4];
+    int ubox[kS6MaxLevels][4];
     long long warp_begin;
     long long rb[kS6MaxLevels];  // pull (reduce into level l) tiles
     long long gb[kS6MaxLevels];  // Gaussian (reduce into level l) tiles
