@@ -792,10 +792,9 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
         const bool inside = x < C.w && y < C.h;
         const size_t p = inside ? static_cast<size_t>(y) * C.w + x : 0;
         const bool coarse = j.top >= 2;
-        // Every load of the tile before the first barrier: this pixel's raw
-        // pull sums, the staged level-1 raw sums and weights, the level 2.
-        // The raw sums (a / b) are never written here: the result goes to
-        // ga / gb, so neighbouring tiles read unchanged halos.
+        // This pixel's raw pull sums and weights first. The raw sums (a / b)
+        // are never written here: the result goes to ga / gb, so
+        // neighbouring tiles read unchanged halos.
         float ra[3], rb[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -803,71 +802,77 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
             rb[c] = C.b[c * C.n + p];
         }
         const float wa = C.va[p], wb = C.vb[p];
-        float sa[kGSIt][4], sb[kGSIt][4];
-#pragma unroll
-        for (int it = 0; it < kGSIt; ++it) {
-            const int i = static_cast<int>(threadIdx.x) + it * kBlock;
-            if (i >= kGSN) continue;
-            const int cx = clampi(x0 - 2 + i % kGPW, 0, C.w - 1), cy = clampi(y0 - 2 + i / kGPW, 0, C.h - 1);
-            const size_t q = static_cast<size_t>(static_cast<unsigned>(cy) * static_cast<unsigned>(C.w) +
-                                                 static_cast<unsigned>(cx));
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                sa[it][c] = C.a[c * C.n + q];
-                sb[it][c] = C.b[c * C.n + q];
-            }
-            sa[it][3] = C.va[q];
-            sb[it][3] = C.vb[q];
-        }
-        StageRGB<kG2W, kG2W, kG2H> la, lb;
-        if (coarse) {
-            const S6Level& D = j.lv[2];
-            la.load(D.a, D.n, D.w, D.h, x0 / 2 - 2, y0 / 2 - 2);
-            lb.load(D.b, D.n, D.w, D.h, x0 / 2 - 2, y0 / 2 - 2);
-            la.store(&p2[0][0][0]);
-            lb.store(&p2[1][0][0]);
-        }
         // An image's pushed level 1 is needed only if some output of the
-        // tile misses it somewhere in its footprint (weight below 1).
+        // tile misses it somewhere in its footprint (weight below 1); only
+        // then are its staged level-1 pixels and level 2 read.
         const bool need_a = __syncthreads_or(inside && wa != 1.0f);
         const bool need_b = __syncthreads_or(inside && wb != 1.0f);
-        // Normalise the staged pixels, or fill them from the expanded level 2.
+        if (need_a || need_b) {
+            float sa[kGSIt][4], sb[kGSIt][4];
 #pragma unroll
-        for (int it = 0; it < kGSIt; ++it) {
-            const int i = static_cast<int>(threadIdx.x) + it * kBlock;
-            if (i >= kGSN) continue;
-            const int cx = clampi(x0 - 2 + i % kGPW, 0, C.w - 1), cy = clampi(y0 - 2 + i / kGPW, 0, C.h - 1);
-            const bool fill_a = need_a && !(sa[it][3] > 0), fill_b = need_b && !(sb[it][3] > 0);
-            float ea[3] = {0, 0, 0}, eb[3] = {0, 0, 0};
-            if (coarse && (fill_a || fill_b)) {
-                Expand e;
-                expand_setup(cx, cy, j.lv[2].w, j.lv[2].h, x0 / 2 - 2, y0 / 2 - 2, kG2W, kG2H, e);
-                if (fill_a) expand_rgb<kG2W>(&p2[0][0][0], e, ea);
-                if (fill_b) expand_rgb<kG2W>(&p2[1][0][0], e, eb);
+            for (int it = 0; it < kGSIt; ++it) {
+                const int i = static_cast<int>(threadIdx.x) + it * kBlock;
+                if (i >= kGSN) continue;
+                const int cx = clampi(x0 - 2 + i % kGPW, 0, C.w - 1), cy = clampi(y0 - 2 + i / kGPW, 0, C.h - 1);
+                const unsigned q = static_cast<unsigned>(cy) * static_cast<unsigned>(C.w) + static_cast<unsigned>(cx);
+                if (need_a) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) sa[it][c] = C.a[c * C.n + q];
+                    sa[it][3] = C.va[q];
+                }
+                if (need_b) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) sb[it][c] = C.b[c * C.n + q];
+                    sb[it][3] = C.vb[q];
+                }
             }
-            // normalize_color divides each channel by the weight; one
-            // correctly rounded reciprocal and three products differ from it
-            // by at most an ulp, far below what the blend can show.
-            if (need_a) {
-                const float r = sa[it][3] > 0 ? 1.0f / sa[it][3] : 0.0f;
-                float4 v;
-                v.x = sa[it][3] > 0 ? sa[it][0] * r : ea[0];
-                v.y = sa[it][3] > 0 ? sa[it][1] * r : ea[1];
-                v.z = sa[it][3] > 0 ? sa[it][2] * r : ea[2];
-                v.w = 0.0f;
-                pp[0][i / kGPW][i % kGPW] = v;
+            if (coarse) {
+                const S6Level& D = j.lv[2];
+                StageRGB<kG2W, kG2W, kG2H> la, lb;
+                if (need_a) la.load(D.a, D.n, D.w, D.h, x0 / 2 - 2, y0 / 2 - 2);
+                if (need_b) lb.load(D.b, D.n, D.w, D.h, x0 / 2 - 2, y0 / 2 - 2);
+                if (need_a) la.store(&p2[0][0][0]);
+                if (need_b) lb.store(&p2[1][0][0]);
+                __syncthreads();
             }
-            if (need_b) {
-                const float r = sb[it][3] > 0 ? 1.0f / sb[it][3] : 0.0f;
-                float4 v;
-                v.x = sb[it][3] > 0 ? sb[it][0] * r : eb[0];
-                v.y = sb[it][3] > 0 ? sb[it][1] * r : eb[1];
-                v.z = sb[it][3] > 0 ? sb[it][2] * r : eb[2];
-                v.w = 0.0f;
-                pp[1][i / kGPW][i % kGPW] = v;
+            // Normalise the staged pixels, or fill them from the expanded level 2.
+#pragma unroll
+            for (int it = 0; it < kGSIt; ++it) {
+                const int i = static_cast<int>(threadIdx.x) + it * kBlock;
+                if (i >= kGSN) continue;
+                const int cx = clampi(x0 - 2 + i % kGPW, 0, C.w - 1), cy = clampi(y0 - 2 + i / kGPW, 0, C.h - 1);
+                const bool fill_a = need_a && !(sa[it][3] > 0), fill_b = need_b && !(sb[it][3] > 0);
+                float ea[3] = {0, 0, 0}, eb[3] = {0, 0, 0};
+                if (coarse && (fill_a || fill_b)) {
+                    Expand e;
+                    expand_setup(cx, cy, j.lv[2].w, j.lv[2].h, x0 / 2 - 2, y0 / 2 - 2, kG2W, kG2H, e);
+                    if (fill_a) expand_rgb<kG2W>(&p2[0][0][0], e, ea);
+                    if (fill_b) expand_rgb<kG2W>(&p2[1][0][0], e, eb);
+                }
+                // normalize_color divides each channel by the weight; one
+                // correctly rounded reciprocal and three products differ from it
+                // by at most an ulp, far below what the blend can show.
+                if (need_a) {
+                    const float r = sa[it][3] > 0 ? 1.0f / sa[it][3] : 0.0f;
+                    float4 v;
+                    v.x = sa[it][3] > 0 ? sa[it][0] * r : ea[0];
+                    v.y = sa[it][3] > 0 ? sa[it][1] * r : ea[1];
+                    v.z = sa[it][3] > 0 ? sa[it][2] * r : ea[2];
+                    v.w = 0.0f;
+                    pp[0][i / kGPW][i % kGPW] = v;
+                }
+                if (need_b) {
+                    const float r = sb[it][3] > 0 ? 1.0f / sb[it][3] : 0.0f;
+                    float4 v;
+                    v.x = sb[it][3] > 0 ? sb[it][0] * r : eb[0];
+                    v.y = sb[it][3] > 0 ? sb[it][1] * r : eb[1];
+                    v.z = sb[it][3] > 0 ? sb[it][2] * r : eb[2];
+                    v.w = 0.0f;
+                    pp[1][i / kGPW][i % kGPW] = v;
+                }
             }
+            __syncthreads();
         }
-        if (need_a || need_b) __syncthreads();
         if (inside) {
             // The footprint is all valid where the weight is 1 and all
             // missing where it is 0 (the weights are exact sums of taps).
@@ -1009,10 +1014,12 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
         // kept only where A is valid (B likewise).
         float ga[kEPer][3], gb[kEPer][3], ua[kEPer], ub[kEPer];
         unsigned f[kEPer];
-        bool any_blend = false;
+        bool any_own_a = false, any_own_b = false;
         if (l == 0) {
-            // Flags first; colours only where both images are valid (the
-            // other pixels' output came from the warp).
+            // Flags first; then, where both images are valid (the other
+            // pixels' output came from the warp), the owner's colour only:
+            // an overlap pixel's result is its owner's colour plus the
+            // expansion of the owner's difference plane.
             unsigned ab[kEPer][3];
 #pragma unroll
             for (int k = 0; k < kEPer; ++k) {
@@ -1024,14 +1031,16 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
                 const int y = y0 + static_cast<int>(threadIdx.x) / kEW + k * (kBlock / kEW);
                 ab[k][0] = ab[k][1] = ab[k][2] = 0;
                 gb[k][0] = gb[k][1] = gb[k][2] = 0;
-                if ((f[k] & (kVA | kVB)) == (kVA | kVB)) {
-                    const unsigned p = static_cast<unsigned>(y) * static_cast<unsigned>(L.w) + x;
+                const bool both = (f[k] & (kVA | kVB)) == (kVA | kVB);
+                if (both && (f[k] & kUA)) {
                     const std::uint8_t* src = a_pixel(j, x, y);
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        ab[k][c] = src[c];
-                        gb[k][c] = L.b[c * L.n + p];
-                    }
+                    for (int c = 0; c < 3; ++c) ab[k][c] = src[c];
+                }
+                if (both && !(f[k] & kUA)) {
+                    const unsigned p = static_cast<unsigned>(y) * static_cast<unsigned>(L.w) + x;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) gb[k][c] = L.b[c * L.n + p];
                 }
             }
 #pragma unroll
@@ -1040,7 +1049,9 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
                 for (int c = 0; c < 3; ++c) ga[k][c] = lut[ab[k][c]];
                 ua[k] = (f[k] & kUA) ? 1.0f : 0.0f;
                 ub[k] = (f[k] & kUB) ? 1.0f : 0.0f;
-                any_blend |= (f[k] & (kVA | kVB)) == (kVA | kVB);
+                const bool both = (f[k] & (kVA | kVB)) == (kVA | kVB);
+                any_own_a |= both && (f[k] & kUA);
+                any_own_b |= both && !(f[k] & kUA);
             }
         }
 #pragma unroll
@@ -1057,7 +1068,6 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
                 }
                 ua[k] = L.ua[p];
                 ub[k] = L.ub[p];
-                any_blend |= inside;
             }
         }
         // The coarse level behind the tile: at level 0 only for tiles with
@@ -1067,7 +1077,10 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
         if (coarse) {
             const S6Level& C = j.lv[l + 1];
             StageRGB<kCW, kCW, kCH> s0, s1, s2;
-            staged = l > 0 || __syncthreads_or(any_blend);
+            // Level 0: each difference plane only if some pixel's owner uses it.
+            const bool need_a = l == 0 && __syncthreads_or(any_own_a);
+            const bool need_b = l == 0 && __syncthreads_or(any_own_b);
+            staged = l > 0 || need_a || need_b;
             if (staged && l > 0) {
                 s0.load(C.ga, C.n, C.w, C.h, cx0, cy0);
                 s1.load(C.gb, C.n, C.w, C.h, cx0, cy0);
@@ -1078,10 +1091,10 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
                 __syncthreads();
             } else if (staged) {
                 // Level 0 expands z1 - ga1 (kept in z) and z1 - gb1 (kept in a).
-                s0.load(C.z, C.n, C.w, C.h, cx0, cy0);
-                s1.load(C.a, C.n, C.w, C.h, cx0, cy0);
-                s0.store(&cc[0][0][0]);
-                s1.store(&cc[1][0][0]);
+                if (need_a) s0.load(C.z, C.n, C.w, C.h, cx0, cy0);
+                if (need_b) s1.load(C.a, C.n, C.w, C.h, cx0, cy0);
+                if (need_a) s0.store(&cc[0][0][0]);
+                if (need_b) s1.store(&cc[1][0][0]);
                 __syncthreads();
             }
         }
