@@ -135,8 +135,7 @@ template <class Begin>
 __device__ __forceinline__ int next_job_by(const dev::JobSpan<S6Job>& span, long long t, Begin begin, int ji,
                                            int jobs) {
     if (!span.table) return 0;
-    while (ji + 1 < jobs && dev::ld_const(&begin(span.table[ji + 1])) <= t) ++ji;
-    return ji;
+    return dev::next_job_at(t, [&](int i) { return dev::ld_const(&begin(span.table[i])); }, ji, jobs);
 }
 
 constexpr unsigned kVA = 1u, kVB = 2u, kUA = 4u, kUB = 8u;

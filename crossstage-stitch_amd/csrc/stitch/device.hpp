@@ -94,14 +94,28 @@ __device__ inline int find_job(const JobSpan<Job>& span, long long t, long long 
     return lo;
 }
 
-// Job owning tile t when a workgroup visits its tiles in increasing order:
-// advances from the previous answer `ji` instead of searching the table
-// again (same result as find_job: the last job whose prefix is <= t).
+// Job owning tile t when a workgroup visits its tiles in increasing order,
+// from the previous answer `ji` (same result as find_job: the last job whose
+// prefix is <= t). The common case - t still in job ji - costs one load;
+// otherwise a binary search over the jobs after ji. (A linear walk from job
+// 0 costs one dependent scalar load per job skipped, which dominates kernels
+// whose workgroups take one or two tiles each.)
+template <class Begin>
+__device__ inline int next_job_at(long long t, Begin begin_of, int ji, int jobs) {
+    if (ji + 1 >= jobs || begin_of(ji + 1) > t) return ji;
+    int lo = ji + 1, hi = jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin_of(mid) <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 template <class Job>
 __device__ inline int next_job(const JobSpan<Job>& span, long long t, long long Job::*begin, int ji, int jobs) {
     if (!span.table) return 0;
-    while (ji + 1 < jobs && ld_const(&(span.table[ji + 1].*begin)) <= t) ++ji;
-    return ji;
+    return next_job_at(t, [&](int i) { return ld_const(&(span.table[i].*begin)); }, ji, jobs);
 }
 
 // Tile walk of a persistent grid, XCD by XCD. The dispatcher hands
