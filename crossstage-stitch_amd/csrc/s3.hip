@@ -82,11 +82,35 @@ __global__ void __launch_bounds__(256) s3_blur(dev::JobSpan<S3Job> span, dev::To
     const int tid = threadIdx.x;
     int ji = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    const long long stride = gridDim.x;
+    // The workgroup's tiles are blockIdx.x + k * gridDim.x. Most are not
+    // marked, so their marks are read 64 at a time: lane k of every wave
+    // looks up tile base + k * gridDim.x (its job by a binary search from
+    // the current one) and the ballot lists the marked ones, the same list
+    // in every wave. (Reading one mark per tile in turn would leave a
+    // memory round trip per skipped tile.)
+    for (long long base = blockIdx.x; base < total; base += 64 * stride) {
+        const long long tk = base + dev::lane_id() * stride;
+        bool marked = false;
+        if (tk < total) {
+            int jk = ji;
+            if (span.table) {
+                int hi = njobs - 1;
+                while (jk < hi) {
+                    const int mid = (jk + hi + 1) >> 1;
+                    if (span.table[mid].blur_begin <= tk) jk = mid;
+                    else hi = mid - 1;
+                }
+            }
+            const S3Job& jj = span.table ? span.table[jk] : span.single;
+            using GlobalU32 = const __attribute__((address_space(1))) unsigned;
+            marked = ((GlobalU32*)jj.marks)[tk - jj.blur_begin] == jj.epoch;
+        }
+        for (unsigned long long todo = __ballot(marked); todo; todo &= todo - 1) {
+        const long long t = base + static_cast<long long>(__ffsll(static_cast<long long>(todo)) - 1) * stride;
         ji = dev::next_job(span, t, &S3Job::blur_begin, ji, njobs);
         const S3Job& j = dev::job_at(span, ji);
         const unsigned local = static_cast<unsigned>(t - j.blur_begin), tiles_x = static_cast<unsigned>(j.tiles_x);
-        if (__builtin_amdgcn_readfirstlane(j.marks[local]) != j.epoch) continue;
         const int x0 = static_cast<int>(local % tiles_x) * kS3TileW;
         const int y0 = static_cast<int>(local / tiles_x) * kS3TileH;
         if (((reinterpret_cast<std::uintptr_t>(j.img) | static_cast<std::uintptr_t>(j.pitch)) & 3u) == 0) {
@@ -138,6 +162,7 @@ __global__ void __launch_bounds__(256) s3_blur(dev::JobSpan<S3Job> span, dev::To
             j.blurred[static_cast<size_t>(gy) * j.w + gx] = static_cast<std::uint8_t>(min(max(v, 0), 255));
         }
         __syncthreads();
+        }
     }
 }
 
