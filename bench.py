@@ -49,7 +49,7 @@ POOL_BYTES = 48 << 30
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 # Measured HBM bytes per launch of the dominant kernel on this workload
 # (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration).
-TRAFFIC_PROFILE = ROOT / "profiles" / "s6_warp_mixed64_traffic.md"
+TRAFFIC_PROFILE = ROOT / "profiles" / "s6_warp_pull_mixed64_traffic.md"
 COPY_CALIBRATION_BYTES = 1 << 30
 SERVE_BUDGET = 64 << 30
 
@@ -164,10 +164,12 @@ def measured_traffic(kernel, algorithmic_bytes):
     if not TRAFFIC_PROFILE.exists():
         return None
     text = TRAFFIC_PROFILE.read_text()
-    fields = dict(re.findall(r"^(\w+):\s*(.+?)\s*$", text, re.M))
-    if fields.get("kernel") != kernel or fields.get("algorithmic_bytes_per_launch") != str(int(algorithmic_bytes)):
-        return None
-    return int(fields["traffic_bytes_per_launch"])
+    entries = re.findall(r"^kernel:\s*(.+?)\s*\nalgorithmic_bytes_per_launch:\s*(\d+)\s*\n"
+                         r"traffic_bytes_per_launch:\s*(\d+)\s*$", text, re.M)
+    for name, alg, traffic in entries:
+        if name == kernel and alg == str(int(algorithmic_bytes)):
+            return int(traffic)
+    return None
 
 
 def roofline(stats):
