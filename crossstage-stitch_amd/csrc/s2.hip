@@ -377,16 +377,20 @@ __global__ void __launch_bounds__(256) s2_scatter(dev::JobSpan<S2Job> span, dev:
     const long long total = work.get();
     int ji = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    // A chunk that selects nothing (the count pass saw no maximum at or
+    // above the boundary score) writes nothing.
+    dev::sparse_walk(
+        total,
+        [&](long long tk) {
+            const S2Job& jj = dev::lane_job(span, tk, &S2Job::chunk_begin, ji, njobs);
+            const long long c = tk - jj.chunk_begin;
+            return (jj.chunk_counts[2 * c] | jj.chunk_counts[2 * c + 1]) != 0;
+        },
+        [&](long long t) {
         ji = dev::next_job(span, t, &S2Job::chunk_begin, ji, njobs);
         const S2Job& j = dev::job_at(span, ji);
         const long long c = t - j.chunk_begin;
         const long long pixels = static_cast<long long>(j.w) * j.h;
-        // A chunk that selects nothing (the count pass saw no maximum at or
-        // above the boundary score) writes nothing.
-        if ((__builtin_amdgcn_readfirstlane(j.chunk_counts[2 * c]) |
-             __builtin_amdgcn_readfirstlane(j.chunk_counts[2 * c + 1])) == 0)
-            continue;
         const SelectParams p = *j.params;
         const long long first = c * kS2Chunk + threadIdx.x * 16;
         std::uint32_t bytes[4];
@@ -406,7 +410,7 @@ __global__ void __launch_bounds__(256) s2_scatter(dev::JobSpan<S2Job> span, dev:
                 }
             }
         }
-    }
+        });
 }
 
 __device__ std::uint32_t order_bits(float value) {
@@ -444,21 +448,26 @@ __global__ void __launch_bounds__(256) s2_harris(dev::JobSpan<S2Job> span, dev::
     const long long total = work.get();
     int ji = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        ji = dev::next_job(span, t, &S2Job::cand_begin, ji, njobs);
-        const S2Job& j = dev::job_at(span, ji);
-        const int slot = static_cast<int>(t - j.cand_begin) * 256 + threadIdx.x;
-        if (slot >= j.candidates) continue;
-        if (static_cast<std::uint32_t>(slot) >= j.params->selected) {
-            j.keys[slot] = 0;
-            continue;
-        }
-        const auto px = static_cast<std::uint32_t>(j.keys[slot]);
-        const int x = static_cast<int>(px % static_cast<std::uint32_t>(j.w));
-        const int y = static_cast<int>(px / static_cast<std::uint32_t>(j.w));
-        const float r = harris_response(j.img, j.pitch, x, y, j.harris_k);
-        j.keys[slot] = (static_cast<std::uint64_t>(order_bits(r)) << 32) | (0xffffffffu - px);
-    }
+    // Only the slots below the selected count hold keys (the later ones are
+    // never read).
+    dev::sparse_walk(
+        total,
+        [&](long long tk) {
+            const S2Job& jj = dev::lane_job(span, tk, &S2Job::cand_begin, ji, njobs);
+            const long long slot0 = (tk - jj.cand_begin) * 256;
+            return slot0 < jj.candidates && slot0 < static_cast<long long>(jj.params->selected);
+        },
+        [&](long long t) {
+            ji = dev::next_job(span, t, &S2Job::cand_begin, ji, njobs);
+            const S2Job& j = dev::job_at(span, ji);
+            const int slot = static_cast<int>(t - j.cand_begin) * 256 + threadIdx.x;
+            if (slot >= j.candidates || static_cast<std::uint32_t>(slot) >= j.params->selected) return;
+            const auto px = static_cast<std::uint32_t>(j.keys[slot]);
+            const int x = static_cast<int>(px % static_cast<std::uint32_t>(j.w));
+            const int y = static_cast<int>(px / static_cast<std::uint32_t>(j.w));
+            const float r = harris_response(j.img, j.pitch, x, y, j.harris_k);
+            j.keys[slot] = (static_cast<std::uint64_t>(order_bits(r)) << 32) | (0xffffffffu - px);
+        });
 }
 
 // rank(key) = #keys greater. Tile = 64 keys x one 2048-key segment; 4 waves
@@ -470,15 +479,27 @@ __global__ void __launch_bounds__(256) s2_rank(dev::JobSpan<S2Job> span, dev::To
     const int lane = dev::lane_id(), wave = threadIdx.x / 64;
     int ji = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-        ji = dev::next_job(span, t, &S2Job::rank_begin, ji, njobs);
-        const S2Job& j = dev::job_at(span, ji);
+    const auto split = [](const S2Job& j, long long t, int& key0, int& seg0) {
         const unsigned local = static_cast<unsigned>(t - j.rank_begin);
         const int segs = (j.candidates + kS2RankSegment - 1) / kS2RankSegment;
-        const int kb = static_cast<int>(local / segs), sg = static_cast<int>(local % segs);
+        key0 = static_cast<int>(local / segs) * kS2RankKeys;
+        seg0 = static_cast<int>(local % segs) * kS2RankSegment;
+    };
+    dev::sparse_walk(
+        total,
+        [&](long long tk) {
+            const S2Job& jj = dev::lane_job(span, tk, &S2Job::rank_begin, ji, njobs);
+            int key0, seg0;
+            split(jj, tk, key0, seg0);
+            const int selected = static_cast<int>(jj.params->selected);
+            return key0 < selected && seg0 < selected;
+        },
+        [&](long long t) {
+        ji = dev::next_job(span, t, &S2Job::rank_begin, ji, njobs);
+        const S2Job& j = dev::job_at(span, ji);
+        int key0, seg0;
+        split(j, t, key0, seg0);
         const int selected = static_cast<int>(j.params->selected);
-        const int key0 = kb * kS2RankKeys, seg0 = sg * kS2RankSegment;
-        if (key0 >= selected || seg0 >= selected) continue;
         const int n = min(kS2RankSegment, selected - seg0);
         for (int i = threadIdx.x; i < n; i += 256) seg[i] = j.keys[seg0 + i];
         __syncthreads();
@@ -495,26 +516,33 @@ __global__ void __launch_bounds__(256) s2_rank(dev::JobSpan<S2Job> span, dev::To
             if (r) atomicAdd(&j.ranks[slot], r);
         }
         __syncthreads();
-    }
+        });
 }
 
 __global__ void __launch_bounds__(256) s2_write(dev::JobSpan<S2Job> span, dev::Total work) {
     const long long total = work.get();
     int ji = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    dev::sparse_walk(
+        total,
+        [&](long long tk) {
+            const S2Job& jj = dev::lane_job(span, tk, &S2Job::cand_begin, ji, njobs);
+            const long long slot0 = (tk - jj.cand_begin) * 256;
+            return slot0 < jj.candidates && slot0 < static_cast<long long>(jj.params->selected);
+        },
+        [&](long long t) {
         ji = dev::next_job(span, t, &S2Job::cand_begin, ji, njobs);
         const S2Job& j = dev::job_at(span, ji);
         const int slot = static_cast<int>(t - j.cand_begin) * 256 + threadIdx.x;
-        if (slot >= j.candidates || static_cast<std::uint32_t>(slot) >= j.params->selected) continue;
+        if (slot >= j.candidates || static_cast<std::uint32_t>(slot) >= j.params->selected) return;
         const std::uint32_t rank = j.ranks[slot];
-        if (rank >= static_cast<std::uint32_t>(j.K)) continue;
+        if (rank >= static_cast<std::uint32_t>(j.K)) return;
         const std::uint64_t key = j.keys[slot];
         const std::uint32_t px = 0xffffffffu - static_cast<std::uint32_t>(key);
         j.out_x[rank] = static_cast<float>(px % static_cast<std::uint32_t>(j.w));
         j.out_y[rank] = static_cast<float>(px / static_cast<std::uint32_t>(j.w));
         if (j.out_resp) j.out_resp[rank] = from_order_bits(static_cast<std::uint32_t>(key >> 32));
-    }
+        });
 }
 
 bool valid_image_shape(int w, int h) { return w > 0 && h > 0 && w <= INT_MAX / h; }

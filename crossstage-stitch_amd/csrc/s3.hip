@@ -198,16 +198,25 @@ __global__ void __launch_bounds__(64 * kS3KeypointsPerBlock) s3_describe(dev::Jo
     const int wave = threadIdx.x / 64;
     int ji = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    // Tiles are sized by the keypoint capacity; only those below the
+    // detected count hold keypoints.
+    dev::sparse_walk(
+        total,
+        [&](long long tk) {
+            const S3Job& jj = dev::lane_job(span, tk, &S3Job::kp_begin, ji, njobs);
+            const long long index0 = (tk - jj.kp_begin) * kS3KeypointsPerBlock;
+            return index0 < jj.capacity && index0 < *jj.d_count;
+        },
+        [&](long long t) {
         ji = dev::next_job(span, t, &S3Job::kp_begin, ji, njobs);
         const S3Job& j = dev::job_at(span, ji);
         const int index = static_cast<int>(t - j.kp_begin) * kS3KeypointsPerBlock + wave;
-        if (index >= j.capacity || index >= *j.d_count) continue;
+        if (index >= j.capacity || index >= *j.d_count) return;
         std::uint8_t* row = j.desc + static_cast<size_t>(index) * j.desc_stride;
         int x, y;
         if (!keypoint_centre(j, index, x, y)) {
             if (lane < 32) row[lane] = 0;
-            continue;
+            return;
         }
         const std::uint8_t* centre = j.img + static_cast<size_t>(y) * j.pitch + x;
         int m10 = 0, m01 = 0;
@@ -241,7 +250,7 @@ __global__ void __launch_bounds__(64 * kS3KeypointsPerBlock) s3_describe(dev::Jo
             bits[r] = __ballot(t0 < t1);
         }
         if (lane < 32) row[lane] = static_cast<std::uint8_t>(bits[lane >> 3] >> (8 * (lane & 7)));
-    }
+        });
 }
 
 std::size_t align256(std::size_t v) { return (v + 255) / 256 * 256; }

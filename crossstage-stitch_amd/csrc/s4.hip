@@ -68,15 +68,26 @@ __global__ void __launch_bounds__(256) s4_dist(dev::JobSpan<S4Job> span, dev::To
     const int lane = dev::lane_id(), wave = threadIdx.x / 64;
     int ji = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    // Tiles are sized by the keypoint capacities; only those inside the
+    // detected counts hold work.
+    dev::sparse_walk(
+        total,
+        [&](long long tk) {
+            const S4Job& jj = dev::lane_job(span, tk, &S4Job::dist_begin, ji, njobs);
+            const unsigned local = static_cast<unsigned>(tk - jj.dist_begin), segs = static_cast<unsigned>(jj.segs);
+            int na, nb;
+            return counts_valid(jj, &na, &nb) && static_cast<int>(local / segs) * kS4QueryBlock < na &&
+                   static_cast<int>(local % segs) * jj.segment < nb;
+        },
+        [&](long long t) {
         ji = dev::next_job(span, t, &S4Job::dist_begin, ji, njobs);
         const S4Job& j = dev::job_at(span, ji);
         const unsigned local = static_cast<unsigned>(t - j.dist_begin), segs = static_cast<unsigned>(j.segs);
         const int qb = static_cast<int>(local / segs), sg = static_cast<int>(local % segs);
         int na, nb;
-        if (!counts_valid(j, &na, &nb)) continue;
+        if (!counts_valid(j, &na, &nb)) return;
         const int q0 = qb * kS4QueryBlock, t0 = sg * j.segment;
-        if (q0 >= na || t0 >= nb) continue;
+        if (q0 >= na || t0 >= nb) return;
         const bool kCross = kCrossAny && j.cross_check;
         const int rows = min(j.segment, nb - t0);
         for (int i = threadIdx.x; i < 2 * rows; i += 256)
@@ -136,7 +147,7 @@ __global__ void __launch_bounds__(256) s4_dist(dev::JobSpan<S4Job> span, dev::To
                 j.part_col[static_cast<size_t>(qb) * j.cap_b + t0 + i] = colmin[i];
         }
         __syncthreads();
-    }
+        });
 }
 
 __device__ Top2 empty_top2() { return {INFINITY, INFINITY, -1, -1}; }

@@ -143,6 +143,38 @@ __device__ inline TileWalk tile_walk(long long total) {
 
 __device__ inline int lane_id() { return __lane_id(); }
 
+// Job owning tile t, looked up by one lane on its own (vector loads, a
+// binary search over the jobs from `from`), for predicates that lanes
+// evaluate over different tiles at once.
+template <class Job>
+__device__ inline const Job& lane_job(const JobSpan<Job>& span, long long t, long long Job::*begin, int from, int jobs) {
+    if (!span.table) return span.single;
+    int lo = from, hi = jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (span.table[mid].*begin <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return span.table[lo];
+}
+
+// Walks the workgroup's tiles blockIdx.x + k * gridDim.x (k = 0, 1, ...) in
+// groups of 64: lane k evaluates keep(t) for the group's k-th tile, and
+// body(t) runs, in increasing order, for the tiles kept - the same list in
+// every wave, so body may use barriers. For kernels whose tile range is
+// sized by a capacity and mostly empty: one memory round trip per 64 tiles
+// finds the work, instead of one per tile.
+template <class Keep, class Body>
+__device__ inline void sparse_walk(long long total, Keep keep, Body body) {
+    const long long stride = gridDim.x;
+    for (long long base = blockIdx.x; base < total; base += 64 * stride) {
+        const long long tk = base + static_cast<long long>(lane_id()) * stride;
+        const bool kept = tk < total && keep(tk);
+        for (unsigned long long todo = __ballot(kept); todo; todo &= todo - 1)
+            body(base + static_cast<long long>(__ffsll(static_cast<long long>(todo)) - 1) * stride);
+    }
+}
+
 template <class T>
 __device__ inline T wave_sum(T v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
