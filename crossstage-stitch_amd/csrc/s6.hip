@@ -191,28 +191,41 @@ __device__ void flush_stats(const S6Job& j, unsigned long long u, unsigned long 
 // box that can hold B). Multiband: level-0 flags and B's colour where valid.
 // Feather: the output and the union mask. Union/overlap counters for both.
 // The border scores are only needed where both images are valid.
-__device__ __forceinline__ void warp_pixel(const S6Job& j, int x, int y, const float* lut, unsigned long long& uni,
-                                           unsigned long long& ovl) {
+// Terms of a canvas row shared by its pixels: the y products of the inverse
+// map's three rows and A's vertical border score. Each is the same rounded
+// value the per-pixel expression would compute (products are rounded on
+// their own, and the sums keep the reference's left-to-right order).
+struct WarpRow {
+    double ya, ty0, ty1, ty2, sya;
+};
+
+__device__ __forceinline__ WarpRow warp_row(const S6Job& j, int y) {
+    const double ya = static_cast<double>(y + j.origin_y);
+    return {ya, j.inverse[1] * ya, j.inverse[4] * ya, j.inverse[7] * ya, fmin(ya + 0.5, j.ha - 0.5 - ya) / j.ha};
+}
+
+__device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int x, int y, const float* lut,
+                                           unsigned long long& uni, unsigned long long& ovl) {
     const size_t p = static_cast<size_t>(y) * j.width + x;
     const int ax = x + j.origin_x, ay = y + j.origin_y;
-    const double xa = static_cast<double>(ax), ya = static_cast<double>(ay);
+    const double xa = static_cast<double>(ax), ya = r.ya;
     float ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
     const bool ma = ax >= 0 && ax <= j.wa - 1 && ay >= 0 && ay <= j.ha - 1;
     bool mb = false;
     double xb = 0, yb = 0;
     if (x >= j.bx0 && x <= j.bx1 && y >= j.by0 && y <= j.by1) {
         const double* inv = j.inverse;
-        const double den = inv[6] * xa + inv[7] * ya + inv[8];
-        const double dsc = fabs(inv[6] * xa) + fabs(inv[7] * ya) + fabs(inv[8]);
+        const double den = inv[6] * xa + r.ty2 + inv[8];
+        const double dsc = fabs(inv[6] * xa) + fabs(r.ty2) + fabs(inv[8]);
         if (isfinite(den) && dsc > 0 && fabs(den) > j.denominator_tolerance * dsc) {
-            xb = (inv[0] * xa + inv[1] * ya + inv[2]) / den;
-            yb = (inv[3] * xa + inv[4] * ya + inv[5]) / den;
+            xb = (inv[0] * xa + r.ty0 + inv[2]) / den;
+            yb = (inv[3] * xa + r.ty1 + inv[5]) / den;
             mb = sample(j.b, j.wb, j.hb, j.pitch_b, xb, yb, rb, nullptr, j.snap_tolerance);
         }
     }
     double sa = 0, sb = 0;
     if (ma && mb) {
-        sa = fmin(fmin(xa + 0.5, j.wa - 0.5 - xa) / j.wa, fmin(ya + 0.5, j.ha - 0.5 - ya) / j.ha);
+        sa = fmin(fmin(xa + 0.5, j.wa - 0.5 - xa) / j.wa, r.sya);
         sb = fmin(fmin(xb + 0.5, j.wb - 0.5 - xb) / j.wb, fmin(yb + 0.5, j.hb - 0.5 - yb) / j.hb);
     }
     if (j.top >= 0) {
@@ -288,10 +301,11 @@ __global__ void __launch_bounds__(kBlock) s6_warp(dev::JobSpan<S6Job> span, dev:
         const unsigned segs = (static_cast<unsigned>(j.width) + kWarpSeg - 1) / kWarpSeg;
         const unsigned row = local / segs;
         const int x0 = static_cast<int>((local - row * segs) * kWarpSeg + threadIdx.x);
+        const WarpRow wr = warp_row(j, static_cast<int>(row));
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int x = x0 + k * kBlock;
-            if (x < j.width) warp_pixel(j, x, static_cast<int>(row), lut, uni, ovl);
+            if (x < j.width) warp_pixel(j, wr, x, static_cast<int>(row), lut, uni, ovl);
         }
     }
     if (cur >= 0 && dev::job_at(span, cur).stats) flush_stats(dev::job_at(span, cur), uni, ovl, red);
