@@ -820,7 +820,7 @@ void BatchRunner::initialize(int max_pairs, std::size_t pool_bytes, bool use_gra
         s.render_pool.initialize(slot_bytes > pair_bytes_total ? slot_bytes - pair_bytes_total : kAlign);
     }
     features_.initialize(max_pairs, use_graphs);
-    render_.initialize(max_pairs, use_graphs);
+    for (RenderEngine& r : render_) r.initialize(max_pairs, use_graphs);
 }
 
 namespace {
@@ -830,13 +830,15 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 }
 }  // namespace
 
-void BatchRunner::complete(Slot& s) {
+void BatchRunner::complete(int slot) {
+    Slot& s = slots_[slot];
+    RenderEngine& render = render_[slot];
     if (!s.pending) return;
-    render_.wait();
-    last_render_ms_ = s.owner.empty() ? 0.0 : render_.last_ms();
+    render.wait();
+    last_render_ms_ = s.owner.empty() ? 0.0 : render.last_ms();
     for (std::size_t k = 0; k < s.owner.size(); ++k) {
         PairOutcome& o = s.out[s.owner[k]];
-        o.render = render_.results()[k];
+        o.render = render.results()[k];
         if (o.render.status != Status::OK) {
             o.status = o.render.status;
             o.fail_stage = "s6";
@@ -862,7 +864,7 @@ void BatchRunner::submit(const PairSpec* pairs, int n, std::vector<PairOutcome>*
     const auto t0 = Clock::now();
     Slot& prev = slots_[cur_ ^ 1];
     Slot& s = slots_[cur_];
-    complete(s);  // two batches back: long done
+    complete(cur_);  // two batches back: long done
     if (previous) previous->clear();
     const bool had_prev = prev.pending;
     s.pair_pool.reset();
@@ -886,10 +888,6 @@ void BatchRunner::submit(const PairSpec* pairs, int n, std::vector<PairOutcome>*
     features_.submit(tasks.data(), n);
     features_.wait();
     const auto t1 = Clock::now();
-    // The previous batch's render must finish before its results are read
-    // and before the render tables are rebuilt.
-    complete(prev);
-    if (previous && had_prev) *previous = prev.out;
     s.render_pool.reset();
     std::vector<RenderTask> renders;
     for (int i = 0; i < n; ++i) {
@@ -933,8 +931,12 @@ void BatchRunner::submit(const PairSpec* pairs, int n, std::vector<PairOutcome>*
         s.owner.push_back(i);
     }
     const auto t2 = Clock::now();
-    render_.submit(renders.data(), static_cast<int>(renders.size()));
+    // This batch's S6 goes onto its slot's render stream before the previous
+    // batch's S6 (on the other slot's stream) is waited for and read.
+    render_[cur_].submit(renders.data(), static_cast<int>(renders.size()));
     s.pending = true;
+    complete(cur_ ^ 1);
+    if (previous && had_prev) *previous = prev.out;
     cur_ ^= 1;
     if (timings) {
         timings->feature_ms = features_.last_ms();
@@ -946,7 +948,7 @@ void BatchRunner::submit(const PairSpec* pairs, int n, std::vector<PairOutcome>*
 
 void BatchRunner::finish(std::vector<PairOutcome>* out, BatchTimings* timings) {
     Slot& last = slots_[cur_ ^ 1];
-    complete(last);
+    complete(cur_ ^ 1);
     if (out) *out = last.out;
     if (timings) timings->render_ms = last_render_ms_;
 }
@@ -970,9 +972,33 @@ void BatchRunner::apply(const Tuning& t) {
     tuning_.s4_segment = jobs::s4_segment();
     tuning_.workgroups_per_cu = jobs::workgroups_per_cu();
     features_.invalidate();
-    render_.invalidate();
     features_.set_use_graph(t.graphs);
-    render_.set_use_graph(t.graphs);
+    for (RenderEngine& r : render_) {
+        r.invalidate();
+        r.set_use_graph(t.graphs);
+    }
+}
+
+void BatchRunner::set_probe(bool on) {
+    for (RenderEngine& r : render_) r.set_probe(on);
+}
+
+const std::vector<RenderEngine::ProbeStat>& BatchRunner::probe_stats() {
+    probe_merged_.clear();
+    for (const RenderEngine& r : render_)
+        for (const RenderEngine::ProbeStat& e : r.probe_stats()) {
+            auto it = std::find_if(probe_merged_.begin(), probe_merged_.end(), [&](const RenderEngine::ProbeStat& m) {
+                return m.kernel == e.kernel && m.level == e.level;
+            });
+            if (it == probe_merged_.end()) {
+                probe_merged_.push_back(e);
+            } else {
+                it->launches += e.launches;
+                it->ms += e.ms;
+                it->bytes += e.bytes;
+            }
+        }
+    return probe_merged_;
 }
 
 namespace {

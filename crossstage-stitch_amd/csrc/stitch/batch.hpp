@@ -302,8 +302,9 @@ public:
     Tuning autotune(const PairSpec* pairs, int n, const std::string& cache_path = "");
     void apply(const Tuning& t);
     const Tuning& tuning() const { return tuning_; }
-    void set_probe(bool on) { render_.set_probe(on); }
-    const std::vector<RenderEngine::ProbeStat>& probe_stats() const { return render_.probe_stats(); }
+    void set_probe(bool on);
+    // Both slots' render engines, summed per (kernel, level).
+    const std::vector<RenderEngine::ProbeStat>& probe_stats();
     // Summed over both slots.
     void pool_use(std::size_t* capacity, std::size_t* in_use, std::size_t* peak) const;
 
@@ -316,9 +317,14 @@ private:
         std::vector<int> owner;
         bool pending{false};
     };
-    void complete(Slot& s);
+    void complete(int slot);
     FeatureEngine features_;
-    RenderEngine render_;
+    // One render engine (stream, job tables, graphs) per slot: a batch's S6
+    // is enqueued before the previous batch's S6 has finished, so the render
+    // work of consecutive batches runs back to back (or overlapped) with no
+    // idle gap for the host's geometry and table preparation.
+    RenderEngine render_[2];
+    std::vector<RenderEngine::ProbeStat> probe_merged_;
     Slot slots_[2];
     int cur_{0};
     double last_render_ms_{0};
