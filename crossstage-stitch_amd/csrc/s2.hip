@@ -88,23 +88,18 @@ __device__ int fast_score_lds(const std::uint8_t* g, int threshold) {
         return (y & (x >> 8)) != 0;
     };
     if (!arc9(dark) && !arc9(bright)) return 0;
-    // cornerScore: the best over the 16 arcs of 9 of min(d) and of -max(d),
-    // by doubling windows (2, 4, 8, then the ninth pixel).
-    int lo2[16], hi2[16], lo4[16], hi4[16], best = INT_MIN;
+    // cornerScore: the best over the 16 arcs of 9 of min(d) and of -max(d).
+    // Each arc is scanned directly (only d[] stays live: this path is rare,
+    // and fewer registers keep the kernel's occupancy up).
+    int best = INT_MIN;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-        lo2[k] = min(d[k], d[(k + 1) & 15]);
-        hi2[k] = max(d[k], d[(k + 1) & 15]);
-    }
+        int lo = d[k], hi = d[k];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        lo4[k] = min(lo2[k], lo2[(k + 2) & 15]);
-        hi4[k] = max(hi2[k], hi2[(k + 2) & 15]);
-    }
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const int lo = min(min(lo4[k], lo4[(k + 4) & 15]), d[(k + 8) & 15]);
-        const int hi = max(max(hi4[k], hi4[(k + 4) & 15]), d[(k + 8) & 15]);
+        for (int i = 1; i < 9; ++i) {
+            lo = min(lo, d[(k + i) & 15]);
+            hi = max(hi, d[(k + i) & 15]);
+        }
         best = max(best, max(lo, -hi));
     }
     return best > threshold ? best - 1 : 0;
@@ -169,6 +164,7 @@ __global__ void __launch_bounds__(256) s2_fast_nms(dev::JobSpan<S2Job> span, dev
         // the candidate test runs on every lane (indexes clamped into the
         // tile, conditions combined without branches) and the full score only
         // where it passed.
+        #pragma unroll 1
         for (int base = 0; base < kScoreH * kScoreW; base += 256) {
             const int i = base + tid, ic = min(i, kScoreH * kScoreW - 1);
             const int sx = ic % kScoreW, sy = ic / kScoreW;
@@ -181,6 +177,7 @@ __global__ void __launch_bounds__(256) s2_fast_nms(dev::JobSpan<S2Job> span, dev
         }
         __syncthreads();
         static_assert(kS2TileW * kS2TileH % 256 == 0, "NMS trips");
+        #pragma unroll 1
         for (int base = 0; base < kS2TileW * kS2TileH; base += 256) {
             const int i = base + tid;
             const int lx = i % kS2TileW, ly = i / kS2TileW;
