@@ -203,6 +203,9 @@ SIGNATURES = {
     "cs_s6_workspace_bytes": (C.c_int, [P(Geometry), C.c_int, P(sz)]),
     "cs_s6_render": (C.c_int, [P(Image), P(Image), P(Geometry), C.c_int, P(Image), P(Image), vp, sz, vp,
                                P(RenderStatistics)]),
+    "cs_extract_scene": (C.c_int, [P(Image), P(Image), vp, vp, vp, C.c_int]),
+    "cs_extract_gate": (C.c_int, [P(Image), P(Image), P(Keypoints), P(Keypoints), P(Matches), vp,
+                                  P(HomographyResult), P(ScaleInfo), P(ScaleInfo), vp, vp, vp, vp, C.c_uint]),
     "cs_runner_create": (vp, []),
     "cs_runner_destroy": (None, [vp]),
     "cs_runner_initialize": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -216,6 +219,8 @@ SIGNATURES = {
     "cs_runner_outputs": (C.c_int, [vp, P(Image), P(Image), P(Keypoints), P(Keypoints), P(Descriptors),
                                     P(Descriptors), P(Matches), P(vp), P(HomographyResult)]),
     "cs_runner_stream": (vp, [vp]),
+    "cs_runner_set_collect_features": (C.c_int, [vp, C.c_int]),
+    "cs_runner_features": (C.c_int, [vp, vp, vp, P(C.c_int), P(C.c_double)]),
     "cs_runner_attempt_trace": (C.c_int, [vp, P(AttemptRecord), C.c_int, P(C.c_int)]),
     "cs_controller_create_legacy": (vp, [C.c_int]),
     "cs_controller_destroy": (None, [vp]),
@@ -242,6 +247,11 @@ SIGNATURES = {
     "cs_server_run_trace": (C.c_int, [vp, P(Request), C.c_int, P(Completion), P(ServeStats)]),
     "cs_server_release": (C.c_int, [vp, C.c_uint64]),
 }
+
+# Controller observation sizes (src/controller/features.hpp:10-13).
+SCENE_FEATURES = 32
+GATE_FEATURES = 38
+FEATURE_DEVICE_DOUBLES = 136
 
 _lib = None
 

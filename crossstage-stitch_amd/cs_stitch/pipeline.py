@@ -99,6 +99,20 @@ class PipelineRunner:
     def set_flags(self, reuse=True, probe=True):
         F.load().cs_runner_set_flags(self.handle, int(reuse), int(probe))
 
+    def set_collect_features(self, enabled=True):
+        """Scene features once each pair is on the device, gate features after
+        each successful S5 of execute_run (PipelineRunner::set_collect_features)."""
+        _check(F.load().cs_runner_set_collect_features(self.handle, int(enabled)), "runner.set_collect_features")
+
+    def features(self):
+        """(scene[32], gate[38], features_present, t_features) of the last run."""
+        scene = np.zeros(F.SCENE_FEATURES)
+        gate = np.zeros(F.GATE_FEATURES)
+        present, t = C.c_int(), C.c_double()
+        _check(F.load().cs_runner_features(self.handle, scene.ctypes.data, gate.ctypes.data, C.byref(present),
+                                           C.byref(t)), "runner.features")
+        return scene, gate, bool(present.value), t.value
+
     def outputs(self):
         """Device views of the last run's outputs (valid until the next run)."""
         canvas, mask = F.Image(), F.Image()

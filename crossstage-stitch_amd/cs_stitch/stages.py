@@ -285,3 +285,36 @@ def s6_render(a: torch.Tensor, b: torch.Tensor, g: F.Geometry, mode: int, stats:
                           C.cast(counters.data_ptr(), C.POINTER(F.RenderStatistics)) if stats else None),
            "s6_render")
     return out, mask, counters
+
+
+# ---------------------------------------------------------------- controller observations
+
+def extract_scene(ga: torch.Tensor, gb: torch.Tensor, relations: bool = True):
+    """Scene features of two GRAY8 device images (controller::extract_scene,
+    src/controller/features.cu:125-135): (status, 32 doubles or None)."""
+    L = F.load()
+    dev = torch.zeros(F.FEATURE_DEVICE_DOUBLES, dtype=torch.float64, device="cuda")
+    host = torch.zeros(F.GATE_FEATURES, dtype=torch.float64, pin_memory=True)
+    ia, ib = image(ga), image(gb)
+    st = L.cs_extract_scene(C.byref(ia), C.byref(ib), dev.data_ptr(), host.data_ptr(), stream_handle(),
+                            int(relations))
+    torch.cuda.current_stream().synchronize()
+    return st, (host[: F.SCENE_FEATURES].numpy().copy() if st == F.OK else None)
+
+
+def extract_gate(a: torch.Tensor, b: torch.Tensor, kp_a: F.Keypoints, kp_b: F.Keypoints, matches: F.Matches,
+                 mask_ptr: int, h: F.HomographyResult, sa: F.ScaleInfo, sb: F.ScaleInfo, forward_ptr: int | None,
+                 relation_mask: int = 7):
+    """Gate features of a finished S5 (controller::extract_gate,
+    src/controller/features.cu:136-170): (status, 38 doubles or None).
+    Keypoints, matches, the inlier mask and the forward top-2 view are
+    device views (C structs / pointers)."""
+    L = F.load()
+    dev = torch.zeros(F.FEATURE_DEVICE_DOUBLES, dtype=torch.float64, device="cuda")
+    host = torch.zeros(F.GATE_FEATURES, dtype=torch.float64, pin_memory=True)
+    ia, ib = image(a), image(b)
+    st = L.cs_extract_gate(C.byref(ia), C.byref(ib), C.byref(kp_a), C.byref(kp_b), C.byref(matches), mask_ptr,
+                           C.byref(h), C.byref(sa), C.byref(sb), dev.data_ptr(), host.data_ptr(), stream_handle(),
+                           forward_ptr, relation_mask)
+    torch.cuda.current_stream().synchronize()
+    return st, (host.numpy().copy() if st == F.OK else None)

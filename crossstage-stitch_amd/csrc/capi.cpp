@@ -5,6 +5,7 @@
 
 #include <exception>
 
+#include "stitch/features.hpp"
 #include "stitch/stages.hpp"
 #include "stitch/tuning.hpp"
 
@@ -247,6 +248,28 @@ int cs_s6_render(const cs_image* a, const cs_image* b, const cs_geometry* g, int
     return guard([&] {
         return s6::render(img(a), img(b), geo(g), static_cast<s6::BlendMode>(mode), img(out), img(mask), ws, bytes,
                           st(stream), reinterpret_cast<s6::RenderStatistics*>(stats));
+    });
+}
+
+static_assert(CS_SCENE_FEATURES == features::kSceneCount && CS_GATE_FEATURES == features::kGateCount &&
+                  CS_FEATURE_DEVICE_DOUBLES == features::kDeviceDoubles,
+              "feature counts");
+static_assert(sizeof(cs_top2) == sizeof(s4::Top2), "Top2 layout");
+
+int cs_extract_scene(const cs_image* a, const cs_image* b, double* device, double* host, void* stream, int relations) {
+    if (!a || !b) return CS_INVALID_ARGUMENT;
+    return guard([&] { return features::extract_scene(img(a), img(b), device, host, st(stream), relations != 0); });
+}
+
+int cs_extract_gate(const cs_image* a, const cs_image* b, const cs_keypoints* ka, const cs_keypoints* kb,
+                    const cs_matches* m, const uint8_t* mask, const cs_homography_result* h, const cs_scale_info* sa,
+                    const cs_scale_info* sb, double* device, double* host, void* stream, const cs_top2* forward,
+                    unsigned relation_mask) {
+    if (!a || !b || !ka || !kb || !m || !h || !sa || !sb) return CS_INVALID_ARGUMENT;
+    return guard([&] {
+        return features::extract_gate(img(a), img(b), kps(ka), kps(kb), mt(m), mask,
+                                      *reinterpret_cast<const HomographyResult*>(h), scale(sa), scale(sb), device,
+                                      host, st(stream), reinterpret_cast<const s4::Top2*>(forward), relation_mask);
     });
 }
 

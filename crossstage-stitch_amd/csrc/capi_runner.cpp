@@ -1,4 +1,5 @@
 // extern "C" layer for the runner, the controller and the synthetic inputs.
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -11,6 +12,7 @@ using namespace stitch;
 
 struct cs_runner {
     harness::PipelineRunner impl;
+    harness::RunRecord last;  // the last run's record (its feature observations)
 };
 
 struct cs_controller {
@@ -164,6 +166,7 @@ int cs_runner_execute_run(cs_runner* r, const cs_harness_config* c, cs_run_recor
         harness::RunRecord rec;
         r->impl.execute_run(config(c), &rec);
         put_record(rec, r->impl.last_gpu_span_ms(), out);
+        r->last = std::move(rec);
         return CS_OK;
     } catch (...) {
         return map_exception();
@@ -176,10 +179,27 @@ int cs_runner_execute_adaptive_run(cs_runner* r, cs_controller* ctl, uint64_t se
         harness::RunRecord rec;
         r->impl.execute_adaptive_run(ctl->impl, seed, &rec);
         put_record(rec, -1, out);
+        r->last = std::move(rec);
         return CS_OK;
     } catch (...) {
         return map_exception();
     }
+}
+
+int cs_runner_set_collect_features(cs_runner* r, int enabled) {
+    if (!r) return CS_INVALID_ARGUMENT;
+    r->impl.set_collect_features(enabled != 0);
+    return CS_OK;
+}
+
+int cs_runner_features(cs_runner* r, double* scene, double* gate, int* present, double* t_features) {
+    if (!r) return CS_INVALID_ARGUMENT;
+    const harness::RunRecord& rec = r->last;
+    if (scene) std::copy(rec.scene_features.begin(), rec.scene_features.end(), scene);
+    if (gate) std::copy(rec.gate_features.begin(), rec.gate_features.end(), gate);
+    if (present) *present = rec.features_present;
+    if (t_features) *t_features = rec.t_features;
+    return CS_OK;
 }
 
 int cs_runner_attempt_trace(cs_runner* r, cs_attempt_record* out, int capacity, int* count) {

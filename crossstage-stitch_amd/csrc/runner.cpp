@@ -56,8 +56,9 @@ PipelineRunner::~PipelineRunner() {
                     static_cast<void*>(matches_.distance), static_cast<void*>(matches_.d_count),
                     static_cast<void*>(inlier_mask_), static_cast<void*>(canvas_.data),
                     static_cast<void*>(canvas_mask_.data), static_cast<void*>(stats_), ws_s2_a_, ws_s2_b_, ws_s3_,
-                    ws_s4_, ws_s5_, ws_s6_, ws_probe_})
+                    ws_s4_, ws_s5_, ws_s6_, ws_probe_, static_cast<void*>(d_features_)})
         if (p) hipFree(p);
+    if (pinned_features_) hipHostFree(pinned_features_);
     if (pinned_h_) hipHostFree(pinned_h_);
     if (pinned_stats_) hipHostFree(pinned_stats_);
     if (pinned_counts_) hipHostFree(pinned_counts_);
@@ -74,6 +75,7 @@ PipelineMemoryPlan PipelineRunner::query_memory(int w, int h, int k, int r, int 
     plan.device_bytes = add(plan.device_bytes, add(image_bytes(cw, ch, 3), image_bytes(cw, ch, 1)));
     plan.device_bytes = add(plan.device_bytes, add(multiply(k, 6 * sizeof(float) + 64 + 3 * sizeof(int) + 1),
                                                    3 * sizeof(int) + sizeof(s6::RenderStatistics) + 32));
+    plan.device_bytes = add(plan.device_bytes, features::kDeviceDoubles * sizeof(double));
     auto workspace = [&](std::size_t bytes, int count = 1) {
         plan.workspace_bytes = add(plan.workspace_bytes, multiply(bytes, count));
         if (bytes) plan.device_bytes = add(plan.device_bytes, multiply(add(bytes, 256), count));
@@ -172,6 +174,10 @@ void PipelineRunner::initialize(int max_w, int max_h, int max_k, int max_r, int 
     bytes_s6_ = plan.s6_workspace_bytes;
     if (bytes_s6_) ws_s6_ = alloc(bytes_s6_);
     ws_probe_ = alloc(32);
+    d_features_ = static_cast<double*>(alloc(features::kDeviceDoubles * sizeof(double)));
+    check(hipHostMalloc(reinterpret_cast<void**>(&pinned_features_), features::kGateCount * sizeof(double),
+                        hipHostMallocDefault),
+          "pin");
     use_a_ = img_a_;
     use_b_ = img_b_;
     initialized_ = true;
@@ -195,6 +201,24 @@ void PipelineRunner::upload_pair(const std::uint8_t* a, int wa, int ha, const st
     check(hipMemcpy2DAsync(use_a_.data, use_a_.pitch_bytes, a, wa * 3, wa * 3, ha, hipMemcpyHostToDevice, stream_), "h2d");
     check(hipMemcpy2DAsync(use_b_.data, use_b_.pitch_bytes, b, wb * 3, wb * 3, hb, hipMemcpyHostToDevice, stream_), "h2d");
     check(hipStreamSynchronize(stream_), "sync");
+    if (collect_features_) collect_scene();
+}
+
+// Scene observations of the pair on the device: S1 at S = 0.25 into the gray
+// buffers, then the scene features (harness/pipeline_runner.cu:295-305).
+void PipelineRunner::collect_scene() {
+    ScaleInfo a{}, b{};
+    make_scale_info(wa_, ha_, 0.25, &a);
+    make_scale_info(wb_, hb_, 0.25, &b);
+    gray_a_.width = a.work_w;
+    gray_a_.height = a.work_h;
+    gray_b_.width = b.work_w;
+    gray_b_.height = b.work_h;
+    if (s1::s1_preprocess(use_a_, use_b_, 0.25, &gray_a_, &gray_b_, &a, &b, stream_) != Status::OK ||
+        features::extract_scene(gray_a_, gray_b_, d_features_, pinned_features_, stream_, true) != Status::OK)
+        throw std::runtime_error("Scene feature collection failed");
+    check(hipStreamSynchronize(stream_), "sync");
+    std::copy_n(pinned_features_, features::kSceneCount, collected_scene_.begin());
 }
 
 void PipelineRunner::use_device_pair(const ImageView& a, const ImageView& b) {
@@ -206,6 +230,7 @@ void PipelineRunner::use_device_pair(const ImageView& a, const ImageView& b) {
     ha_ = a.height;
     wb_ = b.width;
     hb_ = b.height;
+    if (collect_features_) collect_scene();
 }
 
 void PipelineRunner::execute_run(const HarnessConfig& cfg, RunRecord* rec) {
@@ -217,6 +242,7 @@ void PipelineRunner::execute_run(const HarnessConfig& cfg, RunRecord* rec) {
     rec->R = cfg.R;
     rec->blend_mode = static_cast<int>(cfg.blend_mode);
     rec->attempt_count = 1;
+    if (collect_features_) rec->scene_features = collected_scene_;
     auto fail = [&](const char* status, const char* stage) {
         rec->status = status;
         rec->fail_stage = stage;
@@ -286,6 +312,23 @@ void PipelineRunner::execute_run(const HarnessConfig& cfg, RunRecord* rec) {
         rec->t_e2e = ms_since(t0);
         stage_times(false);
         return fail(status_name(h.status), "s5");
+    }
+    if (collect_features_) {
+        // Gate observations of this attempt (harness/pipeline_runner.cu:445-458).
+        const auto begin = Clock::now();
+        const s4::Top2* forward = nullptr;
+        if (s4::s4_forward_neighbors(ka.capacity, kb.capacity, {cfg.ratio_threshold, cfg.cross_check}, ws_s4_,
+                                     bytes_s4_, &forward) != Status::OK)
+            throw std::runtime_error("Missing matching feature observations");
+        const Status fs = features::extract_gate(use_a_, use_b_, ka, kb, matches_, inlier_mask_, h, sa, sb, d_features_,
+                                                 pinned_features_, stream_, forward, 7);
+        if (fs == Status::CUDA_ERROR) throw std::runtime_error("Gate feature device error");
+        check(hipStreamSynchronize(stream_), "sync");
+        if (fs == Status::OK) {
+            std::copy_n(pinned_features_, features::kGateCount, rec->gate_features.begin());
+            rec->features_present = true;
+        }
+        rec->t_features = ms_since(begin);
     }
     s6::Geometry geom{};
     Status gs = s6::prepare_geometry(wa_, ha_, wb_, hb_, h.H, s6::GeometryPolicy{}, &geom);

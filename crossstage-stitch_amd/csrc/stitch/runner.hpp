@@ -12,6 +12,7 @@
 #include <string>
 
 #include "stitch/controller.hpp"
+#include "stitch/features.hpp"
 #include "stitch/stages.hpp"
 
 namespace stitch::harness {
@@ -53,6 +54,12 @@ struct RunRecord {
     int cache_hit{0};
     std::string status{"OK"};
     std::string fail_stage{};
+    // Controller observations, filled when feature collection is on
+    // (harness/harness_types.hpp:72-76).
+    bool features_present{false};
+    double t_features{0};
+    features::SceneFeatures scene_features{};
+    features::GateFeatures gate_features{};
 };
 
 // Bounded per-attempt trace of the adaptive run (harness/attempt_trace.hpp:10-31):
@@ -103,6 +110,11 @@ public:
     void use_device_pair(const ImageView& a, const ImageView& b);
     void set_reuse_enabled(bool enabled) { reuse_enabled_ = enabled; }
     void set_probe_enabled(bool enabled) { probe_enabled_ = enabled; }
+    // Scene features at S = 0.25 once each pair is on the device, gate
+    // features after every successful S5 of execute_run (the reference's
+    // set_collect_features, harness/pipeline_runner.hpp:56).
+    void set_collect_features(bool enabled) { collect_features_ = enabled; }
+    bool collect_features() const { return collect_features_; }
 
     void execute_run(const HarnessConfig& config, RunRecord* record);
     void execute_adaptive_run(controller::Controller& controller, std::uint64_t seed, RunRecord* record);
@@ -124,7 +136,12 @@ public:
 private:
     AttemptTrace attempt_trace_{};
     void check(hipError_t e, const char* what);
+    void collect_scene();
     bool reuse_enabled_{true};
+    bool collect_features_{false};
+    features::SceneFeatures collected_scene_{};
+    double* d_features_{nullptr};
+    double* pinned_features_{nullptr};
     bool probe_enabled_{true};
     hipStream_t stream_{nullptr};
     hipEvent_t ev_[8]{};

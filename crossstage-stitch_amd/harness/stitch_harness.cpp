@@ -563,6 +563,8 @@ int main(int argc, char** argv) {
     H::PipelineRunner runner;
     runner.set_reuse_enabled(reuse);
     runner.set_probe_enabled(probe);
+    // Observations are collected by the fixed and oracle modes only (main.cpp:322).
+    runner.set_collect_features(!features_path.empty() && mode != "adaptive");
     try {
         int max_w = 2048, max_h = 2048;
         for (const PairEntry& p : pairs)
@@ -596,7 +598,10 @@ int main(int argc, char** argv) {
     if (!features_path.empty()) {
         features.open(features_path);
         if (!features) return 1;
-        features << "pair_id,config_id,seed,repeat_idx,is_warmup,features_present,t_features\n";
+        features << "pair_id,config_id,seed,repeat_idx,is_warmup,features_present,t_features";
+        for (int i = 0; i < stitch::features::kSceneCount; ++i) features << ",scene_" << i;
+        for (int i = 0; i < stitch::features::kGateCount; ++i) features << ",gate_" << i;
+        features << '\n';
     }
     std::cout << "Starting benchmark harness:\n  Mode: " << mode << "\n  Pairs: " << pairs.size()
               << "\n  Configs: " << (mode == "adaptive" ? policy.ladder.size() : cfgs.size())
@@ -668,14 +673,20 @@ int main(int argc, char** argv) {
                     rec.config_id = c->cfg.config_id;
                 }
                 runs_row(runs, L, rec, blend_out);
-                if (features)
-                    features << pair.pair_id << ',' << rec.config_id << ',' << rec.seed << ',' << L.repeat_idx << ','
-                             << L.is_warmup << ",0,0\n";
+                if (features) {
+                    features << std::setprecision(17) << pair.pair_id << ',' << rec.config_id << ',' << rec.seed
+                             << ',' << L.repeat_idx << ',' << L.is_warmup << ',' << rec.features_present << ','
+                             << rec.t_features;
+                    for (double v : rec.scene_features) features << ',' << v;
+                    for (double v : rec.gate_features) features << ',' << v;
+                    features << '\n';
+                }
                 if (!L.is_warmup) ++measured;
             }
             runs.flush();
             if (trace) trace.flush();
-            return static_cast<bool>(runs);
+            if (features) features.flush();
+            return static_cast<bool>(runs) && (!features_path.size() || static_cast<bool>(features));
         };
         if (mode == "adaptive") {
             if (!emit(nullptr)) return 2;

@@ -126,6 +126,27 @@ int cs_s6_render(const cs_image* a, const cs_image* b, const cs_geometry* geomet
                  const cs_image* output, const cs_image* mask, void* workspace, size_t workspace_bytes,
                  void* stream, cs_render_statistics* statistics);
 
+/* Controller observations (src/controller/features.hpp:10-32,
+ * relational_features.hpp:8-24): scene features of two GRAY8 images before
+ * matching and gate features of a finished S5. device holds
+ * CS_FEATURE_DEVICE_DOUBLES doubles of device memory; host receives 32
+ * (scene) or 38 (gate) doubles, valid once the stream is synchronized.
+ * extract_scene replaces controller::extract_scene (features.cu:125);
+ * extract_gate replaces controller::extract_gate (features.cu:136), with
+ * relation_mask bits 1 matching, 2 geometry, 4 photometric. An unusable
+ * homography or fewer than 4 matches returns INVALID_ARGUMENT (nothing
+ * enqueued); inputs the relations reject return CUDA_ERROR. */
+#define CS_SCENE_FEATURES 32
+#define CS_GATE_FEATURES 38
+#define CS_FEATURE_DEVICE_DOUBLES 136
+int cs_extract_scene(const cs_image* a, const cs_image* b, double* device, double* host, void* stream,
+                     int relations);
+int cs_extract_gate(const cs_image* a, const cs_image* b, const cs_keypoints* keypoints_a,
+                    const cs_keypoints* keypoints_b, const cs_matches* matches, const uint8_t* inlier_mask,
+                    const cs_homography_result* homography, const cs_scale_info* scale_a,
+                    const cs_scale_info* scale_b, double* device, double* host, void* stream,
+                    const cs_top2* forward, unsigned relation_mask);
+
 /* Plain per-pair runner: harness/pipeline_runner.hpp:32-81 (PipelineRunner),
  * harness/harness_types.hpp:14-106 (HarnessConfig, RunRecord). */
 typedef struct cs_runner cs_runner;
@@ -166,6 +187,13 @@ int cs_runner_execute_run(cs_runner* runner, const cs_harness_config* config, cs
 int cs_runner_execute_adaptive_run(cs_runner* runner, cs_controller* controller, uint64_t seed,
                                    cs_run_record* record);
 int cs_runner_set_flags(cs_runner* runner, int reuse_enabled, int probe_enabled);
+/* Feature collection of the plain runner (PipelineRunner::set_collect_features,
+ * harness/pipeline_runner.hpp:56): scene features once a pair is on the
+ * device, gate features after each successful S5 of execute_run. */
+int cs_runner_set_collect_features(cs_runner* runner, int enabled);
+/* Observations of the last execute_run (harness/harness_types.hpp:72-76):
+ * scene (32 doubles) and gate (38 doubles) may be NULL. */
+int cs_runner_features(cs_runner* runner, double* scene, double* gate, int* features_present, double* t_features);
 int cs_runner_outputs(cs_runner* runner, cs_image* canvas, cs_image* mask, cs_keypoints* keypoints_a,
                       cs_keypoints* keypoints_b, cs_descriptors* descriptors_a, cs_descriptors* descriptors_b,
                       cs_matches* matches, uint8_t** inlier_mask, cs_homography_result* homography);

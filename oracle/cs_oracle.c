@@ -1205,3 +1205,519 @@ int orc_s6_render(const uint8_t* a, int pitch_a, const uint8_t* b, int pitch_b, 
     }
     return ORC_OK;
 }
+
+/* ======================================================================
+ * Controller observations: scene and gate features
+ * (src/controller/features.cu:11-171, src/controller/relational_features.cu:6-190)
+ *
+ * Every FP64 statistic is summed in the reference's order: N partial sums
+ * (partial t takes elements t, t + N, t + 2N, ... in increasing order), then
+ * the N partials added pairwise (v[t] += v[t + N/2] for t < N/2, then N/4,
+ * ... 1). N is 128, and 64 for the 7x7 patches.
+ *
+ * Pinning: closed forms in tests/test_features.py (constant, ramp and
+ * checkerboard images give exact means, deviations and gradients; identical
+ * images give NCC 1 at zero shift and zero photometric error under H = I;
+ * a pure translation puts the NCC peak at its shift; hand-counted inlier
+ * grids give the occupancy, largest-bin and entropy values; a brute-force
+ * double sum agrees with the tree sums to rounding). log and hypot come from
+ * the C library; their last-bit agreement with the device's is unpinned.
+ * ====================================================================== */
+
+typedef struct {
+    const uint8_t* data;
+    int w, h, pitch, format; /* format: 0 RGB8, 1 GRAY8 (src/common/types.hpp) */
+} orc_view;
+
+#define OBS_N 128
+#define OBS_PATCH_N 64
+#define OBS_DEVICE_ERROR 3
+
+/* features.cu:12-16 */
+static double obs_gray(const orc_view* im, int x, int y) {
+    const uint8_t* p = im->data + (size_t)y * im->pitch + (size_t)x * (im->format == 0 ? 3 : 1);
+    if (im->format == 1) return p[0] / 255.;
+    return (9798. * p[0] + 19235. * p[1] + 3735. * p[2]) / (32768. * 255.);
+}
+
+/* features.cu:17-23 */
+static double obs_sample(const orc_view* im, double x, double y) {
+    const int ix = (int)x, iy = (int)y;
+    const int jx = ix + 1 < im->w - 1 ? ix + 1 : im->w - 1, jy = iy + 1 < im->h - 1 ? iy + 1 : im->h - 1;
+    const double dx = x - ix, dy = y - iy;
+    return (1 - dy) * ((1 - dx) * obs_gray(im, ix, iy) + dx * obs_gray(im, jx, iy)) +
+           dy * ((1 - dx) * obs_gray(im, ix, jy) + dx * obs_gray(im, jx, jy));
+}
+
+/* features.cu:24-33: pairwise tree over n partials, n a power of two. */
+static double obs_tree(double* v, int n) {
+    for (int step = n / 2; step; step /= 2)
+        for (int t = 0; t < step; ++t) v[t] += v[t + step];
+    return v[0];
+}
+
+/* relational_features.cu:17-20 */
+static double obs_correlation(double a, double b, double aa, double bb, double ab, double n) {
+    const double va = fmax(0., aa - a * a / n), vb = fmax(0., bb - b * b / n);
+    return fmin(1., fmax(-1., (ab - a * b / n) / fmax(1e-12, sqrt(va * vb))));
+}
+
+/* Adjugate inverse, features.cu:115-123. */
+static int obs_inverse(const double* h, double* out) {
+    const double d = h[0] * (h[4] * h[8] - h[5] * h[7]) - h[1] * (h[3] * h[8] - h[5] * h[6]) +
+                     h[2] * (h[3] * h[7] - h[4] * h[6]);
+    if (!finite_d(d) || fabs(d) < 1e-18) return 0;
+    const double adj[9] = {h[4] * h[8] - h[5] * h[7], h[2] * h[7] - h[1] * h[8], h[1] * h[5] - h[2] * h[4],
+                           h[5] * h[6] - h[3] * h[8], h[0] * h[8] - h[2] * h[6], h[2] * h[3] - h[0] * h[5],
+                           h[3] * h[7] - h[4] * h[6], h[1] * h[6] - h[0] * h[7], h[0] * h[4] - h[1] * h[3]};
+    for (int i = 0; i < 9; ++i) {
+        out[i] = adj[i] / d;
+        if (!finite_d(out[i])) return 0;
+    }
+    return 1;
+}
+
+/* relational_features.cu:155-159 */
+static int obs_valid_image(const orc_view* im) {
+    return im->data && im->w >= 2 && im->h >= 2 && im->pitch > 0 && (im->format == 0 || im->format == 1) &&
+           im->pitch >= im->w * (im->format == 0 ? 3 : 1);
+}
+
+/* scene_kernel, features.cu:34-62: region 0/1 whole-image gradient of A/B,
+ * regions 2..9 quadrant (mean, std, gradient). */
+static void obs_scene_region(const orc_view* a, const orc_view* b, int region, double* out) {
+    const int image = region < 2 ? region : (region - 2) / 4;
+    const orc_view* im = image ? b : a;
+    const int tile = region < 2 ? -1 : (region - 2) % 4;
+    const int x0 = tile < 0 ? 0 : (tile % 2) * im->w / 2, y0 = tile < 0 ? 0 : (tile / 2) * im->h / 2;
+    const int x1 = tile < 0 ? im->w : ((tile % 2) + 1) * im->w / 2;
+    const int y1 = tile < 0 ? im->h : ((tile / 2) + 1) * im->h / 2;
+    const int w = x1 - x0, h = y1 - y0;
+    double sum[OBS_N], sq[OBS_N], grad[OBS_N];
+    for (int t = 0; t < OBS_N; ++t) {
+        double s = 0, q = 0, g = 0;
+        for (int i = t; i < w * h; i += OBS_N) {
+            const int x = x0 + i % w, y = y0 + i / w;
+            const double v = obs_gray(im, x, y);
+            s += v;
+            q += v * v;
+            if (x + 1 < x1) g += fabs(v - obs_gray(im, x + 1, y));
+            if (y + 1 < y1) g += fabs(v - obs_gray(im, x, y + 1));
+        }
+        sum[t] = s;
+        sq[t] = q;
+        grad[t] = g;
+    }
+    const double total = obs_tree(sum, OBS_N), squares = obs_tree(sq, OBS_N), gradient = obs_tree(grad, OBS_N);
+    const double mean = total / (w * h), edges = h * (w - 1) + w * (h - 1);
+    if (tile < 0) {
+        out[region] = edges ? gradient / edges : 0.;
+    } else {
+        double* o = out + 2 + image * 12 + tile * 3;
+        o[0] = mean;
+        o[1] = sqrt(fmax(0., squares / (w * h) - mean * mean));
+        o[2] = edges ? gradient / edges : 0.;
+    }
+}
+
+/* pair_kernel and pair_finish, relational_features.cu:27-49: NCC of a 32x32
+ * grid of A against B shifted by (dx, dy) in {-3..3}/8; out[0..5]. */
+static void obs_pair_relations(const orc_view* a, const orc_view* b, double* out) {
+    double score[49], overlap[49];
+    for (int shift = 0; shift < 49; ++shift) {
+        const double dx = (shift % 7 - 3) / 8., dy = (shift / 7 - 3) / 8.;
+        double p[6][OBS_N];
+        for (int t = 0; t < OBS_N; ++t) {
+            double n = 0, sa = 0, sb = 0, aa = 0, bb = 0, ab = 0;
+            for (int i = t; i < 1024; i += OBS_N) {
+                const double x = (i % 32) / 31., y = (i / 32) / 31., u = x + dx, v = y + dy;
+                if (u < 0 || u > 1 || v < 0 || v > 1) continue;
+                const double av = obs_sample(a, x * (a->w - 1), y * (a->h - 1));
+                const double bv = obs_sample(b, u * (b->w - 1), v * (b->h - 1));
+                ++n;
+                sa += av;
+                sb += bv;
+                aa += av * av;
+                bb += bv * bv;
+                ab += av * bv;
+            }
+            p[0][t] = n;
+            p[1][t] = sa;
+            p[2][t] = sb;
+            p[3][t] = aa;
+            p[4][t] = bb;
+            p[5][t] = ab;
+        }
+        double s[6];
+        for (int k = 0; k < 6; ++k) s[k] = obs_tree(p[k], OBS_N);
+        score[shift] = obs_correlation(s[1], s[2], s[3], s[4], s[5], s[0]);
+        overlap[shift] = s[0] / 1024.;
+    }
+    int best = 0;
+    for (int j = 1; j < 49; ++j)
+        if (score[j] > score[best]) best = j;
+    double second = -2;
+    for (int j = 0; j < 49; ++j)
+        if (j != best) second = fmax(second, score[j]);
+    out[0] = score[best];
+    out[1] = score[best] - second;
+    out[2] = (best % 7 - 3) / 8.;
+    out[3] = (best / 7 - 3) / 8.;
+    out[4] = score[24];
+    out[5] = overlap[best];
+}
+
+/* extract_scene, features.cu:125-135: 32 doubles. Returns 0 (OK),
+ * 1 (INVALID_ARGUMENT) or 3 (the reference's CUDA_ERROR for images the
+ * pair relations reject). */
+int orc_scene_features(const uint8_t* da, int wa, int ha, int pa, int fa, const uint8_t* db, int wb, int hb, int pb,
+                       int fb, int relations, double* out) {
+    const orc_view a = {da, wa, ha, pa, fa}, b = {db, wb, hb, pb, fb};
+    if (!da || !db || !out || wa < 2 || wb < 2 || ha < 2 || hb < 2 || fa != 1 || fb != 1) return ORC_INVALID_ARGUMENT;
+    for (int region = 0; region < 10; ++region) obs_scene_region(&a, &b, region, out);
+    for (int k = 26; k < 32; ++k) out[k] = 0;
+    if (relations) {
+        if (!obs_valid_image(&a) || !obs_valid_image(&b)) return OBS_DEVICE_ERROR;
+        obs_pair_relations(&a, &b, out + 26);
+    }
+    return ORC_OK;
+}
+
+/* photo_kernel, features.cu:64-90: out[10..17]. */
+static void obs_photo(const orc_view* a, const orc_view* b, const double* m, double* out) {
+    double p[8][OBS_N];
+    for (int t = 0; t < OBS_N; ++t) {
+        double n = 0, sum_a = 0, sum_b = 0, sq_a = 0, sq_b = 0, cross = 0, abs_error = 0, sq_error = 0;
+        for (int i = t; i < 4096; i += OBS_N) {
+            const double x = (i % 64) * (a->w - 1.) / 63., y = (i / 64) * (a->h - 1.) / 63.;
+            const double z = m[6] * x + m[7] * y + m[8];
+            if (fabs(z) < 1e-12) continue;
+            const double u = (m[0] * x + m[1] * y + m[2]) / z, v = (m[3] * x + m[4] * y + m[5]) / z;
+            if (u < 0 || v < 0 || u > b->w - 1 || v > b->h - 1 || !finite_d(u) || !finite_d(v)) continue;
+            const double av = obs_sample(a, fmax(0., x), fmax(0., y)), bv = obs_sample(b, u, v), d = av - bv;
+            n += 1;
+            sum_a += av;
+            sum_b += bv;
+            sq_a += av * av;
+            sq_b += bv * bv;
+            cross += av * bv;
+            abs_error += fabs(d);
+            sq_error += d * d;
+        }
+        p[0][t] = n;
+        p[1][t] = sum_a;
+        p[2][t] = sum_b;
+        p[3][t] = sq_a;
+        p[4][t] = sq_b;
+        p[5][t] = cross;
+        p[6][t] = abs_error;
+        p[7][t] = sq_error;
+    }
+    double s[8];
+    for (int k = 0; k < 8; ++k) s[k] = obs_tree(p[k], OBS_N);
+    const double den = fmax(1., s[0]), ma = s[1] / den, mb = s[2] / den;
+    const double va = fmax(0., s[3] / den - ma * ma), vb = fmax(0., s[4] / den - mb * mb);
+    const double sd = sqrt(va) + sqrt(vb);
+    out[10] = s[0] / 4096.;
+    out[11] = s[6] / den;
+    out[12] = sqrt(s[7] / den);
+    out[13] = fmin(1., fmax(-1., (s[5] / den - ma * mb) / fmax(1e-12, sqrt(va * vb))));
+    out[14] = fabs(ma - mb);
+    out[15] = sqrt(va);
+    out[16] = sqrt(vb);
+    out[17] = fmin(1e6, s[6] / den / fmax(1e-6, sd));
+}
+
+/* inlier_kernel, features.cu:91-114: out[18..23]; bins from float
+ * coordinates. */
+static void obs_inliers(const float* ax, const float* ay, int cap_a, const float* bx, const float* by, int cap_b,
+                        const int32_t* ia_, const int32_t* ib_, const uint8_t* mask, int count, int wa, int ha,
+                        int wb, int hb, double* out) {
+    int hist[32] = {0};
+    for (int i = 0; i < count; ++i) {
+        if (!mask[i]) continue;
+        const int ia = ia_[i], ib = ib_[i];
+        if (ia < 0 || ib < 0 || ia >= cap_a || ib >= cap_b) continue;
+        int xa = (int)(4 * ax[ia] / wa), ya = (int)(4 * ay[ia] / ha);
+        int xb = (int)(4 * bx[ib] / wb), yb = (int)(4 * by[ib] / hb);
+        xa = xa < 0 ? 0 : xa > 3 ? 3 : xa;
+        ya = ya < 0 ? 0 : ya > 3 ? 3 : ya;
+        xb = xb < 0 ? 0 : xb > 3 ? 3 : xb;
+        yb = yb < 0 ? 0 : yb > 3 ? 3 : yb;
+        ++hist[ya * 4 + xa];
+        ++hist[16 + yb * 4 + xb];
+    }
+    for (int image = 0; image < 2; ++image) {
+        int total = 0, occupied = 0, largest = 0;
+        for (int j = 0; j < 16; ++j) {
+            const int c = hist[image * 16 + j];
+            total += c;
+            occupied += c > 0;
+            largest = c > largest ? c : largest;
+        }
+        double entropy = 0;
+        if (total)
+            for (int j = 0; j < 16; ++j)
+                if (hist[image * 16 + j]) {
+                    const double p = (double)hist[image * 16 + j] / total;
+                    entropy -= p * log(p) / log(16.);
+                }
+        out[18 + image] = occupied / 16.;
+        out[20 + image] = total ? (double)largest / total : 0.;
+        out[22 + image] = entropy;
+    }
+}
+
+/* statistics_kernel, relational_features.cu:103-154: out[0..7] of the
+ * alignment relations. */
+static void obs_statistics(const float* ax, const float* ay, int cap_a, const float* bx, const float* by, int cap_b,
+                           const int32_t* ia_, const int32_t* ib_, const float* distance, const uint8_t* mask,
+                           int num_matches, const double* H, const orc_scale_info* sa, const orc_scale_info* sb,
+                           const orc_top2* forward, unsigned groups, double* out) {
+    static double regional[16][OBS_N];
+    int residual_hist[257] = {0}, distance_hist[257] = {0}, counts[16] = {0};
+    double dsum[OBS_N], inl[OBS_N];
+    memset(regional, 0, sizeof regional);
+    for (int t = 0; t < OBS_N; ++t) {
+        int inliers = 0;
+        double distance_sum = 0;
+        for (int i = t; i < num_matches; i += OBS_N) {
+            const int ia = ia_[i], ib = ib_[i];
+            if (ia < 0 || ib < 0 || ia >= cap_a || ib >= cap_b) continue;
+            if (groups & 1) {
+                const double d = fmin(256., fmax(0., (double)distance[i]));
+                distance_sum += d;
+                ++distance_hist[(int)d];
+            }
+            if (!(groups & 2) || !mask[i]) continue;
+            const double x = (bx[ib] + .5) / sb->sx - .5, y = (by[ib] + .5) / sb->sy - .5;
+            const double z = H[6] * x + H[7] * y + H[8];
+            if (!finite_d(z) || fabs(z) < 1e-12) continue;
+            const double u = (H[0] * x + H[1] * y + H[2]) / z, v = (H[3] * x + H[4] * y + H[5]) / z;
+            const double e = hypot(u - ((ax[ia] + .5) / sa->sx - .5), v - ((ay[ia] + .5) / sa->sy - .5));
+            if (!finite_d(e)) continue;
+            ++residual_hist[(int)(fmin(32., e) * 8)];
+            ++inliers;
+            int tx = (int)(ax[ia] * 4 / sa->work_w), ty = (int)(ay[ia] * 4 / sa->work_h);
+            tx = tx < 0 ? 0 : tx > 3 ? 3 : tx;
+            ty = ty < 0 ? 0 : ty > 3 ? 3 : ty;
+            ++counts[tx + 4 * ty];
+            regional[tx + 4 * ty][t] += fmin(1e6, e);
+        }
+        dsum[t] = distance_sum;
+        inl[t] = inliers;
+    }
+    const double distance_sum = obs_tree(dsum, OBS_N);
+    const int inliers = (int)obs_tree(inl, OBS_N);
+    double maximum = 0, sum = 0, sq = 0;
+    int regions = 0;
+    if (groups & 2)
+        for (int j = 0; j < 16; ++j) {
+            const double total = obs_tree(regional[j], OBS_N);
+            if (counts[j]) {
+                const double v = total / counts[j];
+                maximum = fmax(maximum, v);
+                sum += v;
+                sq += v * v;
+                ++regions;
+            }
+        }
+    double rs[OBS_N], rc[OBS_N], am[OBS_N];
+    for (int t = 0; t < OBS_N; ++t) {
+        double ratio_sum = 0;
+        int ratios = 0, ambiguous = 0;
+        if (forward && (groups & 1))
+            for (int j = t; j < cap_a; j += OBS_N) {
+                const orc_top2 p = forward[j];
+                if (p.first_index < 0 || p.second_index < 0 || !finite_d(p.second_distance)) continue;
+                const double ratio = p.second_distance > 0 ? p.first_distance / p.second_distance : 1.;
+                ratio_sum += ratio;
+                ++ratios;
+                ambiguous += ratio >= .8;
+            }
+        rs[t] = ratio_sum;
+        rc[t] = ratios;
+        am[t] = ambiguous;
+    }
+    const double ratio_sum = obs_tree(rs, OBS_N);
+    const int ratios = (int)obs_tree(rc, OBS_N), ambiguous = (int)obs_tree(am, OBS_N);
+    int seen = 0, p50 = 256, p90 = 256, dp90 = 256;
+    for (int j = 0; j < 257; ++j) {
+        seen += residual_hist[j];
+        if (seen >= ceil(.5 * inliers) && p50 == 256) p50 = j;
+        if (seen >= ceil(.9 * inliers)) {
+            p90 = j;
+            break;
+        }
+    }
+    seen = 0;
+    for (int j = 0; j < 257; ++j) {
+        seen += distance_hist[j];
+        if (seen >= ceil(.9 * num_matches)) {
+            dp90 = j;
+            break;
+        }
+    }
+    out[0] = inliers ? p50 / 8. : 32;
+    out[1] = inliers ? p90 / 8. : 32;
+    out[2] = maximum;
+    out[3] = regions ? sqrt(fmax(0., sq / regions - (sum / regions) * (sum / regions))) : 0;
+    out[4] = distance_sum / fmax(1., (double)num_matches) / 256.;
+    out[5] = dp90 / 256.;
+    out[6] = ratios ? ratio_sum / ratios : 1;
+    out[7] = ratios ? (double)ambiguous / ratios : 1;
+}
+
+/* relational_features.cu:60-66 */
+static int obs_warp(const orc_view* b, const double* m, double x, double y, double* value) {
+    const double z = m[6] * x + m[7] * y + m[8];
+    if (!finite_d(z) || fabs(z) < 1e-12) return 0;
+    const double u = (m[0] * x + m[1] * y + m[2]) / z, v = (m[3] * x + m[4] * y + m[5]) / z;
+    if (!finite_d(u) || !finite_d(v) || u < 0 || v < 0 || u > b->w - 1 || v > b->h - 1) return 0;
+    *value = obs_sample(b, u, v);
+    return 1;
+}
+
+/* patch_kernel and patch_finish, relational_features.cu:67-102: out[8..13]
+ * of the alignment relations. */
+static void obs_patches(const orc_view* a, const orc_view* b, const double* m, double* out) {
+    double ws[16][5];
+    for (int patch = 0; patch < 16; ++patch) {
+        const double cx = (patch % 4 + .5) * a->w / 4., cy = (patch / 4 + .5) * a->h / 4.;
+        double p[11][OBS_PATCH_N];
+        for (int t = 0; t < OBS_PATCH_N; ++t) {
+            double n = 0, sa = 0, sb = 0, aa = 0, bb = 0, ab = 0, mae = 0, edge = 0, dot = 0, ga2 = 0, gb2 = 0;
+            for (int i = t; i < 49; i += OBS_PATCH_N) {
+                const int dx = i % 7 - 3, dy = i / 7 - 3;
+                const double x = cx + dx, y = cy + dy;
+                double bv, bxv, byv;
+                if (x < 0 || y < 0 || x + 1 > a->w - 1 || y + 1 > a->h - 1 || !obs_warp(b, m, x, y, &bv) ||
+                    !obs_warp(b, m, x + 1, y, &bxv) || !obs_warp(b, m, x, y + 1, &byv))
+                    continue;
+                const double av = obs_sample(a, x, y), gx = obs_sample(a, x + 1, y) - av,
+                             gy = obs_sample(a, x, y + 1) - av;
+                bxv -= bv;
+                byv -= bv;
+                ++n;
+                sa += av;
+                sb += bv;
+                aa += av * av;
+                bb += bv * bv;
+                ab += av * bv;
+                mae += fabs(av - bv);
+                edge += fabs(gx - bxv) + fabs(gy - byv);
+                dot += gx * bxv + gy * byv;
+                ga2 += gx * gx + gy * gy;
+                gb2 += bxv * bxv + byv * byv;
+            }
+            p[0][t] = n;
+            p[1][t] = sa;
+            p[2][t] = sb;
+            p[3][t] = aa;
+            p[4][t] = bb;
+            p[5][t] = ab;
+            p[6][t] = mae;
+            p[7][t] = edge;
+            p[8][t] = dot;
+            p[9][t] = ga2;
+            p[10][t] = gb2;
+        }
+        double s[11];
+        for (int k = 0; k < 11; ++k) s[k] = obs_tree(p[k], OBS_PATCH_N);
+        const double den = fmax(s[0], 1.), ma = s[1] / den, mb = s[2] / den;
+        const double va = fmax(0., s[3] / den - ma * ma), vb = fmax(0., s[4] / den - mb * mb);
+        ws[patch][0] = ((2 * ma * mb + .0001) * (2 * (s[5] / den - ma * mb) + .0009)) /
+                       ((ma * ma + mb * mb + .0001) * (va + vb + .0009));
+        ws[patch][1] = s[7] / (2 * den);
+        ws[patch][2] = s[8] / fmax(1e-12, sqrt(s[9] * s[10]));
+        ws[patch][3] = fmin(1e6, s[6] / den / fmax(1e-6, sqrt(va) + sqrt(vb)));
+        ws[patch][4] = s[0] == 49;
+    }
+    int n = 0;
+    double sum = 0, low = 1, eg = 0, co = 0, errors[16];
+    for (int j = 0; j < 16; ++j)
+        if (ws[j][4]) {
+            errors[n++] = ws[j][3];
+            sum += ws[j][0];
+            low = fmin(low, ws[j][0]);
+            eg += ws[j][1];
+            co += ws[j][2];
+        }
+    for (int j = 1; j < n; ++j) {
+        const double v = errors[j];
+        int k = j;
+        while (k && errors[k - 1] > v) {
+            errors[k] = errors[k - 1];
+            --k;
+        }
+        errors[k] = v;
+    }
+    out[8] = n ? sum / n : -1;
+    out[9] = n ? low : -1;
+    out[10] = n ? eg / n : 1;
+    out[11] = n ? co / n : 0;
+    out[12] = n / 16.;
+    out[13] = n ? errors[(int)ceil(.9 * n) - 1] : 1e6;
+}
+
+/* extract_gate, features.cu:136-170 with alignment_relations,
+ * relational_features.cu:168-189: 38 doubles. The homography result is
+ * passed field by field (status must be OK). Returns 0, 1 (INVALID_ARGUMENT:
+ * nothing computed) or 3 (inputs the alignment relations reject). */
+int orc_gate_features(const uint8_t* da, int wa, int ha, int pa, int fa, const uint8_t* db, int wb, int hb, int pb,
+                      int fb, const float* ax, const float* ay, int cap_a, const float* bx, const float* by,
+                      int cap_b, const int32_t* idx_a, const int32_t* idx_b, const float* distance, int capacity,
+                      const uint8_t* mask, int num_matches, int num_inliers, double inlier_ratio,
+                      double inlier_coverage, double reproj_residual, const double* H, const orc_scale_info* sa,
+                      const orc_scale_info* sb, const orc_top2* forward, unsigned relation_mask, double* out) {
+    const orc_view a = {da, wa, ha, pa, fa}, b = {db, wb, hb, pb, fb};
+    double inv[9];
+    if (!da || !db || !out || !mask || num_matches < 4 || num_matches > capacity || !obs_inverse(H, inv))
+        return ORC_INVALID_ARGUMENT;
+    double x[4], y[4], zmin = 1e300, zmax = 0.;
+    out[0] = num_matches / 4000.;
+    out[1] = num_inliers / 4000.;
+    out[2] = inlier_ratio;
+    out[3] = inlier_coverage;
+    out[4] = reproj_residual;
+    out[5] = reproj_residual / (sa->sx < sa->sy ? sa->sy : sa->sx);
+    for (int i = 0; i < 4; ++i) {
+        const double u = (i == 1 || i == 2) ? wb - 1. : 0., v = i >= 2 ? hb - 1. : 0.;
+        const double z = H[6] * u + H[7] * v + H[8];
+        if (!finite_d(z) || fabs(z) < 1e-12) return ORC_INVALID_ARGUMENT;
+        x[i] = (H[0] * u + H[1] * v + H[2]) / z;
+        y[i] = (H[3] * u + H[4] * v + H[5]) / z;
+        if (!finite_d(x[i]) || !finite_d(y[i])) return ORC_INVALID_ARGUMENT;
+        zmin = fmin(zmin, fabs(z));
+        zmax = fmax(zmax, fabs(z));
+    }
+    double area = 0.;
+    for (int i = 0; i < 4; ++i) area += x[i] * y[(i + 1) % 4] - y[i] * x[(i + 1) % 4];
+    double xl = x[0], xh = x[0], yl = y[0], yh = y[0];
+    for (int i = 1; i < 4; ++i) {
+        xl = x[i] < xl ? x[i] : xl;
+        xh = xh < x[i] ? x[i] : xh;
+        yl = y[i] < yl ? y[i] : yl;
+        yh = yh < y[i] ? y[i] : yh;
+    }
+    out[6] = fmin(1e6, fabs(area) / 2. / ((double)wb * hb));
+    out[7] = fmin(1e6, (xh - xl) / wa);
+    out[8] = fmin(1e6, (yh - yl) / ha);
+    out[9] = fmin(1e6, zmax / zmin);
+    obs_photo(&a, &b, inv, out);
+    obs_inliers(ax, ay, cap_a, bx, by, cap_b, idx_a, idx_b, mask, num_matches, sa->work_w, sa->work_h, sb->work_w,
+                sb->work_h, out);
+    for (int k = 24; k < 38; ++k) out[k] = 0;
+    if (relation_mask) {
+        if (!obs_valid_image(&a) || !obs_valid_image(&b) || !idx_a || !idx_b || !distance || !ax || !ay || !bx ||
+            !by || cap_a < 1 || cap_a > 4000 || cap_b < 1 || cap_b > 4000 || !finite_d(sa->sx) ||
+            !finite_d(sa->sy) || !finite_d(sb->sx) || !finite_d(sb->sy) || sa->sx <= 0 || sa->sy <= 0 ||
+            sb->sx <= 0 || sb->sy <= 0 || sa->work_w <= 0 || sa->work_h <= 0 || relation_mask > 7)
+            return OBS_DEVICE_ERROR;
+        if (relation_mask & 3)
+            obs_statistics(ax, ay, cap_a, bx, by, cap_b, idx_a, idx_b, distance, mask, num_matches, H, sa, sb,
+                           forward, relation_mask, out + 24);
+        if (relation_mask & 4) obs_patches(&a, &b, inv, out + 24);
+    }
+    return ORC_OK;
+}

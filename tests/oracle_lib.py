@@ -106,6 +106,13 @@ def lib():
         L.orc_s6_effective_reductions.argtypes = [C.c_int, C.c_int, C.c_int]
         L.orc_s6_render.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(Geometry), C.c_int, u8p, u8p,
                                     C.POINTER(C.c_ulonglong)]
+        vp = C.c_void_p
+        L.orc_scene_features.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, vp]
+        L.orc_gate_features.argtypes = ([vp, C.c_int, C.c_int, C.c_int, C.c_int] * 2 +
+                                        [vp, vp, C.c_int, vp, vp, C.c_int, vp, vp, vp, C.c_int, vp, C.c_int,
+                                         C.c_int, C.c_double, C.c_double, C.c_double, vp, C.POINTER(ScaleInfo),
+                                         C.POINTER(ScaleInfo), vp, C.c_uint, vp])
         _lib = L
     return _lib
 
@@ -241,3 +248,53 @@ def s6_render(a: np.ndarray, b: np.ndarray, g: Geometry, mode: int):
     st = lib().orc_s6_render(a, a.shape[1] * 3, b, b.shape[1] * 3, C.byref(g), mode, out, mask, stats)
     assert st == 0
     return out, mask, (stats[0], stats[1])
+
+
+def _image(im: np.ndarray):
+    """(h, w) GRAY8 or (h, w, 3) RGB8 -> (array, w, h, pitch, format)."""
+    im = np.ascontiguousarray(im, np.uint8)
+    return im, im.shape[1], im.shape[0], im.strides[0], 1 if im.ndim == 2 else 0
+
+
+def scene_features(gray_a: np.ndarray, gray_b: np.ndarray, relations: bool = True):
+    """(status, 32 doubles) of features.cu:125-135."""
+    a, wa, ha, pa, fa = _image(gray_a)
+    b, wb, hb, pb, fb = _image(gray_b)
+    out = np.zeros(32)
+    st = lib().orc_scene_features(a.ctypes.data, wa, ha, pa, fa, b.ctypes.data, wb, hb, pb, fb, int(relations),
+                                  out.ctypes.data)
+    return st, out
+
+
+def gate_features(img_a, img_b, kp_a, kp_b, idx_a, idx_b, distance, mask, num_matches, num_inliers,
+                  inlier_ratio, inlier_coverage, reproj_residual, h, sa, sb, forward=None, relation_mask=7,
+                  capacity=None):
+    """(status, 38 doubles) of features.cu:136-170. kp_a / kp_b: (capacity, 2)
+    float32 keypoint x, y arrays (rows up to the keypoint capacity); forward:
+    a list of (first_distance, second_distance, first_index, second_index)
+    per row of kp_a, or None."""
+    a, wa, ha, pa, fa = _image(img_a)
+    b, wb, hb, pb, fb = _image(img_b)
+    kax = np.ascontiguousarray(kp_a[:, 0], np.float32)
+    kay = np.ascontiguousarray(kp_a[:, 1], np.float32)
+    kbx = np.ascontiguousarray(kp_b[:, 0], np.float32)
+    kby = np.ascontiguousarray(kp_b[:, 1], np.float32)
+    ia = np.ascontiguousarray(idx_a, np.int32)
+    ib = np.ascontiguousarray(idx_b, np.int32)
+    dist = np.ascontiguousarray(distance, np.float32)
+    m = np.ascontiguousarray(mask, np.uint8)
+    hm = np.ascontiguousarray(h, np.float64).reshape(9)
+    fw = None
+    if forward is not None:
+        fw = (Top2 * max(len(forward), 1))()
+        for i, t in enumerate(forward):
+            fw[i] = Top2(*t)
+    out = np.zeros(38)
+    st = lib().orc_gate_features(a.ctypes.data, wa, ha, pa, fa, b.ctypes.data, wb, hb, pb, fb,
+                                 kax.ctypes.data, kay.ctypes.data, len(kax), kbx.ctypes.data, kby.ctypes.data,
+                                 len(kbx), ia.ctypes.data, ib.ctypes.data, dist.ctypes.data,
+                                 len(ia) if capacity is None else capacity, m.ctypes.data, num_matches,
+                                 num_inliers, inlier_ratio, inlier_coverage, reproj_residual, hm.ctypes.data,
+                                 C.byref(sa), C.byref(sb), C.cast(fw, C.c_void_p) if fw is not None else None,
+                                 relation_mask, out.ctypes.data)
+    return st, out

@@ -77,15 +77,19 @@ def test_fixed_mode_rows_match_the_runner(tmp_path):
             {"config_id": "small", "S": 0.5, "K": 500, "R": 256, "blend_mode": "feather"}]
     (tmp_path / "configs.json").write_text(json.dumps(cfgs))
     out = tmp_path / "runs.csv"
+    feats = tmp_path / "features.csv"
     r = run("--manifest", tmp_path / "manifest.json", "--configs", tmp_path / "configs.json", "--out", out,
-            "--warmup", 1, "--repeats", 2, "--run-id", "t1", "--seed", 5)
+            "--warmup", 1, "--repeats", 2, "--run-id", "t1", "--seed", 5, "--features-out", feats)
     assert r.returncode == 0, r.stderr
     rows = read_csv(out)
+    frows = read_csv(feats)
+    assert len(frows) == len(rows) and len(frows[0]) == 7 + 32 + 38
     assert len(rows) == 2 * 2 * 3
     assert [row["pair_id"] for row in rows[::6]] == ["pair_0", "pair_1"]
     assert {row["config_id"] for row in rows} == {"S1.00_K800_R512_Bmultiband5", "small"}
     assert [row["is_warmup"] for row in rows[:3]] == ["1", "0", "0"]
     runner = P.PipelineRunner(640, 480, max_k=4000, max_r=2048)
+    runner.set_collect_features(True)
     for i, pid in enumerate(entries):
         a, b, _ = P.synth_pair(31000 + i, 640, 480, overlap=(0.5, 0.7))
         runner.upload_pair(a, b)
@@ -102,6 +106,14 @@ def test_fixed_mode_rows_match_the_runner(tmp_path):
                 if rec.status == b"OK":
                     assert [float(row[f"h{k}"]) for k in range(9)] == list(rec.H)
                     assert int(row["canvas_pixels"]) == rec.canvas_pixels
+            # The observations CSV carries the runner's features to the last bit.
+            scene, gate, present, _ = runner.features()
+            fgot = [row for row in frows if row["pair_id"] == pid and row["config_id"] == got[0]["config_id"]]
+            assert len(fgot) == 3
+            for row in fgot:
+                assert row["features_present"] == str(int(present))
+                assert [float(row[f"scene_{k}"]) for k in range(32)] == list(scene)
+                assert [float(row[f"gate_{k}"]) for k in range(38)] == list(gate)
     runner.close()
 
 
