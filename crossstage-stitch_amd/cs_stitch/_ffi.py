@@ -228,6 +228,11 @@ SIGNATURES = {
                                 P(C.c_int)]),
     "cs_controller_walk": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
                                      P(HomographyResult), P(C.c_int), C.c_int, P(C.c_double)]),
+    "cs_controller_create_policy": (vp, [C.c_char_p, C.c_char_p, sz]),
+    "cs_controller_observations": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_uint), P(C.c_int)]),
+    "cs_controller_walk_features": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, vp,
+                                              P(HomographyResult), P(C.c_int), vp, P(C.c_int), C.c_int,
+                                              P(C.c_double)]),
     "cs_synth_pair": (C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, vp, vp,
                                 P(C.c_double)]),
     "cs_batch_create": (C.c_int, [C.c_int, C.c_uint64, C.c_int, P(vp)]),

@@ -40,23 +40,54 @@ def record_dict(r: F.RunRecord) -> dict:
 
 
 class Controller:
-    def __init__(self, blind: bool = False):
-        self.handle = F.load().cs_controller_create_legacy(int(blind))
-        if not self.handle:
-            raise RuntimeError("controller policy rejected")
+    """The built-in legacy-v1 ladder (blind: legacy-v1-blind-s05), or a
+    controller-policy-v2/v3/v4 document given as policy_json."""
 
-    def walk(self, grad_a, grad_b, wa, ha, wb, hb, feedback, exec_status):
+    def __init__(self, blind: bool = False, policy_json: str | None = None):
+        if policy_json is None:
+            self.handle = F.load().cs_controller_create_legacy(int(blind))
+            if not self.handle:
+                raise RuntimeError("controller policy rejected")
+        else:
+            err = C.create_string_buffer(512)
+            self.handle = F.load().cs_controller_create_policy(policy_json.encode(), err, len(err))
+            if not self.handle:
+                raise ValueError(f"controller policy rejected: {err.value.decode()}")
+
+    def observations(self):
+        """(uses_learned_features, uses_pair_relations, relation_mask, needs_gate_features)."""
+        a, b, d = C.c_int(), C.c_int(), C.c_int()
+        m = C.c_uint()
+        F.load().cs_controller_observations(self.handle, C.byref(a), C.byref(b), C.byref(m), C.byref(d))
+        return bool(a.value), bool(b.value), m.value, bool(d.value)
+
+    def walk(self, grad_a, grad_b, wa, ha, wb, hb, feedback, exec_status, scene=None, gates=None):
+        """begin_pair then observe over the feedbacks; scene: 32 doubles or
+        None; gates: per feedback 38 doubles or None. Rows are (action,
+        reason, S, K, R), the first the initial knobs."""
         n = len(feedback)
         fb = (F.HomographyResult * max(n, 1))(*feedback)
         st = (C.c_int * max(n, 1))(*exec_status)
         trace = (C.c_double * (5 * (n + 2)))()
-        rows = F.load().cs_controller_walk(self.handle, grad_a, grad_b, wa, ha, wb, hb, fb, st, n, trace)
+        sc = np.ascontiguousarray(scene, np.float64) if scene is not None else None
+        g = np.zeros((max(n, 1), F.GATE_FEATURES))
+        present = (C.c_int * max(n, 1))()
+        for i, row in enumerate(gates or []):
+            if row is not None:
+                g[i] = row
+                present[i] = 1
+        rows = F.load().cs_controller_walk_features(self.handle, grad_a, grad_b, wa, ha, wb, hb,
+                                                    sc.ctypes.data if sc is not None else None, fb, st,
+                                                    g.ctypes.data, present, n, trace)
         return [tuple(trace[5 * i:5 * i + 5]) for i in range(rows)]
 
     def __del__(self):
-        if getattr(self, "handle", None):
-            F.load().cs_controller_destroy(self.handle)
-            self.handle = None
+        try:
+            if getattr(self, "handle", None):
+                F.load().cs_controller_destroy(self.handle)
+                self.handle = None
+        except (AttributeError, TypeError):  # interpreter teardown: the library module is already gone
+            pass
 
 
 class PipelineRunner:

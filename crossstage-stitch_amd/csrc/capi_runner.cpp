@@ -313,14 +313,66 @@ int cs_plan_retry(double s0, int k0, int r0, int b0, double s1, int k1, int r1, 
 
 int cs_controller_walk(cs_controller* c, double grad_a, double grad_b, int wa, int ha, int wb, int hb,
                        const cs_homography_result* feedback, const int* exec_status, int n, double* trace) {
-    if (!c || !trace) return -1;
+    return cs_controller_walk_features(c, grad_a, grad_b, wa, ha, wb, hb, nullptr, feedback, exec_status, nullptr,
+                                       nullptr, n, trace);
+}
+
+cs_controller* cs_controller_create_policy(const char* text, char* error, size_t error_len) {
+    auto say = [&](const char* what) {
+        if (error && error_len) {
+            std::strncpy(error, what, error_len - 1);
+            error[error_len - 1] = 0;
+        }
+    };
+    if (!text) {
+        say("no policy text");
+        return nullptr;
+    }
+    try {
+        auto* c = new cs_controller{};
+        c->policy = controller::parse_controller_policy(text);
+        if (c->policy.configure(c->impl) != Status::OK) {
+            delete c;
+            say("Invalid Controller policy or ladder");
+            return nullptr;
+        }
+        say("");
+        return c;
+    } catch (const std::exception& e) {
+        say(e.what());
+        return nullptr;
+    } catch (...) {
+        say("policy error");
+        return nullptr;
+    }
+}
+
+int cs_controller_observations(cs_controller* c, int* learned, int* pair_relations, unsigned* relation_mask,
+                               int* needs_gate) {
+    if (!c) return CS_INVALID_ARGUMENT;
+    if (learned) *learned = c->impl.uses_learned_features();
+    if (pair_relations) *pair_relations = c->impl.uses_pair_relations();
+    if (relation_mask) *relation_mask = c->impl.relation_mask();
+    if (needs_gate) *needs_gate = c->impl.needs_gate_features();
+    return CS_OK;
+}
+
+int cs_controller_walk_features(cs_controller* c, double grad_a, double grad_b, int wa, int ha, int wb, int hb,
+                                const double* scene, const cs_homography_result* feedback, const int* exec_status,
+                                const double* gate, const int* gate_present, int n, double* trace) {
+    if (!c || !trace || (n > 0 && (!feedback || !exec_status))) return -1;
     auto& ctl = c->impl;
     ctl.reset_pair();
     ScaleInfo pa{}, pb{};
     make_scale_info(wa, ha, ctl.probe_scale(), &pa);
     make_scale_info(wb, hb, ctl.probe_scale(), &pb);
+    controller::SceneSignals signals{{pa, grad_a}, {pb, grad_b}};
+    if (scene) {
+        std::copy(scene, scene + features::kSceneCount, signals.features.begin());
+        signals.features_present = true;
+    }
     controller::Knobs knobs{};
-    if (ctl.begin_pair({{pa, grad_a}, {pb, grad_b}}, &knobs) != Status::OK) return -2;
+    if (ctl.begin_pair(signals, &knobs) != Status::OK) return -2;
     int rows = 0;
     auto put = [&](int action, int reason, const controller::Knobs& k) {
         double* t = trace + 5 * rows++;
@@ -334,6 +386,11 @@ int cs_controller_walk(cs_controller* c, double grad_a, double grad_b, int wa, i
     for (int i = 0; i < n; ++i) {
         controller::Feedback fb{knobs, static_cast<Status>(exec_status[i]),
                                 *reinterpret_cast<const HomographyResult*>(&feedback[i])};
+        if (gate && gate_present && gate_present[i]) {
+            std::copy(gate + static_cast<std::size_t>(i) * features::kGateCount,
+                      gate + static_cast<std::size_t>(i + 1) * features::kGateCount, fb.features.begin());
+            fb.features_present = true;
+        }
         controller::Decision d{};
         if (ctl.observe(fb, &d) != Status::OK) {
             put(-3, -3, knobs);

@@ -393,20 +393,32 @@ void PipelineRunner::execute_adaptive_run(controller::Controller& ctl, std::uint
     gray_a_.height = pa.work_h;
     gray_b_.width = pb.work_w;
     gray_b_.height = pb.work_h;
+    // Learned policies read the scene features at the probe scale; their
+    // first two values are the probe gradients (pipeline_runner.cu:607-645).
+    const bool learned = ctl.uses_learned_features();
     if (probe_enabled_) {
         const Status p = timed(rec->t_probe, [&] {
             const Status s = s1::s1_preprocess(use_a_, use_b_, probe_s, &gray_a_, &gray_b_, &pa, &pb, stream_);
             if (s != Status::OK) return s;
+            if (learned)
+                return features::extract_scene(gray_a_, gray_b_, d_features_, pinned_features_, stream_,
+                                               ctl.uses_pair_relations());
             return s1::s1_compute_gradient(gray_a_, gray_b_, ws_probe_, pinned_grad_, pinned_grad_ + 1, stream_);
         });
         rec->t_s1 = rec->t_probe;
         if (p != Status::OK) return finish(status_name(p), "s1");
+        if (learned) {
+            pinned_grad_[0] = pinned_features_[0];
+            pinned_grad_[1] = pinned_features_[1];
+            std::copy_n(pinned_features_, features::kSceneCount, rec->scene_features.begin());
+        }
         rec->probe_gradient_a = pinned_grad_[0];
         rec->probe_gradient_b = pinned_grad_[1];
     } else {
+        if (learned) return finish("INVALID_ARGUMENT", "controller");
         pinned_grad_[0] = pinned_grad_[1] = 1.0;
     }
-    const controller::SceneSignals scene{{pa, pinned_grad_[0]}, {pb, pinned_grad_[1]}};
+    const controller::SceneSignals scene{{pa, pinned_grad_[0]}, {pb, pinned_grad_[1]}, rec->scene_features, learned};
     controller::Knobs knobs{};
     {
         const auto c0 = Clock::now();
@@ -529,7 +541,30 @@ void PipelineRunner::execute_adaptive_run(controller::Controller& ctl, std::uint
         tr.homography = *pinned_h_;
         tr.has_homography = true;
         if (s5s == Status::OK) completed_r = knobs.R;
-        const controller::Feedback fb{knobs, s5s, *pinned_h_};
+        controller::Feedback fb{knobs, s5s, *pinned_h_};
+        if (ctl.needs_gate_features()) {
+            // Gate observations of this attempt (pipeline_runner.cu:780-806).
+            const auto f0 = Clock::now();
+            const unsigned relations = ctl.relation_mask();
+            const s4::Top2* forward = nullptr;
+            if ((relations & features::kMatchingRelations) &&
+                s4::s4_forward_neighbors(ka.capacity, kb.capacity, {0.8, false}, ws_s4_, bytes_s4_, &forward) !=
+                    Status::OK)
+                throw std::runtime_error("Missing adaptive matching observations");
+            const Status fs = features::extract_gate(use_a_, use_b_, ka, kb, matches_, inlier_mask_, *pinned_h_, sa, sb,
+                                                     d_features_, pinned_features_, stream_, forward, relations);
+            if (fs == Status::CUDA_ERROR) throw std::runtime_error("Gate feature device error");
+            const auto w0 = Clock::now();
+            check(hipStreamSynchronize(stream_), "sync");
+            rec->t_stream_wait += ms_since(w0);
+            if (fs == Status::OK) {
+                std::copy_n(pinned_features_, features::kGateCount, fb.features.begin());
+                fb.features_present = true;
+            }
+            rec->t_features += ms_since(f0);
+        }
+        rec->gate_features = fb.features;
+        rec->features_present = fb.features_present;
         controller::Decision d{};
         const auto c0 = Clock::now();
         const Status observed = ctl.observe(fb, &d);

@@ -7,8 +7,8 @@
 // (harness/attempt_trace.hpp:60-98) and the observations CSV (main.cpp:283-300).
 // Images are binary PPM (P6, maxval 255), which is what the reference reads
 // when it is built without OpenCV (harness/image_loader.hpp:25-95).
-// Learned controller policies (JSON policy files) are not part of this
-// build; the built-in legacy-v1 and legacy-v1-blind-s05 ladders are.
+// Adaptive mode takes the built-in legacy-v1 / legacy-v1-blind-s05 ladders
+// or a controller-policy-v2/v3/v4 JSON file (harness/controller_policy.hpp).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "stitch/controller.hpp"
+#include "stitch/json.hpp"
 #include "stitch/runner.hpp"
 
 namespace {
@@ -37,153 +38,8 @@ using stitch::s6::BlendMode;
 namespace H = stitch::harness;
 namespace Ctl = stitch::controller;
 
-// ------------------------------------------------------------------ JSON
-struct Json {
-    enum Kind { Null, Bool, Number, String, Array, Object } kind{Null};
-    bool b{false};
-    double num{0};
-    std::string str;
-    std::vector<Json> arr;
-    std::vector<std::pair<std::string, Json>> obj;  // in file order
-
-    const Json* find(const std::string& key) const {
-        for (const auto& kv : obj)
-            if (kv.first == key) return &kv.second;
-        return nullptr;
-    }
-};
-
-class JsonReader {
-public:
-    explicit JsonReader(std::string text) : s_(std::move(text)) {}
-    Json document() {
-        Json v = value();
-        space();
-        if (i_ != s_.size()) fail("trailing characters");
-        return v;
-    }
-
-private:
-    [[noreturn]] void fail(const char* what) const {
-        throw std::runtime_error(std::string("JSON: ") + what + " at offset " + std::to_string(i_));
-    }
-    void space() {
-        while (i_ < s_.size() && (s_[i_] == ' ' || s_[i_] == '\t' || s_[i_] == '\n' || s_[i_] == '\r')) ++i_;
-    }
-    bool take(char c) {
-        space();
-        if (i_ < s_.size() && s_[i_] == c) {
-            ++i_;
-            return true;
-        }
-        return false;
-    }
-    void expect(char c) {
-        if (!take(c)) fail("unexpected character");
-    }
-    bool word(const char* w) {
-        const std::size_t n = std::char_traits<char>::length(w);
-        if (s_.compare(i_, n, w) == 0) {
-            i_ += n;
-            return true;
-        }
-        return false;
-    }
-    std::string text() {
-        expect('"');
-        std::string out;
-        while (true) {
-            if (i_ >= s_.size()) fail("unterminated string");
-            const char c = s_[i_++];
-            if (c == '"') return out;
-            if (c != '\\') {
-                out.push_back(c);
-                continue;
-            }
-            if (i_ >= s_.size()) fail("unterminated escape");
-            const char e = s_[i_++];
-            switch (e) {
-                case '"': case '\\': case '/': out.push_back(e); break;
-                case 'b': out.push_back('\b'); break;
-                case 'f': out.push_back('\f'); break;
-                case 'n': out.push_back('\n'); break;
-                case 'r': out.push_back('\r'); break;
-                case 't': out.push_back('\t'); break;
-                case 'u': {
-                    if (i_ + 4 > s_.size()) fail("short unicode escape");
-                    const unsigned cp = static_cast<unsigned>(std::stoul(s_.substr(i_, 4), nullptr, 16));
-                    i_ += 4;
-                    // UTF-8 for the basic plane (paths and ids only need that).
-                    if (cp < 0x80) {
-                        out.push_back(static_cast<char>(cp));
-                    } else if (cp < 0x800) {
-                        out.push_back(static_cast<char>(0xC0 | (cp >> 6)));
-                        out.push_back(static_cast<char>(0x80 | (cp & 0x3F)));
-                    } else {
-                        out.push_back(static_cast<char>(0xE0 | (cp >> 12)));
-                        out.push_back(static_cast<char>(0x80 | ((cp >> 6) & 0x3F)));
-                        out.push_back(static_cast<char>(0x80 | (cp & 0x3F)));
-                    }
-                    break;
-                }
-                default: fail("bad escape");
-            }
-        }
-    }
-    Json value() {
-        space();
-        if (i_ >= s_.size()) fail("unexpected end");
-        Json v;
-        const char c = s_[i_];
-        if (c == '{') {
-            ++i_;
-            v.kind = Json::Object;
-            if (take('}')) return v;
-            do {
-                space();
-                std::string key = text();
-                expect(':');
-                v.obj.emplace_back(std::move(key), value());
-            } while (take(','));
-            expect('}');
-        } else if (c == '[') {
-            ++i_;
-            v.kind = Json::Array;
-            if (take(']')) return v;
-            do v.arr.push_back(value());
-            while (take(','));
-            expect(']');
-        } else if (c == '"') {
-            v.kind = Json::String;
-            v.str = text();
-        } else if (word("true")) {
-            v.kind = Json::Bool;
-            v.b = true;
-        } else if (word("false")) {
-            v.kind = Json::Bool;
-        } else if (word("null")) {
-            v.kind = Json::Null;
-        } else {
-            const char* start = s_.c_str() + i_;
-            char* end = nullptr;
-            v.num = std::strtod(start, &end);
-            if (end == start) fail("bad value");
-            v.kind = Json::Number;
-            i_ += static_cast<std::size_t>(end - start);
-        }
-        return v;
-    }
-    std::string s_;
-    std::size_t i_{0};
-};
-
-Json read_json(const std::string& path) {
-    std::ifstream f(path, std::ios::binary);
-    if (!f) throw std::runtime_error("cannot open " + path);
-    std::stringstream ss;
-    ss << f.rdbuf();
-    return JsonReader(ss.str()).document();
-}
+using stitch::json::Json;
+using stitch::json::read_json;
 
 // ----------------------------------------------------------------- inputs
 struct PairEntry {
@@ -419,7 +275,7 @@ void usage() {
                  "Modes:\n"
                  "  --mode <fixed|oracle|adaptive>  Execution mode (default: fixed)\n"
                  "  --adaptive | --oracle           Shortcuts for --mode\n"
-                 "  --controller-policy <name>      Adaptive ladder: legacy-v1 or legacy-v1-blind-s05\n"
+                 "  --controller-policy <path>      Required adaptive policy JSON, legacy-v1, or legacy-v1-blind-s05\n"
                  "  --trace-out <path>              Per-attempt controller trace CSV (adaptive)\n"
                  "  --features-out <path>           Controller observations CSV\n"
                  "  --no-reuse                      Rerun S4/S5 on R-only retries\n"
@@ -447,6 +303,7 @@ int main(int argc, char** argv) {
     double S = 0.5;
     int K = 1000, R = 256, warmup = 2, repeats = 10;
     std::uint64_t seed = 0;
+    bool seed_given = false;
     bool reuse = true, probe = true;
     const char* rev = std::getenv("STITCH_GIT_REV");
     const std::string git_rev = rev ? rev : "HEAD";
@@ -478,6 +335,7 @@ int main(int argc, char** argv) {
                 if (v.empty() || v.find_first_not_of("0123456789") != std::string::npos)
                     throw std::invalid_argument("invalid --seed value");
                 seed = std::stoull(v);
+                seed_given = true;
             } else if (a == "--img-a") img_a = next();
             else if (a == "--img-b") img_b = next();
             else if (a == "--pair-id") pair_id = next();
@@ -502,7 +360,8 @@ int main(int argc, char** argv) {
         return 1;
     }
     std::error_code ec;
-    for (const std::string& in : {manifest_path, configs_path, img_a, img_b, trace_path, features_path})
+    const std::string policy_file = policy_name.rfind("legacy-v1", 0) == 0 ? std::string() : policy_name;
+    for (const std::string& in : {manifest_path, configs_path, policy_file, img_a, img_b, trace_path, features_path})
         if (!in.empty() && std::filesystem::weakly_canonical(in, ec) == std::filesystem::weakly_canonical(out_path, ec)) {
             std::cerr << "Output path error: --out aliases another file\n";
             return 1;
@@ -541,20 +400,21 @@ int main(int argc, char** argv) {
     Ctl::Controller controller;
     Ctl::ControllerPolicy policy;
     if (mode == "adaptive") {
-        if (policy_name == "legacy-v1") {
-            policy = Ctl::legacy_controller_policy();
-        } else if (policy_name == "legacy-v1-blind-s05") {
-            policy = Ctl::legacy_blind_s05_policy();
-        } else {
-            std::cerr << "Controller policy error: "
-                      << (policy_name.empty() ? "adaptive mode requires --controller-policy"
-                                              : "only the built-in legacy-v1 and legacy-v1-blind-s05 ladders are "
-                                                "supported (learned policy files are not part of this build)")
-                      << "\n";
-            return 1;
-        }
-        if (policy.configure(controller) != stitch::Status::OK) {
-            std::cerr << "Controller policy error: failed to initialize the controller\n";
+        // Built-in ladders by name, otherwise a policy file (main.cpp:293-314);
+        // a policy's provenance seed replaces the default and must agree with --seed.
+        try {
+            if (policy_name.empty())
+                throw std::runtime_error("Adaptive mode requires --controller-policy <path> or explicit legacy-v1");
+            policy = Ctl::load_controller_policy(policy_name);
+            if (policy.seed) {
+                if (seed_given && seed != *policy.seed)
+                    throw std::runtime_error("CLI seed differs from policy provenance.seed");
+                seed = *policy.seed;
+            }
+            if (policy.configure(controller) != stitch::Status::OK)
+                throw std::runtime_error("failed to initialize Controller");
+        } catch (const std::exception& e) {
+            std::cerr << "Controller policy error: " << e.what() << "\n";
             return 1;
         }
     }

@@ -227,6 +227,23 @@ int cs_plan_retry(double s0, int k0, int r0, int b0, double s1, int k1, int r1, 
 int cs_controller_walk(cs_controller* controller, double gradient_a, double gradient_b, int width_a, int height_a,
                        int width_b, int height_b, const cs_homography_result* feedback, const int* exec_status,
                        int n, double* trace);
+/* Controller from a policy document: schema controller-policy-v2 (rules),
+ * -v3 (selector and gate trees) or -v4 (pooled boosted models), as
+ * harness/controller_policy.hpp:123-206 parses it. On failure returns NULL
+ * and writes the reason to error (error_len bytes, may be 0). */
+cs_controller* cs_controller_create_policy(const char* json_text, char* error, size_t error_len);
+/* What the configured policy reads (controller.hpp:72-77): learned features,
+ * the pair relations of the scene features, the gate relation mask, and
+ * whether the current rung needs gate features. */
+int cs_controller_observations(cs_controller* controller, int* uses_learned_features, int* uses_pair_relations,
+                               unsigned* relation_mask, int* needs_gate_features);
+/* cs_controller_walk with observations: scene (32 doubles, NULL = absent) goes
+ * to begin_pair; gate row i (38 doubles) goes with feedback i when
+ * gate_present[i] is nonzero (gate or gate_present NULL = absent). */
+int cs_controller_walk_features(cs_controller* controller, double gradient_a, double gradient_b, int width_a,
+                                int height_a, int width_b, int height_b, const double* scene,
+                                const cs_homography_result* feedback, const int* exec_status, const double* gate,
+                                const int* gate_present, int n, double* trace);
 
 /* Batched form (no counterpart in the reference): ragged batches of pairs,
  * each stage one persistent launch over the whole batch, captured in HIP

@@ -109,4 +109,68 @@ int ref_legacy_walk(int blind, double grad_a, double grad_b, int wa, int ha, int
     return rows;
 }
 
+// Parses a policy document with the reference's loader: 0 accepted, 1
+// rejected (the reason in error).
+int ref_policy_check(const char* text, char* error, int error_len) {
+    try {
+        (void)harness::parse_controller_policy(harness::JsonValue::parse(text));
+        if (error_len > 0) error[0] = 0;
+        return 0;
+    } catch (const std::exception& e) {
+        if (error_len > 0) {
+            std::strncpy(error, e.what(), static_cast<std::size_t>(error_len) - 1);
+            error[error_len - 1] = 0;
+        }
+        return 1;
+    }
+}
+
+// ref_legacy_walk for a policy document, with observations: scene (32
+// doubles or NULL) for begin_pair, gate row i (38 doubles) with feedback i
+// when present[i]. Returns the number of rows, -1 when the policy is
+// rejected, -2 when begin_pair fails.
+int ref_policy_walk(const char* text, double grad_a, double grad_b, int wa, int ha, int wb, int hb, const double* scene,
+                    const HomographyResult* feedback, const int* exec_status, const double* gate, const int* present,
+                    int n, double* trace) {
+    harness::ControllerPolicy policy;
+    try {
+        policy = harness::parse_controller_policy(harness::JsonValue::parse(text));
+    } catch (const std::exception&) {
+        return -1;
+    }
+    controller::Controller c;
+    if (policy.configure(c) != Status::OK) return -1;
+    c.reset_pair();
+    ScaleInfo pa{}, pb{};
+    make_scale_info(wa, ha, c.probe_scale(), &pa);
+    make_scale_info(wb, hb, c.probe_scale(), &pb);
+    controller::SceneSignals signals{{pa, grad_a}, {pb, grad_b}, {}, false};
+    if (scene) {
+        std::memcpy(signals.features.data(), scene, sizeof(double) * signals.features.size());
+        signals.features_present = true;
+    }
+    controller::Knobs knobs{};
+    if (c.begin_pair(signals, &knobs) != Status::OK) return -2;
+    int rows = 0;
+    auto put = [&](int action, int reason, const controller::Knobs& k) {
+        double* t = trace + 5 * rows++;
+        t[0] = action; t[1] = reason; t[2] = k.S; t[3] = k.K; t[4] = k.R;
+    };
+    put(-1, -1, knobs);
+    for (int i = 0; i < n; ++i) {
+        controller::Feedback fb{knobs, static_cast<Status>(exec_status[i]), feedback[i], {}, false};
+        if (gate && present && present[i]) {
+            std::memcpy(fb.features.data(), gate + static_cast<std::size_t>(i) * fb.features.size(),
+                        sizeof(double) * fb.features.size());
+            fb.features_present = true;
+        }
+        controller::Decision d{};
+        if (c.observe(fb, &d) != Status::OK) { put(-3, -3, knobs); break; }
+        put(static_cast<int>(d.action), static_cast<int>(d.reason), d.next);
+        if (d.action != controller::Action::RETRY) break;
+        knobs = d.next;
+    }
+    return rows;
+}
+
 }  // extern "C"

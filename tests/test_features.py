@@ -329,25 +329,75 @@ def test_runner_collects_features():
     st0, want_scene = O.scene_features(O.s1_preprocess(a, 0.25), O.s1_preprocess(b, 0.25))
     assert st0 == 0
     assert_features_close(scene, want_scene)
+    assert_features_close(gate, oracle_gate_of_runner(runner, rec, a, b, S, K, 7))
+    runner.close()
+
+
+def oracle_gate_of_runner(runner, rec, a, b, S, K, relation_mask, max_k=4000):
+    """The oracle's gate features of the runner's last S5, from its device
+    outputs (keypoints, matches, mask, homography); the forward top-2 view is
+    recomputed on the host from its descriptors."""
+    from cs_stitch import pipeline as P
     out = runner.outputs()
-    nm = rec.num_matches
-    kpa = np.stack([P.download(out["ka"].x, 4 * K).view(np.float32), P.download(out["ka"].y, 4 * K).view(np.float32)], 1)
-    kpb = np.stack([P.download(out["kb"].x, 4 * K).view(np.float32), P.download(out["kb"].y, 4 * K).view(np.float32)], 1)
+    hres = out["h"]
+    nm = hres.num_matches
+    kp = {}
+    for key in ("ka", "kb"):
+        kp[key] = np.stack([P.download(out[key].x, 4 * K).view(np.float32),
+                            P.download(out[key].y, 4 * K).view(np.float32)], 1)
     m = out["matches"]
     ia = P.download(m.idx_a, 4 * nm).view(np.int32)
     ib = P.download(m.idx_b, 4 * nm).view(np.int32)
     dist = P.download(m.distance, 4 * nm).view(np.float32)
     mask = P.download(out["inlier_mask"], nm)
-    na, nb = rec.num_kp_a, rec.num_kp_b
-    da = P.download(out["da"].data, 32 * na).reshape(-1, 32)
-    db = P.download(out["db"].data, 32 * nb).reshape(-1, 32)
-    fwd = O.s4_top2(da, db) + [(math.inf, math.inf, -1, -1)] * (K - na)
-    _, sa = O.scale_info(w, h, S)
-    _, sb = O.scale_info(w, h, S)
-    hres = out["h"]
-    st1, want_gate = O.gate_features(a, b, kpa, kpb, ia, ib, dist, mask, nm, hres.num_inliers, hres.inlier_ratio,
-                                     hres.inlier_coverage, hres.reproj_residual, np.array(hres.H[:]), sa, sb, fwd, 7,
-                                     capacity=4000)
-    assert st1 == 0
-    assert_features_close(gate, want_gate)
+    fwd = None
+    if relation_mask & 1:
+        na, nb = (int(P.download(out[k].d_count, 4).view(np.int32)[0]) for k in ("ka", "kb"))
+        da = P.download(out["da"].data, 32 * na).reshape(-1, 32)
+        db = P.download(out["db"].data, 32 * nb).reshape(-1, 32)
+        fwd = O.s4_top2(da, db) + [(math.inf, math.inf, -1, -1)] * (K - na)
+    _, sa = O.scale_info(a.shape[1], a.shape[0], S)
+    _, sb = O.scale_info(b.shape[1], b.shape[0], S)
+    st, want = O.gate_features(a, b, kp["ka"], kp["kb"], ia, ib, dist, mask, nm, hres.num_inliers, hres.inlier_ratio,
+                               hres.inlier_coverage, hres.reproj_residual, np.array(hres.H[:]), sa, sb, fwd,
+                               relation_mask, capacity=max_k)
+    assert st == 0
+    return want
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["v3", "v4"])
+def test_adaptive_runner_with_learned_policy(kind):
+    """Adaptive runs under learned policies: the probe collects the scene
+    features (pair relations when the pooled policy asks), the first rung is
+    the controller's choice from them, gate features of the last attempt match
+    the oracle on the runner's own outputs, and the run ends in a terminal
+    status of the decision logic."""
+    import json
+    from cs_stitch import pipeline as P
+    from test_policies import random_policy
+    rng = np.random.default_rng(5 if kind == "v3" else 6)
+    doc = random_policy(rng, kind)
+    if kind == "v4":
+        doc.update(pair_relations=True, relation_mask=7)
+    ctl = P.Controller(policy_json=json.dumps(doc))
+    learned, pair_relations, relation_mask, _ = ctl.observations()
+    assert learned and pair_relations == (kind == "v4")
+    w, h = 640, 480
+    a, b, _ = P.synth_pair(91, w, h, overlap=(0.5, 0.7))
+    runner = P.PipelineRunner(w, h, max_k=4000, max_r=2048, canvas_width=2048, canvas_height=1024)
+    runner.upload_pair(a, b)
+    rec = runner.execute_adaptive_run(ctl, seed=0)
+    assert rec.status.decode() in ("OK", "BUDGET_EXHAUSTED", "REJECTED"), (rec.status, rec.fail_stage)
+    scene, gate, present, _ = runner.features()
+    st, want = O.scene_features(O.s1_preprocess(a, 0.25), O.s1_preprocess(b, 0.25), pair_relations)
+    assert st == 0
+    assert_features_close(scene, want)
+    assert (rec.probe_gradient_a, rec.probe_gradient_b) == (scene[0], scene[1])
+    trace = runner.attempt_trace()
+    first = ctl.walk(scene[0], scene[1], w, h, w, h, [], [], scene=scene)[0]
+    assert (trace[0].S, trace[0].K, trace[0].R) == first[2:]
+    if present:
+        last = trace[-1]
+        assert_features_close(gate, oracle_gate_of_runner(runner, rec, a, b, last.S, last.K, relation_mask))
     runner.close()

@@ -39,7 +39,15 @@ def test_help_and_argument_errors(tmp_path):
     policy.write_text("{}")
     r = run("--adaptive", "--controller-policy", policy, "--img-a", "a.ppm", "--img-b", "b.ppm",
             "--out", tmp_path / "r.csv")
-    assert r.returncode == 1 and "learned policy" in r.stderr
+    assert r.returncode == 1 and "schema_version" in r.stderr
+    # A policy file's provenance seed must agree with --seed.
+    import json
+    from test_policies import V2
+    doc = dict(V2, provenance={"seed": 42})
+    policy.write_text(json.dumps(doc))
+    r = run("--adaptive", "--controller-policy", policy, "--seed", "7", "--img-a", "a.ppm", "--img-b", "b.ppm",
+            "--out", tmp_path / "r.csv")
+    assert r.returncode == 1 and "provenance.seed" in r.stderr
     # Neither a manifest nor a pair; a manifest that is not an object.
     assert run("--out", tmp_path / "r.csv").returncode == 1
     bad = tmp_path / "m.json"
