@@ -248,6 +248,7 @@ SIGNATURES = {
     "cs_batch_pair_bytes": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_uint64)]),
     "cs_batch_render_bytes": (C.c_int, [C.c_int, C.c_int, C.c_int, P(C.c_uint64)]),
     "cs_server_create": (C.c_int, [P(ServerConfig), P(vp)]),
+    "cs_server_create_policy": (C.c_int, [P(ServerConfig), C.c_char_p, P(vp)]),
     "cs_server_destroy": (None, [vp]),
     "cs_server_run_trace": (C.c_int, [vp, P(Request), C.c_int, P(Completion), P(ServeStats)]),
     "cs_server_release": (C.c_int, [vp, C.c_uint64]),

@@ -138,10 +138,13 @@ def poisson_arrivals(rate_per_s: float, n: int, seed: int) -> list[float]:
 
 class Server:
     def __init__(self, hbm_budget: int = 64 << 30, max_batch: int = 64, max_live: int = 256, adaptive: bool = True,
-                 graphs: bool = True, canvas_factor: float = 4.0):
+                 graphs: bool = True, canvas_factor: float = 4.0, policy_json: str | None = None):
+        """policy_json: a controller-policy-v2/v3/v4 document for the adaptive
+        controller (None: the built-in legacy-v1 ladder)."""
         self.handle = C.c_void_p()
         cfg = F.ServerConfig(hbm_budget, max_batch, max_live, int(adaptive), int(graphs), canvas_factor)
-        _check(F.load().cs_server_create(C.byref(cfg), C.byref(self.handle)), "server_create")
+        _check(F.load().cs_server_create_policy(C.byref(cfg), policy_json.encode() if policy_json else None,
+                                                C.byref(self.handle)), "server_create")
 
     def run_trace(self, requests):
         """Replays the trace (sorted by arrival); returns (completions in completion order, stats)."""

@@ -283,8 +283,17 @@ int cs_batch_render_bytes(int canvas_width, int canvas_height, int mode, uint64_
     }
 }
 
-int cs_server_create(const cs_server_config* c, cs_server** out) {
+int cs_server_create(const cs_server_config* c, cs_server** out) { return cs_server_create_policy(c, nullptr, out); }
+
+int cs_server_create_policy(const cs_server_config* c, const char* policy_json, cs_server** out) {
     if (!c || !out || c->max_batch <= 0 || c->max_live <= 0 || !(c->canvas_factor > 0)) return CS_INVALID_ARGUMENT;
+    if (policy_json) {
+        try {
+            (void)controller::parse_controller_policy(policy_json);
+        } catch (...) {
+            return CS_INVALID_ARGUMENT;
+        }
+    }
     *out = nullptr;
     try {
         auto* s = new cs_server{};
@@ -295,6 +304,7 @@ int cs_server_create(const cs_server_config* c, cs_server** out) {
         cfg.adaptive = c->adaptive != 0;
         cfg.use_graphs = c->use_graphs != 0;
         cfg.canvas_factor = c->canvas_factor;
+        if (policy_json) cfg.policy_json = policy_json;
         try {
             s->impl.initialize(cfg);
         } catch (...) {

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <stdexcept>
 #include <thread>
 
 namespace stitch::serve {
@@ -74,11 +75,29 @@ Server::Server() = default;
 Server::~Server() {
     features_.wait();
     render_.wait();
+    if (obs_stream_) hipStreamSynchronize(obs_stream_);
+    if (obs_device_) hipFree(obs_device_);
+    if (obs_host_) hipHostFree(obs_host_);
+    if (obs_stream_) hipStreamDestroy(obs_stream_);
 }
 
 void Server::initialize(const ServerConfig& config) {
     config_ = config;
-    policy_ = controller::legacy_controller_policy();
+    policy_ = config.policy_json.empty() ? controller::legacy_controller_policy()
+                                         : controller::parse_controller_policy(config.policy_json);
+    {
+        controller::Controller probe;
+        if (policy_.configure(probe) != Status::OK) throw admission::Error("INVALID_POLICY");
+        learned_ = probe.uses_learned_features();
+    }
+    if (learned_) {
+        if (hipStreamCreateWithFlags(&obs_stream_, hipStreamNonBlocking) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void**>(&obs_device_), features::kDeviceDoubles * sizeof(double)) !=
+                hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void**>(&obs_host_), features::kGateCount * sizeof(double),
+                          hipHostMallocDefault) != hipSuccess)
+            throw std::runtime_error("observation buffers");
+    }
     for (const auto& k : policy_.ladder) {
         max_k_ = std::max(max_k_, k.K);
         max_r_ = std::max(max_r_, k.R);
@@ -160,7 +179,24 @@ void Server::run_trace(const Request* requests, int n, std::vector<Completion>* 
         L.done.num_kp_b = r.num_kp_b;
         if (L.phase == Phase::ProbeRunning) {
             if (r.status != Status::OK) return finish(L, status_name(r.status), "s1");
-            const controller::SceneSignals scene{{r.sa, r.grad_a}, {r.sb, r.grad_b}};
+            controller::SceneSignals scene{{r.sa, r.grad_a}, {r.sb, r.grad_b}};
+            if (learned_) {
+                // Scene features of the probe's gray pair; their first two
+                // values are the probe gradients (pipeline_runner.cu:607-645).
+                ImageView ga = L.buf.gray_a, gb = L.buf.gray_b;
+                ga.width = r.sa.work_w;
+                ga.height = r.sa.work_h;
+                gb.width = r.sb.work_w;
+                gb.height = r.sb.work_h;
+                const Status fs = features::extract_scene(ga, gb, obs_device_, obs_host_, obs_stream_,
+                                                          L.ctl.uses_pair_relations());
+                if (fs != Status::OK || hipStreamSynchronize(obs_stream_) != hipSuccess)
+                    return finish(L, status_name(fs != Status::OK ? fs : Status::CUDA_ERROR), "s1");
+                std::copy_n(obs_host_, features::kSceneCount, scene.features.begin());
+                scene.features_present = true;
+                scene.a.mean_abs_gradient = scene.features[0];
+                scene.b.mean_abs_gradient = scene.features[1];
+            }
             if (const Status b = L.ctl.begin_pair(scene, &L.knobs); b != Status::OK)
                 return finish(L, status_name(b), "controller");
             L.attempt = 0;
@@ -186,8 +222,29 @@ void Server::run_trace(const Request* requests, int n, std::vector<Completion>* 
         }
         if (s5s == Status::OK) L.completed_r = L.knobs.R;
         L.done.h = h;
+        controller::Feedback fb{L.knobs, s5s, h};
+        if (learned_ && L.ctl.needs_gate_features()) {
+            // Gate observations of this attempt (pipeline_runner.cu:780-806).
+            const unsigned relations = L.ctl.relation_mask();
+            const s4::Top2* forward = nullptr;
+            if ((relations & features::kMatchingRelations) &&
+                s4::s4_forward_neighbors(L.knobs.K, L.knobs.K, {0.8, false}, L.buf.ws_s4, L.buf.bytes_s4, &forward) !=
+                    Status::OK)
+                return finish(L, "CUDA_ERROR", "controller");
+            Keypoints ka = L.buf.kp_a, kb = L.buf.kp_b;
+            ka.capacity = kb.capacity = L.knobs.K;
+            const Status fs = features::extract_gate(L.req.a, L.req.b, ka, kb, L.buf.matches, L.buf.inlier_mask, h,
+                                                     r.sa, r.sb, obs_device_, obs_host_, obs_stream_, forward,
+                                                     relations);
+            if (fs == Status::CUDA_ERROR || hipStreamSynchronize(obs_stream_) != hipSuccess)
+                return finish(L, "CUDA_ERROR", "controller");
+            if (fs == Status::OK) {
+                std::copy_n(obs_host_, features::kGateCount, fb.features.begin());
+                fb.features_present = true;
+            }
+        }
         controller::Decision d{};
-        const Status observed = L.ctl.observe({L.knobs, s5s, h}, &d);
+        const Status observed = L.ctl.observe(fb, &d);
         if (observed != Status::OK) return finish(L, status_name(observed), s5s == Status::OK ? "controller" : "s5");
         if (d.action == controller::Action::RETRY) {
             L.previous = L.knobs;

@@ -32,6 +32,11 @@ struct ServerConfig {
     bool adaptive{true};                  // legacy-v1 controller; false: the request's fixed knobs
     bool use_graphs{true};
     double canvas_factor{4.0};            // canvas pixels reserved at admission, per input pixel
+    // Controller policy document (controller-policy-v2/v3/v4); empty: the
+    // built-in legacy-v1 ladder. Learned policies get their scene features
+    // after the batched probe and their gate features after each batched S5,
+    // extracted per request on the server's observation stream.
+    std::string policy_json{};
 };
 
 struct Request {
@@ -86,6 +91,10 @@ private:
     struct Live;
     ServerConfig config_{};
     controller::ControllerPolicy policy_{};
+    bool learned_{false};
+    hipStream_t obs_stream_{nullptr};
+    double* obs_device_{nullptr};  // features::kDeviceDoubles
+    double* obs_host_{nullptr};    // features::kGateCount, pinned
     int max_k_{0}, max_r_{0};
     batch::DevicePool pool_;
     batch::FeatureEngine features_;

@@ -365,6 +365,11 @@ typedef struct {
 } cs_serve_stats;
 
 int cs_server_create(const cs_server_config* config, cs_server** out);
+/* A server whose adaptive controller runs a policy document
+ * (controller-policy-v2/v3/v4; NULL: legacy-v1). Learned policies get their
+ * scene and gate observations per request. A rejected document returns
+ * CS_INVALID_ARGUMENT. */
+int cs_server_create_policy(const cs_server_config* config, const char* policy_json, cs_server** server);
 void cs_server_destroy(cs_server* server);
 /* Requests sorted by arrival; completions (n of them) in completion order. */
 int cs_server_run_trace(cs_server* server, const cs_request* requests, int n, cs_completion* completions,

@@ -375,12 +375,8 @@ def test_adaptive_runner_with_learned_policy(kind):
     status of the decision logic."""
     import json
     from cs_stitch import pipeline as P
-    from test_policies import random_policy
-    rng = np.random.default_rng(5 if kind == "v3" else 6)
-    doc = random_policy(rng, kind)
-    if kind == "v4":
-        doc.update(pair_relations=True, relation_mask=7)
-    ctl = P.Controller(policy_json=json.dumps(doc))
+    from test_policies import gated_v3_policy, gated_v4_policy
+    ctl = P.Controller(policy_json=json.dumps(gated_v3_policy() if kind == "v3" else gated_v4_policy()))
     learned, pair_relations, relation_mask, _ = ctl.observations()
     assert learned and pair_relations == (kind == "v4")
     w, h = 640, 480
@@ -395,6 +391,7 @@ def test_adaptive_runner_with_learned_policy(kind):
     assert_features_close(scene, want)
     assert (rec.probe_gradient_a, rec.probe_gradient_b) == (scene[0], scene[1])
     trace = runner.attempt_trace()
+    print(kind, rec.status, [(t.S, t.K, t.R, t.action, t.reason) for t in trace], present)
     first = ctl.walk(scene[0], scene[1], w, h, w, h, [], [], scene=scene)[0]
     assert (trace[0].S, trace[0].K, trace[0].R) == first[2:]
     if present:

@@ -326,3 +326,33 @@ def test_server_fixed_knobs_matches_batch():
         assert np.array_equal(np.array(c.homography.H[:]), ref[i][1]["H"]), i
         srv.release(i)
     srv.close()
+
+
+@pytest.mark.parametrize("kind", ["v3", "v4"])
+def test_server_learned_policy_matches_plain_runner(kind):
+    """Served requests under a learned policy (scene features after the
+    batched probe, gate features after each batched S5) end exactly as the
+    plain adaptive runner under the same policy."""
+    import json
+    from cs_stitch import batch as B
+    from cs_stitch import pipeline as P
+    from test_policies import gated_v3_policy, gated_v4_policy
+    text = json.dumps(gated_v3_policy() if kind == "v3" else gated_v4_policy())
+    pairs = make_pairs(4, seed0=9300, sizes=[(640, 480), (800, 600)], overlap=(0.5, 0.8))
+    srv = B.Server(hbm_budget=12 << 30, max_batch=4, policy_json=text)
+    done, st = srv.run_trace(serve_requests(pairs, [0.0] * 4, [5] * 4, [False] * 4))
+    assert len(done) == 4 and st.completed + st.failed == 4
+    runner = P.PipelineRunner(800, 600, max_k=4000, max_r=2048, canvas_width=4096, canvas_height=4096)
+    for c in done:
+        ta, tb = pairs[c.id]
+        runner.use_device_pair(ta, tb)
+        rec = runner.execute_adaptive_run(P.Controller(policy_json=text), seed=5)
+        print(kind, c.id, c.status, c.attempts, (c.S, c.K, c.R))
+        assert rec.status == c.status and rec.attempt_count == c.attempts
+        assert (rec.S, rec.K, rec.R) == (c.S, c.K, c.R)
+        if rec.status == b"OK":
+            assert np.array_equal(np.array(rec.H[:]), np.array(c.homography.H[:]))
+    runner.close()
+    srv.close()
+    with pytest.raises(Exception):
+        B.Server(hbm_budget=1 << 30, policy_json='{"schema_version": "controller-policy-v9"}')

@@ -276,3 +276,51 @@ def test_learned_walks_match_reference_controller(kind):
             assert mine == [tuple(trace[5 * i:5 * i + 5]) for i in range(rows)], (text, mine)
             walks += 1
     assert walks == 600
+
+
+def _ladder(rungs):
+    return [{"S": s, "K": k, "R": r, "blend_mode": "feather"} for s, k, r in rungs]
+
+
+def _checks(n):
+    return [{"min_inliers": 8, "min_inlier_ratio": 0.05, "min_inlier_coverage": 0.0, "max_residual_s1": 6.0}] * n
+
+
+def gated_v3_policy():
+    """Selector always picks rung 0; rung 0's gate rejects whenever the
+    photometric overlap share (gate feature 10) is finite, rung 1's accepts
+    unless the photometric NCC (feature 13) is below -2. Exercises gate
+    features on a retry."""
+    rungs = [(.5, 1000, 128), (.75, 2000, 512), (1.0, 2000, 1024)]
+    return {"schema_version": "controller-policy-v3", "policy_id": "gated-v3", "probe_scale": 0.25,
+            "ladder": _ladder(rungs), "self_checks": _checks(3),
+            "scene_rules": [{"min_pair_gradient": 0, "first_index": 0}], "feature_schema": "controller-features-v1",
+            "selector": [{"value": 0}],
+            "gates": [[{"feature": 10, "threshold": 2.0, "left": 1, "right": 2}, {"value": 0}, {"value": 1}],
+                      [{"feature": 13, "threshold": -2.0, "left": 1, "right": 2}, {"value": 0}, {"value": 1}],
+                      [{"value": 1}]]}
+
+
+def gated_v4_policy():
+    """Latency 1 ms for K/2000 <= 0.6 and 10 ms otherwise, pre-quality 0.993:
+    the cheapest non-reference rung starts. The post-quality model accepts
+    while the photometric overlap share (input 43 = gate feature 10) is
+    finite. Pair and all alignment relations are collected."""
+    rungs = [(.5, 1000, 128), (.75, 2000, 512), (1.0, 2000, 1024)]
+    sig = {"bias": 0, "transform": "sigmoid", "calibration": {"slope": 1, "intercept": 0}}
+    return {"schema_version": "controller-policy-v4", "policy_id": "gated-v4", "probe_scale": 0.25,
+            "ladder": _ladder(rungs), "self_checks": _checks(3),
+            "scene_rules": [{"min_pair_gradient": 0, "first_index": 0}], "feature_schema": "controller-features-v2",
+            "reference_index": 2, "pre_threshold": 0.0, "gate_threshold": 0.5, "extra_ms": 0.5,
+            "pair_relations": True, "relation_mask": 7,
+            "models": {"pre_quality": dict(sig, features=39, trees=[[{"leaf": 5}]]),
+                       "post_quality": dict(sig, features=77, trees=[[{"feature": 43, "threshold": 2.0, "left": 1,
+                                                                       "right": 2}, {"leaf": 5}, {"leaf": -5}]]),
+                       "latency": {"features": 39, "bias": 0, "transform": "exp",
+                                   "calibration": {"slope": 1, "intercept": 0},
+                                   "trees": [[{"feature": 31, "threshold": 0.6, "left": 1, "right": 2},
+                                              {"leaf": 0.0}, {"leaf": 2.30258509}]]}}}
+
+
+def test_gated_policies_load():
+    assert accepted(gated_v3_policy()) and accepted(gated_v4_policy())
