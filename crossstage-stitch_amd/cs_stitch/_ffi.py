@@ -83,6 +83,11 @@ class RenderStatistics(C.Structure):
     _fields_ = [("union_pixels", C.c_uint64), ("overlap_pixels", C.c_uint64)]
 
 
+class PyramidConfig(C.Structure):
+    _fields_ = [("levels", C.c_int), ("scale_factor", C.c_double), ("fast_threshold", C.c_int),
+                ("max_keypoints", C.c_int), ("harris_k", C.c_float)]
+
+
 class HarnessConfig(C.Structure):
     _fields_ = [("S", C.c_double), ("K", C.c_int), ("R", C.c_int), ("blend_mode", C.c_int),
                 ("fast_threshold", C.c_int), ("harris_k", C.c_float), ("ratio_threshold", C.c_double),
@@ -203,6 +208,10 @@ SIGNATURES = {
     "cs_s6_workspace_bytes": (C.c_int, [P(Geometry), C.c_int, P(sz)]),
     "cs_s6_render": (C.c_int, [P(Image), P(Image), P(Geometry), C.c_int, P(Image), P(Image), vp, sz, vp,
                                P(RenderStatistics)]),
+    "cs_pyramid_level_counts": (C.c_int, [P(PyramidConfig), P(C.c_int)]),
+    "cs_pyramid_workspace_bytes": (C.c_int, [C.c_int, C.c_int, C.c_double, P(PyramidConfig), P(sz)]),
+    "cs_pyramid_detect_describe": (C.c_int, [P(Image), C.c_double, P(PyramidConfig), P(Keypoints), P(Descriptors),
+                                             vp, P(ScaleInfo), vp, sz, vp]),
     "cs_extract_scene": (C.c_int, [P(Image), P(Image), vp, vp, vp, C.c_int]),
     "cs_extract_gate": (C.c_int, [P(Image), P(Image), P(Keypoints), P(Keypoints), P(Matches), vp,
                                   P(HomographyResult), P(ScaleInfo), P(ScaleInfo), vp, vp, vp, vp, C.c_uint]),

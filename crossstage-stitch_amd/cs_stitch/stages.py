@@ -318,3 +318,32 @@ def extract_gate(a: torch.Tensor, b: torch.Tensor, kp_a: F.Keypoints, kp_b: F.Ke
                            forward_ptr, relation_mask)
     torch.cuda.current_stream().synchronize()
     return st, (host.numpy().copy() if st == F.OK else None)
+
+
+# ---------------------------------------------------------------- multi-level detection
+
+def pyramid_level_counts(levels: int, scale_factor: float, max_keypoints: int) -> list[int]:
+    cfg = F.PyramidConfig(levels, scale_factor, 20, max_keypoints, 0.04)
+    out = (C.c_int * max(levels, 1))()
+    _check(F.load().cs_pyramid_level_counts(C.byref(cfg), out), "pyramid_level_counts")
+    return list(out[:levels])
+
+
+def pyramid_detect_describe(rgb: torch.Tensor, scale: float, levels: int, scale_factor: float, fast_threshold: int,
+                            max_keypoints: int, harris_k: float = 0.04):
+    """Keypoints (level-0 work coordinates), 32-byte descriptors and the level
+    of each keypoint, over an S1 pyramid of the device RGB image."""
+    L = F.load()
+    cfg = F.PyramidConfig(levels, scale_factor, fast_threshold, max_keypoints, harris_k)
+    n = C.c_size_t()
+    _check(L.cs_pyramid_workspace_bytes(rgb.shape[1], rgb.shape[0], scale, C.byref(cfg), C.byref(n)),
+           "pyramid_workspace_bytes")
+    ws = workspace(n.value)
+    kp = KeypointSet.empty(max_keypoints)
+    desc = torch.zeros((max(max_keypoints, 1), 32), dtype=torch.uint8, device="cuda")
+    lv = torch.zeros(max(max_keypoints, 1), dtype=torch.uint8, device="cuda")
+    si = F.ScaleInfo()
+    img, k, d = image(rgb), kp.c(max_keypoints), F.Descriptors(F.BINARY_U8, 32, 32, desc.data_ptr())
+    _check(L.cs_pyramid_detect_describe(C.byref(img), scale, C.byref(cfg), C.byref(k), C.byref(d), lv.data_ptr(),
+                                        C.byref(si), ws.data_ptr(), n.value, stream_handle()), "pyramid_detect_describe")
+    return kp, desc, lv, si

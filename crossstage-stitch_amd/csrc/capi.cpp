@@ -6,6 +6,7 @@
 #include <exception>
 
 #include "stitch/features.hpp"
+#include "stitch/pyramid.hpp"
 #include "stitch/stages.hpp"
 #include "stitch/tuning.hpp"
 
@@ -248,6 +249,33 @@ int cs_s6_render(const cs_image* a, const cs_image* b, const cs_geometry* g, int
     return guard([&] {
         return s6::render(img(a), img(b), geo(g), static_cast<s6::BlendMode>(mode), img(out), img(mask), ws, bytes,
                           st(stream), reinterpret_cast<s6::RenderStatistics*>(stats));
+    });
+}
+
+static pyramid::Config pyramid_config(const cs_pyramid_config* c) {
+    return {c->levels, c->scale_factor, c->fast_threshold, c->max_keypoints, c->harris_k};
+}
+
+int cs_pyramid_level_counts(const cs_pyramid_config* c, int* counts) {
+    if (!c) return CS_INVALID_ARGUMENT;
+    return static_cast<int>(pyramid::level_counts(pyramid_config(c), counts));
+}
+
+int cs_pyramid_workspace_bytes(int w, int h, double scale, const cs_pyramid_config* c, size_t* bytes) {
+    if (!c) return CS_INVALID_ARGUMENT;
+    return static_cast<int>(pyramid::workspace_bytes(w, h, scale, pyramid_config(c), bytes));
+}
+
+int cs_pyramid_detect_describe(const cs_image* rgb, double scale, const cs_pyramid_config* c, const cs_keypoints* kp,
+                               const cs_descriptors* d, uint8_t* levels, cs_scale_info* level0, void* ws,
+                               size_t bytes, void* stream) {
+    if (!rgb || !c || !kp || !d) return CS_INVALID_ARGUMENT;
+    return guard([&] {
+        ScaleInfo s0{};
+        const Status r = pyramid::detect_describe(img(rgb), scale, pyramid_config(c), kps(kp), desc(d), levels, &s0,
+                                                  ws, bytes, st(stream));
+        if (r == Status::OK && level0) put_scale(s0, level0);
+        return r;
     });
 }
 

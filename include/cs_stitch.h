@@ -126,6 +126,19 @@ int cs_s6_render(const cs_image* a, const cs_image* b, const cs_geometry* geomet
                  const cs_image* output, const cs_image* mask, void* workspace, size_t workspace_bytes,
                  void* stream, cs_render_statistics* statistics);
 
+/* Multi-level detection (no counterpart in the reference, whose S2/S3 are
+ * single-level): level l is S1 of the RGB input at scale S / f^l, detected
+ * and described with S2/S3 on its share of K (OpenCV's ORB split: geometric
+ * series with ratio 1/f, last level the remainder); keypoints are returned in
+ * level-0 work coordinates, level by level. levels (K bytes) and level0 may
+ * be NULL. Levels under 64 px on a side are skipped. */
+typedef struct { int levels; double scale_factor; int fast_threshold, max_keypoints; float harris_k; } cs_pyramid_config;
+int cs_pyramid_level_counts(const cs_pyramid_config* config, int* counts);
+int cs_pyramid_workspace_bytes(int width, int height, double scale, const cs_pyramid_config* config, size_t* bytes);
+int cs_pyramid_detect_describe(const cs_image* rgb, double scale, const cs_pyramid_config* config,
+                               const cs_keypoints* keypoints, const cs_descriptors* descriptors, uint8_t* levels,
+                               cs_scale_info* level0, void* workspace, size_t workspace_bytes, void* stream);
+
 /* Controller observations (src/controller/features.hpp:10-32,
  * relational_features.hpp:8-24): scene features of two GRAY8 images before
  * matching and gate features of a finished S5. device holds

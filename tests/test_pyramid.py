@@ -1,0 +1,88 @@
+"""Multi-level detection (an S1 pyramid over S2/S3), a build-defined
+extension: the reference detects on one level only, so nothing of the
+reference pins levels above 0 ("parity unpinned" against the reference).
+
+The oracle is the composition of the pinned single-level oracle stages: S1
+at scale S / f^l, S2 with the level's share of K, S3, and the map back to
+level-0 work coordinates. The level split is checked on closed forms
+(f = 2 gives exact geometric shares). GPU results must equal the
+composition bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+
+def split(levels, f, k):
+    """Keypoints per level: rounded geometric shares, remainder last."""
+    ratio = 1.0 / f
+    per = k * (1.0 - ratio) / (1.0 - ratio ** levels)
+    out, total = [], 0
+    for _ in range(levels - 1):
+        c = int(np.rint(per))
+        out.append(c)
+        total += c
+        per *= ratio
+    out.append(max(k - total, 0))
+    return out
+
+
+def test_level_split_closed_forms():
+    from cs_stitch import stages as st
+    assert st.pyramid_level_counts(2, 2.0, 3000) == [2000, 1000]
+    assert st.pyramid_level_counts(3, 2.0, 7) == [4, 2, 1]
+    assert st.pyramid_level_counts(1, 1.2, 5000) == [5000]
+    for levels, f, k in ((8, 1.2, 5000), (4, 1.5, 1000), (16, 1.05, 37), (8, 1.2, 0)):
+        c = st.pyramid_level_counts(levels, f, k)
+        assert c == split(levels, f, k) and sum(c) == k
+        assert all(a >= b for a, b in zip(c[:-2], c[1:-1]))
+
+
+def oracle_pyramid(rgb, S, levels, f, thr, k, harris_k=0.04):
+    h, w, _ = rgb.shape
+    _, s0 = O.scale_info(w, h, S)
+    xs, ys, rs, ds, lv = [], [], [], [], []
+    for l, n in enumerate(split(levels, f, k)):
+        scale = S / f ** l
+        st, sl = O.scale_info(w, h, scale)
+        if n == 0 or st != 0 or sl.work_w < 64 or sl.work_h < 64:
+            continue
+        g = O.s1_preprocess(rgb, scale)
+        kp = O.s2_detect(g, thr, n, harris_k)
+        d = O.s3_describe(g, kp)
+        x = kp[:, 0].astype(np.float64)
+        y = kp[:, 1].astype(np.float64)
+        xs.append(((x + 0.5) / sl.sx * s0.sx - 0.5).astype(np.float32))
+        ys.append(((y + 0.5) / sl.sy * s0.sy - 0.5).astype(np.float32))
+        rs.append(kp[:, 2])
+        ds.append(d)
+        lv.append(np.full(len(kp), l, np.uint8))
+    cat = lambda a, dt: np.concatenate(a) if a else np.zeros(0, dt)
+    return (cat(xs, np.float32), cat(ys, np.float32), cat(rs, np.float32),
+            np.concatenate(ds) if ds else np.zeros((0, 32), np.uint8), cat(lv, np.uint8))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w,h,S,levels,f,k", [(640, 480, 1.0, 4, 1.2, 2000), (1280, 720, 0.5, 8, 1.2, 3000),
+                                              (800, 600, 0.75, 3, 2.0, 1000), (640, 480, 1.0, 1, 1.2, 1500)])
+def test_pyramid_matches_composed_oracle(w, h, S, levels, f, k):
+    import torch
+    from cs_stitch import pipeline as P
+    from cs_stitch import stages as st
+    a, _, _ = P.synth_pair(6100 + w, w, h, overlap=(0.5, 0.7))
+    t = st.pitched(h, w, 3)
+    t.copy_(torch.from_numpy(a).cuda())
+    kp, desc, lv, si = st.pyramid_detect_describe(t, S, levels, f, 20, k)
+    torch.cuda.synchronize()
+    n, x, y, r = kp.host()
+    ox, oy, orr, od, ol = oracle_pyramid(a, S, levels, f, 20, k)
+    print(w, h, S, levels, f, k, "keypoints", n, "per level", np.bincount(ol, minlength=levels).tolist())
+    assert n == len(ox) and n > 0
+    assert np.array_equal(x.numpy(), ox) and np.array_equal(y.numpy(), oy) and np.array_equal(r.numpy(), orr)
+    assert np.array_equal(desc[:n].cpu().numpy(), od)
+    assert np.array_equal(lv[:n].cpu().numpy(), ol)
+    _, s0 = O.scale_info(w, h, S)
+    assert (si.work_w, si.work_h) == (s0.work_w, s0.work_h)
+    # Every keypoint lies inside the level-0 image.
+    assert x.min() >= 0 and y.min() >= 0 and x.max() < s0.work_w and y.max() < s0.work_h
