@@ -92,7 +92,8 @@ class HarnessConfig(C.Structure):
     _fields_ = [("S", C.c_double), ("K", C.c_int), ("R", C.c_int), ("blend_mode", C.c_int),
                 ("fast_threshold", C.c_int), ("harris_k", C.c_float), ("ratio_threshold", C.c_double),
                 ("cross_check", C.c_int), ("msac_batch_size", C.c_int), ("msac_refinements", C.c_int),
-                ("msac_threshold", C.c_double), ("seed", C.c_uint64)]
+                ("msac_threshold", C.c_double), ("seed", C.c_uint64), ("pyramid_levels", C.c_int),
+                ("pyramid_scale_factor", C.c_double)]
 
 
 class AttemptRecord(C.Structure):

@@ -25,10 +25,11 @@ def synth_pair(seed: int, width: int, height: int, overlap=(0.3, 0.5), kind: int
 
 
 def harness_config(S=1.0, K=1000, R=256, blend="feather", fast_threshold=20, harris_k=0.04, ratio=0.8,
-                   cross_check=False, batch=64, refinements=1, threshold=3.0, seed=0) -> F.HarnessConfig:
+                   cross_check=False, batch=64, refinements=1, threshold=3.0, seed=0, pyramid_levels=1,
+                   pyramid_scale_factor=1.2) -> F.HarnessConfig:
     mode = F.BLEND_NAMES[blend] if isinstance(blend, str) else int(blend)
     return F.HarnessConfig(S, K, R, mode, fast_threshold, harris_k, ratio, int(cross_check), batch, refinements,
-                           threshold, seed)
+                           threshold, seed, pyramid_levels, pyramid_scale_factor)
 
 
 def record_dict(r: F.RunRecord) -> dict:

@@ -48,6 +48,8 @@ harness::HarnessConfig config(const cs_harness_config* c) {
     h.msac_refinements = c->msac_refinements;
     h.msac_threshold = c->msac_threshold;
     h.seed = c->seed;
+    h.pyramid_levels = c->pyramid_levels > 1 ? c->pyramid_levels : 1;
+    h.pyramid_scale_factor = c->pyramid_levels > 1 ? c->pyramid_scale_factor : 1.2;
     return h;
 }
 

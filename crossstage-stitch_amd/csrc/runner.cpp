@@ -2,6 +2,8 @@
 // Flow of the reference's harness/pipeline_runner.cu:110-885.
 #include "stitch/runner.hpp"
 
+#include "stitch/pyramid.hpp"
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -56,7 +58,7 @@ PipelineRunner::~PipelineRunner() {
                     static_cast<void*>(matches_.distance), static_cast<void*>(matches_.d_count),
                     static_cast<void*>(inlier_mask_), static_cast<void*>(canvas_.data),
                     static_cast<void*>(canvas_mask_.data), static_cast<void*>(stats_), ws_s2_a_, ws_s2_b_, ws_s3_,
-                    ws_s4_, ws_s5_, ws_s6_, ws_probe_, static_cast<void*>(d_features_)})
+                    ws_s4_, ws_s5_, ws_s6_, ws_probe_, static_cast<void*>(d_features_), ws_pyramid_})
         if (p) hipFree(p);
     if (pinned_features_) hipHostFree(pinned_features_);
     if (pinned_h_) hipHostFree(pinned_h_);
@@ -76,6 +78,12 @@ PipelineMemoryPlan PipelineRunner::query_memory(int w, int h, int k, int r, int 
     plan.device_bytes = add(plan.device_bytes, add(multiply(k, 6 * sizeof(float) + 64 + 3 * sizeof(int) + 1),
                                                    3 * sizeof(int) + sizeof(s6::RenderStatistics) + 32));
     plan.device_bytes = add(plan.device_bytes, features::kDeviceDoubles * sizeof(double));
+    {
+        std::size_t pyramid_bytes = 0;
+        if (pyramid::workspace_bytes(w, h, 1.0, {pyramid::kMaxLevels, 1.2, 20, k, .04f}, &pyramid_bytes) != Status::OK)
+            throw Error("ERR_CAPACITY");
+        plan.device_bytes = add(plan.device_bytes, pyramid_bytes);
+    }
     auto workspace = [&](std::size_t bytes, int count = 1) {
         plan.workspace_bytes = add(plan.workspace_bytes, multiply(bytes, count));
         if (bytes) plan.device_bytes = add(plan.device_bytes, multiply(add(bytes, 256), count));
@@ -174,6 +182,10 @@ void PipelineRunner::initialize(int max_w, int max_h, int max_k, int max_r, int 
     bytes_s6_ = plan.s6_workspace_bytes;
     if (bytes_s6_) ws_s6_ = alloc(bytes_s6_);
     ws_probe_ = alloc(32);
+    if (pyramid::workspace_bytes(max_w, max_h, 1.0, {pyramid::kMaxLevels, 1.2, 20, max_k, .04f}, &bytes_pyramid_) !=
+        Status::OK)
+        throw admission::Error("ERR_CAPACITY");
+    ws_pyramid_ = alloc(bytes_pyramid_);
     d_features_ = static_cast<double*>(alloc(features::kDeviceDoubles * sizeof(double)));
     check(hipHostMalloc(reinterpret_cast<void**>(&pinned_features_), features::kGateCount * sizeof(double),
                         hipHostMallocDefault),
@@ -264,15 +276,28 @@ void PipelineRunner::execute_run(const HarnessConfig& cfg, RunRecord* rec) {
     check(hipEventRecord(ev_[1], stream_), "event");
     Keypoints ka = kp_a_, kb = kp_b_;
     if (cfg.K >= 0 && cfg.K <= kp_a_.capacity) ka.capacity = kb.capacity = cfg.K;
-    const s2::Config c2{cfg.fast_threshold, cfg.K, cfg.harris_k};
-    Status sa2 = s2::s2_detect(gray_a_, c2, ka, ws_s2_a_, bytes_s2_, stream_);
-    Status sb2 = s2::s2_detect(gray_b_, c2, kb, ws_s2_b_, bytes_s2_, stream_);
-    if (sa2 != Status::OK || sb2 != Status::OK) return fail(status_name(sa2 != Status::OK ? sa2 : sb2), "s2");
-    check(hipEventRecord(ev_[2], stream_), "event");
-    Status sa3 = s3::s3_describe(gray_a_, ka, desc_a_, ws_s3_, bytes_s3_, stream_);
-    Status sb3 = s3::s3_describe(gray_b_, kb, desc_b_, ws_s3_, bytes_s3_, stream_);
-    if (sa3 != Status::OK || sb3 != Status::OK) return fail(status_name(sa3 != Status::OK ? sa3 : sb3), "s3");
-    check(hipEventRecord(ev_[3], stream_), "event");
+    if (cfg.pyramid_levels > 1) {
+        // Pyramid detection and description (S2 time covers both; S3 is 0).
+        const pyramid::Config pc{cfg.pyramid_levels, cfg.pyramid_scale_factor, cfg.fast_threshold, cfg.K, cfg.harris_k};
+        Status sp = pyramid::detect_describe(use_a_, cfg.S, pc, ka, desc_a_, nullptr, nullptr, ws_pyramid_,
+                                             bytes_pyramid_, stream_);
+        if (sp == Status::OK)
+            sp = pyramid::detect_describe(use_b_, cfg.S, pc, kb, desc_b_, nullptr, nullptr, ws_pyramid_,
+                                          bytes_pyramid_, stream_);
+        if (sp != Status::OK) return fail(status_name(sp), "s2");
+        check(hipEventRecord(ev_[2], stream_), "event");
+        check(hipEventRecord(ev_[3], stream_), "event");
+    } else {
+        const s2::Config c2{cfg.fast_threshold, cfg.K, cfg.harris_k};
+        Status sa2 = s2::s2_detect(gray_a_, c2, ka, ws_s2_a_, bytes_s2_, stream_);
+        Status sb2 = s2::s2_detect(gray_b_, c2, kb, ws_s2_b_, bytes_s2_, stream_);
+        if (sa2 != Status::OK || sb2 != Status::OK) return fail(status_name(sa2 != Status::OK ? sa2 : sb2), "s2");
+        check(hipEventRecord(ev_[2], stream_), "event");
+        Status sa3 = s3::s3_describe(gray_a_, ka, desc_a_, ws_s3_, bytes_s3_, stream_);
+        Status sb3 = s3::s3_describe(gray_b_, kb, desc_b_, ws_s3_, bytes_s3_, stream_);
+        if (sa3 != Status::OK || sb3 != Status::OK) return fail(status_name(sa3 != Status::OK ? sa3 : sb3), "s3");
+        check(hipEventRecord(ev_[3], stream_), "event");
+    }
     s = s4::s4_match(ka, desc_a_, kb, desc_b_, {cfg.ratio_threshold, cfg.cross_check}, matches_, ws_s4_, bytes_s4_,
                      stream_);
     if (s != Status::OK) return fail(status_name(s), "s4");

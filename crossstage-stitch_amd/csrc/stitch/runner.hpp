@@ -31,6 +31,10 @@ struct HarnessConfig {
     int msac_refinements{1};
     double msac_threshold{3.0};
     std::uint64_t seed{0};
+    // Multi-level detection (an extension; 1 = the reference's single level):
+    // S2/S3 run on an S1 pyramid with this many levels and scale factor.
+    int pyramid_levels{1};
+    double pyramid_scale_factor{1.2};
 };
 
 struct RunRecord {
@@ -165,6 +169,8 @@ private:
     void* ws_s5_{nullptr};
     void* ws_s6_{nullptr};
     void* ws_probe_{nullptr};
+    void* ws_pyramid_{nullptr};
+    std::size_t bytes_pyramid_{0};
     double* pinned_grad_{nullptr};
     std::size_t bytes_s2_{0}, bytes_s3_{0}, bytes_s4_{0}, bytes_s5_{0}, bytes_s6_{0};
     bool initialized_{false};

@@ -104,6 +104,9 @@ std::vector<Config> configs(const std::string& path) {
         if (const Json* v = item.find("S")) c.cfg.S = v->num;
         if (const Json* v = item.find("K")) c.cfg.K = static_cast<int>(v->num);
         if (const Json* v = item.find("R")) c.cfg.R = static_cast<int>(v->num);
+        // Extension keys: multi-level detection.
+        if (const Json* v = item.find("pyramid_levels")) c.cfg.pyramid_levels = std::max(1, static_cast<int>(v->num));
+        if (const Json* v = item.find("pyramid_scale_factor")) c.cfg.pyramid_scale_factor = v->num;
         if (const Json* v = item.find("blend_mode")) c.blend = v->str;
         c.cfg.blend_mode = blend_of(c.blend);
         const Json* id = item.find("config_id");

@@ -172,6 +172,8 @@ typedef struct {
     int cross_check, msac_batch_size, msac_refinements;
     double msac_threshold;
     uint64_t seed;
+    int pyramid_levels;           /* 0 or 1: single-level S2/S3 (the reference's) */
+    double pyramid_scale_factor;  /* used when pyramid_levels > 1 */
 } cs_harness_config;
 typedef struct {
     double S;

@@ -86,3 +86,35 @@ def test_pyramid_matches_composed_oracle(w, h, S, levels, f, k):
     assert (si.work_w, si.work_h) == (s0.work_w, s0.work_h)
     # Every keypoint lies inside the level-0 image.
     assert x.min() >= 0 and y.min() >= 0 and x.max() < s0.work_w and y.max() < s0.work_h
+
+
+@pytest.mark.gpu
+def test_runner_with_pyramid_matches_oracle_chain():
+    """The plain runner with pyramid_levels = 4: keypoints, descriptors and
+    matches of the pyramid, then S5, equal the oracle chain run on the
+    composed pyramid."""
+    from cs_stitch import pipeline as P
+    w, h, S, K, R, levels = 960, 720, 1.0, 2000, 512, 4
+    a, b, _ = P.synth_pair(6200, w, h, overlap=(0.5, 0.7))
+    runner = P.PipelineRunner(w, h, max_k=4000, max_r=2048, canvas_width=4096, canvas_height=2048)
+    runner.upload_pair(a, b)
+    rec = runner.execute_run(P.harness_config(S=S, K=K, R=R, pyramid_levels=levels, pyramid_scale_factor=1.2))
+    pa = oracle_pyramid(a, S, levels, 1.2, 20, K)
+    pb = oracle_pyramid(b, S, levels, 1.2, 20, K)
+    assert (rec.num_kp_a, rec.num_kp_b) == (len(pa[0]), len(pb[0]))
+    ia, ib, dist = O.s4_match(pa[3], pb[3], 0.8, False)
+    _, sa = O.scale_info(w, h, S)
+    _, sb = O.scale_info(w, h, S)
+    res, mask, _ = O.s5_estimate_kp(pa[0], pa[1], pb[0], pb[1], ia, ib, sa, sb, R, 1, 3.0, 0)
+    out = runner.outputs()
+    nm = rec.num_matches
+    print("pyramid runner", rec.status, rec.num_kp_a, rec.num_kp_b, nm, rec.num_inliers)
+    assert nm == len(ia) and rec.num_inliers == res.num_inliers
+    assert np.array_equal(P.download(out["matches"].idx_a, 4 * nm).view(np.int32), ia)
+    assert np.array_equal(P.download(out["matches"].idx_b, 4 * nm).view(np.int32), ib)
+    assert np.array_equal(P.download(out["inlier_mask"], nm), mask[:nm])
+    assert out["h"].status == res.status
+    if res.status == 0:
+        assert np.array_equal(np.array(out["h"].H[:]), np.array(res.H[:]))
+        assert rec.status == b"OK"
+    runner.close()
