@@ -80,20 +80,23 @@ def build(jobs: int = 0, verbose: bool = False) -> Path:
 
 HARNESS_SRC = PKG / "harness" / "stitch_harness.cpp"
 HARNESS = PKG / "bin" / "stitch_harness"
+STRESS_SRC = PKG / "harness" / "stitch_stress.cpp"
+STRESS = PKG / "bin" / "stitch_stress"
 
 
 def build_harness(verbose: bool = False) -> Path:
-    """The stitch_harness command-line driver, linked against libcsstitch.so
-    (found next to it through the run path)."""
+    """The stitch_harness and stitch_stress command-line drivers, linked
+    against libcsstitch.so (found next to them through the run path)."""
     HARNESS.parent.mkdir(exist_ok=True)
-    cmd = [HIPCC, *COMMON, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", str(HARNESS_SRC),
-           "-o", str(HARNESS), f"-L{OUT.parent}", "-lcsstitch", "-Wl,-rpath,$ORIGIN/../cs_stitch",
-           "-L/opt/rocm/lib", "-lamdhip64", "-lpthread"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"harness build failed\n{r.stdout}\n{r.stderr}")
+    for src, exe in ((HARNESS_SRC, HARNESS), (STRESS_SRC, STRESS)):
+        cmd = [HIPCC, *COMMON, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", str(src),
+               "-o", str(exe), f"-L{OUT.parent}", "-lcsstitch", "-Wl,-rpath,$ORIGIN/../cs_stitch",
+               "-L/opt/rocm/lib", "-lamdhip64", "-lpthread"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{exe.name} build failed\n{r.stdout}\n{r.stderr}")
     return HARNESS
 
 
