@@ -147,3 +147,33 @@ def test_adaptive_campaign_with_attempt_trace(tmp_path):
     # A trace path must be fresh.
     assert run("--manifest", tmp_path / "manifest.json", "--adaptive", "--controller-policy", "legacy-v1",
                "--out", tmp_path / "r2.csv", "--trace-out", trace).returncode == 1
+
+
+@pytest.mark.gpu
+def test_adaptive_campaign_with_learned_policy_file(tmp_path):
+    """A controller-policy-v3 file drives the adaptive campaign; the features
+    CSV carries the scene and last-attempt gate observations the runner
+    collected, equal to the Python runner's under the same policy."""
+    import json
+    from cs_stitch import pipeline as P
+    from test_policies import gated_v3_policy
+    pair_files(tmp_path, 1)
+    policy = tmp_path / "policy.json"
+    policy.write_text(json.dumps(gated_v3_policy()))
+    out, trace, feats = tmp_path / "runs.csv", tmp_path / "attempts.csv", tmp_path / "features.csv"
+    r = run("--manifest", tmp_path / "manifest.json", "--adaptive", "--controller-policy", policy, "--out", out,
+            "--trace-out", trace, "--features-out", feats, "--warmup", 0, "--repeats", 1, "--seed", 5)
+    assert r.returncode == 0, r.stderr
+    rows, frows = read_csv(out), read_csv(feats)
+    assert len(rows) == 1 and rows[0]["policy_id"] == "gated-v3"
+    a, b, _ = P.synth_pair(31000, 640, 480, overlap=(0.5, 0.7))
+    runner = P.PipelineRunner(640, 480, max_k=4000, max_r=2048)
+    runner.upload_pair(a, b)
+    rec = runner.execute_adaptive_run(P.Controller(policy_json=policy.read_text()), seed=5)
+    scene, gate, present, _ = runner.features()
+    assert rows[0]["status"] == rec.status.decode() and int(rows[0]["attempt_count"]) == rec.attempt_count
+    row = frows[0]
+    assert row["features_present"] == str(int(present))
+    assert [float(row[f"scene_{k}"]) for k in range(32)] == list(scene)
+    assert [float(row[f"gate_{k}"]) for k in range(38)] == list(gate)
+    runner.close()
