@@ -78,3 +78,21 @@ def test_adaptive_runner_legacy_policy():
         assert rec2.status == rec.status and rec2.attempt_count == rec.attempt_count
         assert list(rec2.H) == list(rec.H) or status != "OK"
         runner.close()
+
+
+def test_large_stress_adjacent_pair_matches_oracle():
+    """One adjacent pair of the large-scale strip: 3840x2160 frames at 30 %
+    overlap, 16000 keypoints per image (16000 x 16000 Hamming), RANSAC 8192,
+    7-band blend; every stage against the oracle chain."""
+    from cs_stitch import pipeline as P
+    w, h = 3840, 2160
+    # A seed whose pair canvas (6046 x 2289) fits the 16.7 MP canvas budget;
+    # the whole 8-frame strip does not (it is not rendered here).
+    a, b, _ = P.synth_pair(20261204, w, h, overlap=(0.3, 0.3))
+    runner = P.PipelineRunner(w, h, max_k=16000, max_r=8192, canvas_width=6400, canvas_height=2600,
+                              workspace_mode=P.F.MULTIBAND7)
+    rec, ref = pipeline_oracle.check_runner(runner, a, b, 1.0, 16000, 8192, "multiband_7", False)
+    print("stress pair", rec.status, rec.num_kp_a, rec.num_kp_b, rec.num_matches, rec.num_inliers,
+          rec.canvas_width, rec.canvas_height)
+    assert ref["result"].status == 0 and rec.status == b"OK"
+    runner.close()

@@ -118,3 +118,35 @@ def test_runner_with_pyramid_matches_oracle_chain():
         assert np.array_equal(np.array(out["h"].H[:]), np.array(res.H[:]))
         assert rec.status == b"OK"
     runner.close()
+
+
+@pytest.mark.gpu
+def test_reference_cpu_case_with_pyramid_end_to_end():
+    """The reference's CPU-runnable case: 512x512, FAST 20, 1000 keypoints on
+    an 8-level pyramid at scale 1.2, ratio 0.8, RANSAC 2000 at 3 px, 5-band
+    blend. Every stage equals the oracle chain on the composed pyramid; the
+    panorama is within 1 LSB with the mask exact."""
+    from cs_stitch import pipeline as P
+    w = h = 512
+    a, b, _ = P.synth_pair(20261201, w, h, overlap=(0.6, 0.8))
+    runner = P.PipelineRunner(w, h, max_k=4000, max_r=2048, canvas_width=2048, canvas_height=2048)
+    runner.upload_pair(a, b)
+    rec = runner.execute_run(P.harness_config(S=1.0, K=1000, R=2000, blend="multiband_5", pyramid_levels=8,
+                                              pyramid_scale_factor=1.2))
+    pa = oracle_pyramid(a, 1.0, 8, 1.2, 20, 1000)
+    pb = oracle_pyramid(b, 1.0, 8, 1.2, 20, 1000)
+    ia, ib, _ = O.s4_match(pa[3], pb[3], 0.8, False)
+    _, s0 = O.scale_info(w, h, 1.0)
+    res, mask, _ = O.s5_estimate_kp(pa[0], pa[1], pb[0], pb[1], ia, ib, s0, s0, 2000, 1, 3.0, 0)
+    print("reference CPU case", rec.status, rec.num_kp_a, rec.num_kp_b, rec.num_matches, rec.num_inliers)
+    assert (rec.num_kp_a, rec.num_kp_b, rec.num_matches) == (len(pa[0]), len(pb[0]), len(ia))
+    assert res.status == 0 and rec.status == b"OK" and rec.num_inliers == res.num_inliers
+    out = runner.outputs()
+    assert np.array_equal(np.array(out["h"].H[:]), np.array(res.H[:]))
+    st, geo = O.prepare_geometry(w, h, w, h, np.array(res.H[:]))
+    assert st == 0
+    pano, pmask, _ = O.s6_render(a, b, geo, O.MODES["multiband_5"])
+    got = P.download_image(out["canvas"], 3)
+    assert np.array_equal(P.download_image(out["mask"], 1), pmask)
+    assert np.abs(got.astype(int) - pano.astype(int)).max() <= 1
+    runner.close()
