@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(kAppendThreads)
     if (threadIdx.x == 0) {
         base = *total;
         n = min(max(*level.d_count, 0), level.capacity);
-        n = min(n, out.capacity - base);
+        n = max(0, min(n, out.capacity - base));
     }
     __syncthreads();
     for (int i = static_cast<int>(threadIdx.x); i < n; i += kAppendThreads) {
