@@ -215,7 +215,8 @@ def assert_features_close(got, want):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("shape_a,shape_b,kind", [((120, 160), (120, 160), "texture"), ((67, 93), (71, 88), "noise"),
-                                                  ((2, 2), (3, 5), "noise"), ((270, 480), (270, 480), "texture")])
+                                                  ((2, 2), (3, 5), "noise"), ((270, 480), (270, 480), "texture"),
+                                                  ((64, 80), (64, 80), "constant")])
 @pytest.mark.parametrize("relations", [True, False])
 def test_extract_scene_matches_oracle(shape_a, shape_b, kind, relations):
     import torch
@@ -224,6 +225,11 @@ def test_extract_scene_matches_oracle(shape_a, shape_b, kind, relations):
     if kind == "texture":
         a = tex_gray(shape_a[1], shape_a[0], 1)
         b = tex_gray(shape_b[1], shape_b[0], 2)
+    elif kind == "constant":
+        # Zero variances: the NCC terms are rounding residues over the 1e-12
+        # floor, so only identical arithmetic agrees here.
+        a = np.full(shape_a, 77, np.uint8)
+        b = np.full(shape_b, 200, np.uint8)
     else:
         a = rng.integers(0, 256, shape_a, dtype=np.uint8)
         b = rng.integers(0, 256, shape_b, dtype=np.uint8)
@@ -299,6 +305,11 @@ def test_extract_gate_matches_oracle(seed, w, h, S, K, R, cross):
         assert status == ost == 0, (status, ost)
         print(seed, relation_mask, "bit-equal:", int((got == want).sum()), "of 38")
         assert_features_close(got, want)
+    # Fewer than 4 matches: INVALID_ARGUMENT, as the oracle says.
+    few = P.F.HomographyResult.from_buffer_copy(bytes(hres))
+    few.num_matches = 3
+    assert st.extract_gate(d["ra"], d["rb"], d["ka"].c(K), d["kb"].c(K), d["m"].c(), d["s5"].mask.data_ptr(), few,
+                           d["sa"], d["sb"], fptr, 7)[0] == P.F.INVALID_ARGUMENT
     # A singular homography is refused before anything is enqueued.
     bad = P.F.HomographyResult.from_buffer_copy(bytes(hres))
     for i in range(9):
