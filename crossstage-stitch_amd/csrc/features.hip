@@ -41,10 +41,12 @@ __device__ double bilinear(const ImageView& im, double x, double y) {
            dy * ((1 - dx) * gray_of(im, ix, jy) + dx * gray_of(im, jx, jy));
 }
 
+// Pairwise tree over the first N threads' values (threads beyond N, in a
+// wider block, take no part); every thread gets the total.
 template <int N>
 __device__ double block_sum(double v, double* scratch) {
     const int t = static_cast<int>(threadIdx.x);
-    scratch[t] = v;
+    if (t < N) scratch[t] = v;
     __syncthreads();
     for (int step = N / 2; step > 0; step /= 2) {
         if (t < step) scratch[t] += scratch[t + step];
@@ -168,27 +170,54 @@ __global__ void pair_peak(const double* workspace, double* out) {
 }
 
 // A 64x64 grid over A through the inverse homography into B: overlap share
-// and photometric agreement (features.cu:65-91).
-__global__ void __launch_bounds__(kThreads) photo_consistency(ImageView a, ImageView b, Mat3 inverse, double* out) {
-    __shared__ double scratch[kThreads];
+// and photometric agreement (features.cu:65-91). The samples are evaluated
+// by all 1024 threads, 1024 at a time, into LDS; then threads 0..127 add
+// them to their partial sums in the same order as one thread per partial
+// walking i = t, t + 128, ... would (the sums are unchanged).
+constexpr int kPhotoThreads = 1024;
+constexpr int kPhotoSamples = 4096;
+
+__global__ void __launch_bounds__(kPhotoThreads) photo_consistency(ImageView a, ImageView b, Mat3 inverse, double* out) {
+    __shared__ double sample_a[kPhotoThreads], sample_b[kPhotoThreads], scratch[kThreads];
+    __shared__ bool sample_ok[kPhotoThreads];
+    const int t = static_cast<int>(threadIdx.x);
     double n = 0, sum_a = 0, sum_b = 0, sq_a = 0, sq_b = 0, cross = 0, abs_error = 0, sq_error = 0;
-    for (int i = static_cast<int>(threadIdx.x); i < 4096; i += kThreads) {
-        const double x = (i % 64) * (a.width - 1.0) / 63.0, y = (i / 64) * (a.height - 1.0) / 63.0;
-        const double* m = inverse.v;
-        const double z = m[6] * x + m[7] * y + m[8];
-        if (fabs(z) < 1e-12) continue;
-        const double u = (m[0] * x + m[1] * y + m[2]) / z, v = (m[3] * x + m[4] * y + m[5]) / z;
-        if (u < 0 || v < 0 || u > b.width - 1 || v > b.height - 1 || !isfinite(u) || !isfinite(v)) continue;
-        const double av = bilinear(a, fmax(0.0, x), fmax(0.0, y)), bv = bilinear(b, u, v), d = av - bv;
-        n += 1;
-        sum_a += av;
-        sum_b += bv;
-        sq_a += av * av;
-        sq_b += bv * bv;
-        cross += av * bv;
-        abs_error += fabs(d);
-        sq_error += d * d;
+    for (int round = 0; round < kPhotoSamples; round += kPhotoThreads) {
+        {
+            const int i = round + t;
+            const double x = (i % 64) * (a.width - 1.0) / 63.0, y = (i / 64) * (a.height - 1.0) / 63.0;
+            const double* m = inverse.v;
+            const double z = m[6] * x + m[7] * y + m[8];
+            bool ok = !(fabs(z) < 1e-12);
+            double u = 0, v = 0;
+            if (ok) {
+                u = (m[0] * x + m[1] * y + m[2]) / z;
+                v = (m[3] * x + m[4] * y + m[5]) / z;
+                ok = !(u < 0 || v < 0 || u > b.width - 1 || v > b.height - 1 || !isfinite(u) || !isfinite(v));
+            }
+            sample_ok[t] = ok;
+            if (ok) {
+                sample_a[t] = bilinear(a, fmax(0.0, x), fmax(0.0, y));
+                sample_b[t] = bilinear(b, u, v);
+            }
+        }
+        __syncthreads();
+        if (t < kThreads)
+            for (int k = t; k < kPhotoThreads; k += kThreads) {
+                if (!sample_ok[k]) continue;
+                const double av = sample_a[k], bv = sample_b[k], d = av - bv;
+                n += 1;
+                sum_a += av;
+                sum_b += bv;
+                sq_a += av * av;
+                sq_b += bv * bv;
+                cross += av * bv;
+                abs_error += fabs(d);
+                sq_error += d * d;
+            }
+        __syncthreads();
     }
+    // Threads 128..1023 hold zeros and take no part in the tree below.
     n = block_sum<kThreads>(n, scratch);
     sum_a = block_sum<kThreads>(sum_a, scratch);
     sum_b = block_sum<kThreads>(sum_b, scratch);
@@ -197,7 +226,7 @@ __global__ void __launch_bounds__(kThreads) photo_consistency(ImageView a, Image
     cross = block_sum<kThreads>(cross, scratch);
     abs_error = block_sum<kThreads>(abs_error, scratch);
     sq_error = block_sum<kThreads>(sq_error, scratch);
-    if (threadIdx.x == 0) {
+    if (t == 0) {
         const double den = fmax(1.0, n), ma = sum_a / den, mb = sum_b / den;
         const double va = fmax(0.0, sq_a / den - ma * ma), vb = fmax(0.0, sq_b / den - mb * mb);
         const double sd = sqrt(va) + sqrt(vb);
@@ -520,7 +549,7 @@ Status extract_gate(const ImageView& a, const ImageView& b, const Keypoints& ka,
                         sb.sy > 0 && sa.work_w > 0 && sa.work_h > 0 && relation_mask <= 7;
         if (!ok) return Status::CUDA_ERROR;
     }
-    photo_consistency<<<1, kThreads, 0, stream>>>(a, b, inv, device);
+    photo_consistency<<<1, kPhotoThreads, 0, stream>>>(a, b, inv, device);
     inlier_spread<<<1, kThreads, 0, stream>>>(ka, kb, matches, mask, h.num_matches, sa.work_w, sa.work_h, sb.work_w,
                                               sb.work_h, device);
     fill_zero<<<1, 64, 0, stream>>>(device + 24, kAlignmentRelations);
