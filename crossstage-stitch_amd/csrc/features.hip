@@ -82,9 +82,15 @@ __global__ void fill_zero(double* p, int n) {
 
 // Blocks 0 and 1: the mean absolute gradient of A and B. Blocks 2-9: per
 // image (A then B) and quadrant (row-major), the mean, standard deviation
-// and gradient of the gray levels (features.cu:34-61).
-__global__ void __launch_bounds__(kThreads) scene_stats(ImageView a, ImageView b, double* out) {
-    __shared__ double scratch[kThreads];
+// and gradient of the gray levels (features.cu:34-61). Pixels are read by
+// all 1024 threads, 1024 at a time, into LDS; threads 0..127 then add them
+// in the order of one thread per partial walking i = t, t + 128, ...
+constexpr int kSceneThreads = 1024;
+
+__global__ void __launch_bounds__(kSceneThreads) scene_stats(ImageView a, ImageView b, double* out) {
+    __shared__ double value[kSceneThreads], right[kSceneThreads], down[kSceneThreads], scratch[kThreads];
+    __shared__ unsigned char kind[kSceneThreads];  // bit 0: pixel, bit 1: right neighbour, bit 2: lower neighbour
+    const int t = static_cast<int>(threadIdx.x);
     const int region = static_cast<int>(blockIdx.x);
     const bool whole = region < 2;
     const int image = whole ? region : (region - 2) / 4, quad = whole ? 0 : (region - 2) % 4;
@@ -92,21 +98,46 @@ __global__ void __launch_bounds__(kThreads) scene_stats(ImageView a, ImageView b
     const int x0 = whole ? 0 : (quad % 2) * im.width / 2, y0 = whole ? 0 : (quad / 2) * im.height / 2;
     const int x1 = whole ? im.width : (quad % 2 + 1) * im.width / 2;
     const int y1 = whole ? im.height : (quad / 2 + 1) * im.height / 2;
-    const int w = x1 - x0, h = y1 - y0;
+    const int w = x1 - x0, h = y1 - y0, count = w * h;
     double sum = 0, sq = 0, grad = 0;
-    for (int i = static_cast<int>(threadIdx.x); i < w * h; i += kThreads) {
-        const int x = x0 + i % w, y = y0 + i / w;
-        const double v = gray_of(im, x, y);
-        sum += v;
-        sq += v * v;
-        if (x + 1 < x1) grad += fabs(v - gray_of(im, x + 1, y));
-        if (y + 1 < y1) grad += fabs(v - gray_of(im, x, y + 1));
+    for (int round = 0; round < count; round += kSceneThreads) {
+        {
+            const int i = round + t;
+            unsigned char k = 0;
+            if (i < count) {
+                const int x = x0 + i % w, y = y0 + i / w;
+                const double v = gray_of(im, x, y);
+                value[t] = v;
+                k = 1;
+                if (x + 1 < x1) {
+                    right[t] = fabs(v - gray_of(im, x + 1, y));
+                    k |= 2;
+                }
+                if (y + 1 < y1) {
+                    down[t] = fabs(v - gray_of(im, x, y + 1));
+                    k |= 4;
+                }
+            }
+            kind[t] = k;
+        }
+        __syncthreads();
+        if (t < kThreads)
+            for (int j = t; j < kSceneThreads; j += kThreads) {
+                const unsigned char k = kind[j];
+                if (!k) continue;
+                const double v = value[j];
+                sum += v;
+                sq += v * v;
+                if (k & 2) grad += right[j];
+                if (k & 4) grad += down[j];
+            }
+        __syncthreads();
     }
     sum = block_sum<kThreads>(sum, scratch);
     sq = block_sum<kThreads>(sq, scratch);
     grad = block_sum<kThreads>(grad, scratch);
-    if (threadIdx.x == 0) {
-        const double pixels = static_cast<double>(w * h), edges = static_cast<double>(h * (w - 1) + w * (h - 1));
+    if (t == 0) {
+        const double pixels = static_cast<double>(count), edges = static_cast<double>(h * (w - 1) + w * (h - 1));
         const double g = edges != 0 ? grad / edges : 0.0;
         if (whole) {
             out[region] = g;
@@ -497,7 +528,7 @@ Status extract_scene(const ImageView& a, const ImageView& b, double* device, dou
     if (!a.data || !b.data || !device || !host || a.width < 2 || b.width < 2 || a.height < 2 || b.height < 2 ||
         a.format != PixelFormat::GRAY8 || b.format != PixelFormat::GRAY8)
         return Status::INVALID_ARGUMENT;
-    scene_stats<<<10, kThreads, 0, stream>>>(a, b, device);
+    scene_stats<<<10, kSceneThreads, 0, stream>>>(a, b, device);
     fill_zero<<<1, 64, 0, stream>>>(device + 26, kPairRelations);
     if (launched() != Status::OK) return Status::CUDA_ERROR;
     if (relations) {
