@@ -2,29 +2,37 @@
 the plain per-pair GPU path, on one MI355X. Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --stage s4|s1      # one stage alone (profiling runs)
 
 Workload (value): the batched form on a ragged batch of 64 synthetic pairs
 - resolutions drawn from {640x480, 1280x720, 1920x1080, 3840x2160}, keypoint
 caps from U[500, 8000], overlap U[20 %, 80 %], cross-checked Hamming matching,
 RANSAC 4096, 6-band multiband blend. A step is that batch through S1-S6, with
-the inputs resident in HBM before the timed region; `value` is the stream of
-such batches (each batch's S1-S5 overlap the previous batch's S6 on a second
-stream), and "batched" times one batch at a time. The same 64 pairs run
-through the plain per-pair runner (the reference's execute_run flow: one
-pair at a time, per-stage launches, two host synchronizations) for the ratio.
-"c1_1080p" runs the 1080p load the same way: 64 pairs shaped like the
-single-pair 1080p config (1920x1080, 30-50 % overlap, K 5000, cross-check,
-RANSAC 4096, multiband_6), batched and streamed, against the plain runner on
-the same pairs. The served form replays Poisson
-arrival traces (adaptive legacy-v1 controller, 64 GB HBM budget, textured,
-low-texture, low-overlap and repetitive scenes) and reports p50/p99.
+the inputs resident in HBM before the timed region; `value` counts the pairs
+that stitched (status OK) per second of the stream of such batches (each
+batch's S1-S5 overlap the previous batch's S6 on a second stream); pairs that
+fail (a canvas over the 16,777,216-pixel budget) are listed, and the rate
+over all pairs is reported beside it. "batched" times one batch at a time.
+The same 64 pairs run through the plain per-pair runner (the reference's
+execute_run flow: one pair at a time, per-stage launches, two host
+synchronizations) for the ratio. "c1_1080p" runs the 1080p load the same
+way: 64 pairs shaped like the single-pair 1080p config (1920x1080, 30-50 %
+overlap, K 5000, cross-check, RANSAC 4096, multiband_6), batched and
+streamed, against the plain runner on the same pairs. The served form
+replays Poisson arrival traces with the same size mix and textured,
+low-texture, low-overlap and repetitive scenes under a 64 GB HBM budget:
+once with the adaptive legacy-v1 controller, once with fixed knobs (the
+batch's), and saturated (every request at t = 0) so that the budget binds;
+`pairs_per_s` there counts OK completions.
 
-"roofline": after the timed regions, K more batches run with direct launches
-and HIP events around every S6 kernel on the render stream; the kernel with
-the largest summed time is reported with its compulsory HBM bytes per launch
-(computed from each canvas's pixel counts) over its mean launch duration,
-against the 8 TB/s HBM peak. "traffic" is the measured HBM bytes per launch
-of that kernel from the committed rocprofv3 PMC summary, when it covers it.
+"roofline": the S6 stage of one batch (one render-graph replay: every S6
+kernel of every canvas) against the 8 TB/s HBM peak, with the compulsory
+bytes 3(A + B) + 4P per stitched canvas (A, B input pixels, P canvas pixels)
+over its event-timed duration on the render stream. "whole_path" adds the
+other stages' compulsory bytes per pair - S1 4(A + B), S2/S3 (A + B) +
+44(Ka + Kb), S4 32(Ka + Kb) + 12M - over ms_per_step. "s6_kernels" times
+every S6 kernel with HIP events (direct launches) and lists each one's design
+bytes (what it reads and writes by construction).
 """
 from __future__ import annotations
 
@@ -32,6 +40,7 @@ import argparse
 import gc
 import json
 import os
+import platform
 import sys
 import time
 from pathlib import Path
@@ -47,11 +56,16 @@ R, BLEND = 4096, "multiband_6"
 SEED0 = 20261022
 POOL_BYTES = 48 << 30
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
-# Measured HBM bytes per launch of the dominant kernel on this workload
-# (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration).
-TRAFFIC_PROFILE = ROOT / "profiles" / "s6_warp_pull_mixed64_traffic.md"
+# Integer VALU: 256 CUs x 64 lanes x 2.4 GHz (one 32-bit op per lane per clock).
+INT_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# Measured HBM bytes of the S6 stage of one batch (rocprofv3 FETCH_SIZE +
+# WRITE_SIZE over the S6 kernels, scaled by a copy-kernel calibration).
+TRAFFIC_PROFILE = ROOT / "profiles" / "s6_stage_mixed64_traffic.md"
 COPY_CALIBRATION_BYTES = 1 << 30
 SERVE_BUDGET = 64 << 30
+# Poisson traces: (rate per second, requests).
+SERVE_TRACES = [(400, 10000), (200, 2000), (100, 2000), (50, 2000)]
+SERVE_FIXED_TRACES = [(400, 10000), (100, 2000)]
 
 
 def parse():
@@ -60,7 +74,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--pairs", type=int, default=64, help="pairs per batch")
-    ap.add_argument("--serve-seconds", type=float, default=1.5, help="length of each arrival trace")
+    ap.add_argument("--serve-scale", type=float, default=1.0, help="fraction of each trace's requests to replay")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -68,11 +82,15 @@ def parse():
     ap.add_argument("--no-stream", action="store_true",
                     help="skip the stream of batches, so no two batches' kernels overlap (kernel-trace runs); "
                          "value is then the one-batch-at-a-time rate")
+    ap.add_argument("--no-probe", action="store_true", help="skip the per-kernel S6 probe")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU oracle sample")
     ap.add_argument("--tune-cache", default=None)
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 1 GiB device copy (the reference kernel for PMC byte counters)")
+    ap.add_argument("--stage", choices=["s4", "s1"], default=None,
+                    help="time one stage alone: s4 = 16000 x 16000 cross-checked Hamming match, "
+                         "s1 = gray conversion of a 3840x2160 pair")
     return ap.parse_args()
 
 
@@ -111,20 +129,30 @@ def specs_for(dev, work):
             zip(dev, work)]
 
 
+def status_counts(outcomes):
+    out = {}
+    for o in outcomes:
+        key = "OK" if o.status == 0 else f"{o.fail_stage.decode()}:{o.status}"
+        out[key] = out.get(key, 0) + 1
+    return out
+
+
 def time_batched(br, specs, steps, warmup):
     import torch
     for _ in range(warmup):
         br.run(specs)
     torch.cuda.synchronize()
-    times, last = [], None
+    times, render, feature, last = [], [], [], None
     t0 = time.perf_counter()
     for _ in range(steps):
         s = time.perf_counter()
         last, tm = br.run(specs)
         times.append(time.perf_counter() - s)
+        render.append(tm.render_ms)
+        feature.append(tm.feature_ms)
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
-    return total, times, last, tm
+    return total, times, last, tm, float(np.mean(render)), float(np.mean(feature))
 
 
 def time_stream(br, specs, steps, warmup):
@@ -147,7 +175,7 @@ def time_stream(br, specs, steps, warmup):
 def probe_kernels(br, specs, steps):
     """Per-kernel S6 timing over `steps` batches: direct launches with HIP
     events around every kernel on the render stream, plus each launch's
-    compulsory HBM bytes computed from the canvases and their pixel counts."""
+    design bytes computed from the canvases and their pixel counts."""
     import torch
     br.set_probe(True)
     for _ in range(steps):
@@ -158,37 +186,61 @@ def probe_kernels(br, specs, steps):
     return stats
 
 
-def measured_traffic(kernel, algorithmic_bytes):
-    """Bytes per launch from the committed PMC summary, or None. The summary
-    names the kernel and the compulsory bytes per launch it was measured at;
-    a different kernel version or workload gives other bytes, and then the
-    measurement does not apply."""
+def s6_compulsory(outcomes):
+    """3(A + B) + 4P over the canvases that were rendered, with the pixel sums."""
+    ab = p = n = 0
+    for o in outcomes:
+        if o.status != 0:
+            continue
+        g = o.geometry
+        ab += g.width_a * g.height_a + g.width_b * g.height_b
+        p += g.width * g.height
+        n += 1
+    return 3 * ab + 4 * p, ab, p, n
+
+
+def path_compulsory(outcomes, specs):
+    """Compulsory bytes of the whole path per batch: S1 4(A + B), S2/S3
+    (A + B) + 44(Ka + Kb), S4 32(Ka + Kb) + 12M for every pair, S6 for the
+    rendered canvases."""
+    total = 0
+    for o, s in zip(outcomes, specs):
+        ab = s.a.width * s.a.height + s.b.width * s.b.height
+        kk = o.num_kp_a + o.num_kp_b
+        total += 4 * ab + ab + 44 * kk + 32 * kk + 12 * o.homography.num_matches
+    return total + s6_compulsory(outcomes)[0]
+
+
+def measured_traffic(algorithmic_bytes):
+    """S6-stage HBM bytes per batch from the committed PMC summary, or None.
+    The summary states the compulsory bytes of the batch it was measured on;
+    another workload or kernel set reads other bytes, and then it does not
+    apply."""
     import re
     if not TRAFFIC_PROFILE.exists():
         return None
     text = TRAFFIC_PROFILE.read_text()
-    entries = re.findall(r"^kernel:\s*(.+?)\s*\nalgorithmic_bytes_per_launch:\s*(\d+)\s*\n"
-                         r"traffic_bytes_per_launch:\s*(\d+)\s*$", text, re.M)
-    for name, alg, traffic in entries:
-        if name == kernel and alg == str(int(algorithmic_bytes)):
-            return int(traffic)
+    m = re.search(r"^algorithmic_bytes_per_launch:\s*(\d+)\s*\ntraffic_bytes_per_launch:\s*(\d+)\s*$", text, re.M)
+    if m and m.group(1) == str(int(algorithmic_bytes)):
+        return int(m.group(2))
     return None
 
 
-def roofline(stats):
-    dom = max(stats, key=lambda e: e["total_ms"])
-    launch_ms = dom["total_ms"] / dom["launches"]
-    per_launch = dom["algorithmic_bytes"] / dom["launches"]
-    achieved = per_launch / (launch_ms * 1e-3) / 1e9
-    name = f"{dom['kernel']} level {dom['level']}"
-    traffic = measured_traffic(name, per_launch)
-    top = sorted(stats, key=lambda e: -e["total_ms"])[:10]
+def roofline(outcomes, render_ms, stats):
+    nbytes, ab, p, n = s6_compulsory(outcomes)
+    achieved = nbytes / (render_ms * 1e-3) / 1e9
+    top = sorted(stats, key=lambda e: -e["total_ms"])[:12] if stats else []
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=name,
-                launch_ms=round(launch_ms, 4), algorithmic_bytes=int(per_launch),
-                kernels=[dict(kernel=f"{e['kernel']} level {e['level']}", launches=e["launches"],
-                              ms_per_launch=round(e["total_ms"] / e["launches"], 4),
-                              gbs=round(e["algorithmic_bytes"] / (e["total_ms"] * 1e-3) / 1e9, 1)) for e in top])
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(nbytes),
+                kernel="s6 render stage (every S6 kernel of one batch, one graph replay)",
+                launch_ms=round(render_ms, 4), algorithmic_bytes=int(nbytes),
+                units=dict(canvases=n, input_pixels=int(ab), canvas_pixels=int(p),
+                           bytes_per_canvas="3(A+B)+4P"),
+                s6_kernels=[dict(kernel=f"{e['kernel']} level {e['level']}", launches=e["launches"],
+                                 ms_per_launch=round(e["total_ms"] / e["launches"], 4),
+                                 design_bytes_per_launch=int(e["algorithmic_bytes"] / e["launches"]),
+                                 design_gbs=round(e["algorithmic_bytes"] / (e["total_ms"] * 1e-3) / 1e9, 1))
+                            for e in top])
 
 
 def plain_runners(dev, work, outcomes):
@@ -203,6 +255,8 @@ def plain_runners(dev, work, outcomes):
         key = (w, h, (cw + 255) // 256 * 256, (ch + 127) // 128 * 128)
         if key[2] * key[3] > 16777216:
             key = (w, h, cw, ch)
+        if key[2] * key[3] > 16777216:  # the batch found no canvas that fits: any box rejects it
+            key = (w, h, w, h)
         assign.append(key)
         boxes.setdefault(key, 0)
     runners = {}
@@ -231,13 +285,14 @@ def time_plain(dev, work, runners, assign, passes=1):
         if rep:
             t_total += time.perf_counter() - t0
     n = passes * len(dev)
-    return dict(pairs_per_s=n / t_total, ms_per_pair=1e3 * t_total / n, p50_ms=float(np.percentile(lat, 50)),
-                p99_ms=float(np.percentile(lat, 99)), ok=int(ok), n=n)
+    return dict(pairs_per_s=ok / t_total, all_pairs_per_s=n / t_total, ms_per_pair=1e3 * t_total / n,
+                p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)), ok=int(ok), n=n)
 
 
 def serve_pool(n):
     """Served scenes: sizes as the batch; half textured at 20-80 % overlap, the
-    rest hard: low texture (blobs / 10), low overlap (10-25 %), repetitive."""
+    rest hard: low texture (blobs / 10), low overlap (10-25 %), repetitive.
+    Each scene has its own keypoint cap for the fixed-knob pass."""
     rng = np.random.default_rng(SEED0 + 7)
     out = []
     for i in range(n):
@@ -246,40 +301,73 @@ def serve_pool(n):
         kind, ov = (0, float(rng.uniform(0.2, 0.8))) if c < 3 else (
             (1, float(rng.uniform(0.2, 0.8))) if c == 3 else ((0, float(rng.uniform(0.1, 0.25))) if c == 4 else
                                                                (2, float(rng.uniform(0.2, 0.8)))))
-        out.append((w, h, kind, ov))
+        out.append((w, h, kind, ov, int(rng.integers(500, 8001))))
     return out
 
 
-def run_serving(dev, rates, seconds):
+def trace_summary(done, st, n):
+    statuses = {}
+    for c in done:
+        statuses[c.status.decode()] = statuses.get(c.status.decode(), 0) + 1
+    ok = statuses.get("OK", 0)
+    secs = st.makespan_ms / 1e3
+    return dict(requests=n, pairs_per_s=round(ok / secs, 2), requests_per_s=round(n / secs, 2),
+                p50_ms=round(st.p50_ms, 3), p99_ms=round(st.p99_ms, 3), ok=ok, statuses=statuses,
+                budget_exhausted_share=round(statuses.get("BUDGET_EXHAUSTED", 0) / n, 4),
+                mean_feature_batch=round(st.mean_feature_batch, 2), mean_render_batch=round(st.mean_render_batch, 2),
+                peak_pool_gb=round(st.peak_pool_bytes / 2 ** 30, 2))
+
+
+def run_serving(dev, pool, scale):
     from cs_stitch import batch as B
-    srv = B.Server(hbm_budget=SERVE_BUDGET, max_batch=64, max_live=256, adaptive=True)
     res = {}
-    warm = [B.request(i, *dev[i % len(dev)], 0.0, seed=0) for i in range(min(8, len(dev)))]
-    srv.run_trace(warm)
-    for rate in rates:
-        n = max(8, int(rate * seconds))
-        arr = B.poisson_arrivals(rate, n, seed=int(rate))
-        reqs = [B.request(i, *dev[i % len(dev)], t, seed=0) for i, t in enumerate(arr)]
-        done, st = srv.run_trace(reqs)
-        statuses = {}
-        for c in done:
-            statuses[c.status.decode()] = statuses.get(c.status.decode(), 0) + 1
-        res[f"poisson_{int(rate)}"] = dict(requests=n, pairs_per_s=round(n / (st.makespan_ms / 1e3), 2),
-                                           p50_ms=round(st.p50_ms, 3), p99_ms=round(st.p99_ms, 3),
-                                           mean_feature_batch=round(st.mean_feature_batch, 2),
-                                           mean_render_batch=round(st.mean_render_batch, 2), statuses=statuses,
-                                           peak_pool_gb=round(st.peak_pool_bytes / 2 ** 30, 2))
-        log("served", rate, res[f"poisson_{int(rate)}"])
-    # Saturated: every request queued at t = 0 (capacity of the loop).
-    n = 4 * len(dev)
-    reqs = [B.request(i, *dev[i % len(dev)], 0.0, seed=0) for i in range(n)]
-    done, st = srv.run_trace(reqs)
-    res["saturated"] = dict(requests=n, pairs_per_s=round(n / (st.makespan_ms / 1e3), 2), p50_ms=round(st.p50_ms, 3),
-                            p99_ms=round(st.p99_ms, 3), mean_feature_batch=round(st.mean_feature_batch, 2),
-                            mean_render_batch=round(st.mean_render_batch, 2))
-    log("served saturated", res["saturated"])
-    srv.close()
+    for adaptive in (True, False):
+        srv = B.Server(hbm_budget=SERVE_BUDGET, max_batch=64, max_live=256, adaptive=adaptive)
+        fixed = [B.pair_spec(ta, tb, S=1.0, K=k, R=R, blend=BLEND, cross_check=True)
+                 for (ta, tb), (_, _, _, _, k) in zip(dev, pool)]
+
+        def req(i, t):
+            j = i % len(dev)
+            return B.request(i, *dev[j], t, seed=0, fixed=None if adaptive else fixed[j])
+
+        name = "legacy_v1" if adaptive else "fixed_knobs"
+        srv.run_trace([req(i, 0.0) for i in range(min(16, len(dev)))])  # warm the graphs
+        part = {}
+        for rate, count in (SERVE_TRACES if adaptive else SERVE_FIXED_TRACES):
+            n = max(16, int(count * scale))
+            arr = B.poisson_arrivals(rate, n, seed=int(rate) + (0 if adaptive else 7))
+            done, st = srv.run_trace([req(i, t) for i, t in enumerate(arr)])
+            part[f"poisson_{int(rate)}"] = trace_summary(done, st, n)
+            log("served", name, rate, part[f"poisson_{int(rate)}"])
+        # Saturated: every request queued at t = 0 (capacity of the loop; the
+        # HBM budget decides how many are in flight).
+        n = 4 * len(dev)
+        done, st = srv.run_trace([req(i, 0.0) for i in range(n)])
+        part["saturated"] = trace_summary(done, st, n)
+        log("served", name, "saturated", part["saturated"])
+        srv.close()
+        res[name] = part
+    res["workload"] = (f"{len(dev)} resident scenes cycled by the requests: sizes from {{640x480,1280x720,"
+                       f"1920x1080,3840x2160}}; half textured (20-80 % overlap), the rest low texture, low overlap "
+                       f"(10-25 %) or repetitive; legacy_v1: the adaptive controller; fixed_knobs: K U[500,8000], "
+                       f"R {R}, cross-check, {BLEND}; HBM budget {SERVE_BUDGET >> 30} GB")
     return res
+
+
+def host_cpu():
+    model = platform.processor() or ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return dict(cpu_model=model, host_cores=os.cpu_count(), cores_available=avail)
 
 
 def cpu_baseline(pairs, work, seconds):
@@ -294,7 +382,7 @@ def cpu_baseline(pairs, work, seconds):
             break
     dt = time.perf_counter() - t0
     sizes = ", ".join(f"{w}x{h}" for (w, h, _, _) in work[:n])
-    return dict(value=round(n / dt, 4), unit="pairs/s", cores=1, kind="port",
+    return dict(value=round(n / dt, 4), unit="pairs/s", cores=1, kind="port", **host_cpu(),
                 sample=f"first {n} pair(s) of the batch ({sizes}; K caps {[k for _, _, k, _ in work[:n]]}, R={R}, "
                        f"cross-check, {BLEND}) through the single-thread C oracle, {dt:.1f} s")
 
@@ -302,11 +390,7 @@ def cpu_baseline(pairs, work, seconds):
 def cpu_threads():
     # The host share of one GPU: 16 threads at most, fewer where the process
     # may use fewer cores.
-    try:
-        avail = len(os.sched_getaffinity(0))
-    except AttributeError:
-        avail = os.cpu_count() or 1
-    return max(1, min(16, avail))
+    return max(1, min(16, host_cpu()["cores_available"]))
 
 
 def cpu_baseline_all_cores(pairs, work, seconds):
@@ -331,9 +415,69 @@ def cpu_baseline_all_cores(pairs, work, seconds):
         for f in cf.as_completed(pending):
             f.result()
     dt = time.perf_counter() - t0
-    return dict(value=round(n / dt, 4), unit="pairs/s", cores=threads, kind="port",
+    return dict(value=round(n / dt, 4), unit="pairs/s", cores=threads, kind="port", **host_cpu(),
                 sample=f"first {n} pair(s) of the batch (R={R}, cross-check, {BLEND}) through the C oracle on "
                        f"{threads} threads (one pair per thread), {dt:.1f} s")
+
+
+def stage_bench(kind, steps, warmup):
+    """One stage alone on its largest configured shape, timed with HIP events
+    on the stream it runs on (torch's current stream)."""
+    import torch
+    from cs_stitch import stages as st
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    if kind == "s4":
+        n = 16000
+        g = torch.Generator().manual_seed(SEED0)
+        da = torch.randint(0, 256, (n, 32), dtype=torch.uint8, generator=g).cuda()
+        db = torch.randint(0, 256, (n, 32), dtype=torch.uint8, generator=g).cuda()
+        ca = torch.tensor([n], dtype=torch.int32, device="cuda")
+        cb = torch.tensor([n], dtype=torch.int32, device="cuda")
+        out, ws = st.s4_match(da, ca, db, cb, 0.8, True)
+        for _ in range(warmup):
+            st.s4_match(da, ca, db, cb, 0.8, True, out=out, ws=ws)
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(steps):
+            st.s4_match(da, ca, db, cb, 0.8, True, out=out, ws=ws)
+        ev[1].record()
+        torch.cuda.synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / steps
+        ops = 16 * n * n * 2
+        return dict(stage="s4", workload=f"{n} x {n} random 256-bit descriptors, cross-check, ratio 0.8",
+                    ms_per_launch=round(ms, 4), int_ops=ops, tops=round(ops / ms / 1e9, 2),
+                    peak_tops=round(INT_PEAK_TOPS, 1), frac=round(ops / ms / 1e9 / INT_PEAK_TOPS, 4))
+    w, h = 3840, 2160
+    a, b, _ = synth(SEED0, w, h, 0.5)
+    ta, tb = upload(a, b)
+    for _ in range(warmup):
+        st.s1_preprocess(ta, tb, 1.0)
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(steps):
+        st.s1_preprocess(ta, tb, 1.0)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / steps
+    nbytes = 4 * 2 * w * h
+    return dict(stage="s1", workload=f"{w}x{h} RGB8 pair to gray at S=1", ms_per_launch=round(ms, 4),
+                bytes=nbytes, gbs=round(nbytes / ms / 1e6, 1), peak_gbs=HBM_PEAK_GBS,
+                frac=round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4))
+
+
+def children():
+    """Processes whose parent is this one (left-behind workers show here)."""
+    me, out = os.getpid(), []
+    for d in Path("/proc").iterdir():
+        if not d.name.isdigit():
+            continue
+        try:
+            fields = (d / "stat").read_text().rsplit(")", 1)[1].split()
+            if int(fields[1]) == me:
+                out.append((int(d.name), (d / "cmdline").read_bytes().replace(b"\0", b" ").decode()[:80]))
+        except (OSError, ValueError, IndexError):
+            pass
+    return out
 
 
 def main():
@@ -347,6 +491,10 @@ def main():
     from cs_stitch import _ffi
     from cs_stitch import batch as B
     _ffi.load()
+    if args.stage:
+        r = stage_bench(args.stage, max(args.steps, 10), max(args.warmup, 3))
+        print(json.dumps(r))
+        return
     if args.calibrate_copy:
         src = torch.empty(COPY_CALIBRATION_BYTES, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
@@ -360,6 +508,7 @@ def main():
     torch.cuda.synchronize()
     specs = specs_for(dev, work)
     br = B.BatchRunner(max_pairs=args.pairs, pool_bytes=POOL_BYTES)
+    tune = None
     if not args.no_tune:
         tune = br.autotune(specs, args.tune_cache)
         log("tuning", tune.s4_segment, tune.workgroups_per_cu, tune.graphs)
@@ -368,23 +517,31 @@ def main():
     # it stays off while batches are timed.
     gc.collect()
     gc.disable()
-    total, times, outcomes, tm = time_batched(br, specs, args.steps, args.warmup)
+    total, times, outcomes, tm, render_ms, feature_ms = time_batched(br, specs, args.steps, args.warmup)
     ok = sum(o.status == 0 for o in outcomes)
-    batched = dict(pairs_per_s=args.pairs * args.steps / total, ms_per_step=1e3 * total / args.steps,
-                   p50_ms=1e3 * float(np.percentile(times, 50)), p99_ms=1e3 * float(np.percentile(times, 99)),
-                   feature_ms=tm.feature_ms, render_ms=tm.render_ms, host_ms=tm.host_ms, ok=ok,
+    batched = dict(pairs_per_s=ok * args.steps / total, all_pairs_per_s=args.pairs * args.steps / total,
+                   ms_per_step=1e3 * total / args.steps, p50_ms=1e3 * float(np.percentile(times, 50)),
+                   p99_ms=1e3 * float(np.percentile(times, 99)), feature_ms=feature_ms, render_ms=render_ms,
+                   host_ms=tm.host_ms, ok=ok, statuses=status_counts(outcomes),
                    steps_ms=[round(1e3 * t, 3) for t in times])
     log("batched", batched)
     stream = None
     if not args.no_stream:
         s_total, s_last = time_stream(br, specs, args.steps, args.warmup)
         assert [o.status for o in s_last] == [o.status for o in outcomes]
-        stream = dict(pairs_per_s=args.pairs * args.steps / s_total, ms_per_step=1e3 * s_total / args.steps,
-                      ok=sum(o.status == 0 for o in s_last))
+        s_ok = sum(o.status == 0 for o in s_last)
+        stream = dict(pairs_per_s=s_ok * args.steps / s_total, all_pairs_per_s=args.pairs * args.steps / s_total,
+                      ms_per_step=1e3 * s_total / args.steps, ok=s_ok)
         log("stream", stream)
     headline = stream if stream is not None else batched
-    roof = roofline(probe_kernels(br, specs, args.steps))
-    log("roofline", {k: v for k, v in roof.items() if k != "kernels"})
+    stats = None if args.no_probe else probe_kernels(br, specs, args.steps)
+    roof = roofline(outcomes, render_ms, stats)
+    whole = path_compulsory(outcomes, specs)
+    whole_path = dict(algorithmic_bytes=int(whole), ms_per_step=round(headline["ms_per_step"], 4),
+                      achieved=round(whole / (headline["ms_per_step"] * 1e-3) / 1e9, 1), unit="GB/s",
+                      frac=round(whole / (headline["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      bytes_per_pair="S1 4(A+B) + S2/S3 (A+B)+44(Ka+Kb) + S4 32(Ka+Kb)+12M + S6 3(A+B)+4P")
+    log("roofline", {k: v for k, v in roof.items() if k != "s6_kernels"}, "whole path", whole_path)
     # The 1080p load: a batch of 64 pairs shaped like the single-pair 1080p
     # config (1920x1080, 30-50 % overlap, K 5000, cross-check, RANSAC 4096,
     # multiband_6), against the plain runner on the same pairs.
@@ -398,12 +555,13 @@ def main():
         torch.cuda.synchronize()
         specs1 = specs_for(dev1, work1)
         out1 = br.run(specs1)[0]
-        t_b1, times_b1, last1, _ = time_batched(br, specs1, args.steps, args.warmup)
+        t_b1, times_b1, last1, _, _, _ = time_batched(br, specs1, args.steps, args.warmup)
+        ok1 = sum(o.status == 0 for o in last1)
         t_s1 = time_stream(br, specs1, args.steps, args.warmup)[0] if not args.no_stream else t_b1
         c1 = dict(workload=f"{args.pairs} pairs of 1920x1080, overlap U[0.3,0.5], K 5000, cross-check, RANSAC {R}, "
-                           f"{BLEND}", stream_pairs_per_s=args.pairs * args.steps / t_s1,
-                  batched_pairs_per_s=args.pairs * args.steps / t_b1,
-                  batched_p50_ms=1e3 * float(np.percentile(times_b1, 50)), ok=sum(o.status == 0 for o in last1))
+                           f"{BLEND}", stream_pairs_per_s=ok1 * args.steps / t_s1,
+                  batched_pairs_per_s=ok1 * args.steps / t_b1,
+                  batched_p50_ms=1e3 * float(np.percentile(times_b1, 50)), ok=ok1)
         log("c1_1080p", c1)
     result = dict(stream=stream, batched=batched, c1_1080p=c1)
     if not args.no_plain:
@@ -427,9 +585,9 @@ def main():
     gc.enable()
     if not args.no_serve:
         pool = serve_pool(args.pairs)
-        sdev = [upload(*synth(SEED0 + 1000 + i, w, h, ov, kind)[:2]) for i, (w, h, kind, ov) in enumerate(pool)]
+        sdev = [upload(*synth(SEED0 + 1000 + i, w, h, ov, kind)[:2]) for i, (w, h, kind, ov, _) in enumerate(pool)]
         torch.cuda.synchronize()
-        result["served"] = run_serving(sdev, [50, 100, 200, 400], args.serve_seconds)
+        result["served"] = run_serving(sdev, pool, args.serve_scale)
     cpu = cpu_single = None
     if not args.no_cpu:
         cpu_single = cpu_baseline(pairs, work, args.cpu_seconds)
@@ -452,13 +610,20 @@ def main():
         "config": {"workload": f"batched form as {'a stream of batches' if stream else 'one batch at a time'}, "
                                f"{args.pairs} pairs per step: sizes from "
                                f"{{640x480,1280x720,1920x1080,3840x2160}}, K caps U[500,8000], overlap U[0.2,0.8], "
-                               f"cross-checked Hamming, RANSAC {R}, {BLEND}", "pairs_per_step": args.pairs},
+                               f"cross-checked Hamming, RANSAC {R}, {BLEND}; value counts the pairs that stitched "
+                               f"(status OK)", "pairs_per_step": args.pairs,
+                   "tuning": None if tune is None else dict(s4_segment=tune.s4_segment,
+                                                            workgroups_per_cu=tune.workgroups_per_cu)},
         **result,
         "roofline": roof,
+        "whole_path": whole_path,
         "cpu_baseline": cpu,
         "cpu_baseline_single_thread": cpu_single,
     }
-    print(json.dumps(line))
+    print(json.dumps(line), flush=True)
+    left = children()
+    if left:
+        log("child processes at exit:", left)
 
 
 if __name__ == "__main__":

@@ -103,7 +103,7 @@ class AttemptRecord(C.Structure):
                 ("homography", HomographyResult), ("t_s1", C.c_double), ("t_s2", C.c_double), ("t_s3", C.c_double),
                 ("t_s4", C.c_double), ("t_s5", C.c_double), ("t_attempt", C.c_double),
                 ("restart_from", C.c_char * 8), ("fail_stage", C.c_char * 8), ("execution_status", C.c_int),
-                ("has_decision", C.c_int), ("has_homography", C.c_int), ("pad", C.c_int)]
+                ("has_decision", C.c_int), ("has_homography", C.c_int), ("has_feedback", C.c_int)]
 
 
 class RunRecord(C.Structure):
@@ -164,7 +164,8 @@ class Completion(C.Structure):
                 ("arrival_ms", C.c_double), ("admitted_ms", C.c_double), ("done_ms", C.c_double),
                 ("attempts", C.c_int), ("S", C.c_double), ("K", C.c_int), ("R", C.c_int), ("blend_mode", C.c_int),
                 ("num_kp_a", C.c_int), ("num_kp_b", C.c_int), ("homography", HomographyResult),
-                ("geometry", Geometry), ("statistics", RenderStatistics), ("canvas", Image), ("mask", Image)]
+                ("geometry", Geometry), ("statistics", RenderStatistics), ("canvas", Image), ("mask", Image),
+                ("reserved_bytes", C.c_uint64)]
 
 
 class ServeStats(C.Structure):

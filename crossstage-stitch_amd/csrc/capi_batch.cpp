@@ -361,6 +361,7 @@ int cs_server_run_trace(cs_server* s, const cs_request* requests, int n, cs_comp
             std::memcpy(&o.statistics, &c.stats, sizeof o.statistics);
             o.canvas = put_image(c.canvas);
             o.mask = put_image(c.mask);
+            o.reserved_bytes = c.reserved_bytes;
         }
         if (stats) {
             cs_serve_stats& o = *stats;

@@ -236,6 +236,7 @@ int cs_runner_attempt_trace(cs_runner* r, cs_attempt_record* out, int capacity, 
         o.execution_status = static_cast<int>(a.execution_status);
         o.has_decision = a.has_decision ? 1 : 0;
         o.has_homography = a.has_homography ? 1 : 0;
+        o.has_feedback = a.has_feedback ? 1 : 0;
     }
     return CS_OK;
 }

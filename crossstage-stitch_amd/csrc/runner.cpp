@@ -563,11 +563,14 @@ void PipelineRunner::execute_adaptive_run(controller::Controller& ctl, std::uint
         tr.generation = generation;
         tr.execution_status = s5s;
         if (s5s != Status::OK) tr.fail_stage = "s5";
-        tr.homography = *pinned_h_;
-        tr.has_homography = true;
+        tr.has_feedback = s5s == Status::OK;
+        if (tr.has_feedback) tr.homography = *pinned_h_;
+        tr.has_homography = tr.has_feedback && tr.homography.status == Status::OK;
         if (s5s == Status::OK) completed_r = knobs.R;
         controller::Feedback fb{knobs, s5s, *pinned_h_};
-        if (ctl.needs_gate_features()) {
+        // Gate observations need a usable homography: without one there is
+        // nothing to observe and the reference skips the extraction.
+        if (ctl.needs_gate_features() && tr.has_homography) {
             // Gate observations of this attempt (pipeline_runner.cu:780-806).
             const auto f0 = Clock::now();
             const unsigned relations = ctl.relation_mask();
@@ -586,8 +589,10 @@ void PipelineRunner::execute_adaptive_run(controller::Controller& ctl, std::uint
                 std::copy_n(pinned_features_, features::kGateCount, fb.features.begin());
                 fb.features_present = true;
             }
-            rec->t_features += ms_since(f0);
+            tr.t_features = ms_since(f0);
+            rec->t_features += tr.t_features;
         }
+        tr.features = fb.features;
         rec->gate_features = fb.features;
         rec->features_present = fb.features_present;
         controller::Decision d{};

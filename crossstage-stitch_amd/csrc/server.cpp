@@ -223,7 +223,7 @@ void Server::run_trace(const Request* requests, int n, std::vector<Completion>* 
         if (s5s == Status::OK) L.completed_r = L.knobs.R;
         L.done.h = h;
         controller::Feedback fb{L.knobs, s5s, h};
-        if (learned_ && L.ctl.needs_gate_features()) {
+        if (learned_ && L.ctl.needs_gate_features() && s5s == Status::OK && h.status == Status::OK) {
             // Gate observations of this attempt (pipeline_runner.cu:780-806).
             const unsigned relations = L.ctl.relation_mask();
             const s4::Top2* forward = nullptr;
@@ -334,6 +334,7 @@ void Server::run_trace(const Request* requests, int n, std::vector<Completion>* 
                 continue;
             }
             L->reserved = need;
+            L->done.reserved_bytes = need;
             reserved += need;
             if (config_.adaptive) {
                 L->ctl.reset_pair();

@@ -79,8 +79,11 @@ struct AttemptRecord {
     const char* restart_from{"none"};
     const char* fail_stage{""};
     Status execution_status{Status::OK};
+    features::GateFeatures features{};  // gate observations given to the controller
+    double t_features{0};
     bool has_decision{false};
-    bool has_homography{false};
+    bool has_feedback{false};    // S5 ran; homography holds its result
+    bool has_homography{false};  // ... and that result has a usable H
 };
 
 struct AttemptTrace {

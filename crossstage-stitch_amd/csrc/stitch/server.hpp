@@ -60,6 +60,7 @@ struct Completion {
     s6::Geometry geometry{};
     s6::RenderStatistics stats{};
     ImageView canvas{}, mask{};  // valid until release(id) when keep_output
+    std::size_t reserved_bytes{0};  // budget held from admission until completion (or release)
     double latency_ms() const { return done_ms - arrival_ms; }
 };
 

@@ -235,7 +235,9 @@ void trace_header(std::ostream& o) {
          "S,K,R,previous_R,eligible_R_old,generation,num_matches,num_inliers,inlier_ratio,"
          "inlier_coverage,reproj_residual,homography_status,action,reason,t_s4,t_s5,t_attempt,"
          "h0,h1,h2,h3,h4,h5,h6,h7,h8,policy_id,B,restart_from,t_s1,t_s2,t_s3,"
-         "execution_status,attempt_fail_stage,terminal_status,terminal_fail_stage,t_features\n";
+         "execution_status,attempt_fail_stage,terminal_status,terminal_fail_stage,t_features";
+    for (int i = 0; i < stitch::features::kGateCount; ++i) o << ",gate_" << i;
+    o << '\n';
 }
 
 void trace_rows(std::ostream& o, const Labels& L, const H::RunRecord& r, const H::AttemptTrace& t) {
@@ -247,7 +249,7 @@ void trace_rows(std::ostream& o, const Labels& L, const H::RunRecord& r, const H
           << L.repeat_idx << ',' << L.is_warmup << ',' << r.seed << ',' << i << ',' << a.knobs.S << ','
           << a.knobs.K << ',' << a.knobs.R << ',' << a.previous_R << ',' << a.previous_R << ',' << a.generation
           << ',';
-        if (a.has_homography)
+        if (a.has_feedback)
             o << h.num_matches << ',' << h.num_inliers << ',' << h.inlier_ratio << ',' << h.inlier_coverage << ','
               << h.reproj_residual << ',' << static_cast<int>(h.status);
         else
@@ -262,7 +264,9 @@ void trace_rows(std::ostream& o, const Labels& L, const H::RunRecord& r, const H
         }
         o << ',' << L.policy_id << ',' << blend_name(a.knobs.B) << ',' << a.restart_from << ',' << a.t_s1 << ','
           << a.t_s2 << ',' << a.t_s3 << ',' << static_cast<int>(a.execution_status) << ',' << a.fail_stage << ','
-          << r.status << ',' << r.fail_stage << ",0\n";
+          << r.status << ',' << r.fail_stage << ',' << a.t_features;
+        for (double v : a.features) o << ',' << v;
+        o << '\n';
     }
 }
 

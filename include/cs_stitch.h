@@ -228,7 +228,9 @@ typedef struct {
     double t_s1, t_s2, t_s3, t_s4, t_s5, t_attempt;
     char restart_from[8];         /* "none", "S1", "S2", "S5" */
     char fail_stage[8];
-    int execution_status, has_decision, has_homography, pad;
+    /* has_feedback: S5 ran (its result is recorded); has_homography: it also
+     * returned a usable H (harness/pipeline_runner.cu:777-779). */
+    int execution_status, has_decision, has_homography, has_feedback;
 } cs_attempt_record;
 int cs_runner_attempt_trace(cs_runner* runner, cs_attempt_record* out, int capacity, int* count);
 
@@ -369,6 +371,7 @@ typedef struct {
     cs_geometry geometry;
     cs_render_statistics statistics;
     cs_image canvas, mask;
+    uint64_t reserved_bytes;  /* HBM budget held from admission to completion */
 } cs_completion;
 typedef struct {
     int completed, failed;

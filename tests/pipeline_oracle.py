@@ -26,6 +26,36 @@ def run(a, b, S, K, R, blend, cross_check, fast=20, ratio=0.8, refinements=1, th
     return out
 
 
+def check_outputs(d, ref, what=""):
+    """Host copies of one pair's device outputs (keys as tests/test_gpu_batch
+    download_outcome: kp_a/kp_b (n, 3) float32, desc_a/desc_b bytes, nm,
+    idx_a/idx_b/dist bytes, H, h_status, inliers, mask_inl, canvas, mask)
+    against the oracle chain: keypoints, descriptors and matches bit-exact,
+    the S5 status, inlier count, H and inlier mask identical, and - when S5
+    and the geometry succeed - the panorama within 1 LSB with the union mask
+    exact."""
+    assert np.array_equal(d["kp_a"], ref["kp_a"]), (what, "kp_a")
+    assert np.array_equal(d["kp_b"], ref["kp_b"]), (what, "kp_b")
+    assert np.array_equal(np.asarray(d["desc_a"]).reshape(-1, 32), ref["desc_a"]), (what, "desc_a")
+    assert np.array_equal(np.asarray(d["desc_b"]).reshape(-1, 32), ref["desc_b"]), (what, "desc_b")
+    nm = int(d["nm"])
+    assert nm == len(ref["idx_a"]) == ref["result"].num_matches, (what, nm, len(ref["idx_a"]))
+    assert np.array_equal(np.asarray(d["idx_a"]).view(np.int32), ref["idx_a"]), (what, "idx_a")
+    assert np.array_equal(np.asarray(d["idx_b"]).view(np.int32), ref["idx_b"]), (what, "idx_b")
+    assert np.array_equal(np.asarray(d["dist"]).view(np.float32), ref["dist"]), (what, "distance")
+    res = ref["result"]
+    assert d["h_status"] == res.status and d["inliers"] == res.num_inliers, (what, d["h_status"], res.status)
+    if res.status == 0:
+        assert np.array_equal(np.asarray(d["H"]), np.array(res.H[:])), (what, "H")
+    assert np.array_equal(np.asarray(d["mask_inl"]), ref["mask"][:nm]), (what, "inlier mask")
+    if res.status != 0 or ref.get("geometry_status", 1) != 0:
+        return
+    pano, pmask = d["canvas"], d["mask"]
+    assert pano.shape == ref["panorama"].shape, (what, pano.shape, ref["panorama"].shape)
+    assert np.array_equal(pmask, ref["pano_mask"]), (what, "union mask")
+    assert np.abs(pano.astype(int) - ref["panorama"].astype(int)).max() <= 1, (what, "panorama")
+
+
 def check_runner(runner, a, b, S, K, R, blend, cross):
     """Uploads the pair, runs the plain runner once and checks every output
     against the oracle chain: bit-exact keypoints, descriptors and matches,

@@ -209,8 +209,22 @@ def test_gate_invalid_inputs():
 
 # ---------------------------------------------------------------- GPU parity
 
+# Gate values computed through log (the inlier entropies, 22-23) or hypot
+# (the alignment-relation residual statistics, 24-31): the device math
+# library may round these differently from the host's, so they are held to
+# the tolerance. Every other value is a fixed sequence of correctly rounded
+# operations (sums in the reference's partial order, sqrt, divisions) and must
+# be bit-equal.
+GATE_LIBM = [22, 23] + list(range(24, 32))
+
+
 def assert_features_close(got, want):
-    np.testing.assert_allclose(got, want, rtol=RTOL, atol=ATOL)
+    got, want = np.asarray(got), np.asarray(want)
+    exact = np.ones(got.shape, bool)
+    if got.shape == (38,):
+        exact[GATE_LIBM] = False
+    np.testing.assert_array_equal(got[exact], want[exact])
+    np.testing.assert_allclose(got[~exact], want[~exact], rtol=RTOL, atol=ATOL)
 
 
 @pytest.mark.gpu
