@@ -491,16 +491,16 @@ def main():
     from cs_stitch import _ffi
     from cs_stitch import batch as B
     _ffi.load()
-    if args.stage:
-        r = stage_bench(args.stage, max(args.steps, 10), max(args.warmup, 3))
-        print(json.dumps(r))
-        return
     if args.calibrate_copy:
         src = torch.empty(COPY_CALIBRATION_BYTES, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.stage:
+        r = stage_bench(args.stage, max(args.steps, 10), max(args.warmup, 3))
+        print(json.dumps(r))
+        return
     work = batch_workload(args.pairs)
     log("synthesizing", len(work), "pairs")
     pairs = [synth(SEED0 + i, w, h, ov) for i, (w, h, _, ov) in enumerate(work)]

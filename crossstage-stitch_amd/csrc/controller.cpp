@@ -202,155 +202,186 @@ std::array<double, kSelectorFeatures> selector_features(const features::SceneFea
     return v;
 }
 
-void Controller::set_pooled_rung(std::size_t index) noexcept {
-    pooled_context_[30] = ladder_[index].S;
-    pooled_context_[31] = ladder_[index].K / 2000.;
-    pooled_context_[32] = ladder_[index].R / 1024.;
+// ------------------------------------------------------------- controller
+// The per-pair decision procedure (contract: src/controller/controller.hpp;
+// rules of src/controller/controller.cpp): the first rung comes from the
+// scene, each attempt is judged by its rung's self-check (and, for learned
+// policies, a quality model), and a failed attempt moves to a later rung
+// until the ladder runs out.
+
+void Controller::describe_rung(std::size_t rung) noexcept {
+    const Knobs& k = ladder_[rung];
+    context_[30] = k.S;
+    context_[31] = k.K / 2000.;
+    context_[32] = k.R / 1024.;
 }
 
 Status Controller::configure(const Policy& policy) noexcept {
-    if (phase_ == Phase::AwaitingFeedback || !detail::valid_policy(policy)) return Status::INVALID_ARGUMENT;
+    if (state_ == State::Running || !detail::valid_policy(policy)) return Status::INVALID_ARGUMENT;
     policy_ = policy;
-    configured_ = true;
+    mode_ = policy.pooled ? Mode::Pooled : (policy.selector.count ? Mode::Selector : Mode::Rules);
+    state_ = State::Unbound;
     ladder_ = nullptr;
-    count_ = position_ = 0;
-    phase_ = Phase::Uninitialized;
+    rungs_ = rung_ = 0;
     return Status::OK;
 }
 
 double Controller::probe_scale() const noexcept {
-    return configured_ ? policy_.probe_scale : std::numeric_limits<double>::quiet_NaN();
+    return mode_ == Mode::Unconfigured ? std::numeric_limits<double>::quiet_NaN() : policy_.probe_scale;
 }
 
 Status Controller::initialize(const Knobs* ladder, std::size_t count) noexcept {
-    if (phase_ == Phase::AwaitingFeedback || !configured_ || !ladder || count == 0 || count != policy_.self_check_count)
+    if (state_ == State::Running || mode_ == Mode::Unconfigured || !ladder || count == 0 ||
+        count != policy_.self_check_count)
         return Status::INVALID_ARGUMENT;
+    // Every rung is a valid, distinct knob set with one blend mode.
     for (std::size_t i = 0; i < count; ++i) {
         if (!detail::valid_knobs(ladder[i]) || ladder[i].B != ladder[0].B) return Status::INVALID_ARGUMENT;
-        for (std::size_t j = 0; j < i; ++j)
-            if (detail::same_knobs(ladder[i], ladder[j])) return Status::INVALID_ARGUMENT;
+        if (std::any_of(ladder, ladder + i, [&](const Knobs& k) { return detail::same_knobs(k, ladder[i]); }))
+            return Status::INVALID_ARGUMENT;
     }
     ladder_ = ladder;
-    count_ = count;
-    position_ = 0;
-    phase_ = Phase::Idle;
+    rungs_ = count;
+    rung_ = 0;
+    state_ = State::Bound;
     return Status::OK;
 }
 
-Status Controller::begin_pair(const SceneSignals& scene, Knobs* first) noexcept {
-    if (phase_ != Phase::Idle || !first || !detail::valid_scene(scene, policy_.probe_scale))
-        return Status::INVALID_ARGUMENT;
-    if (const PooledModels* m = policy_.pooled) {
-        // Start at the rung with the lowest expected cost: its predicted
-        // latency plus, weighted by its predicted failure, the reference
-        // rung's latency and a fixed overhead (controller.cpp:45-69).
-        if (!scene.features_present) return Status::INVALID_ARGUMENT;
-        const auto selector = selector_features(scene.features);
-        std::copy(selector.begin(), selector.end(), pooled_context_.begin());
-        std::copy(scene.features.begin() + 26, scene.features.end(), pooled_context_.begin() + 33);
-        set_pooled_rung(m->reference_index);
-        const double reference_ms = predict_ensemble(m->latency, pooled_context_.data());
-        if (!std::isfinite(reference_ms)) return Status::INVALID_ARGUMENT;
-        double best = reference_ms;
-        position_ = m->reference_index;
-        if (m->pre_threshold < 1.0)
-            for (std::size_t i = 0; i < count_; ++i) {
-                if (i == m->reference_index) continue;
-                set_pooled_rung(i);
-                const double quality = predict_ensemble(m->pre_quality, pooled_context_.data());
-                const double ms = predict_ensemble(m->latency, pooled_context_.data());
-                if (!std::isfinite(quality) || !std::isfinite(ms)) return Status::INVALID_ARGUMENT;
-                const double cost = ms + (1 - quality) * reference_ms + m->extra_ms;
-                if (quality >= m->pre_threshold && cost < best) {
-                    best = cost;
-                    position_ = i;
-                }
-            }
-        set_pooled_rung(position_);
-        *first = ladder_[position_];
-        phase_ = Phase::AwaitingFeedback;
-        return Status::OK;
+// The first rung of a pair, or false when the scene cannot choose one.
+bool Controller::pick_first_rung(const SceneSignals& scene, std::size_t* rung) noexcept {
+    if (mode_ == Mode::Rules) {
+        // Rules hold decreasing gradient floors ending at zero: the first one
+        // the weaker image reaches names the starting rung.
+        const double g = std::fmin(scene.a.mean_abs_gradient, scene.b.mean_abs_gradient);
+        const SceneRule* end = policy_.scene_rules + policy_.scene_rule_count;
+        const SceneRule* hit =
+            std::find_if(policy_.scene_rules, end, [g](const SceneRule& r) { return g >= r.min_pair_gradient; });
+        if (hit == end) return false;
+        *rung = hit->first_index;
+        return true;
     }
-    if (policy_.selector.count) {
-        if (!scene.features_present) return Status::INVALID_ARGUMENT;
-        const auto selector = selector_features(scene.features);
-        const int rung = predict_tree(policy_.selector, selector.data(), kSelectorFeatures);
-        if (rung < 0 || rung >= static_cast<int>(count_)) return Status::INVALID_ARGUMENT;
-        position_ = static_cast<std::size_t>(rung);
-        *first = ladder_[position_];
-        phase_ = Phase::AwaitingFeedback;
-        return Status::OK;
+    if (!scene.features_present) return false;
+    const auto selector = selector_features(scene.features);
+    if (mode_ == Mode::Selector) {
+        const int r = predict_tree(policy_.selector, selector.data(), kSelectorFeatures);
+        if (r < 0 || r >= static_cast<int>(rungs_)) return false;
+        *rung = static_cast<std::size_t>(r);
+        return true;
     }
-    const double gradient = std::fmin(scene.a.mean_abs_gradient, scene.b.mean_abs_gradient);
-    for (std::size_t i = 0; i < policy_.scene_rule_count; ++i) {
-        const auto& rule = policy_.scene_rules[i];
-        if (gradient >= rule.min_pair_gradient) {
-            position_ = rule.first_index;
-            *first = ladder_[position_];
-            phase_ = Phase::AwaitingFeedback;
-            return Status::OK;
+    // Pooled: the reference rung unless another rung is predicted to pass
+    // (quality at least the threshold) at a lower expected cost - its own
+    // predicted latency plus, on failure, a retry at the reference rung and a
+    // fixed overhead.
+    const PooledModels& m = *policy_.pooled;
+    std::copy(selector.begin(), selector.end(), context_.begin());
+    std::copy(scene.features.begin() + 26, scene.features.end(), context_.begin() + 33);
+    describe_rung(m.reference_index);
+    const double reference_ms = predict_ensemble(m.latency, context_.data());
+    if (!std::isfinite(reference_ms)) return false;
+    std::size_t choice = m.reference_index;
+    double lowest = reference_ms;
+    for (std::size_t i = 0; m.pre_threshold < 1.0 && i < rungs_; ++i) {
+        if (i == m.reference_index) continue;
+        describe_rung(i);
+        const double pass = predict_ensemble(m.pre_quality, context_.data());
+        const double ms = predict_ensemble(m.latency, context_.data());
+        if (!std::isfinite(pass) || !std::isfinite(ms)) return false;
+        const double expected = ms + (1 - pass) * reference_ms + m.extra_ms;
+        if (pass >= m.pre_threshold && expected < lowest) {
+            lowest = expected;
+            choice = i;
         }
     }
-    return Status::INVALID_ARGUMENT;
+    describe_rung(choice);
+    *rung = choice;
+    return true;
+}
+
+Status Controller::begin_pair(const SceneSignals& scene, Knobs* first) noexcept {
+    if (state_ != State::Bound || !first || !detail::valid_scene(scene, policy_.probe_scale))
+        return Status::INVALID_ARGUMENT;
+    std::size_t rung = 0;
+    if (!pick_first_rung(scene, &rung)) return Status::INVALID_ARGUMENT;
+    rung_ = rung;
+    *first = ladder_[rung];
+    state_ = State::Running;
+    return Status::OK;
+}
+
+// Verdict on an attempt that ran: the rung's self-check, then the learned
+// quality model where the policy has one.
+Reason Controller::judge(const Feedback& f) const noexcept {
+    const Reason checked = detail::self_check_reason(f.homography, policy_.self_checks[rung_]);
+    if (checked != Reason::SELF_CHECK_PASSED) return checked;
+    if (mode_ == Mode::Pooled && rung_ != policy_.pooled->reference_index) {
+        // 77 inputs: rung context (33), gate features 0..23, pair relations
+        // (6), alignment relations (gate features 24..37).
+        if (!f.features_present) return Reason::QUALITY_PREDICTOR_REJECTED;
+        std::array<double, kPooledGateInputs> in{};
+        auto at = std::copy_n(context_.begin(), 33, in.begin());
+        at = std::copy_n(f.features.begin(), 24, at);
+        at = std::copy_n(context_.begin() + 33, 6, at);
+        std::copy_n(f.features.begin() + 24, 14, at);
+        const double q = predict_ensemble(policy_.pooled->post_quality, in.data());
+        if (!(std::isfinite(q) && q >= policy_.pooled->gate_threshold)) return Reason::QUALITY_PREDICTOR_REJECTED;
+    }
+    if (mode_ == Mode::Selector &&
+        (!f.features_present || predict_tree(policy_.gates[rung_], f.features.data(), features::kGateCount) != 1))
+        return Reason::QUALITY_PREDICTOR_REJECTED;
+    return Reason::SELF_CHECK_PASSED;
+}
+
+// Candidate after `rung` for a retry (rungs_ when there is none).
+std::size_t Controller::successor(std::size_t rung) const noexcept {
+    if (mode_ == Mode::Pooled) {
+        // One fallback, to the reference rung; from there, nothing.
+        const std::size_t ref = policy_.pooled->reference_index;
+        return rung == ref ? rungs_ : ref;
+    }
+    return policy_.retry_target_count ? policy_.retry_targets[rung] : rung + 1;
 }
 
 Status Controller::observe(const Feedback& feedback, Decision* decision) noexcept {
-    if (phase_ != Phase::AwaitingFeedback || !decision || !detail::same_knobs(feedback.executed, ladder_[position_]) ||
+    if (state_ != State::Running || !decision || !detail::same_knobs(feedback.executed, ladder_[rung_]) ||
         !detail::valid_feedback(feedback))
         return Status::INVALID_ARGUMENT;
-    Decision result{Action::REJECT, Reason::EXECUTION_ERROR, feedback.executed};
-    if (feedback.execution_status != Status::OK || feedback.homography.status == Status::INVALID_ARGUMENT) {
-        phase_ = Phase::Finished;
-    } else {
-        result.reason = detail::self_check_reason(feedback.homography, policy_.self_checks[position_]);
-        const PooledModels* m = policy_.pooled;
-        if (m && position_ != m->reference_index && result.reason == Reason::SELF_CHECK_PASSED) {
-            // Quality gate on 77 inputs: the rung context (33), gate features
-            // 0..23, the pair relations (6), the alignment relations (14).
-            std::array<double, kPooledGateInputs> in{};
-            std::copy_n(pooled_context_.begin(), 33, in.begin());
-            std::copy_n(feedback.features.begin(), 24, in.begin() + 33);
-            std::copy_n(pooled_context_.begin() + 33, 6, in.begin() + 57);
-            std::copy_n(feedback.features.begin() + 24, 14, in.begin() + 63);
-            const double quality = feedback.features_present ? predict_ensemble(m->post_quality, in.data()) : -1.0;
-            if (!std::isfinite(quality) || quality < m->gate_threshold) result.reason = Reason::QUALITY_PREDICTOR_REJECTED;
-        }
-        if (policy_.selector.count && result.reason == Reason::SELF_CHECK_PASSED &&
-            (!feedback.features_present ||
-             predict_tree(policy_.gates[position_], feedback.features.data(), features::kGateCount) != 1))
-            result.reason = Reason::QUALITY_PREDICTOR_REJECTED;
-        if (result.reason == Reason::SELF_CHECK_PASSED) {
-            result.action = Action::ACCEPT;
-            phase_ = Phase::Finished;
-        } else {
-            // Pooled policies retry once, at the reference rung; selector
-            // policies follow their retry targets; rule policies the next rung.
-            const bool targets = policy_.retry_target_count != 0;
-            auto after = [&](std::size_t i) { return targets ? policy_.retry_targets[i] : i + 1; };
-            std::size_t next = m ? (position_ == m->reference_index ? count_ : m->reference_index) : after(position_);
-            // Changing R cannot add matches: skip rungs whose inlier minimum
-            // already exceeds the available correspondences.
-            while (next < count_ && ladder_[next].S == feedback.executed.S && ladder_[next].K == feedback.executed.K &&
-                   feedback.homography.num_matches < policy_.self_checks[next].min_inliers)
-                next = after(next);
-            if (next < count_) {
-                result.action = Action::RETRY;
-                result.next = ladder_[next];
-                position_ = next;
-                if (m) set_pooled_rung(next);
-            } else {
-                result.reason = Reason::BUDGET_EXHAUSTED;
-                phase_ = Phase::Finished;
-            }
-        }
+    const HomographyResult& h = feedback.homography;
+    if (feedback.execution_status != Status::OK || h.status == Status::INVALID_ARGUMENT) {
+        *decision = {Action::REJECT, Reason::EXECUTION_ERROR, feedback.executed};
+        state_ = State::Done;
+        return Status::OK;
     }
-    *decision = result;
+    const Reason reason = judge(feedback);
+    if (reason == Reason::SELF_CHECK_PASSED) {
+        *decision = {Action::ACCEPT, reason, feedback.executed};
+        state_ = State::Done;
+        return Status::OK;
+    }
+    // A rung with the same S and K only draws more hypotheses over the same
+    // matches, so it is passed over when it asks for more inliers than there
+    // are matches.
+    auto hopeless = [&](std::size_t r) {
+        const Knobs& k = ladder_[r];
+        return k.S == feedback.executed.S && k.K == feedback.executed.K &&
+               h.num_matches < policy_.self_checks[r].min_inliers;
+    };
+    std::size_t next = successor(rung_);
+    while (next < rungs_ && hopeless(next))
+        next = mode_ == Mode::Pooled ? next + 1 : successor(next);
+    if (next >= rungs_) {
+        *decision = {Action::REJECT, Reason::BUDGET_EXHAUSTED, feedback.executed};
+        state_ = State::Done;
+        return Status::OK;
+    }
+    rung_ = next;
+    if (mode_ == Mode::Pooled) describe_rung(next);
+    *decision = {Action::RETRY, reason, ladder_[next]};
     return Status::OK;
 }
 
 void Controller::reset_pair() noexcept {
-    position_ = 0;
-    phase_ = ladder_ ? Phase::Idle : Phase::Uninitialized;
+    rung_ = 0;
+    state_ = ladder_ ? State::Bound : State::Unbound;
 }
 
 Status plan_retry(const Knobs& a, const Knobs& b, RetryPlan* plan) noexcept {

@@ -1172,45 +1172,52 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
     }
 }
 
+// A geometry the kernels can run: a canvas of addressable size that holds
+// all of A (A sits at -origin, so the origin is never positive), non-empty
+// inputs, a finite non-zero inverse map and tolerances in their ranges.
 bool valid_geometry(const s6::Geometry& g) {
-    if (g.width <= 0 || g.height <= 0 ||
-        static_cast<std::size_t>(g.width) > std::numeric_limits<std::size_t>::max() / static_cast<std::size_t>(g.height) ||
-        g.width_a <= 0 || g.height_a <= 0 || g.width_b <= 0 || g.height_b <= 0 || g.origin_x > 0 || g.origin_y > 0)
+    const bool sizes = g.width > 0 && g.height > 0 && g.width_a > 0 && g.height_a > 0 && g.width_b > 0 &&
+                       g.height_b > 0;
+    if (!sizes || static_cast<std::size_t>(g.width) >
+                      std::numeric_limits<std::size_t>::max() / static_cast<std::size_t>(g.height))
         return false;
-    const auto right = static_cast<long long>(g.origin_x) + g.width - 1;
-    const auto bottom = static_cast<long long>(g.origin_y) + g.height - 1;
-    if (right < g.width_a - 1LL || bottom < g.height_a - 1LL) return false;
-    double magnitude = 0;
-    for (double v : g.inverse) {
-        if (!std::isfinite(v)) return false;
-        magnitude = std::max(magnitude, std::fabs(v));
-    }
-    return magnitude > 0 && std::isfinite(g.denominator_tolerance) && g.denominator_tolerance > 0 &&
-           g.denominator_tolerance < 1 && std::isfinite(g.snap_tolerance) && g.snap_tolerance >= 0 &&
-           g.snap_tolerance < 0.5;
+    const bool holds_a = g.origin_x <= 0 && g.origin_y <= 0 &&
+                         static_cast<long long>(g.origin_x) + g.width >= g.width_a &&
+                         static_cast<long long>(g.origin_y) + g.height >= g.height_a;
+    const bool finite_map = std::all_of(std::begin(g.inverse), std::end(g.inverse),
+                                        [](double v) { return std::isfinite(v); });
+    const bool nonzero_map = std::any_of(std::begin(g.inverse), std::end(g.inverse),
+                                         [](double v) { return v != 0; });
+    const bool tolerances = std::isfinite(g.denominator_tolerance) && g.denominator_tolerance > 0 &&
+                            g.denominator_tolerance < 1 && std::isfinite(g.snap_tolerance) &&
+                            g.snap_tolerance >= 0 && g.snap_tolerance < 0.5;
+    return holds_a && finite_map && nonzero_map && tolerances;
 }
 
 bool valid_mode(s6::BlendMode m) { return s6::requested_reductions(m) >= 0; }
 
-struct Range {
-    std::uintptr_t first, last;
+// Byte extent [lo, hi) of a buffer.
+struct Extent {
+    std::uintptr_t lo{0}, hi{0};
+    bool meets(const Extent& o) const { return lo < o.hi && o.lo < hi; }
 };
-bool image_range(const ImageView& v, PixelFormat format, Range& r) {
-    const std::size_t channels = format == PixelFormat::RGB8 ? 3 : 1;
-    if (!v.data || v.width <= 0 || v.height <= 0 || v.pitch_bytes <= 0 || v.format != format) return false;
-    const std::size_t row = static_cast<std::size_t>(v.width) * channels;
-    if (row > static_cast<std::size_t>(v.pitch_bytes)) return false;
-    const auto max = std::numeric_limits<std::uintptr_t>::max();
-    const auto rows = static_cast<std::size_t>(v.height - 1);
-    const auto pitch = static_cast<std::size_t>(v.pitch_bytes);
-    if (rows > (max - row) / pitch) return false;
-    const auto span = rows * pitch + row;
-    r.first = reinterpret_cast<std::uintptr_t>(v.data);
-    if (r.first > max - span) return false;
-    r.last = r.first + span;
+
+// Extent of a pitched image from its first byte to the last pixel byte of
+// its last row; false for an unusable view or one that wraps the address space.
+bool image_extent(const ImageView& v, PixelFormat format, Extent* e) {
+    if (!v.data || v.format != format || v.width <= 0 || v.height <= 0 || v.pitch_bytes <= 0) return false;
+    const std::size_t row = static_cast<std::size_t>(v.width) * (format == PixelFormat::RGB8 ? 3 : 1);
+    const std::size_t pitch = static_cast<std::size_t>(v.pitch_bytes);
+    const std::size_t top = std::numeric_limits<std::uintptr_t>::max();
+    if (row > pitch) return false;
+    const std::size_t below = static_cast<std::size_t>(v.height) - 1;  // rows after the first
+    if (below && below > (top - row) / pitch) return false;
+    const std::size_t span = below * pitch + row;
+    const auto start = reinterpret_cast<std::uintptr_t>(v.data);
+    if (start > top - span) return false;
+    *e = {start, start + span};
     return true;
 }
-bool overlaps(const Range& a, const Range& b) { return a.first < b.last && b.first < a.last; }
 
 std::size_t align256(std::size_t v) { return (v + 255) / 256 * 256; }
 
@@ -1433,31 +1440,35 @@ Status make_s6_job(const ImageView& a, const ImageView& b, const s6::Geometry& g
     if (a.width != g.width_a || a.height != g.height_a || b.width != g.width_b || b.height != g.height_b ||
         output.width != g.width || output.height != g.height || mask.width != g.width || mask.height != g.height)
         return Status::INVALID_ARGUMENT;
-    Range r[4];
-    if (!image_range(a, PixelFormat::RGB8, r[0]) || !image_range(b, PixelFormat::RGB8, r[1]) ||
-        !image_range(output, PixelFormat::RGB8, r[2]) || !image_range(mask, PixelFormat::GRAY8, r[3]))
+    // Aliasing rules (src/s6/s6.cu:338-366): the inputs may share memory with
+    // each other but nothing the render writes - panorama, mask, workspace
+    // (when the mode uses one) and statistics - may touch any other buffer.
+    Extent in_a, in_b, pano, union_mask;
+    if (!image_extent(a, PixelFormat::RGB8, &in_a) || !image_extent(b, PixelFormat::RGB8, &in_b) ||
+        !image_extent(output, PixelFormat::RGB8, &pano) || !image_extent(mask, PixelFormat::GRAY8, &union_mask))
         return Status::INVALID_ARGUMENT;
-    for (int i = 0; i < 4; ++i)
-        for (int k = i + 1; k < 4; ++k)
-            if (!(i == 0 && k == 1) && overlaps(r[i], r[k])) return Status::INVALID_ARGUMENT;
+    const Extent images[4] = {in_a, in_b, pano, union_mask};
+    const auto touches_image = [&](const Extent& e, int skip) {
+        for (int i = 0; i < 4; ++i)
+            if (i != skip && e.meets(images[i])) return true;
+        return false;
+    };
+    if (touches_image(pano, 2) || touches_image(union_mask, 3)) return Status::INVALID_ARGUMENT;
     if (available < layout.bytes || (layout.bytes && !ws)) return Status::INSUFFICIENT_WORKSPACE;
-    if (layout.bytes) {
-        const auto address = reinterpret_cast<std::uintptr_t>(ws);
-        if (address % 256 || address > std::numeric_limits<std::uintptr_t>::max() - available)
-            return Status::INVALID_ARGUMENT;
-        const Range arena{address, address + available};
-        for (const auto& x : r)
-            if (overlaps(x, arena)) return Status::INVALID_ARGUMENT;
+    Extent arena{};
+    const bool have_arena = ws && available;
+    if (have_arena) {
+        arena.lo = reinterpret_cast<std::uintptr_t>(ws);
+        if (arena.lo > std::numeric_limits<std::uintptr_t>::max() - available) return Status::INVALID_ARGUMENT;
+        arena.hi = arena.lo + available;
     }
+    if (layout.bytes && (arena.lo % 256 || touches_image(arena, -1))) return Status::INVALID_ARGUMENT;
     if (stats) {
-        const auto address = reinterpret_cast<std::uintptr_t>(stats);
-        if (address % alignof(s6::RenderStatistics)) return Status::INVALID_ARGUMENT;
-        const Range counters{address, address + sizeof(s6::RenderStatistics)};
-        for (const auto& x : r)
-            if (overlaps(x, counters)) return Status::INVALID_ARGUMENT;
-        if (ws && available && overlaps(counters, Range{reinterpret_cast<std::uintptr_t>(ws),
-                                                        reinterpret_cast<std::uintptr_t>(ws) + available}))
-            return Status::INVALID_ARGUMENT;
+        const auto at = reinterpret_cast<std::uintptr_t>(stats);
+        if (at % alignof(s6::RenderStatistics)) return Status::INVALID_ARGUMENT;
+        const Extent counters{at, at + sizeof(s6::RenderStatistics)};
+        // A supplied workspace counts even when the mode does not use it.
+        if (touches_image(counters, -1) || (have_arena && counters.meets(arena))) return Status::INVALID_ARGUMENT;
     }
     S6Job j{};
     j.a = a.data;

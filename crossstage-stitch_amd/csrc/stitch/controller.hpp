@@ -158,25 +158,37 @@ public:
     Status begin_pair(const SceneSignals& scene, Knobs* first) noexcept;
     Status observe(const Feedback& feedback, Decision* decision) noexcept;
     void reset_pair() noexcept;
-    std::size_t ladder_position() const noexcept { return position_; }
+    std::size_t ladder_position() const noexcept { return rung_; }
     // Which observations the configured policy reads (controller.hpp:72-77).
-    bool uses_learned_features() const noexcept { return configured_ && (policy_.selector.count || policy_.pooled); }
-    bool uses_pair_relations() const noexcept { return configured_ && policy_.pooled && policy_.pooled->pair_relations; }
-    unsigned relation_mask() const noexcept { return configured_ && policy_.pooled ? policy_.pooled->relation_mask : 0; }
+    bool uses_learned_features() const noexcept { return mode_ == Mode::Selector || mode_ == Mode::Pooled; }
+    bool uses_pair_relations() const noexcept { return mode_ == Mode::Pooled && policy_.pooled->pair_relations; }
+    unsigned relation_mask() const noexcept { return mode_ == Mode::Pooled ? policy_.pooled->relation_mask : 0; }
     bool needs_gate_features() const noexcept {
-        return uses_learned_features() && (!policy_.pooled || position_ != policy_.pooled->reference_index);
+        return mode_ == Mode::Selector || (mode_ == Mode::Pooled && rung_ != policy_.pooled->reference_index);
     }
 
 private:
-    enum class Phase { Uninitialized, Idle, AwaitingFeedback, Finished };
+    // How the first rung is chosen and an attempt judged: by the scene
+    // gradient rules alone, by a selector tree with per-rung gate trees, or
+    // by the pooled cost and quality models.
+    enum class Mode { Unconfigured, Rules, Selector, Pooled };
+    // Per pair: Bound -> (begin_pair) Running -> (observe: ACCEPT / REJECT)
+    // Done; a RETRY stays Running on the next rung.
+    enum class State { Unbound, Bound, Running, Done };
     Policy policy_{};
-    bool configured_ = false;
+    Mode mode_ = Mode::Unconfigured;
+    State state_ = State::Unbound;
     const Knobs* ladder_ = nullptr;
-    std::size_t count_ = 0;
-    std::size_t position_ = 0;
-    Phase phase_ = Phase::Uninitialized;
-    std::array<double, kPooledContext> pooled_context_{};
-    void set_pooled_rung(std::size_t index) noexcept;
+    std::size_t rungs_ = 0;
+    std::size_t rung_ = 0;
+    // Inputs of the pooled models: selector features (0..29), the rung's
+    // S, K / 2000 and R / 1024 (30..32), the pair relations (33..38).
+    std::array<double, kPooledContext> context_{};
+
+    bool pick_first_rung(const SceneSignals& scene, std::size_t* rung) noexcept;
+    Reason judge(const Feedback& feedback) const noexcept;
+    std::size_t successor(std::size_t rung) const noexcept;
+    void describe_rung(std::size_t rung) noexcept;
 };
 
 enum class RestartFrom : int { S1 = 1, S2 = 2, S5 = 5 };

@@ -33,7 +33,7 @@ MIXED = [
     (31100, 3840, 2160, 8000, (0.3, 0.5)),
     (31101, 3840, 2160, 6000, (0.5, 0.8)),
     (31009, 3840, 2160, 3000, (0.2, 0.25)),  # canvas ~7500x2850 (21 MP): over the budget
-    (31103, 3840, 2160, 7500, (0.4, 0.8)),
+    (31117, 3840, 2160, 7500, (0.4, 0.8)),
     (31104, 1920, 1080, 8000, (0.3, 0.5)),
     (31105, 1920, 1080, 5000, (0.2, 0.8)),
     (31106, 1920, 1080, 2500, (0.2, 0.8)),
@@ -89,14 +89,11 @@ def test_mixed_batch_matches_oracle_and_plain_runner():
     for i in ORACLE_CHECKED:
         assert statuses[i] == 0, (i, statuses[i], stages[i])
     # Every pair through the plain per-pair runner, one runner per pair sized
-    # to the pair (its canvas box is the batch's geometry, or the budget's
-    # largest box when the batch found none).
+    # to the pair (its canvas box is the batch's geometry, or the input size
+    # when the batch rendered nothing).
     for i, ((ta, tb), c, o) in enumerate(zip(dev, cfgs, outs)):
         w, h = MIXED[i][1], MIXED[i][2]
-        if o.status == 0:
-            cw, ch = o.geometry.width, o.geometry.height
-        else:
-            cw, ch = 8192, 2048
+        cw, ch = (o.geometry.width, o.geometry.height) if o.status == 0 else (w, h)
         runner = P.PipelineRunner(w, h, max_k=c["K"], max_r=R, canvas_width=cw, canvas_height=ch,
                                   workspace_mode=P.F.BLEND_NAMES[BLEND])
         runner.use_device_pair(ta, tb)
