@@ -13,6 +13,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libcsstitch.so"
 
 OK, INVALID_ARGUMENT, INSUFFICIENT_WORKSPACE, DEVICE_ERROR, NOT_ENOUGH_POINTS, DEGENERATE = range(6)
 RGB8, GRAY8 = 0, 1
+# Fixed device array behind cs_s4_match_noalloc (CS_S4_NOALLOC_SCRATCH_BYTES).
+S4_NOALLOC_SCRATCH_BYTES = 36 << 20
 BINARY_U8, FLOAT32 = 0, 1
 FEATHER, MULTIBAND3, MULTIBAND5, MULTIBAND6, MULTIBAND7 = range(5)
 BLEND_NAMES = {"feather": FEATHER, "multiband_3": MULTIBAND3, "multiband_5": MULTIBAND5,
@@ -200,6 +202,8 @@ SIGNATURES = {
     "cs_s4_match": (C.c_int, [P(Keypoints), P(Descriptors), P(Keypoints), P(Descriptors), P(S4Config), P(Matches),
                               vp, sz, vp]),
     "cs_s4_forward_neighbors": (C.c_int, [C.c_int, C.c_int, P(S4Config), vp, sz, P(vp)]),
+    "cs_s4_match_noalloc": (C.c_int, [P(Keypoints), P(Descriptors), P(Keypoints), P(Descriptors), P(S4Config),
+                                      P(Matches), vp]),
     "cs_s4_set_segment": (C.c_int, [C.c_int]),
     "cs_s5_workspace_bytes": (C.c_int, [P(S5Config), P(sz)]),
     "cs_s5_estimate": (C.c_int, [P(Keypoints), P(Keypoints), P(Matches), P(ScaleInfo), P(ScaleInfo), P(S5Config),

@@ -105,6 +105,19 @@ int cs_s4_match(const cs_keypoints* a, const cs_descriptors* da, const cs_keypoi
                 void* workspace, size_t workspace_bytes, void* stream);
 int cs_s4_forward_neighbors(int capacity_a, int capacity_b, const cs_s4_config* config, const void* workspace,
                             size_t workspace_bytes, const cs_top2** output);
+/* cs_s4_match without a workspace argument and without any allocation. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The workspace is a fixed device array of
+ * CS_S4_NOALLOC_SCRATCH_BYTES held by each GPU until the process ends, enough
+ * for 16384 x 16384 descriptors with cross-check; capacities whose
+ * cs_s4_workspace_bytes exceed it return CS_INSUFFICIENT_WORKSPACE. Results
+ * equal cs_s4_match's. All calls on one GPU share the array: calls that
+ * overlap on one GPU race. */
+#define CS_S4_NOALLOC_SCRATCH_BYTES ((size_t)36 << 20)
+int cs_s4_match_noalloc(const cs_keypoints* a, const cs_descriptors* da, const cs_keypoints* b,
+                        const cs_descriptors* db, const cs_s4_config* config, const cs_matches* output,
+                        void* stream);
 /* Train rows per LDS segment of the S4 fast path (64..512, multiple of 64). */
 int cs_s4_set_segment(int rows);
 
