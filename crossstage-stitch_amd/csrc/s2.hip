@@ -467,13 +467,62 @@ __global__ void __launch_bounds__(256) s2_harris(dev::JobSpan<S2Job> span, dev::
         });
 }
 
-// rank(key) = #keys greater. Tile = 64 keys x one 2048-key segment; 4 waves
-// split the segment and the partial counts are added atomically.
+// Sorts each kS2RankSegment-key segment of the selected candidates in place,
+// descending, with a bitonic network in LDS (unused slots padded with 0,
+// below every key: a key's high word is order_bits of a finite response,
+// never 0). Keys are unique (the low word holds the pixel index), so after
+// this pass a key's rank among all keys is its position in its own segment
+// plus the number of greater keys in each other segment.
+__global__ void __launch_bounds__(256) s2_sort(dev::JobSpan<S2Job> span, dev::Total work) {
+    const long long total = work.get();
+    __shared__ std::uint64_t seg[kS2RankSegment];
+    int ji = 0;
+    const int njobs = span.jobs();
+    dev::sparse_walk(
+        total,
+        [&](long long tk) {
+            const S2Job& jj = dev::lane_job(span, tk, &S2Job::sort_begin, ji, njobs);
+            return (tk - jj.sort_begin) * kS2RankSegment < static_cast<long long>(jj.params->selected);
+        },
+        [&](long long t) {
+            ji = dev::next_job(span, t, &S2Job::sort_begin, ji, njobs);
+            const S2Job& j = dev::job_at(span, ji);
+            const int seg0 = static_cast<int>(t - j.sort_begin) * kS2RankSegment;
+            const int n = min(kS2RankSegment, static_cast<int>(j.params->selected) - seg0);
+            // The network spans the smallest power of two holding the keys.
+            int span_n = 2;
+            while (span_n < n) span_n <<= 1;
+            for (int i = threadIdx.x; i < span_n; i += 256) seg[i] = i < n ? j.keys[seg0 + i] : 0ull;
+            __syncthreads();
+            for (int k = 2; k <= span_n; k <<= 1) {
+                for (int d = k >> 1; d > 0; d >>= 1) {
+                    // Pair i compares elements lo and lo + d; blocks of k
+                    // alternate direction so that every merge sees a bitonic
+                    // sequence, and the final block (all of it) ends descending.
+                    for (int i = threadIdx.x; i < span_n / 2; i += 256) {
+                        const int lo = 2 * i - (i & (d - 1));
+                        const bool descending = (lo & k) == 0;
+                        const std::uint64_t x = seg[lo], y = seg[lo + d];
+                        if ((x < y) == descending) {
+                            seg[lo] = y;
+                            seg[lo + d] = x;
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            for (int i = threadIdx.x; i < n; i += 256) j.keys[seg0 + i] = seg[i];
+            __syncthreads();
+        });
+}
+
+// rank(key) = #keys greater, over segment-sorted keys. Tile = 256 keys x one
+// sorted segment: the segment goes to LDS and each thread counts the
+// segment's keys above its own by binary search (descending order: they are
+// a prefix). Partial counts of the segments add up atomically.
 __global__ void __launch_bounds__(256) s2_rank(dev::JobSpan<S2Job> span, dev::Total work) {
     const long long total = work.get();
     __shared__ std::uint64_t seg[kS2RankSegment];
-    __shared__ std::uint32_t part[4][kS2RankKeys];
-    const int lane = dev::lane_id(), wave = threadIdx.x / 64;
     int ji = 0;
     const int njobs = span.jobs();
     const auto split = [](const S2Job& j, long long t, int& key0, int& seg0) {
@@ -492,27 +541,26 @@ __global__ void __launch_bounds__(256) s2_rank(dev::JobSpan<S2Job> span, dev::To
             return key0 < selected && seg0 < selected;
         },
         [&](long long t) {
-        ji = dev::next_job(span, t, &S2Job::rank_begin, ji, njobs);
-        const S2Job& j = dev::job_at(span, ji);
-        int key0, seg0;
-        split(j, t, key0, seg0);
-        const int selected = static_cast<int>(j.params->selected);
-        const int n = min(kS2RankSegment, selected - seg0);
-        for (int i = threadIdx.x; i < n; i += 256) seg[i] = j.keys[seg0 + i];
-        __syncthreads();
-        const int slot = key0 + lane;
-        const std::uint64_t key = slot < selected ? j.keys[slot] : ~0ull;
-        const int per = (n + 3) / 4;
-        const int b = wave * per, e = min(n, b + per);
-        std::uint32_t rank = 0;
-        for (int i = b; i < e; ++i) rank += seg[i] > key;
-        part[wave][lane] = rank;
-        __syncthreads();
-        if (wave == 0 && slot < selected) {
-            const std::uint32_t r = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
-            if (r) atomicAdd(&j.ranks[slot], r);
-        }
-        __syncthreads();
+            ji = dev::next_job(span, t, &S2Job::rank_begin, ji, njobs);
+            const S2Job& j = dev::job_at(span, ji);
+            int key0, seg0;
+            split(j, t, key0, seg0);
+            const int selected = static_cast<int>(j.params->selected);
+            const int n = min(kS2RankSegment, selected - seg0);
+            for (int i = threadIdx.x; i < n; i += 256) seg[i] = j.keys[seg0 + i];
+            __syncthreads();
+            const int slot = key0 + threadIdx.x;
+            if (slot < selected) {
+                const std::uint64_t key = j.keys[slot];
+                int lo = 0, hi = n;  // first position whose key is not above `key`
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (seg[mid] > key) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo) atomicAdd(&j.ranks[slot], static_cast<std::uint32_t>(lo));
+            }
+            __syncthreads();
         });
 }
 
@@ -614,19 +662,21 @@ Status make_s2_job(const ImageView& image, const s2::Config& cfg, const Keypoint
 }
 
 S2Totals s2_prefix(S2Job* js, int n) {
-    S2Totals t{n, 0, 0, 0, 0};
+    S2Totals t{n, 0, 0, 0, 0, 0};
     for (int i = 0; i < n; ++i) {
         S2Job& j = js[i];
         j.fast_begin = t.fast;
         j.chunk_begin = t.chunks;
         j.cand_begin = t.cands;
+        j.sort_begin = t.sort;
         j.rank_begin = t.rank;
         if (!j.active) continue;
         t.fast += j.tiles;
         t.chunks += j.chunks;
         t.cands += (j.candidates + 255) / 256;
-        t.rank += static_cast<long long>((j.candidates + kS2RankKeys - 1) / kS2RankKeys) *
-                  ((j.candidates + kS2RankSegment - 1) / kS2RankSegment);
+        const long long segments = (j.candidates + kS2RankSegment - 1) / kS2RankSegment;
+        t.sort += segments;
+        t.rank += static_cast<long long>((j.candidates + kS2RankKeys - 1) / kS2RankKeys) * segments;
     }
     return t;
 }
@@ -639,14 +689,15 @@ hipError_t launch_s2(const S2Job* table, int count, const S2Totals& t, const S2J
     };
     s2_init<<<count, 256, 0, stream>>>(span);
     if (t.fast > 0) {
-        s2_fast_nms<<<persistent_grid(t.fast), 256, 0, stream>>>(span, total(&S2Totals::fast));
+        s2_fast_nms<<<persistent_grid(t.fast, jobs::Stage::S2), 256, 0, stream>>>(span, total(&S2Totals::fast));
         s2_select<<<count, 64, 0, stream>>>(span);
-        s2_count<<<persistent_grid(t.chunks), 256, 0, stream>>>(span, total(&S2Totals::chunks));
+        s2_count<<<persistent_grid(t.chunks, jobs::Stage::S2), 256, 0, stream>>>(span, total(&S2Totals::chunks));
         s2_scan<<<count, 256, 0, stream>>>(span);
-        s2_scatter<<<persistent_grid(t.chunks), 256, 0, stream>>>(span, total(&S2Totals::chunks));
-        s2_harris<<<persistent_grid(t.cands), 256, 0, stream>>>(span, total(&S2Totals::cands));
-        s2_rank<<<persistent_grid(t.rank), 256, 0, stream>>>(span, total(&S2Totals::rank));
-        s2_write<<<persistent_grid(t.cands), 256, 0, stream>>>(span, total(&S2Totals::cands));
+        s2_scatter<<<persistent_grid(t.chunks, jobs::Stage::S2), 256, 0, stream>>>(span, total(&S2Totals::chunks));
+        s2_harris<<<persistent_grid(t.cands, jobs::Stage::S2), 256, 0, stream>>>(span, total(&S2Totals::cands));
+        s2_sort<<<persistent_grid(t.sort, jobs::Stage::S2), 256, 0, stream>>>(span, total(&S2Totals::sort));
+        s2_rank<<<persistent_grid(t.rank, jobs::Stage::S2), 256, 0, stream>>>(span, total(&S2Totals::rank));
+        s2_write<<<persistent_grid(t.cands, jobs::Stage::S2), 256, 0, stream>>>(span, total(&S2Totals::cands));
     }
     return hipGetLastError();
 }

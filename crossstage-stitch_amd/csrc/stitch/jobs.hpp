@@ -22,14 +22,15 @@ struct S1Job {
     int orig_w, orig_h, work_w, work_h;
     float inv_sx, inv_sy, scale_area;
     int mode;      // 0: 1x1, 1: 2x2, 2: 4x4, 3: general area filter
-    int tiles_x;   // tiles of kS1TileW x kS1TileH output pixels
+    int tiles_x;   // tiles of kS1TileW x tile_h output pixels
+    int tile_h;    // s1_tile_rows() when the job was made; a launch's jobs share it
     long long tile_begin;
 };
 constexpr int kS1TileW = 64, kS1TileH = 64;
 
 Status make_s1_job(const ImageView& in, const ImageView& out, const ScaleInfo& si, S1Job* job);
 inline long long s1_tiles(const S1Job& j) {
-    return static_cast<long long>(j.tiles_x) * ((j.work_h + kS1TileH - 1) / kS1TileH);
+    return static_cast<long long>(j.tiles_x) * ((j.work_h + j.tile_h - 1) / j.tile_h);
 }
 hipError_t launch_s1(const S1Job* table, int count, long long total_tiles, const S1Job* single,
                      hipStream_t stream, const dev::Replay<long long>* replay = nullptr);
@@ -73,17 +74,20 @@ struct S2Job {
     int active;                    // 0: image too small or K == 0 -> count 0 only
     int tiles_x, tiles;            // FAST/NMS tiles
     int chunks;                    // compaction chunks of kS2Chunk pixels
-    long long fast_begin, chunk_begin, cand_begin, rank_begin;
+    long long fast_begin, chunk_begin, cand_begin, sort_begin, rank_begin;
 };
 constexpr int kS2TileW = 64, kS2TileH = 32, kS2Chunk = 4096;
-constexpr int kS2RankKeys = 64;      // keys ranked per workgroup
-constexpr int kS2RankSegment = 2048; // comparand keys per workgroup
+// Top-K order: the candidate keys are sorted in segments of kS2RankSegment
+// (one workgroup each, in LDS), then each key's rank is its position in its
+// own segment plus a binary search in every other one.
+constexpr int kS2RankKeys = 256;     // keys ranked per workgroup (one per thread)
+constexpr int kS2RankSegment = 2048; // keys per sorted segment
 
 std::size_t s2_layout_bytes(int w, int h, int K, std::size_t offsets[7]);
 Status make_s2_job(const ImageView& image, const s2::Config& cfg, const Keypoints& out, void* ws,
                    std::size_t ws_bytes, S2Job* job);
 struct S2Totals {
-    long long jobs_tiles, fast, chunks, cands, rank;
+    long long jobs_tiles, fast, chunks, cands, sort, rank;
 };
 // Fills the per-kernel tile prefixes of jobs[0..n) and returns the totals.
 S2Totals s2_prefix(S2Job* jobs, int n);

@@ -92,8 +92,12 @@ def test_s2_s3_golden_vectors(T):
     assert n >= 40
 
 
+# The top-K order sorts 2048-key segments and merges them by binary search:
+# K 1000 (one segment of 2000 candidates), 5000 (5 segments), 8000 and 16000
+# (8 and 16 segments, the large strip's keypoint count).
 @pytest.mark.parametrize("seed,h,w,k", [(1, 480, 640, 1000), (2, 720, 1280, 5000), (3, 1080, 1920, 5000),
-                                        (4, 300, 451, 37), (5, 2160, 3840, 8000)])
+                                        (4, 300, 451, 37), (5, 2160, 3840, 8000), (6, 2160, 3840, 16000),
+                                        (7, 1080, 1920, 1024)])
 def test_s2_s3_match_oracle(T, seed, h, w, k):
     torch, st = T
     gray = images.gray_of(images.texture_rgb(seed, h, w, blobs=200, rects=50))
