@@ -511,7 +511,8 @@ def main():
     tune = None
     if not args.no_tune:
         tune = br.autotune(specs, args.tune_cache)
-        log("tuning", tune.s4_segment, tune.workgroups_per_cu, tune.graphs)
+        log("tuning", tune.s4_segment, tune.workgroups_per_cu, tune.graphs, list(tune.stage_workgroups_per_cu),
+            tune.s1_tile_rows)
     # Python's cyclic collector can pause the host thread for tens of ms in
     # the middle of a timed step (many tracked objects once torch is loaded);
     # it stays off while batches are timed.
@@ -612,8 +613,11 @@ def main():
                                f"{{640x480,1280x720,1920x1080,3840x2160}}, K caps U[500,8000], overlap U[0.2,0.8], "
                                f"cross-checked Hamming, RANSAC {R}, {BLEND}; value counts the pairs that stitched "
                                f"(status OK)", "pairs_per_step": args.pairs,
-                   "tuning": None if tune is None else dict(s4_segment=tune.s4_segment,
-                                                            workgroups_per_cu=tune.workgroups_per_cu)},
+                   "tuning": None if tune is None else dict(
+                       s4_segment=tune.s4_segment, workgroups_per_cu=tune.workgroups_per_cu,
+                       stage_workgroups_per_cu=dict(zip(("s1", "s2", "s3", "s4", "s5", "s6"),
+                                                        list(tune.stage_workgroups_per_cu))),
+                       s1_tile_rows=tune.s1_tile_rows)},
         **result,
         "roofline": roof,
         "whole_path": whole_path,

@@ -148,7 +148,8 @@ class KernelProbe(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("s4_segment", C.c_int), ("workgroups_per_cu", C.c_int), ("graphs", C.c_int)]
+    _fields_ = [("s4_segment", C.c_int), ("workgroups_per_cu", C.c_int), ("graphs", C.c_int),
+                ("stage_workgroups_per_cu", C.c_int * 6), ("s1_tile_rows", C.c_int)]
 
 
 class ServerConfig(C.Structure):

@@ -74,9 +74,11 @@ class BatchRunner:
                                           cache_path.encode() if cache_path else None, C.byref(t)), "batch_autotune")
         return t
 
-    def apply(self, s4_segment=128, workgroups_per_cu=8, graphs=True):
-        _check(F.load().cs_batch_apply(self.handle, C.byref(F.Tuning(s4_segment, workgroups_per_cu, int(graphs)))),
-               "batch_apply")
+    def apply(self, s4_segment=128, workgroups_per_cu=8, graphs=True, stage_workgroups=(0, 0, 0, 0, 0, 0),
+              s1_tile_rows=64):
+        """Launch shapes; stage_workgroups overrides workgroups_per_cu per stage S1..S6 where non-zero."""
+        t = F.Tuning(s4_segment, workgroups_per_cu, int(graphs), (C.c_int * 6)(*stage_workgroups), s1_tile_rows)
+        _check(F.load().cs_batch_apply(self.handle, C.byref(t)), "batch_apply")
 
     def set_probe(self, on: bool):
         """Per-kernel S6 timing: direct launches with HIP events around each kernel."""

@@ -342,7 +342,13 @@ void FeatureEngine::launch(bool replay) {
         cap.n_s4 = cap.n_s5 = max_tasks_;
         cap.n_copy = 4 * max_tasks_;
         cap.t_s1 = cap.t_grad = kReplayTiles;
-        cap.t2 = {kReplayTiles, kReplayTiles, kReplayTiles, kReplayTiles, kReplayTiles};
+        // Every totals field gets the capacity (a field left at 0 would size
+        // its kernel's captured grid to a single workgroup).
+        static_assert(sizeof(jobs::S2Totals) == 6 * sizeof(long long), "S2Totals fields");
+        static_assert(sizeof(jobs::S3Totals) == 3 * sizeof(long long), "S3Totals fields");
+        static_assert(sizeof(jobs::S4Totals) == 6 * sizeof(long long), "S4Totals fields");
+        static_assert(sizeof(jobs::S5Totals) == 2 * sizeof(long long), "S5Totals fields");
+        cap.t2 = {kReplayTiles, kReplayTiles, kReplayTiles, kReplayTiles, kReplayTiles, kReplayTiles};
         cap.t3 = {kReplayTiles, kReplayTiles, kReplayTiles};
         cap.t4 = {kReplayTiles, kReplayTiles, kReplayTiles, kReplayTiles, kReplayTiles, kReplayTiles};
         cap.t5 = {kReplayTiles, kReplayTiles};
@@ -373,7 +379,7 @@ void FeatureEngine::capture() {
     if (graph_) hipGraphDestroy(graph_);
     exec_ = nullptr;
     graph_ = nullptr;
-    persistent_grid(1);  // device query outside the capture
+    persistent_grid(1, jobs::Stage::S1);  // device query outside the capture
     check(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed), "begin capture");
     try {
         launch(true);
@@ -700,7 +706,7 @@ hipGraphExec_t RenderEngine::graph_for(int top, int blend_top) {
     }
     auto* d = reinterpret_cast<const Header*>(dev_);
     const dev::Replay<jobs::S6Totals> r6{&d->n_s6, &d->t6};
-    persistent_grid(1);
+    persistent_grid(1, jobs::Stage::S6);  // device query outside the capture
     hipGraph_t g = nullptr;
     hipGraphExec_t x = nullptr;
     check(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed), "begin capture");
@@ -968,9 +974,16 @@ void BatchRunner::run(const PairSpec* pairs, int n, std::vector<PairOutcome>* ou
 void BatchRunner::apply(const Tuning& t) {
     jobs::s4_set_segment(t.s4_segment);
     jobs::set_workgroups_per_cu(t.workgroups_per_cu);
+    for (int s = 0; s < jobs::kStages; ++s)
+        if (t.stage_workgroups[s]) jobs::set_workgroups_per_cu(static_cast<jobs::Stage>(s), t.stage_workgroups[s]);
+    jobs::s1_set_tile_rows(t.s1_tile_rows);
     tuning_ = t;
     tuning_.s4_segment = jobs::s4_segment();
-    tuning_.workgroups_per_cu = jobs::workgroups_per_cu();
+    tuning_.workgroups_per_cu = jobs::workgroups_per_cu(jobs::Stage::S2);
+    if (t.workgroups_per_cu >= 1 && t.workgroups_per_cu <= 64) tuning_.workgroups_per_cu = t.workgroups_per_cu;
+    for (int s = 0; s < jobs::kStages; ++s)
+        tuning_.stage_workgroups[s] = jobs::workgroups_per_cu(static_cast<jobs::Stage>(s));
+    tuning_.s1_tile_rows = jobs::s1_tile_rows();
     features_.invalidate();
     features_.set_use_graph(t.graphs);
     for (RenderEngine& r : render_) {
@@ -1028,12 +1041,24 @@ bool load_cache(const std::string& path, const std::string& key, const std::stri
     if (p == std::string::npos) return false;
     const auto end = text.find('}', p);
     const std::string entry = text.substr(p, end == std::string::npos ? std::string::npos : end - p);
-    int seg = 0, wpc = 0, graphs = 1;
+    int seg = 0, wpc = 0, graphs = 1, rows = 64;
     if (!read_int(entry, "s4_segment", &seg) || !read_int(entry, "workgroups_per_cu", &wpc)) return false;
     read_int(entry, "graphs", &graphs);
+    read_int(entry, "s1_tile_rows", &rows);
+    int stages[jobs::kStages] = {};
+    const auto list = entry.find("\"stage_workgroups_per_cu\"");
+    if (list != std::string::npos) {
+        const auto open = entry.find('[', list);
+        if (open == std::string::npos ||
+            std::sscanf(entry.c_str() + open, "[%d,%d,%d,%d,%d,%d]", &stages[0], &stages[1], &stages[2], &stages[3],
+                        &stages[4], &stages[5]) != jobs::kStages)
+            return false;
+    }
     t->s4_segment = seg;
     t->workgroups_per_cu = wpc;
     t->graphs = graphs != 0;
+    t->s1_tile_rows = rows;
+    std::copy(stages, stages + jobs::kStages, t->stage_workgroups);
     return true;
 }
 
@@ -1045,7 +1070,10 @@ void store_cache(const std::string& path, const std::string& key, const std::str
     }
     std::ostringstream entry;
     entry << "\"" << key << "|" << shape << "\": {\"s4_segment\": " << t.s4_segment
-          << ", \"workgroups_per_cu\": " << t.workgroups_per_cu << ", \"graphs\": " << (t.graphs ? 1 : 0) << "}";
+          << ", \"workgroups_per_cu\": " << t.workgroups_per_cu << ", \"graphs\": " << (t.graphs ? 1 : 0)
+          << ", \"s1_tile_rows\": " << t.s1_tile_rows << ", \"stage_workgroups_per_cu\": [";
+    for (int s = 0; s < jobs::kStages; ++s) entry << (s ? "," : "") << t.stage_workgroups[s];
+    entry << "]}";
     const std::string want = "\"" + key + "|" + shape + "\"";
     const auto p = text.find(want);
     if (p != std::string::npos) {
@@ -1092,8 +1120,13 @@ Tuning BatchRunner::autotune(const PairSpec* pairs, int n, const std::string& ca
         }
         return bestms;
     };
-    // Occupancy first (every stage), then the S4 train segment (S4 only).
+    // Occupancy of every stage together first, then each stage's own
+    // occupancy in turn with the others held (S1-S5 judged by the feature
+    // phase, S6 by the render phase), then the S1 tile height and the S4
+    // train segment. Each step keeps the current choice unless another one
+    // is faster.
     double best_ms = 1e30;
+    for (int s = 0; s < jobs::kStages; ++s) best.stage_workgroups[s] = 0;
     for (int wpc : {8, 16, 24, 32}) {
         Tuning t = best;
         t.workgroups_per_cu = wpc;
@@ -1103,11 +1136,47 @@ Tuning BatchRunner::autotune(const PairSpec* pairs, int n, const std::string& ca
             best.workgroups_per_cu = wpc;
         }
     }
+    for (int s = 0; s < jobs::kStages; ++s) best.stage_workgroups[s] = best.workgroups_per_cu;
+    const auto phase_ms = [&](const Tuning& t, bool render) {
+        apply(t);
+        BatchTimings tm{};
+        double lowest = 1e30;
+        for (int rep = 0; rep < 3; ++rep) {
+            run(pairs, n, &out, &tm);
+            if (rep > 0) lowest = std::min(lowest, render ? tm.render_ms : tm.feature_ms);
+        }
+        return lowest;
+    };
+    for (int s = 0; s < jobs::kStages; ++s) {
+        const bool render = s == static_cast<int>(jobs::Stage::S6);
+        double stage_best = phase_ms(best, render);
+        for (int wpc : {4, 8, 16, 24, 32}) {
+            if (wpc == best.stage_workgroups[s]) continue;
+            Tuning t = best;
+            t.stage_workgroups[s] = wpc;
+            const double ms = phase_ms(t, render);
+            if (ms < stage_best) {
+                stage_best = ms;
+                best.stage_workgroups[s] = wpc;
+            }
+        }
+    }
+    {
+        double rows_best = phase_ms(best, false);
+        for (int rows : {16, 32, 64}) {
+            if (rows == best.s1_tile_rows) continue;
+            Tuning t = best;
+            t.s1_tile_rows = rows;
+            const double ms = phase_ms(t, false);
+            if (ms < rows_best) {
+                rows_best = ms;
+                best.s1_tile_rows = rows;
+            }
+        }
+    }
     best_ms = 1e30;
-    const int wpc = best.workgroups_per_cu;
     for (int seg : {64, 128, 256, 512}) {
         Tuning t = best;
-        t.workgroups_per_cu = wpc;
         t.s4_segment = seg;
         const double ms = measure(t, true);
         if (ms < best_ms) {

@@ -205,7 +205,14 @@ int cs_batch_autotune(cs_batch* b, const cs_pair_spec* pairs, int n, const char*
         std::vector<batch::PairSpec> specs(n);
         for (int i = 0; i < n; ++i) specs[i] = spec(pairs[i]);
         const batch::Tuning t = b->impl.autotune(specs.data(), n, cache_path ? cache_path : "");
-        if (chosen) *chosen = {t.s4_segment, t.workgroups_per_cu, t.graphs ? 1 : 0};
+        if (chosen) {
+            *chosen = cs_tuning{};
+            chosen->s4_segment = t.s4_segment;
+            chosen->workgroups_per_cu = t.workgroups_per_cu;
+            chosen->graphs = t.graphs ? 1 : 0;
+            for (int s = 0; s < 6; ++s) chosen->stage_workgroups_per_cu[s] = t.stage_workgroups[s];
+            chosen->s1_tile_rows = t.s1_tile_rows;
+        }
         return CS_OK;
     } catch (...) {
         return map_exception();
@@ -215,7 +222,13 @@ int cs_batch_autotune(cs_batch* b, const cs_pair_spec* pairs, int n, const char*
 int cs_batch_apply(cs_batch* b, const cs_tuning* t) {
     if (!b || !t) return CS_INVALID_ARGUMENT;
     try {
-        b->impl.apply({t->s4_segment, t->workgroups_per_cu, t->graphs != 0});
+        batch::Tuning tuning{};
+        tuning.s4_segment = t->s4_segment;
+        tuning.workgroups_per_cu = t->workgroups_per_cu;
+        tuning.graphs = t->graphs != 0;
+        for (int s = 0; s < 6; ++s) tuning.stage_workgroups[s] = t->stage_workgroups_per_cu[s];
+        tuning.s1_tile_rows = t->s1_tile_rows ? t->s1_tile_rows : 64;
+        b->impl.apply(tuning);
         return CS_OK;
     } catch (...) {
         return map_exception();

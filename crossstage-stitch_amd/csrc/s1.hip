@@ -57,9 +57,10 @@ __device__ std::uint8_t s1_pixel(const S1Job& p, int xd, int yd) {
     return static_cast<std::uint8_t>(static_cast<int>(rintf(total * p.scale_area)));
 }
 
-// Persistent walk over 64 x 16 output tiles of every job; four rows per
-// thread, so each thread has four pixels' loads in flight.
-constexpr int kRowsPerThread = kS1TileH / (256 / kS1TileW);
+// Persistent walk over 64 x kRows output tiles of every job (kRows 16, 32
+// or 64, a launch-shape knob): the word path reads four pixels per thread
+// and row step, the scalar path kRows / 4 rows per thread.
+constexpr int kRowsPerThread = kS1TileH / (256 / kS1TileW);  // gradient tiles (64 x 64)
 static_assert(kRowsPerThread * (256 / kS1TileW) == kS1TileH, "S1 tile");
 
 // Four horizontally adjacent pixels of a full-size (mode 0) job from three
@@ -73,13 +74,15 @@ __device__ __forceinline__ unsigned gray4(unsigned w0, unsigned w1, unsigned w2)
     return g0 | (g1 << 8) | (g2 << 16) | (g3 << 24);
 }
 
+template <int kTileH>
 __device__ __forceinline__ void s1_tile_vec4(const S1Job& p, int tx, int ty) {
-    constexpr int kQuads = kS1TileW / 4, kRows = 256 / kQuads, kIt = kS1TileH / kRows;
+    constexpr int kQuads = kS1TileW / 4, kRows = 256 / kQuads, kIt = kTileH / kRows;
+    static_assert(kIt * kRows == kTileH, "S1 word-path tile");
     const int xd = tx * kS1TileW + 4 * static_cast<int>(threadIdx.x % kQuads);
     unsigned w[kIt][3];
 #pragma unroll
     for (int k = 0; k < kIt; ++k) {
-        const int yd = ty * kS1TileH + static_cast<int>(threadIdx.x / kQuads) + k * kRows;
+        const int yd = ty * kTileH + static_cast<int>(threadIdx.x / kQuads) + k * kRows;
         if (yd < p.work_h && xd + 3 < p.work_w) {
             const auto* s = reinterpret_cast<const unsigned*>(p.src + static_cast<size_t>(yd) * p.src_pitch + 3 * xd);
             w[k][0] = s[0];
@@ -89,7 +92,7 @@ __device__ __forceinline__ void s1_tile_vec4(const S1Job& p, int tx, int ty) {
     }
 #pragma unroll
     for (int k = 0; k < kIt; ++k) {
-        const int yd = ty * kS1TileH + static_cast<int>(threadIdx.x / kQuads) + k * kRows;
+        const int yd = ty * kTileH + static_cast<int>(threadIdx.x / kQuads) + k * kRows;
         if (yd >= p.work_h || xd >= p.work_w) continue;
         std::uint8_t* d = p.dst + static_cast<size_t>(yd) * p.dst_pitch + xd;
         if (xd + 3 < p.work_w) {
@@ -101,7 +104,9 @@ __device__ __forceinline__ void s1_tile_vec4(const S1Job& p, int tx, int ty) {
     }
 }
 
+template <int kTileH>
 __global__ void __launch_bounds__(256) s1_kernel(dev::JobSpan<S1Job> span, dev::Total work) {
+    constexpr int kScalarRows = kTileH / (256 / kS1TileW);
     const long long total = work.get();
     int j = 0;
     const int njobs = span.jobs();
@@ -114,19 +119,19 @@ __global__ void __launch_bounds__(256) s1_kernel(dev::JobSpan<S1Job> span, dev::
         if (p.mode == 0 && ((reinterpret_cast<std::uintptr_t>(p.src) | reinterpret_cast<std::uintptr_t>(p.dst) |
                              static_cast<std::uintptr_t>(p.src_pitch) | static_cast<std::uintptr_t>(p.dst_pitch)) &
                             3u) == 0) {
-            s1_tile_vec4(p, tx, ty);
+            s1_tile_vec4<kTileH>(p, tx, ty);
             continue;
         }
         const int xd = tx * kS1TileW + static_cast<int>(threadIdx.x % kS1TileW);
-        std::uint8_t v[kRowsPerThread];
+        std::uint8_t v[kScalarRows];
 #pragma unroll
-        for (int k = 0; k < kRowsPerThread; ++k) {
-            const int yd = ty * kS1TileH + static_cast<int>(threadIdx.x / kS1TileW) + k * (256 / kS1TileW);
+        for (int k = 0; k < kScalarRows; ++k) {
+            const int yd = ty * kTileH + static_cast<int>(threadIdx.x / kS1TileW) + k * (256 / kS1TileW);
             v[k] = xd < p.work_w && yd < p.work_h ? s1_pixel(p, xd, yd) : 0;
         }
 #pragma unroll
-        for (int k = 0; k < kRowsPerThread; ++k) {
-            const int yd = ty * kS1TileH + static_cast<int>(threadIdx.x / kS1TileW) + k * (256 / kS1TileW);
+        for (int k = 0; k < kScalarRows; ++k) {
+            const int yd = ty * kTileH + static_cast<int>(threadIdx.x / kS1TileW) + k * (256 / kS1TileW);
             if (xd < p.work_w && yd < p.work_h) p.dst[static_cast<size_t>(yd) * p.dst_pitch + xd] = v[k];
         }
     }
@@ -227,6 +232,7 @@ Status make_s1_job(const ImageView& in, const ImageView& out, const ScaleInfo& s
     p->scale_area = 1.0f / (p->inv_sx * p->inv_sy);
     p->mode = determine_mode(in.width, in.height, out.width, out.height);
     p->tiles_x = (out.width + kS1TileW - 1) / kS1TileW;
+    p->tile_h = s1_tile_rows();
     p->tile_begin = 0;
     return Status::OK;
 }
@@ -235,7 +241,15 @@ hipError_t launch_s1(const S1Job* table, int count, long long total, const S1Job
                      const dev::Replay<long long>* replay) {
     if (total <= 0) return hipSuccess;
     dev::JobSpan<S1Job> span{table, count, single ? *single : S1Job{}, replay ? replay->count : nullptr};
-    s1_kernel<<<persistent_grid(total), 256, 0, stream>>>(span, dev::Total(total, replay ? replay->totals : nullptr));
+    // Jobs take their tile height from the knob when they are made, and the
+    // engines make them just before launching.
+    const unsigned grid = persistent_grid(total, jobs::Stage::S1);
+    const dev::Total work(total, replay ? replay->totals : nullptr);
+    switch (s1_tile_rows()) {
+        case 16: s1_kernel<16><<<grid, 256, 0, stream>>>(span, work); break;
+        case 32: s1_kernel<32><<<grid, 256, 0, stream>>>(span, work); break;
+        default: s1_kernel<64><<<grid, 256, 0, stream>>>(span, work); break;
+    }
     return hipGetLastError();
 }
 
@@ -244,7 +258,7 @@ hipError_t launch_gradient(const GradJob* table, int count, long long total, con
     if (total <= 0) return hipSuccess;
     dev::JobSpan<GradJob> span{table, count, single ? *single : GradJob{}, replay ? replay->count : nullptr};
     gradient_clear_jobs<<<count, 64, 0, stream>>>(span);
-    gradient_kernel<<<persistent_grid(total), 256, 0, stream>>>(span, dev::Total(total, replay ? replay->totals : nullptr));
+    gradient_kernel<<<persistent_grid(total, jobs::Stage::S1), 256, 0, stream>>>(span, dev::Total(total, replay ? replay->totals : nullptr));
     gradient_normalize_jobs<<<count, 64, 0, stream>>>(span);
     return hipGetLastError();
 }
@@ -302,7 +316,7 @@ Status gradient(const ImageView* views, int n, void* ws, double* const* pinned, 
         GradJob g{views[i].data, views[i].pitch_bytes, views[i].width, views[i].height, accum + i, out + i,
                   (views[i].width + kS1TileW - 1) / kS1TileW, 0};
         const long long tiles = static_cast<long long>(g.tiles_x) * ((g.h + kS1TileH - 1) / kS1TileH);
-        gradient_kernel<<<persistent_grid(tiles), 256, 0, stream>>>(dev::JobSpan<GradJob>{nullptr, 1, g}, tiles);
+        gradient_kernel<<<persistent_grid(tiles, jobs::Stage::S1), 256, 0, stream>>>(dev::JobSpan<GradJob>{nullptr, 1, g}, tiles);
     }
     gradient_normalize<<<1, 64, 0, stream>>>(accum, views[0].width, views[0].height, views[n - 1].width,
                                              views[n - 1].height, n, out);

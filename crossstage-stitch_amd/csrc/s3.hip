@@ -319,10 +319,10 @@ hipError_t launch_s3(const S3Job* table, int count, const S3Totals& t, const S3J
         return dev::Total(t.*m, replay ? &(replay->totals->*m) : nullptr);
     };
     if (t.mark > 0)
-        s3_mark<<<persistent_grid(t.mark), kS3MarkBlock, 0, stream>>>(span, total(&S3Totals::mark));
-    if (t.blur > 0) s3_blur<<<persistent_grid(t.blur), 256, 0, stream>>>(span, total(&S3Totals::blur));
+        s3_mark<<<persistent_grid(t.mark, jobs::Stage::S3), kS3MarkBlock, 0, stream>>>(span, total(&S3Totals::mark));
+    if (t.blur > 0) s3_blur<<<persistent_grid(t.blur, jobs::Stage::S3), 256, 0, stream>>>(span, total(&S3Totals::blur));
     if (t.kp > 0)
-        s3_describe<<<persistent_grid(t.kp, 16), 64 * kS3KeypointsPerBlock, 0, stream>>>(span, total(&S3Totals::kp));
+        s3_describe<<<persistent_grid(t.kp, jobs::Stage::S3, 16), 64 * kS3KeypointsPerBlock, 0, stream>>>(span, total(&S3Totals::kp));
     return hipGetLastError();
 }
 

@@ -363,12 +363,24 @@ void s4_set_segment(int rows) {
 int s4_segment() { return g_segment; }
 
 namespace {
-int g_workgroups = 8;
+int g_workgroups[kStages] = {8, 8, 8, 8, 8, 8};
+int g_s1_rows = kS1TileH;
 }
 void set_workgroups_per_cu(int eighths) {
-    if (eighths >= 1 && eighths <= 64) g_workgroups = eighths;
+    for (int s = 0; s < kStages; ++s) set_workgroups_per_cu(static_cast<Stage>(s), eighths);
 }
-int workgroups_per_cu() { return g_workgroups; }
+void set_workgroups_per_cu(Stage stage, int eighths) {
+    const int s = static_cast<int>(stage);
+    if (s >= 0 && s < kStages && eighths >= 1 && eighths <= 64) g_workgroups[s] = eighths;
+}
+int workgroups_per_cu(Stage stage) {
+    const int s = static_cast<int>(stage);
+    return s >= 0 && s < kStages ? g_workgroups[s] : 8;
+}
+void s1_set_tile_rows(int rows) {
+    if (rows == 16 || rows == 32 || rows == 64) g_s1_rows = rows;
+}
+int s1_tile_rows() { return g_s1_rows; }
 
 // Offsets: forward, reverse, rev_first, part_fwd, part_col.
 std::size_t s4_layout_bytes(int cap_a, int cap_b, bool cc, int segment, std::size_t off[5]) {
@@ -461,15 +473,15 @@ hipError_t launch_s4(const S4Job* table, int count, const S4Totals& t, const S4J
     // Tables may mix jobs with and without cross-check; the kernel tests each job.
     const bool any_cross = single ? static_cast<bool>(single->cross_check) : true;
     if (t.dist > 0) {
-        if (any_cross) s4_dist<true><<<persistent_grid(t.dist), 256, 0, stream>>>(span, total(&S4Totals::dist));
-        else s4_dist<false><<<persistent_grid(t.dist), 256, 0, stream>>>(span, total(&S4Totals::dist));
-        s4_qmerge<<<persistent_grid(t.qmerge), 256, 0, stream>>>(span, total(&S4Totals::qmerge));
-        if (t.cmerge > 0) s4_cmerge<<<persistent_grid(t.cmerge), 256, 0, stream>>>(span, total(&S4Totals::cmerge));
+        if (any_cross) s4_dist<true><<<persistent_grid(t.dist, jobs::Stage::S4), 256, 0, stream>>>(span, total(&S4Totals::dist));
+        else s4_dist<false><<<persistent_grid(t.dist, jobs::Stage::S4), 256, 0, stream>>>(span, total(&S4Totals::dist));
+        s4_qmerge<<<persistent_grid(t.qmerge, jobs::Stage::S4), 256, 0, stream>>>(span, total(&S4Totals::qmerge));
+        if (t.cmerge > 0) s4_cmerge<<<persistent_grid(t.cmerge, jobs::Stage::S4), 256, 0, stream>>>(span, total(&S4Totals::cmerge));
     }
     if (t.generic_fwd > 0)
-        s4_generic<<<persistent_grid(t.generic_fwd), 256, 0, stream>>>(span, total(&S4Totals::generic_fwd), 0, nullptr);
+        s4_generic<<<persistent_grid(t.generic_fwd, jobs::Stage::S4), 256, 0, stream>>>(span, total(&S4Totals::generic_fwd), 0, nullptr);
     if (t.generic_rev > 0)
-        s4_generic<<<persistent_grid(t.generic_rev), 256, 0, stream>>>(span, total(&S4Totals::generic_rev), 1, nullptr);
+        s4_generic<<<persistent_grid(t.generic_rev, jobs::Stage::S4), 256, 0, stream>>>(span, total(&S4Totals::generic_rev), 1, nullptr);
     s4_compact<<<count, 1024, 0, stream>>>(span);
     return hipGetLastError();
 }
@@ -509,7 +521,7 @@ Status s4_knn_top2(const Keypoints& a, const Descriptors& da, const Keypoints& b
     j.dim = da.dim;
     j.binary = da.type == DescriptorType::BINARY_U8;
     const long long tiles = (j.cap_a + 255) / 256;
-    s4_generic<<<persistent_grid(tiles), 256, 0, stream>>>(dev::JobSpan<S4Job>{nullptr, 1, j}, tiles, 0, output);
+    s4_generic<<<persistent_grid(tiles, jobs::Stage::S4), 256, 0, stream>>>(dev::JobSpan<S4Job>{nullptr, 1, j}, tiles, 0, output);
     return hipGetLastError() == hipSuccess ? Status::OK : Status::CUDA_ERROR;
 }
 

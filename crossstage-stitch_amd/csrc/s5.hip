@@ -728,8 +728,8 @@ hipError_t launch_s5(const S5Job* table, int count, const S5Totals& t, const S5J
         return dev::Total(t.*m, replay ? &(replay->totals->*m) : nullptr);
     };
     s5_prepare<<<count, 256, 0, stream>>>(span);
-    if (t.gen > 0) s5_generate<<<persistent_grid(t.gen), 256, 0, stream>>>(span, total(&S5Totals::gen));
-    if (t.score > 0) s5_score<<<persistent_grid(t.score, 16), 256, 0, stream>>>(span, total(&S5Totals::score));
+    if (t.gen > 0) s5_generate<<<persistent_grid(t.gen, jobs::Stage::S5), 256, 0, stream>>>(span, total(&S5Totals::gen));
+    if (t.score > 0) s5_score<<<persistent_grid(t.score, jobs::Stage::S5, 16), 256, 0, stream>>>(span, total(&S5Totals::score));
     s5_select<<<count, 256, 0, stream>>>(span);
     s5_finish<<<count, 256, 0, stream>>>(span);
     return hipGetLastError();

@@ -1682,34 +1682,34 @@ hipError_t launch_s6(const S6Job* table, int count, const S6Totals& t, const S6J
     if (any_stats) timed(LaunchProbe::kClear, 0, [&] { s6_clear_stats<<<count, 64, 0, stream>>>(span); });
     if (t.warp > 0)
         timed(LaunchProbe::kWarp, 0, [&] {
-            s6_warp<<<persistent_grid(t.warp), kBlock, 0, stream>>>(span, total(&t.warp, d ? &d->warp : nullptr));
+            s6_warp<<<persistent_grid(t.warp, jobs::Stage::S6), kBlock, 0, stream>>>(span, total(&t.warp, d ? &d->warp : nullptr));
         });
     for (int l = 1; l <= t.top; ++l) {
         if (t.pull[l] <= 0) continue;
         const dev::Total w = total(&t.pull[l], d ? &d->pull[l] : nullptr);
         timed(LaunchProbe::kPull, l, [&] {
-            if (l == 1) s6_reduce<2><<<persistent_grid(t.pull[l]), kBlock, 0, stream>>>(span, w, l);
-            else s6_reduce<0><<<persistent_grid(t.pull[l]), kBlock, 0, stream>>>(span, w, l);
+            if (l == 1) s6_reduce<2><<<persistent_grid(t.pull[l], jobs::Stage::S6), kBlock, 0, stream>>>(span, w, l);
+            else s6_reduce<0><<<persistent_grid(t.pull[l], jobs::Stage::S6), kBlock, 0, stream>>>(span, w, l);
         });
     }
     for (int l = t.top; l >= 2; --l)  // level 1 is pushed inside s6_gauss1
         if (t.push[l] > 0)
             timed(LaunchProbe::kPush, l, [&] {
-                s6_push<<<persistent_grid(t.push[l]), kBlock, 0, stream>>>(
+                s6_push<<<persistent_grid(t.push[l], jobs::Stage::S6), kBlock, 0, stream>>>(
                     span, total(&t.push[l], d ? &d->push[l] : nullptr), l);
             });
     for (int l = 1; l <= t.blend_top; ++l) {
         if (t.gauss[l] <= 0) continue;
         const dev::Total w = total(&t.gauss[l], d ? &d->gauss[l] : nullptr);
         timed(LaunchProbe::kGauss, l, [&] {
-            if (l == 1) s6_gauss1<<<persistent_grid(t.gauss[l]), kBlock, 0, stream>>>(span, w);
-            else s6_reduce<1><<<persistent_grid(t.gauss[l]), kBlock, 0, stream>>>(span, w, l);
+            if (l == 1) s6_gauss1<<<persistent_grid(t.gauss[l], jobs::Stage::S6), kBlock, 0, stream>>>(span, w);
+            else s6_reduce<1><<<persistent_grid(t.gauss[l], jobs::Stage::S6), kBlock, 0, stream>>>(span, w, l);
         });
     }
     for (int l = t.blend_top; l >= 0; --l)
         if (t.mix[l] > 0)
             timed(LaunchProbe::kMix, l, [&] {
-                s6_mix<<<persistent_grid(t.mix[l]), kBlock, 0, stream>>>(
+                s6_mix<<<persistent_grid(t.mix[l], jobs::Stage::S6), kBlock, 0, stream>>>(
                     span, total(&t.mix[l], d ? &d->mix[l] : nullptr), l);
             });
     return hipGetLastError();

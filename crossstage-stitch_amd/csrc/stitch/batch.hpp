@@ -272,8 +272,11 @@ struct BatchTimings {
 
 struct Tuning {
     int s4_segment{128};
-    int workgroups_per_cu{8};
+    int workgroups_per_cu{8};  // every stage's persistent grids, unless stage_workgroups overrides it
     bool graphs{true};
+    // Per stage S1..S6: workgroups per CU in eighths of the default (0: workgroups_per_cu).
+    int stage_workgroups[6]{0, 0, 0, 0, 0, 0};
+    int s1_tile_rows{64};  // output rows per S1 tile: 16, 32 or 64
 };
 
 class BatchRunner {

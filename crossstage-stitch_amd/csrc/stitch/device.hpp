@@ -200,7 +200,7 @@ __device__ inline int wave_prefix(bool pred, int* total) {
 
 // Grid size for a persistent launch over `tiles` tiles of `threads` threads:
 // enough workgroups to fill every CU a few times, never more than there are tiles.
-inline unsigned persistent_grid(long long tiles, int per_cu = 8) {
+inline unsigned persistent_grid(long long tiles, stitch::jobs::Stage stage, int per_cu = 8) {
     static int cus = 0;
     if (!cus) {
         int dev = 0;
@@ -209,7 +209,7 @@ inline unsigned persistent_grid(long long tiles, int per_cu = 8) {
             cus = prop.multiProcessorCount;
         if (cus <= 0) cus = 256;
     }
-    long long cap = static_cast<long long>(cus) * per_cu * stitch::jobs::workgroups_per_cu() / 8;
+    long long cap = static_cast<long long>(cus) * per_cu * stitch::jobs::workgroups_per_cu(stage) / 8;
     if (cap < cus) cap = cus;
     const long long g = tiles < cap ? tiles : cap;
     return static_cast<unsigned>(g > 0 ? g : 1);

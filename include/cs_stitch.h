@@ -317,7 +317,14 @@ typedef struct {
     double total_ms;
     double algorithmic_bytes;
 } cs_kernel_probe;
-typedef struct { int s4_segment, workgroups_per_cu, graphs; } cs_tuning;
+/* Launch shapes: the S4 train segment, workgroups per CU (eighths of each
+ * kernel's default) for every stage and, where non-zero, per stage S1..S6,
+ * graph replay, and the S1 tile height (16, 32 or 64 rows). */
+typedef struct {
+    int s4_segment, workgroups_per_cu, graphs;
+    int stage_workgroups_per_cu[6];
+    int s1_tile_rows;
+} cs_tuning;
 
 int cs_batch_create(int max_pairs, uint64_t pool_bytes, int use_graphs, cs_batch** out);
 void cs_batch_destroy(cs_batch* batch);

@@ -132,13 +132,17 @@ def test_batch_independent_of_graphs_tuning_and_company():
     br = B.BatchRunner(max_pairs=8, pool_bytes=6 << 30)
     ref, _ = run_batch(br, pairs, cfgs)
     other = make_pairs(3, seed0=7300, sizes=[(1280, 720)])
-    for graphs, seg, wpc in ((False, 128, 8), (True, 64, 4), (True, 512, 16), (True, 128, 8)):
-        br.apply(s4_segment=seg, workgroups_per_cu=wpc, graphs=graphs)
+    # (graph replay, S4 segment, workgroups per CU, per-stage workgroups S1..S6, S1 tile rows)
+    shapes = ((False, 128, 8, (0,) * 6, 64), (True, 64, 4, (0,) * 6, 64), (True, 512, 16, (0,) * 6, 32),
+              (True, 256, 8, (4, 32, 16, 8, 24, 4), 16), (False, 128, 8, (32, 4, 4, 32, 8, 16), 16),
+              (True, 128, 8, (0,) * 6, 64))
+    for graphs, seg, wpc, stages, rows in shapes:
+        br.apply(s4_segment=seg, workgroups_per_cu=wpc, graphs=graphs, stage_workgroups=stages, s1_tile_rows=rows)
         run_batch(br, other, [knobs(0)] * 3)  # different work between replays
         got, _ = run_batch(br, pairs, cfgs)
         for i in range(6):
             assert got[i][0] == ref[i][0]
-            same(ref[i][1], got[i][1], f"graphs={graphs} seg={seg} wpc={wpc} pair {i}")
+            same(ref[i][1], got[i][1], f"graphs={graphs} seg={seg} wpc={wpc} stages={stages} rows={rows} pair {i}")
     alone, _ = run_batch(br, pairs[2:3], cfgs[2:3])
     same(ref[2][1], alone[0][1], "alone")
     br.close()
