@@ -215,6 +215,11 @@ SIGNATURES = {
     "cs_s6_workspace_bytes": (C.c_int, [P(Geometry), C.c_int, P(sz)]),
     "cs_s6_render": (C.c_int, [P(Image), P(Image), P(Geometry), C.c_int, P(Image), P(Image), vp, sz, vp,
                                P(RenderStatistics)]),
+    "cs_s6_prepare_chain": (C.c_int, [C.c_int, P(C.c_int), P(C.c_int), P(C.c_double), P(GeometryPolicy),
+                                      P(Geometry)]),
+    "cs_s6_chain_workspace_bytes": (C.c_int, [C.c_int, P(Geometry), C.c_int, P(sz)]),
+    "cs_s6_render_chain": (C.c_int, [C.c_int, P(Image), P(Geometry), C.c_int, P(Image), P(Image), vp, sz, vp,
+                                     P(RenderStatistics)]),
     "cs_pyramid_level_counts": (C.c_int, [P(PyramidConfig), P(C.c_int)]),
     "cs_pyramid_workspace_bytes": (C.c_int, [C.c_int, C.c_int, C.c_double, P(PyramidConfig), P(sz)]),
     "cs_pyramid_detect_describe": (C.c_int, [P(Image), C.c_double, P(PyramidConfig), P(Keypoints), P(Descriptors),
@@ -251,6 +256,7 @@ SIGNATURES = {
                                               P(C.c_double)]),
     "cs_synth_pair": (C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, vp, vp,
                                 P(C.c_double)]),
+    "cs_synth_strip": (C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, vp, P(C.c_double)]),
     "cs_batch_create": (C.c_int, [C.c_int, C.c_uint64, C.c_int, P(vp)]),
     "cs_batch_destroy": (None, [vp]),
     "cs_batch_run": (C.c_int, [vp, P(PairSpec), C.c_int, P(PairOutcome), P(BatchTimings)]),

@@ -24,6 +24,15 @@ def synth_pair(seed: int, width: int, height: int, overlap=(0.3, 0.5), kind: int
     return a, b, np.array(h[:]).reshape(3, 3)
 
 
+def synth_strip(seed: int, n: int, width: int, height: int, overlap: float = 0.4, kind: int = TEXTURED):
+    """(frames, h_adj): n uint8 (H, W, 3) frames and the n - 1 true homographies
+    frame k + 1 -> frame k."""
+    frames = np.empty((n, height, width, 3), np.uint8)
+    h = (C.c_double * max(1, 9 * (n - 1)))()
+    _check(F.load().cs_synth_strip(seed, n, width, height, overlap, kind, frames.ctypes.data, h), "synth_strip")
+    return frames, [np.array(h[9 * k: 9 * k + 9]).reshape(3, 3) for k in range(n - 1)]
+
+
 def harness_config(S=1.0, K=1000, R=256, blend="feather", fast_threshold=20, harris_k=0.04, ratio=0.8,
                    cross_check=False, batch=64, refinements=1, threshold=3.0, seed=0, pyramid_levels=1,
                    pyramid_scale_factor=1.2) -> F.HarnessConfig:

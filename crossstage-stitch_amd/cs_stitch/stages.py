@@ -287,6 +287,39 @@ def s6_render(a: torch.Tensor, b: torch.Tensor, g: F.Geometry, mode: int, stats:
     return out, mask, counters
 
 
+def prepare_chain(sizes, h_adj, policy: F.GeometryPolicy | None = None):
+    """Canvas of a chain of frames: sizes [(w, h)] per frame, h_adj the n - 1
+    homographies frame k + 1 -> frame k. Returns (status, [Geometry] * n)."""
+    n = len(sizes)
+    g = (F.Geometry * n)()
+    ws = (C.c_int * n)(*[int(w) for w, _ in sizes])
+    hs = (C.c_int * n)(*[int(h) for _, h in sizes])
+    flat = [float(v) for h in h_adj for v in (h.ravel() if hasattr(h, "ravel") else h)]
+    harr = (C.c_double * max(1, len(flat)))(*flat)
+    st = F.load().cs_s6_prepare_chain(n, ws, hs, harr, C.byref(policy) if policy else None, g)
+    return st, list(g)
+
+
+def s6_render_chain(frames, geoms, mode: int, stats: bool = True, pad: int = 64):
+    """N-frame render of device RGB8 frames onto the chain canvas: (out, mask, counters)."""
+    L = F.load()
+    n = len(frames)
+    g = (F.Geometry * n)(*geoms)
+    nb = C.c_size_t()
+    _check(L.cs_s6_chain_workspace_bytes(n, g, mode, C.byref(nb)), "s6_chain_workspace_bytes")
+    ws = workspace(nb.value)
+    out = pitched(g[0].height, g[0].width, 3, pad)
+    mask = pitched(g[0].height, g[0].width, 1, pad)
+    counters = torch.zeros(2, dtype=torch.int64, device="cuda") if stats else None
+    views = (F.Image * n)(*[image(f) for f in frames])
+    io, im = image(out), image(mask)
+    _check(L.cs_s6_render_chain(n, views, g, mode, C.byref(io), C.byref(im), ws.data_ptr(), nb.value,
+                                stream_handle(),
+                                C.cast(counters.data_ptr(), C.POINTER(F.RenderStatistics)) if stats else None),
+           "s6_render_chain")
+    return out, mask, counters
+
+
 # ---------------------------------------------------------------- controller observations
 
 def extract_scene(ga: torch.Tensor, gb: torch.Tensor, relations: bool = True):

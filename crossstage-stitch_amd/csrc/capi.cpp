@@ -252,6 +252,45 @@ int cs_s6_render(const cs_image* a, const cs_image* b, const cs_geometry* g, int
     });
 }
 
+int cs_s6_prepare_chain(int n, const int* widths, const int* heights, const double* h_adj,
+                        const cs_geometry_policy* p, cs_geometry* out) {
+    if (!out || n < 1 || n > s6::kMaxChainFrames) return CS_INVALID_ARGUMENT;
+    s6::GeometryPolicy pol{};
+    pol.max_canvas_pixels = s6::kChainCanvasPixels;
+    if (p) {
+        pol.denominator_tolerance = p->denominator_tolerance;
+        pol.inverse_tolerance = p->inverse_tolerance;
+        pol.snap_tolerance = p->snap_tolerance;
+        pol.max_canvas_pixels = static_cast<std::size_t>(p->max_canvas_pixels);
+    }
+    s6::Geometry g[s6::kMaxChainFrames]{};
+    const Status r = s6::prepare_chain_geometry(n, widths, heights, h_adj, pol, g);
+    for (int k = 0; k < n; ++k) put_geo(g[k], &out[k]);
+    return static_cast<int>(r);
+}
+
+int cs_s6_chain_workspace_bytes(int n, const cs_geometry* g, int mode, size_t* bytes) {
+    if (!g || n < 1 || n > s6::kMaxChainFrames) return CS_INVALID_ARGUMENT;
+    s6::Geometry v[s6::kMaxChainFrames]{};
+    for (int k = 0; k < n; ++k) v[k] = geo(&g[k]);
+    return guard([&] { return s6::chain_workspace_bytes(n, v, static_cast<s6::BlendMode>(mode), bytes); });
+}
+
+int cs_s6_render_chain(int n, const cs_image* frames, const cs_geometry* g, int mode, const cs_image* out,
+                       const cs_image* mask, void* ws, size_t bytes, void* stream, cs_render_statistics* stats) {
+    if (!frames || !g || !out || !mask || n < 1 || n > s6::kMaxChainFrames) return CS_INVALID_ARGUMENT;
+    s6::Geometry v[s6::kMaxChainFrames]{};
+    ImageView f[s6::kMaxChainFrames]{};
+    for (int k = 0; k < n; ++k) {
+        v[k] = geo(&g[k]);
+        f[k] = img(&frames[k]);
+    }
+    return guard([&] {
+        return s6::render_chain(n, f, v, static_cast<s6::BlendMode>(mode), img(out), img(mask), ws, bytes,
+                                st(stream), reinterpret_cast<s6::RenderStatistics*>(stats));
+    });
+}
+
 static pyramid::Config pyramid_config(const cs_pyramid_config* c) {
     return {c->levels, c->scale_factor, c->fast_threshold, c->max_keypoints, c->harris_k};
 }

@@ -229,4 +229,69 @@ Status prepare_geometry(int wa, int ha, int wb, int hb, const double* h, const G
     return Status::OK;
 }
 
+// Chain canvas (no counterpart in the reference, which stitches pairs).
+// Frame k reaches frame 0 through G_k = G_{k-1} h_adj[k-1], kept normalised
+// by its largest coefficient (FP64, products summed left to right). Each
+// frame's geometry against frame 0 comes from prepare_geometry with an
+// unbounded budget; the chain canvas is the union of those canvases (each
+// already holds frame 0), and only the union must fit the policy's budget.
+Status prepare_chain_geometry(int n, const int* widths, const int* heights, const double* h_adj,
+                              const GeometryPolicy& policy, Geometry* out) {
+    if (!out || n < 1 || n > kMaxChainFrames || !widths || !heights || (n > 1 && !h_adj))
+        return Status::INVALID_ARGUMENT;
+    for (int k = 0; k < n; ++k) {
+        out[k] = Geometry{};
+        if (widths[k] <= 0 || heights[k] <= 0) return Status::INVALID_ARGUMENT;
+    }
+    GeometryPolicy unbounded = policy;
+    unbounded.max_canvas_pixels = std::numeric_limits<std::size_t>::max();
+    double to0[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    long long left = 0, top = 0, right = widths[0], bottom = heights[0];  // half-open box in frame-0 pixels
+    for (int k = 1; k < n; ++k) {
+        const double* step = h_adj + 9 * (k - 1);
+        double next[9];
+        double peak = 0;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double acc = 0;
+                for (int i = 0; i < 3; ++i) acc += to0[3 * r + i] * step[3 * i + c];
+                next[3 * r + c] = acc;
+                if (!std::isfinite(acc)) return Status::DEGENERATE;
+                peak = std::max(peak, std::fabs(acc));
+            }
+        if (peak == 0) return Status::DEGENERATE;
+        for (int i = 0; i < 9; ++i) to0[i] = next[i] / peak;
+        const Status s = prepare_geometry(widths[0], heights[0], widths[k], heights[k], to0, unbounded, &out[k]);
+        if (s != Status::OK) return s;
+        left = std::min<long long>(left, out[k].origin_x);
+        top = std::min<long long>(top, out[k].origin_y);
+        right = std::max<long long>(right, static_cast<long long>(out[k].origin_x) + out[k].width);
+        bottom = std::max<long long>(bottom, static_cast<long long>(out[k].origin_y) + out[k].height);
+    }
+    const long long cols = right - left, rows = bottom - top;
+    if (cols > std::numeric_limits<int>::max() || rows > std::numeric_limits<int>::max() ||
+        static_cast<std::size_t>(cols) > policy.max_canvas_pixels / static_cast<std::size_t>(rows)) {
+        for (int k = 0; k < n; ++k) out[k] = Geometry{};
+        return Status::INSUFFICIENT_WORKSPACE;
+    }
+    for (int k = 0; k < n; ++k) {
+        Geometry& g = out[k];
+        g.width = static_cast<int>(cols);
+        g.height = static_cast<int>(rows);
+        g.origin_x = static_cast<int>(left);
+        g.origin_y = static_cast<int>(top);
+        g.width_a = widths[0];
+        g.height_a = heights[0];
+        g.width_b = widths[k];
+        g.height_b = heights[k];
+        g.denominator_tolerance = policy.denominator_tolerance;
+        g.snap_tolerance = policy.snap_tolerance;
+        if (k == 0) {
+            std::fill(std::begin(g.inverse), std::end(g.inverse), 0.0);
+            g.inverse[0] = g.inverse[4] = g.inverse[8] = 1;
+        }
+    }
+    return Status::OK;
+}
+
 }  // namespace stitch::s6

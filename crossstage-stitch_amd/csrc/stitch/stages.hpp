@@ -134,6 +134,25 @@ Status s6_workspace_bytes(const Geometry& geometry, BlendMode mode, std::size_t*
 Status render(const ImageView& a, const ImageView& b, const Geometry& geometry, BlendMode mode,
               const ImageView& output, const ImageView& mask, void* workspace, std::size_t workspace_bytes,
               hipStream_t stream, RenderStatistics* statistics = nullptr);
+
+// Chains of frames onto one canvas (no counterpart in the reference, which
+// renders pairs). h_adj holds n - 1 homographies, h_adj[9k..] mapping frame
+// k + 1 to frame k. out[k] (n entries) share the chain canvas and frame 0's
+// size; out[k].inverse maps frame-0-original to frame-k-original
+// coordinates (the identity for frame 0). The default chain budget is
+// kChainCanvasPixels; pass a policy to change it.
+constexpr int kMaxChainFrames = 64;
+constexpr std::size_t kChainCanvasPixels = std::size_t{1} << 28;
+Status prepare_chain_geometry(int n, const int* widths, const int* heights, const double* h_adj,
+                              const GeometryPolicy& policy, Geometry* out);
+// N-frame render: frame 0 at its integer offset, frames 1.. through their
+// inverse maps, feather or multiband blending generalised from the pair
+// render (ownership to the highest border score, lowest frame on ties; band
+// mix over every frame's pyramids). frames[k] is RGB8 of g[k]'s frame size.
+Status chain_workspace_bytes(int n, const Geometry* g, BlendMode mode, std::size_t* bytes);
+Status render_chain(int n, const ImageView* frames, const Geometry* g, BlendMode mode, const ImageView& output,
+                    const ImageView& mask, void* workspace, std::size_t workspace_bytes, hipStream_t stream,
+                    RenderStatistics* statistics = nullptr);
 }  // namespace s6
 
 }  // namespace stitch

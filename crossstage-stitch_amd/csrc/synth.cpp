@@ -12,6 +12,7 @@
 // All randomness comes from one SplitMix64 stream seeded per pair, so every
 // process produces the same bytes.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -225,5 +226,85 @@ extern "C" int cs_synth_pair(std::uint64_t seed, int width, int height, double o
             }
         }
     if (h_ba) std::memcpy(h_ba, h, sizeof h);
+    return CS_OK;
+}
+
+// A horizontal strip of n frames: frame k + 1 is seen from frame k through a
+// random homography drawn like a pair's B (rotation, scale, perspective,
+// vertical shift) whose horizontal shift puts `overlap` of frame k under
+// frame k + 1. Frame 0 is the world crop at the origin and the world covers
+// every frame's footprint, with the pair generator's texture density per
+// frame area; frames 1.. get the noise and brightness offset of a pair's B.
+// h_adj[9k..9k+8] maps frame k + 1 to frame k.
+extern "C" int cs_synth_strip(std::uint64_t seed, int n, int width, int height, double overlap, int kind,
+                              std::uint8_t* frames, double* h_adj) {
+    if (n < 1 || n > 64 || width < 8 || height < 8 || !frames || !(overlap > 0) || overlap >= 1 || kind < 0 ||
+        kind > 2)
+        return CS_INVALID_ARGUMENT;
+    Rng rng{seed};
+    const double W = width, H = height;
+    const double pscale = std::min(1.0, 1920.0 / W);
+    std::vector<std::array<double, 9>> to0(n);  // frame k -> frame 0
+    to0[0] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int k = 1; k < n; ++k) {
+        const double th = rng.uniform(-5, 5) * 3.141592653589793 / 180.0, s = rng.uniform(0.9, 1.1);
+        const double p1 = rng.uniform(-1e-4, 1e-4) * pscale, p2 = rng.uniform(-1e-4, 1e-4) * pscale;
+        const double ty = rng.uniform(-0.05, 0.05) * H;
+        double h[9] = {s * std::cos(th), -s * std::sin(th), 0, s * std::sin(th), s * std::cos(th), ty, p1, p2, 1};
+        auto covered = [&](double tx) {
+            h[2] = tx;
+            std::vector<P2> q = {apply(h, 0, 0), apply(h, W, 0), apply(h, W, H), apply(h, 0, H)};
+            return clipped_area(q, W, H) / (W * H);
+        };
+        double lo = -0.5 * W, hi = 1.2 * W;
+        for (int it = 0; it < 80; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            if (covered(mid) > overlap) lo = mid;
+            else hi = mid;
+        }
+        covered(0.5 * (lo + hi));
+        if (h_adj) std::memcpy(h_adj + 9 * (k - 1), h, sizeof h);
+        const auto& g = to0[k - 1];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double acc = 0;
+                for (int i = 0; i < 3; ++i) acc += g[3 * r + i] * h[3 * i + c];
+                to0[k][3 * r + c] = acc;
+            }
+    }
+    double minx = 0, miny = 0, maxx = W, maxy = H;
+    for (int k = 1; k < n; ++k)
+        for (P2 c : {apply(to0[k].data(), 0, 0), apply(to0[k].data(), W, 0), apply(to0[k].data(), W, H),
+                     apply(to0[k].data(), 0, H)}) {
+            minx = std::min(minx, c.x);
+            miny = std::min(miny, c.y);
+            maxx = std::max(maxx, c.x);
+            maxy = std::max(maxy, c.y);
+        }
+    World wd;
+    wd.x0 = static_cast<int>(std::floor(minx)) - 2;
+    wd.y0 = static_cast<int>(std::floor(miny)) - 2;
+    wd.w = static_cast<int>(std::ceil(maxx)) + 3 - wd.x0;
+    wd.h = static_cast<int>(std::ceil(maxy)) + 3 - wd.y0;
+    render_world(wd, rng, W * H, kind);
+    const std::size_t frame_bytes = static_cast<std::size_t>(width) * height * 3;
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x)
+            for (int c = 0; c < 3; ++c) frames[(static_cast<std::size_t>(y) * width + x) * 3 + c] = clamp8(wd.at(x, y, c));
+    for (int k = 1; k < n; ++k) {
+        std::uint8_t* out = frames + frame_bytes * k;
+        const double offset = rng.uniform(-10, 10);
+        for (int y = 0; y < height; ++y)
+            for (int x = 0; x < width; ++x) {
+                const P2 p = apply(to0[k].data(), x, y);
+                const int x0 = static_cast<int>(std::floor(p.x)), y0 = static_cast<int>(std::floor(p.y));
+                const double fx = p.x - x0, fy = p.y - y0;
+                for (int c = 0; c < 3; ++c) {
+                    const double v = wd.at(x0, y0, c) * (1 - fx) * (1 - fy) + wd.at(x0 + 1, y0, c) * fx * (1 - fy) +
+                                     wd.at(x0, y0 + 1, c) * (1 - fx) * fy + wd.at(x0 + 1, y0 + 1, c) * fx * fy;
+                    out[(static_cast<std::size_t>(y) * width + x) * 3 + c] = clamp8(v + 2.0 * rng.normal() + offset);
+                }
+            }
+    }
     return CS_OK;
 }

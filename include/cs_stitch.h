@@ -139,6 +139,24 @@ int cs_s6_render(const cs_image* a, const cs_image* b, const cs_geometry* geomet
                  const cs_image* output, const cs_image* mask, void* workspace, size_t workspace_bytes,
                  void* stream, cs_render_statistics* statistics);
 
+/* S6 over a chain of n frames (1..64; no counterpart in the reference, which
+ * renders pairs; the pair render's blend generalised to n frames, equal to
+ * cs_s6_render for two). h_adj holds n - 1 homographies, h_adj[9k..] mapping
+ * frame k + 1 to frame k. out (n entries) share the chain canvas and frame
+ * 0's size; out[k].inverse maps frame-0-original to frame-k-original
+ * coordinates. policy may be NULL: default tolerances and a canvas budget of
+ * 2^28 pixels. Frames 1.. are sampled through their inverse maps; ownership
+ * goes to the highest border score (lowest frame on ties), feather averages
+ * by score, multiband mixes every frame's bands. render_chain uploads two
+ * small tables from host memory, so unlike cs_s6_render it is not meant for
+ * graph capture. frames[k] is RGB8 of out[k].width_b x height_b. */
+int cs_s6_prepare_chain(int n, const int* widths, const int* heights, const double* h_adj,
+                        const cs_geometry_policy* policy, cs_geometry* out);
+int cs_s6_chain_workspace_bytes(int n, const cs_geometry* geometry, int mode, size_t* bytes);
+int cs_s6_render_chain(int n, const cs_image* frames, const cs_geometry* geometry, int mode,
+                       const cs_image* output, const cs_image* mask, void* workspace, size_t workspace_bytes,
+                       void* stream, cs_render_statistics* statistics);
+
 /* Multi-level detection (no counterpart in the reference, whose S2/S3 are
  * single-level): level l is S1 of the RGB input at scale S / f^l, detected
  * and described with S2/S3 on its share of K (OpenCV's ORB split: geometric
@@ -419,6 +437,12 @@ int cs_server_release(cs_server* server, uint64_t id);
  * kind 0 textured, 1 low texture, 2 repetitive; h_ba = true H (B -> A). */
 int cs_synth_pair(uint64_t seed, int width, int height, double overlap_lo, double overlap_hi, int kind,
                   uint8_t* rgb_a, uint8_t* rgb_b, double* h_ba);
+/* Seeded synthetic strip of n frames (1..64; no counterpart in the
+ * reference): frames holds n * width * height * 3 bytes, frame k + 1 sees
+ * `overlap` (0..1) of frame k; h_adj (may be NULL) gets the n - 1 true
+ * adjacent homographies, h_adj[9k..] mapping frame k + 1 to frame k. */
+int cs_synth_strip(uint64_t seed, int n, int width, int height, double overlap, int kind, uint8_t* frames,
+                   double* h_adj);
 
 #ifdef __cplusplus
 }

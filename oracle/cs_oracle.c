@@ -1207,6 +1207,249 @@ int orc_s6_render(const uint8_t* a, int pitch_a, const uint8_t* b, int pitch_b, 
 }
 
 /* ======================================================================
+ * S6 over a chain of frames (no counterpart in the reference, which renders
+ * pairs): frame 0 sits on the canvas at an integer offset, as A does, and
+ * every other frame k is sampled through its own inverse map g[k].inverse
+ * (frame-0-original -> frame-k-original). The pair algorithm of
+ * orc_s6_render generalised to n frames:
+ *   - ownership goes to the valid frame with the largest border score, the
+ *     lowest index on ties (for two frames: A if s_a >= s_b);
+ *   - feather: sum_k s_k c_k / sum_k s_k over the valid frames, summed in
+ *     frame order;
+ *   - multiband: every frame's pull-push extension and Gaussian pyramid as
+ *     in the pair render, ownership pyramids per frame, and the band mix
+ *     sum_k U_k L_k / sum_k U_k with the numerator fused from the last frame
+ *     down (for two frames: fmaf(U_a, L_a, U_b L_b)) and the denominator
+ *     summed in frame order;
+ *   - single-source pixels keep their frame's colour, overlap (two or more
+ *     frames valid) takes the collapsed mix.
+ * For n = 2 every expression is the pair render's, so the result equals
+ * orc_s6_render bit for bit (checked by the tests); beyond two frames the
+ * blend is this build's definition. All g[k] share the canvas (width,
+ * height, origin) and frame 0's size (width_a, height_a); g[k].width_b /
+ * height_b give frame k's size. out / mask as in orc_s6_render.
+ * ====================================================================== */
+/* Canvas of a chain of n frames (no counterpart in the reference): h holds
+ * the n - 1 adjacent homographies, h[k] mapping frame k + 1 to frame k.
+ * Frame k maps to frame 0 through G_k = G_{k-1} h[k-1] (G_0 = I), each
+ * product normalised by its largest coefficient; frame k's pairwise
+ * geometry with frame 0 is orc_s6_prepare_geometry of G_k (unbounded
+ * budget), and the chain canvas is the union of those boxes, at most
+ * max_pixels. out[k] gets the common canvas, frame 0's size, frame k's size
+ * and frame k's inverse map (the identity for k = 0). */
+int orc_s6_prepare_chain(int n, const int* w, const int* hgt, const double* h, double den_tol, double inv_tol,
+                         double snap_tol, unsigned long long max_pixels, orc_geometry* out) {
+    if (n < 1 || n > 64) return ORC_INVALID_ARGUMENT;
+    double G[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    long long x0 = 0, y0 = 0, x1 = w[0], y1 = hgt[0];
+    for (int k = 0; k < n; ++k) {
+        memset(&out[k], 0, sizeof out[k]);
+        if (w[k] <= 0 || hgt[k] <= 0) return ORC_INVALID_ARGUMENT;
+    }
+    for (int k = 1; k < n; ++k) {
+        double P[9], peak = 0;
+        const double* a = h + 9 * (k - 1);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double acc = 0;
+                for (int i = 0; i < 3; ++i) acc += G[3 * r + i] * a[3 * i + c];
+                P[3 * r + c] = acc;
+            }
+        for (int i = 0; i < 9; ++i) {
+            if (!isfinite(P[i])) return ORC_DEGENERATE;
+            peak = fmax(peak, fabs(P[i]));
+        }
+        if (peak == 0) return ORC_DEGENERATE;
+        for (int i = 0; i < 9; ++i) G[i] = P[i] / peak;
+        const int st = orc_s6_prepare_geometry(w[0], hgt[0], w[k], hgt[k], G, den_tol, inv_tol, snap_tol,
+                                               ~0ull, &out[k]);
+        if (st != ORC_OK) return st;
+        x0 = out[k].origin_x < x0 ? out[k].origin_x : x0;
+        y0 = out[k].origin_y < y0 ? out[k].origin_y : y0;
+        x1 = out[k].origin_x + (long long)out[k].width > x1 ? out[k].origin_x + (long long)out[k].width : x1;
+        y1 = out[k].origin_y + (long long)out[k].height > y1 ? out[k].origin_y + (long long)out[k].height : y1;
+    }
+    const unsigned long long cw = (unsigned long long)(x1 - x0), ch = (unsigned long long)(y1 - y0);
+    if (x1 - x0 > 2147483647LL || y1 - y0 > 2147483647LL || cw * ch > max_pixels) return ORC_INSUFFICIENT_WORKSPACE;
+    for (int k = 0; k < n; ++k) {
+        out[k].width = (int)cw;
+        out[k].height = (int)ch;
+        out[k].origin_x = (int)x0;
+        out[k].origin_y = (int)y0;
+        out[k].width_a = w[0];
+        out[k].height_a = hgt[0];
+        out[k].width_b = w[k];
+        out[k].height_b = hgt[k];
+        out[k].denominator_tolerance = den_tol;
+        out[k].snap_tolerance = snap_tol;
+        if (k == 0) {
+            memset(out[0].inverse, 0, sizeof out[0].inverse);
+            out[0].inverse[0] = out[0].inverse[4] = out[0].inverse[8] = 1;
+        }
+    }
+    return ORC_OK;
+}
+
+int orc_s6_render_chain(int n, const uint8_t* const* img, const int* pitch, const orc_geometry* g, int mode,
+                        uint8_t* out, uint8_t* mask, unsigned long long* stats) {
+    const int requested = orc_s6_requested_reductions(mode);
+    if (requested < 0 || n < 1 || n > 64) return ORC_INVALID_ARGUMENT;
+    const int W = g[0].width, H = g[0].height;
+    const size_t np = (size_t)W * H;
+    float* c0 = (float*)calloc(np * 3 * (size_t)n, sizeof(float));  /* level-0 colours, per frame */
+    float* v0 = (float*)calloc(np * (size_t)n, sizeof(float));      /* level-0 validity */
+    float* u0 = (float*)calloc(np * (size_t)n, sizeof(float));      /* level-0 ownership */
+    int* count = (int*)calloc(np, sizeof(int));
+    unsigned long long uni = 0, ovl = 0;
+    for (size_t p = 0; p < np; ++p) {
+        const int x = (int)(p % W), y = (int)(p / W);
+        const double xa = (double)x + g[0].origin_x, ya = (double)y + g[0].origin_y;
+        double best = 0, fs = 0, fc[3] = {0, 0, 0};
+        int owner = -1, valid = 0;
+        float single[3] = {0, 0, 0};
+        for (int k = 0; k < n; ++k) {
+            float rgb[3] = {0, 0, 0};
+            double s = 0;
+            int m = 0;
+            if (k == 0) {
+                m = sample_rgb(img[0], g[0].width_a, g[0].height_a, pitch[0], xa, ya, rgb, &s, g[0].snap_tolerance);
+            } else {
+                const double* inv = g[k].inverse;
+                const double den = inv[6] * xa + inv[7] * ya + inv[8];
+                const double dsc = fabs(inv[6] * xa) + fabs(inv[7] * ya) + fabs(inv[8]);
+                if (isfinite(den) && dsc > 0 && fabs(den) > g[k].denominator_tolerance * dsc) {
+                    const double xb = (inv[0] * xa + inv[1] * ya + inv[2]) / den;
+                    const double yb = (inv[3] * xa + inv[4] * ya + inv[5]) / den;
+                    m = sample_rgb(img[k], g[k].width_b, g[k].height_b, pitch[k], xb, yb, rgb, &s,
+                                   g[k].snap_tolerance);
+                }
+            }
+            if (!m) continue;
+            ++valid;
+            if (owner < 0 || s > best) {
+                owner = k;
+                best = s;
+            }
+            for (int c = 0; c < 3; ++c) {
+                c0[((size_t)k * np + p) * 3 + c] = rgb[c];
+                fc[c] = valid == 1 ? s * rgb[c] : fc[c] + s * rgb[c];
+                single[c] = rgb[c];
+            }
+            fs = valid == 1 ? s : fs + s;
+            v0[(size_t)k * np + p] = 1.0f;
+        }
+        if (owner >= 0) u0[(size_t)owner * np + p] = 1.0f;
+        count[p] = valid;
+        mask[p] = valid ? 1 : 0;
+        uni += valid > 0;
+        ovl += valid > 1;
+        if (mode == 0)
+            for (int c = 0; c < 3; ++c)
+                out[3 * p + c] = quantize(valid > 1 ? (float)(fc[c] / fs) : single[c]);
+    }
+    if (stats) { stats[0] = uni; stats[1] = ovl; }
+    if (mode == 0) {
+        free(c0); free(v0); free(u0); free(count);
+        return ORC_OK;
+    }
+    int lw[40], lh[40], top = 0;
+    lw[0] = W; lh[0] = H;
+    while (lw[top] > 1 || lh[top] > 1) {
+        lw[top + 1] = lw[top] / 2 + lw[top] % 2;
+        lh[top + 1] = lh[top] / 2 + lh[top] % 2;
+        ++top;
+    }
+    const int bt = requested < top ? requested : top;
+    /* Per frame k and level l: colour C (3), validity V, ownership U (l <= bt). */
+    float** C = (float**)calloc((size_t)n * 40, sizeof(float*));
+    float** V = (float**)calloc((size_t)n * 40, sizeof(float*));
+    float** U = (float**)calloc((size_t)n * 40, sizeof(float*));
+    float* Z[40];
+    for (int k = 0; k < n; ++k) {
+        C[k * 40] = c0 + (size_t)k * np * 3;
+        V[k * 40] = v0 + (size_t)k * np;
+        U[k * 40] = u0 + (size_t)k * np;
+        for (int l = 1; l <= top; ++l) {
+            const size_t m = (size_t)lw[l] * lh[l];
+            C[k * 40 + l] = (float*)malloc(m * 12);
+            V[k * 40 + l] = (float*)malloc(m * 4);
+            U[k * 40 + l] = l <= bt ? (float*)malloc(m * 4) : NULL;
+        }
+    }
+    for (int l = 0; l <= bt; ++l) Z[l] = (float*)malloc((size_t)lw[l] * lh[l] * 12);
+    for (int k = 0; k < n; ++k) {
+        float** Ck = C + k * 40;
+        float** Vk = V + k * 40;
+        float** Uk = U + k * 40;
+        for (int l = 1; l <= top; ++l) {
+            reduce_level(Ck[l - 1], lw[l - 1], lh[l - 1], Ck[l], lw[l], lh[l], 3);
+            reduce_level(Vk[l - 1], lw[l - 1], lh[l - 1], Vk[l], lw[l], lh[l], 1);
+        }
+        for (int l = 0; l <= top; ++l) {
+            const size_t m = (size_t)lw[l] * lh[l];
+            for (size_t p = 0; p < m; ++p)
+                for (int c = 0; c < 3; ++c) Ck[l][3 * p + c] = Vk[l][p] > 0 ? Ck[l][3 * p + c] / Vk[l][p] : 0;
+        }
+        for (int l = top - 1; l >= 0; --l)
+            for (int y = 0; y < lh[l]; ++y)
+                for (int x = 0; x < lw[l]; ++x) {
+                    const size_t p = (size_t)y * lw[l] + x;
+                    if (!(Vk[l][p] > 0))
+                        for (int c = 0; c < 3; ++c) Ck[l][3 * p + c] = expanded(Ck[l + 1], lw[l + 1], lh[l + 1], 3, x, y, c);
+                }
+        for (int l = 1; l <= bt; ++l) {
+            reduce_level(Ck[l - 1], lw[l - 1], lh[l - 1], Ck[l], lw[l], lh[l], 3);
+            reduce_level(Uk[l - 1], lw[l - 1], lh[l - 1], Uk[l], lw[l], lh[l], 1);
+        }
+    }
+    for (int l = 0; l <= bt; ++l) {
+        const int coarse = l < bt;
+        for (int y = 0; y < lh[l]; ++y)
+            for (int x = 0; x < lw[l]; ++x) {
+                const size_t p = (size_t)y * lw[l] + x;
+                float total = U[l][p];
+                for (int k = 1; k < n; ++k) total += U[k * 40 + l][p];
+                for (int c = 0; c < 3; ++c) {
+                    float acc = 0;
+                    for (int k = n - 1; k >= 0; --k) {
+                        const float* Ck = C[k * 40 + l];
+                        const float lk = Ck[3 * p + c] -
+                                         (coarse ? expanded(C[k * 40 + l + 1], lw[l + 1], lh[l + 1], 3, x, y, c) : 0);
+                        const float uk = U[k * 40 + l][p];
+                        acc = k == n - 1 ? uk * lk : fmaf(uk, lk, acc);
+                    }
+                    Z[l][3 * p + c] = total > 0 ? acc / total : 0;
+                }
+            }
+    }
+    for (int l = bt - 1; l >= 0; --l)
+        for (int y = 0; y < lh[l]; ++y)
+            for (int x = 0; x < lw[l]; ++x) {
+                const size_t p = (size_t)y * lw[l] + x;
+                for (int c = 0; c < 3; ++c) Z[l][3 * p + c] += expanded(Z[l + 1], lw[l + 1], lh[l + 1], 3, x, y, c);
+            }
+    for (size_t p = 0; p < np; ++p) {
+        int only = -1;
+        for (int k = 0; k < n && only < 0; ++k)
+            if (v0[(size_t)k * np + p] > 0) only = k;
+        for (int c = 0; c < 3; ++c) {
+            const float color = count[p] > 1 ? Z[0][3 * p + c] : (only >= 0 ? c0[((size_t)only * np + p) * 3 + c] : 0);
+            out[3 * p + c] = quantize(color);
+        }
+    }
+    for (int k = 0; k < n; ++k)
+        for (int l = 1; l <= top; ++l) {
+            free(C[k * 40 + l]);
+            free(V[k * 40 + l]);
+            free(U[k * 40 + l]);
+        }
+    for (int l = 0; l <= bt; ++l) free(Z[l]);
+    free(C); free(V); free(U);
+    free(c0); free(v0); free(u0); free(count);
+    return ORC_OK;
+}
+
+/* ======================================================================
  * Controller observations: scene and gate features
  * (src/controller/features.cu:11-171, src/controller/relational_features.cu:6-190)
  *

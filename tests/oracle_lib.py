@@ -106,6 +106,10 @@ def lib():
         L.orc_s6_effective_reductions.argtypes = [C.c_int, C.c_int, C.c_int]
         L.orc_s6_render.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(Geometry), C.c_int, u8p, u8p,
                                     C.POINTER(C.c_ulonglong)]
+        L.orc_s6_prepare_chain.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                           C.c_double, C.c_double, C.c_double, C.c_ulonglong, C.POINTER(Geometry)]
+        L.orc_s6_render_chain.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int), C.POINTER(Geometry),
+                                          C.c_int, u8p, u8p, C.POINTER(C.c_ulonglong)]
         vp = C.c_void_p
         L.orc_scene_features.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_int,
                                          C.c_int, C.c_int, vp]
@@ -237,6 +241,36 @@ def prepare_geometry(wa, ha, wb, hb, h, den_tol=1e-12, inv_tol=1e-12, snap_tol=1
 
 
 MODES = {"feather": 0, "multiband_3": 1, "multiband_5": 2, "multiband_6": 3, "multiband_7": 4}
+
+
+def prepare_chain(sizes, hs, max_pixels=16777216, den_tol=1e-12, inv_tol=1e-12, snap_tol=1e-10):
+    """Canvas of a chain: sizes [(w, h)] of n frames, hs the n - 1 adjacent
+    homographies (hs[k]: frame k + 1 -> frame k). (status, [Geometry] * n)."""
+    n = len(sizes)
+    w = (C.c_int * n)(*[s[0] for s in sizes])
+    h = (C.c_int * n)(*[s[1] for s in sizes])
+    flat = [float(v) for m in hs for v in np.asarray(m, np.float64).ravel()]
+    harr = (C.c_double * max(len(flat), 1))(*flat)
+    out = (Geometry * n)()
+    st = lib().orc_s6_prepare_chain(n, w, h, harr, den_tol, inv_tol, snap_tol, max_pixels, out)
+    return st, list(out)
+
+
+def s6_render_chain(frames, geoms, mode):
+    """frames: n (h, w, 3) uint8 arrays; geoms from prepare_chain."""
+    n = len(frames)
+    frames = [np.ascontiguousarray(f) for f in frames]
+    g0 = geoms[0]
+    out = np.zeros((g0.height, g0.width, 3), np.uint8)
+    mask = np.zeros((g0.height, g0.width), np.uint8)
+    stats = (C.c_ulonglong * 2)()
+    ptrs = (C.c_void_p * n)(*[f.ctypes.data for f in frames])
+    pitches = (C.c_int * n)(*[f.shape[1] * 3 for f in frames])
+    garr = (Geometry * n)(*[Geometry.from_buffer_copy(bytes(g)) for g in geoms])  # any same-layout struct
+    st = lib().orc_s6_render_chain(n, ptrs, pitches, garr, MODES[mode] if isinstance(mode, str) else mode, out, mask,
+                                   stats)
+    assert st == 0
+    return out, mask, (stats[0], stats[1])
 
 
 def s6_render(a: np.ndarray, b: np.ndarray, g: Geometry, mode: int):
