@@ -2,7 +2,7 @@
 the plain per-pair GPU path, on one MI355X. Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-    python bench.py --stage s4|s1      # one stage alone (profiling runs)
+    python bench.py --stage s4|s1|strip  # one stage, or the strip, alone
 
 Workload (value): the batched form on a ragged batch of 64 synthetic pairs
 - resolutions drawn from {640x480, 1280x720, 1920x1080, 3840x2160}, keypoint
@@ -23,7 +23,10 @@ replays Poisson arrival traces with the same size mix and textured,
 low-texture, low-overlap and repetitive scenes under a 64 GB HBM budget:
 once with the adaptive legacy-v1 controller, once with fixed knobs (the
 batch's), and saturated (every request at t = 0) so that the budget binds;
-`pairs_per_s` there counts OK completions.
+`pairs_per_s` there counts OK completions. "strip" stitches 8 frames of
+3840x2160 in a row (30 % overlap, K 16000, RANSAC 8192, 7 bands) onto one
+canvas of about 22000 x 2450: features once per frame, S4/S5 per adjacent
+pair, one chain render.
 
 "roofline": the S6 stage of one batch (one render-graph replay: every S6
 kernel of every canvas) against the 8 TB/s HBM peak, with the compulsory
@@ -88,9 +91,10 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 1 GiB device copy (the reference kernel for PMC byte counters)")
-    ap.add_argument("--stage", choices=["s4", "s1"], default=None,
+    ap.add_argument("--no-strip", action="store_true", help="skip the 8-frame 3840x2160 strip")
+    ap.add_argument("--stage", choices=["s4", "s1", "strip"], default=None,
                     help="time one stage alone: s4 = 16000 x 16000 cross-checked Hamming match, "
-                         "s1 = gray conversion of a 3840x2160 pair")
+                         "s1 = gray conversion of a 3840x2160 pair, strip = the 8-frame 3840x2160 strip")
     return ap.parse_args()
 
 
@@ -465,6 +469,77 @@ def stage_bench(kind, steps, warmup):
                 frac=round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4))
 
 
+STRIP_FRAMES, STRIP_W, STRIP_H, STRIP_OVERLAP = 8, 3840, 2160, 0.3
+STRIP_K, STRIP_R, STRIP_BLEND = 16000, 8192, "multiband_7"
+
+
+def strip_bench(steps, warmup):
+    """The large-frame strip: 8 frames of 3840x2160 in a row, 30 % of each
+    frame under the next, K 16000, cross-checked Hamming, RANSAC 8192 and a
+    7-band blend of every frame onto one canvas. Features once per frame, S4
+    and S5 per adjacent pair, one chain render; wall-clock per strip, with the
+    event-timed phases beside it."""
+    import torch
+    from cs_stitch import pipeline as P
+    from cs_stitch import stages as S
+    n, w, h = STRIP_FRAMES, STRIP_W, STRIP_H
+    frames, truth = P.synth_strip(SEED0, n, w, h, STRIP_OVERLAP)
+    dev = []
+    for f in frames:
+        t = S.pitched(h, w, 3)
+        t.copy_(torch.from_numpy(f).cuda())
+        dev.append(t)
+    del frames
+    torch.cuda.synchronize()
+    cfg = P.harness_config(S=1.0, K=STRIP_K, R=STRIP_R, blend=STRIP_BLEND, cross_check=True)
+    strip = P.StripStitcher(n, w, h, max_k=STRIP_K, max_r=STRIP_R)
+    for _ in range(max(1, warmup)):
+        strip.run(dev, cfg)
+    recs, status = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st, rec, pairs, counts = strip.run(dev, cfg)
+        recs.append(rec)
+        status.append(st)
+    total = time.perf_counter() - t0
+    rec = recs[-1]
+    canvas = rec.canvas_width * rec.canvas_height
+    # Median distance of estimated and true images over each pair's overlap
+    # (a 64-pixel grid of frame k + 1 that lands in frame k).
+    err = []
+    for k, p in enumerate(pairs):
+        if p.status != 0:
+            err.append(None)
+            continue
+        hm, d = np.array(p.H).reshape(3, 3), []
+        for y in range(0, h, 64):
+            for x in range(0, w, 64):
+                q = truth[k] @ np.array([x, y, 1.0])
+                if 0 <= q[0] / q[2] < w and 0 <= q[1] / q[2] < h:
+                    e = hm @ np.array([x, y, 1.0])
+                    d.append(float(np.hypot(*(e[:2] / e[2] - q[:2] / q[2]))))
+        err.append(round(float(np.median(d)), 3))
+    mean = lambda f: round(float(np.mean([getattr(r, f) for r in recs])), 3)  # noqa: E731
+    render_ms = float(np.mean([r.t_render for r in recs]))
+    render_bytes = 3 * n * w * h + 4 * canvas
+    strip.close()
+    del dev
+    torch.cuda.empty_cache()
+    return dict(
+        workload=f"{n} frames of {w}x{h} in a strip, overlap {STRIP_OVERLAP}, K {STRIP_K}, cross-check, "
+                 f"RANSAC {STRIP_R}, {STRIP_BLEND} onto one canvas", status=status[-1],
+        fail_stage=rec.fail_stage.decode(), strips_per_s=round(steps / total, 3),
+        ms_per_strip=round(1e3 * total / steps, 2), t_features_ms=mean("t_features"), t_match_ms=mean("t_match"),
+        t_estimate_ms=mean("t_estimate"), t_geometry_ms=mean("t_geometry"), t_render_ms=round(render_ms, 3),
+        canvas=f"{rec.canvas_width}x{rec.canvas_height}", canvas_mp=round(canvas / 1e6, 2),
+        union_pixels=int(rec.union_pixels), overlap_pixels=int(rec.overlap_pixels),
+        keypoints=counts, inliers=[p.num_inliers for p in pairs], overlap_error_px=err,
+        render_algorithmic_bytes=render_bytes, render_bytes_formula="3 * sum(frame pixels) + 4 * canvas pixels",
+        render_gbs=round(render_bytes / render_ms / 1e6, 1) if rec.status == 0 else None,
+        render_frac=round(render_bytes / render_ms / 1e6 / HBM_PEAK_GBS, 4) if rec.status == 0 else None,
+        steps=steps)
+
+
 def children():
     """Processes whose parent is this one (left-behind workers show here)."""
     me, out = os.getpid(), []
@@ -497,6 +572,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.stage == "strip":
+        print(json.dumps(strip_bench(args.steps, args.warmup)))
+        return
     if args.stage:
         r = stage_bench(args.stage, max(args.steps, 10), max(args.warmup, 3))
         print(json.dumps(r))
@@ -589,6 +667,9 @@ def main():
         sdev = [upload(*synth(SEED0 + 1000 + i, w, h, ov, kind)[:2]) for i, (w, h, kind, ov, _) in enumerate(pool)]
         torch.cuda.synchronize()
         result["served"] = run_serving(sdev, pool, args.serve_scale)
+    if not args.no_strip:
+        result["strip"] = strip_bench(args.steps, 1)
+        log("strip", result["strip"])
     cpu = cpu_single = None
     if not args.no_cpu:
         cpu_single = cpu_baseline(pairs, work, args.cpu_seconds)

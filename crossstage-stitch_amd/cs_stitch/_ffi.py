@@ -108,6 +108,14 @@ class AttemptRecord(C.Structure):
                 ("has_decision", C.c_int), ("has_homography", C.c_int), ("has_feedback", C.c_int)]
 
 
+class StripRecord(C.Structure):
+    _fields_ = [("status", C.c_int), ("fail_stage", C.c_char * 16), ("fail_pair", C.c_int),
+                ("canvas_width", C.c_int), ("canvas_height", C.c_int), ("origin_x", C.c_int), ("origin_y", C.c_int),
+                ("union_pixels", C.c_uint64), ("overlap_pixels", C.c_uint64),
+                ("t_features", C.c_double), ("t_match", C.c_double), ("t_estimate", C.c_double),
+                ("t_geometry", C.c_double), ("t_render", C.c_double), ("t_total", C.c_double)]
+
+
 class RunRecord(C.Structure):
     _fields_ = [("S", C.c_double), ("K", C.c_int), ("R", C.c_int), ("blend_mode", C.c_int),
                 ("t_s1", C.c_double), ("t_s2", C.c_double), ("t_s3", C.c_double), ("t_s4", C.c_double),
@@ -227,6 +235,12 @@ SIGNATURES = {
     "cs_extract_scene": (C.c_int, [P(Image), P(Image), vp, vp, vp, C.c_int]),
     "cs_extract_gate": (C.c_int, [P(Image), P(Image), P(Keypoints), P(Keypoints), P(Matches), vp,
                                   P(HomographyResult), P(ScaleInfo), P(ScaleInfo), vp, vp, vp, vp, C.c_uint]),
+    "cs_strip_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(vp)]),
+    "cs_strip_destroy": (None, [vp]),
+    "cs_strip_run": (C.c_int, [vp, C.c_int, P(Image), P(HarnessConfig), P(StripRecord), P(HomographyResult),
+                               P(C.c_int)]),
+    "cs_strip_outputs": (C.c_int, [vp, P(Image), P(Image), P(Geometry), C.c_int]),
+    "cs_strip_stream": (vp, [vp]),
     "cs_runner_create": (vp, []),
     "cs_runner_destroy": (None, [vp]),
     "cs_runner_initialize": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,

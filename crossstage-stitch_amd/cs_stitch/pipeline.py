@@ -178,6 +178,44 @@ class PipelineRunner:
             pass
 
 
+class StripStitcher:
+    """n frames in a row onto one canvas (cs_strip_*): features once per frame,
+    S4/S5 per adjacent pair, one chain render. Frames are device RGB8 tensors."""
+
+    def __init__(self, max_frames, max_width, max_height, max_k=16000, max_r=8192):
+        h = C.c_void_p()
+        _check(F.load().cs_strip_create(max_frames, max_width, max_height, max_k, max_r, C.byref(h)), "strip.create")
+        self.handle = h.value
+
+    def run(self, frames, cfg: F.HarnessConfig):
+        """(status, record, [HomographyResult] * (n - 1), keypoint counts)."""
+        n = len(frames)
+        views = (F.Image * n)(*[image(f) for f in frames])
+        rec = F.StripRecord()
+        pairs = (F.HomographyResult * (n - 1))()
+        counts = (C.c_int * n)()
+        st = F.load().cs_strip_run(self.handle, n, views, C.byref(cfg), C.byref(rec), pairs, counts)
+        return st, rec, list(pairs), list(counts)
+
+    def outputs(self, n):
+        """Device views of the last canvas and mask, and the n frame geometries."""
+        canvas, mask = F.Image(), F.Image()
+        g = (F.Geometry * n)()
+        _check(F.load().cs_strip_outputs(self.handle, C.byref(canvas), C.byref(mask), g, n), "strip.outputs")
+        return canvas, mask, list(g)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            F.load().cs_strip_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except (AttributeError, TypeError):
+            pass
+
+
 _hip = None
 
 

@@ -1,5 +1,6 @@
 // extern "C" layer for the runner, the controller and the synthetic inputs.
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -7,6 +8,7 @@
 #include "cs_stitch.h"
 #include "stitch/controller.hpp"
 #include "stitch/runner.hpp"
+#include "stitch/strip.hpp"
 
 using namespace stitch;
 
@@ -19,6 +21,14 @@ struct cs_controller {
     controller::ControllerPolicy policy;
     controller::Controller impl;
 };
+
+struct cs_strip {
+    strip::StripStitcher impl;
+    cs_strip(int f, int w, int h, int k, int r) : impl(f, w, h, k, r) {}
+};
+
+static_assert(sizeof(cs_geometry) == sizeof(s6::Geometry), "Geometry layout");
+static_assert(sizeof(cs_homography_result) == sizeof(HomographyResult), "HomographyResult layout");
 
 namespace {
 
@@ -405,5 +415,69 @@ int cs_controller_walk_features(cs_controller* c, double grad_a, double grad_b, 
     }
     return rows;
 }
+
+int cs_strip_create(int max_frames, int max_w, int max_h, int max_k, int max_r, cs_strip** out) {
+    if (!out) return CS_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (max_frames < 2 || max_frames > s6::kMaxChainFrames || max_w < 8 || max_h < 8 || max_k < 1 || max_r < 1)
+        return CS_INVALID_ARGUMENT;
+    try {
+        *out = new cs_strip(max_frames, max_w, max_h, max_k, max_r);
+        return CS_OK;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+void cs_strip_destroy(cs_strip* s) { delete s; }
+
+int cs_strip_run(cs_strip* s, int n, const cs_image* frames, const cs_harness_config* c, cs_strip_record* rec,
+                 cs_homography_result* pairs, int* keypoints) {
+    if (!s || !frames || !c || !rec || !pairs || n < 2 || n > s6::kMaxChainFrames) return CS_INVALID_ARGUMENT;
+    std::memset(rec, 0, sizeof *rec);
+    try {
+        ImageView views[s6::kMaxChainFrames]{};
+        for (int k = 0; k < n; ++k)
+            views[k] = {frames[k].data, frames[k].width, frames[k].height, frames[k].pitch_bytes,
+                        static_cast<PixelFormat>(frames[k].format)};
+        strip::StripRecord r{};
+        s->impl.run(n, views, config(c), &r, reinterpret_cast<HomographyResult*>(pairs), keypoints);
+        rec->status = static_cast<int>(r.status);
+        std::snprintf(rec->fail_stage, sizeof rec->fail_stage, "%s", r.fail_stage);
+        rec->fail_pair = r.fail_pair;
+        rec->canvas_width = r.canvas_width;
+        rec->canvas_height = r.canvas_height;
+        rec->origin_x = r.origin_x;
+        rec->origin_y = r.origin_y;
+        rec->union_pixels = r.union_pixels;
+        rec->overlap_pixels = r.overlap_pixels;
+        rec->t_features = r.t_features;
+        rec->t_match = r.t_match;
+        rec->t_estimate = r.t_estimate;
+        rec->t_geometry = r.t_geometry;
+        rec->t_render = r.t_render;
+        rec->t_total = r.t_total;
+        return rec->status;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+int cs_strip_outputs(const cs_strip* s, cs_image* canvas, cs_image* mask, cs_geometry* geometry, int n) {
+    if (!s) return CS_INVALID_ARGUMENT;
+    const auto put = [](const ImageView& v, cs_image* o) {
+        if (o) *o = {v.data, v.width, v.height, v.pitch_bytes, static_cast<int>(v.format)};
+    };
+    put(s->impl.canvas(), canvas);
+    put(s->impl.mask(), mask);
+    const auto& g = s->impl.geometry();
+    if (geometry) {
+        if (n != static_cast<int>(g.size())) return CS_INVALID_ARGUMENT;
+        std::memcpy(geometry, g.data(), g.size() * sizeof(cs_geometry));
+    }
+    return CS_OK;
+}
+
+void* cs_strip_stream(cs_strip* s) { return s ? s->impl.stream() : nullptr; }
 
 }  // extern "C"

@@ -230,9 +230,10 @@ extern "C" int cs_synth_pair(std::uint64_t seed, int width, int height, double o
 }
 
 // A horizontal strip of n frames: frame k + 1 is seen from frame k through a
-// random homography drawn like a pair's B (rotation, scale, perspective,
-// vertical shift) whose horizontal shift puts `overlap` of frame k under
-// frame k + 1. Frame 0 is the world crop at the origin and the world covers
+// random homography (rotation U[-1, 1] degrees, scale U[0.98, 1.02],
+// perspective terms U[-1e-5, 1e-5] at 1080p, vertical shift U[-2, 2] % of
+// the height; milder than a pair's because they compound along the chain)
+// whose horizontal shift puts `overlap` of frame k under frame k + 1. Frame 0 is the world crop at the origin and the world covers
 // every frame's footprint, with the pair generator's texture density per
 // frame area; frames 1.. get the noise and brightness offset of a pair's B.
 // h_adj[9k..9k+8] maps frame k + 1 to frame k.
@@ -247,9 +248,9 @@ extern "C" int cs_synth_strip(std::uint64_t seed, int n, int width, int height, 
     std::vector<std::array<double, 9>> to0(n);  // frame k -> frame 0
     to0[0] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     for (int k = 1; k < n; ++k) {
-        const double th = rng.uniform(-5, 5) * 3.141592653589793 / 180.0, s = rng.uniform(0.9, 1.1);
-        const double p1 = rng.uniform(-1e-4, 1e-4) * pscale, p2 = rng.uniform(-1e-4, 1e-4) * pscale;
-        const double ty = rng.uniform(-0.05, 0.05) * H;
+        const double th = rng.uniform(-1, 1) * 3.141592653589793 / 180.0, s = rng.uniform(0.98, 1.02);
+        const double p1 = rng.uniform(-1e-5, 1e-5) * pscale, p2 = rng.uniform(-1e-5, 1e-5) * pscale;
+        const double ty = rng.uniform(-0.02, 0.02) * H;
         double h[9] = {s * std::cos(th), -s * std::sin(th), 0, s * std::sin(th), s * std::cos(th), ty, p1, p2, 1};
         auto covered = [&](double tx) {
             h[2] = tx;

@@ -432,6 +432,35 @@ int cs_server_run_trace(cs_server* server, const cs_request* requests, int n, cs
                         cs_serve_stats* stats);
 int cs_server_release(cs_server* server, uint64_t id);
 
+/* Strip stitching (no counterpart in the reference, whose stitching stops at
+ * pairs; its large-frame stress driver harness/stress_multires.cu:81-200 runs
+ * each adjacent pair on its own canvas of at most 16,777,216 pixels): n
+ * frames in a row, S1-S3 once per frame, S4 and S5 per adjacent pair
+ * (pairs[k]: frame k + 1 -> frame k), the chain canvas from the composed
+ * homographies (budget 2^28 pixels) and one cs_s6_render_chain. Frames are
+ * device RGB8; config as for cs_runner_execute_run (single-level S2/S3).
+ * Frame, feature and match memory is allocated by cs_strip_create; the
+ * canvas and the render workspace grow on demand and stay until the next
+ * growth or cs_strip_destroy. cs_strip_run returns the outcome's status (the
+ * record says which stage and pair failed). */
+typedef struct cs_strip cs_strip;
+typedef struct {
+    int status;
+    char fail_stage[16];
+    int fail_pair;
+    int canvas_width, canvas_height, origin_x, origin_y;
+    uint64_t union_pixels, overlap_pixels;
+    double t_features, t_match, t_estimate, t_geometry, t_render, t_total; /* ms */
+} cs_strip_record;
+int cs_strip_create(int max_frames, int max_w, int max_h, int max_k, int max_r, cs_strip** out);
+void cs_strip_destroy(cs_strip* strip);
+int cs_strip_run(cs_strip* strip, int n, const cs_image* frames, const cs_harness_config* config,
+                 cs_strip_record* record, cs_homography_result* pairs, int* keypoints);
+/* Canvas, mask (device views) and the n frame geometries of the last run;
+ * any output may be NULL. */
+int cs_strip_outputs(const cs_strip* strip, cs_image* canvas, cs_image* mask, cs_geometry* geometry, int n);
+void* cs_strip_stream(cs_strip* strip);
+
 /* Seeded synthetic pair (no counterpart in the reference; bench and test
  * input): host RGB8 buffers of width*height*3, overlap drawn in [lo, hi],
  * kind 0 textured, 1 low texture, 2 repetitive; h_ba = true H (B -> A). */

@@ -17,6 +17,7 @@ STRUCTS = {
     "cs_pair_outcome": "PairOutcome", "cs_batch_timings": "BatchTimings", "cs_tuning": "Tuning",
     "cs_server_config": "ServerConfig", "cs_request": "Request", "cs_completion": "Completion",
     "cs_serve_stats": "ServeStats", "cs_kernel_probe": "KernelProbe", "cs_attempt_record": "AttemptRecord",
+    "cs_strip_record": "StripRecord",
 }
 
 
