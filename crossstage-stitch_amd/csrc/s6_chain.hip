@@ -36,8 +36,14 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kMaxLevels = 40;
-constexpr int kZMargin = 6;   // mix window margin, level pixels
-constexpr int kGTop = 8;      // Gaussian window margin at the blend top
+// Window margins in level pixels. A frame's ownership weight at level l is
+// zero outside its level-0 box scaled by 2^-l and widened by 2 (each reduce
+// widens the support by under one coarse pixel, and the widenings sum to
+// less than 2); the mix zone keeps one more. The Gaussian levels are read
+// over the zone, through the expansion by the mix one level finer (4) and by
+// the level-0 finish (2).
+constexpr int kZMargin = 3;
+constexpr int kGTop = 4;
 
 struct Box {  // half-open [x0, x1) x [y0, y1) in level pixels
     int x0, y0, x1, y1;
@@ -190,13 +196,21 @@ __global__ void __launch_bounds__(kThreads) chain_own(const Frames* fr, int W, i
                 o[c] = quantize(valid > 1 ? static_cast<float>(fc[c] / fs) : single[c]);
     }
     if (stats) {
+        // One atomic pair per block (the grid is capped at a few blocks per CU).
+        __shared__ unsigned long long part[2][kThreads / 64];
         for (int off = 32; off > 0; off >>= 1) {
             uni += __shfl_xor(uni, off, 64);
             ovl += __shfl_xor(ovl, off, 64);
         }
         if ((threadIdx.x & 63) == 0) {
-            if (uni) atomicAdd(&stats[0], uni);
-            if (ovl) atomicAdd(&stats[1], ovl);
+            part[0][threadIdx.x / 64] = uni;
+            part[1][threadIdx.x / 64] = ovl;
+        }
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            unsigned long long v = 0;
+            for (int w = 0; w < kThreads / 64; ++w) v += part[threadIdx.x][w];
+            if (v) atomicAdd(&stats[threadIdx.x], v);
         }
     }
 }
@@ -284,31 +298,70 @@ __global__ void __launch_bounds__(kThreads) chain_push(Plane pull, Plane dst, Pl
     }
 }
 
-// Filled level-0 colour of a frame: its own colour where valid, else the
-// expansion of its filled level 1.
-__device__ __forceinline__ float filled0(const Level0& f, const Plane& f1, int w1, int h1, int x, int y, int c) {
+// Filled level-0 colour of a frame (3 channels): its own colour where valid,
+// else the expansion of its filled level 1 (each channel summed in the
+// expansion's tap order).
+__device__ __forceinline__ void filled0(const Level0& f, const Plane& f1, int w1, int h1, int x, int y, float* v) {
     if (f.box.has(x, y)) {
         const size_t j = static_cast<size_t>(y - f.box.y0) * f.box.w() + (x - f.box.x0);
-        if (f.valid[j]) return f.color[j * 3 + c];
+        if (f.valid[j]) {
+            for (int c = 0; c < 3; ++c) v[c] = f.color[j * 3 + c];
+            return;
+        }
     }
-    return expand_at(f1, w1, h1, x, y, c);
+    float sum[3] = {0, 0, 0};
+    for (int dy = -2; dy <= 2; ++dy) {
+        const int yy = reflect101(y + dy, 2 * h1);
+        if (yy & 1) continue;
+        for (int dx = -2; dx <= 2; ++dx) {
+            const int xx = reflect101(x + dx, 2 * w1);
+            if (xx & 1) continue;
+            const float t = tap(dx) * tap(dy);
+            for (int c = 0; c < 3; ++c) sum[c] = fmaf(t, f1.at(xx / 2, yy / 2, c), sum[c]);
+        }
+    }
+    for (int c = 0; c < 3; ++c) v[c] = 4 * sum[c];
 }
 
-// Gaussian level 1 of the filled frame and its ownership weight.
+// Gaussian level 1 of the filled frame and its ownership weight. A block
+// makes a kG1W x kG1H tile of level-1 pixels from the (2 kG1W + 3) x
+// (2 kG1H + 3) level-0 pixels the 5x5 reduce reads, staged once in LDS
+// (filled colour, and ownership as 0/1), so each filled value is formed once
+// per tile instead of once per tap. Input positions are reflected into the
+// canvas as they are staged; positions past what the tile's real outputs
+// read are clamped (their values are never used).
+constexpr int kG1W = 64, kG1H = 8;
+constexpr int kG1InW = 2 * kG1W + 3, kG1InH = 2 * kG1H + 3;
+
 __global__ void __launch_bounds__(kThreads) chain_gauss1(Level0 f, int k, const std::int8_t* owner, int W0, int H0,
                                                         Plane f1, int w1, int h1, Plane g, Plane u) {
-    GRID_STRIDE(i, g.b.n()) {
-        const int x = g.b.x0 + static_cast<int>(i % g.b.w()), y = g.b.y0 + static_cast<int>(i / g.b.w());
+    __shared__ float s_c[3][kG1InH][kG1InW];
+    __shared__ float s_o[kG1InH][kG1InW];
+    const int tiles_x = (g.b.w() + kG1W - 1) / kG1W;
+    const int ox0 = g.b.x0 + static_cast<int>(blockIdx.x % tiles_x) * kG1W;
+    const int oy0 = g.b.y0 + static_cast<int>(blockIdx.x / tiles_x) * kG1H;
+    const int ix0 = 2 * ox0 - 2, iy0 = 2 * oy0 - 2;
+    for (int i = threadIdx.x; i < kG1InW * kG1InH; i += blockDim.x) {
+        const int lx = i % kG1InW, ly = i / kG1InW;
+        const int xx = reflect101(min(ix0 + lx, W0 + 1), W0), yy = reflect101(min(iy0 + ly, H0 + 1), H0);
+        float v[3];
+        filled0(f, f1, w1, h1, xx, yy, v);
+        for (int c = 0; c < 3; ++c) s_c[c][ly][lx] = v[c];
+        s_o[ly][lx] = owner[static_cast<size_t>(yy) * W0 + xx] == k ? 1.0f : 0.0f;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kG1W * kG1H; i += blockDim.x) {
+        const int lx = i % kG1W, ly = i / kG1W;
+        const int x = ox0 + lx, y = oy0 + ly;
+        if (x >= g.b.x1 || y >= g.b.y1) continue;
         float v[3] = {0, 0, 0}, o = 0;
-        for (int dy = -2; dy <= 2; ++dy) {
-            const int yy = reflect101(2 * y + dy, H0);
+        for (int dy = -2; dy <= 2; ++dy)
             for (int dx = -2; dx <= 2; ++dx) {
-                const int xx = reflect101(2 * x + dx, W0);
                 const float t = tap(dx) * tap(dy);
-                for (int c = 0; c < 3; ++c) v[c] = fmaf(t, filled0(f, f1, w1, h1, xx, yy, c), v[c]);
-                o = fmaf(t, owner[static_cast<size_t>(yy) * W0 + xx] == k ? 1.0f : 0.0f, o);
+                const int sy = 2 * ly + dy + 2, sx = 2 * lx + dx + 2;
+                for (int c = 0; c < 3; ++c) v[c] = fmaf(t, s_c[c][sy][sx], v[c]);
+                o = fmaf(t, s_o[sy][sx], o);
             }
-        }
         float* d = g.ptr(x, y);
         for (int c = 0; c < 3; ++c) d[c] = v[c];
         *u.ptr(x, y) = o;
@@ -397,9 +450,9 @@ __global__ void __launch_bounds__(kThreads) chain_finish(const Frames* fr, const
     }
 }
 
-unsigned grid_for(long long n) {
+unsigned grid_for(long long n, long long cap = 1 << 16) {
     const long long g = (n + kThreads - 1) / kThreads;
-    return static_cast<unsigned>(std::max<long long>(1, std::min<long long>(g, 1 << 16)));
+    return static_cast<unsigned>(std::max<long long>(1, std::min<long long>(g, cap)));
 }
 
 // ------------------------------------------------------------------ layout
@@ -673,7 +726,7 @@ Status render_chain(int n, const ImageView* frames, const Geometry* g, BlendMode
     if (stats) chain_clear_stats<<<1, 64, 0, stream>>>(stats);
     for (int k = 0; k < n; ++k)
         chain_warp<<<grid_for(fr.f[k].box.n()), kThreads, 0, stream>>>(fr.f[k], g[0].origin_x, g[0].origin_y);
-    chain_own<<<grid_for(static_cast<long long>(W) * H), kThreads, 0, stream>>>(
+    chain_own<<<grid_for(static_cast<long long>(W) * H, 256 * 8), kThreads, 0, stream>>>(
         d_frames, W, H, !bands, owner, count, output.data, output.pitch_bytes, mask.data, mask.pitch_bytes, stats);
     if (bands) {
         for (int k = 0; k < n; ++k) {
@@ -691,7 +744,9 @@ Status render_chain(int n, const ImageView* frames, const Geometry* g, BlendMode
                     pull(l), fill(l), up, top ? 1 : P.lw[l + 1], top ? 1 : P.lh[l + 1], top);
             }
             if (P.blend_top >= 1) {
-                chain_gauss1<<<grid_for(F.gauss[1].n()), kThreads, 0, stream>>>(
+                const unsigned g1_tiles = static_cast<unsigned>(((F.gauss[1].w() + kG1W - 1) / kG1W) *
+                                                                ((F.gauss[1].h() + kG1H - 1) / kG1H));
+                chain_gauss1<<<g1_tiles, kThreads, 0, stream>>>(
                     fr.f[k], k, owner, W, H, fill(1), P.lw[1], P.lh[1], mix[1].f[k].g, mix[1].f[k].u);
                 for (int l = 2; l <= P.blend_top; ++l)
                     chain_gauss<<<grid_for(F.gauss[l].n()), kThreads, 0, stream>>>(
