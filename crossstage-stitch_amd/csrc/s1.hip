@@ -110,7 +110,8 @@ __global__ void __launch_bounds__(256) s1_kernel(dev::JobSpan<S1Job> span, dev::
     const long long total = work.get();
     int j = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    for (dev::TileWalk tw = dev::tile_walk(total); tw.t < tw.end; tw.t += tw.step) {
+        const long long t = tw.t;
         j = dev::next_job(span, t, &S1Job::tile_begin, j, njobs);
         const S1Job& p = dev::job_at(span, j);
         const unsigned local = static_cast<unsigned>(t - p.tile_begin);
@@ -144,7 +145,8 @@ __global__ void __launch_bounds__(256) gradient_kernel(dev::JobSpan<GradJob> spa
     const long long total = work.get();
     int j = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    for (dev::TileWalk tw = dev::tile_walk(total); tw.t < tw.end; tw.t += tw.step) {
+        const long long t = tw.t;
         j = dev::next_job(span, t, &GradJob::tile_begin, j, njobs);
         const GradJob& g = dev::job_at(span, j);
         const unsigned local = static_cast<unsigned>(t - g.tile_begin);

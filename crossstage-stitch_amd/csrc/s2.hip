@@ -114,7 +114,8 @@ __global__ void __launch_bounds__(256) s2_fast_nms(dev::JobSpan<S2Job> span, dev
     const int tid = threadIdx.x;
     int ji = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    for (dev::TileWalk tw = dev::tile_walk(total); tw.t < tw.end; tw.t += tw.step) {
+        const long long t = tw.t;
         ji = dev::next_job(span, t, &S2Job::fast_begin, ji, njobs);
         const S2Job& j = dev::job_at(span, ji);
         // Job fields used per pixel, read once per tile.
@@ -324,7 +325,8 @@ __global__ void __launch_bounds__(256) s2_count(dev::JobSpan<S2Job> span, dev::T
     const long long total = work.get();
     int ji = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    for (dev::TileWalk tw = dev::tile_walk(total); tw.t < tw.end; tw.t += tw.step) {
+        const long long t = tw.t;
         ji = dev::next_job(span, t, &S2Job::chunk_begin, ji, njobs);
         const S2Job& j = dev::job_at(span, ji);
         const long long c = t - j.chunk_begin;

@@ -58,7 +58,8 @@ __global__ void __launch_bounds__(kS3MarkBlock) s3_mark(dev::JobSpan<S3Job> span
     const long long total = work.get();
     int ji = 0;
     const int njobs = span.jobs();
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    for (dev::TileWalk tw = dev::tile_walk(total); tw.t < tw.end; tw.t += tw.step) {
+        const long long t = tw.t;
         ji = dev::next_job(span, t, &S3Job::mark_begin, ji, njobs);
         const S3Job& j = dev::job_at(span, ji);
         const int index = static_cast<int>(t - j.mark_begin) * kS3MarkBlock + static_cast<int>(threadIdx.x);
