@@ -244,16 +244,18 @@ __device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int
         // written here; the band mix writes the overlap.
         j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = ma || mb ? 1 : 0;
         if (!(ma && mb)) {
+            std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
             if (ma) {
+                // A's level-0 colour is float(v / 255), which quantises back
+                // to v for every byte v: the output is A's bytes.
                 unsigned lo, hi;
                 load_rgb<true>(a_pixel(j, x, y), false, lo, hi);
-                ra[0] = lut[byte_of(lo, 0)];
-                ra[1] = lut[byte_of(lo, 1)];
-                ra[2] = lut[byte_of(lo, 2)];
-            }
-            std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) out[c] = quantize(ma ? ra[c] : rb[c]);
+                for (int c = 0; c < 3; ++c) out[c] = static_cast<std::uint8_t>(byte_of(lo, c));
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) out[c] = quantize(rb[c]);
+            }
         }
     } else {
         if (ma) {
