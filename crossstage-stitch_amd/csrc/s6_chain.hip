@@ -20,13 +20,15 @@
 // The canvas keeps the owner and valid-count maps and one mix plane per
 // blended level. Windows are sized so that every value that reaches the
 // output is read inside its window; reads are clamped to the window in any
-// case, so no access leaves its allocation. One launch per frame and level.
+// case, so no access leaves its allocation. One launch per level covers
+// every frame (grid y = frame), from tables uploaded in one copy.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <new>
 #include <vector>
 
 #include "stitch/stages.hpp"
@@ -125,9 +127,38 @@ __device__ __forceinline__ std::uint8_t quantize(float x) {
     for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < (n); \
          i += static_cast<long long>(gridDim.x) * blockDim.x)
 
-// Level 0 of one frame over its box.
-__global__ void __launch_bounds__(kThreads) chain_warp(Level0 f, int origin_x, int origin_y) {
-    GRID_STRIDE(i, f.box.n()) {
+struct Frames {
+    Level0 f[kMaxChainFrames];
+    int n;
+};
+
+// Per-frame arguments of one launch; a launch covers every frame, frame k
+// taking the blocks with blockIdx.y == k, which stride over its window.
+struct PullJob {  // pull into level l >= 2
+    Plane src, dst;
+    int fw, fh;   // level l - 1
+};
+struct PushJob {  // fill level l
+    Plane pull, dst, coarse;
+    int cw, ch;   // level l + 1
+};
+struct Gauss1Job {
+    Plane f1, g, u;
+};
+struct GaussJob {  // Gaussian into level l >= 2
+    Plane gs, us, g, u;
+    int fw, fh;
+};
+
+#define FRAME_STRIDE(i, n) \
+    for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < (n); \
+         i += static_cast<long long>(gridDim.x) * blockDim.x)
+
+// Level 0 of every frame over its box.
+__global__ void __launch_bounds__(kThreads) chain_warp(const Frames* fr, int origin_x, int origin_y) {
+    // A copy: stores to the frame's planes could otherwise alias the table.
+    const Level0 f = fr->f[blockIdx.y];
+    FRAME_STRIDE(i, f.box.n()) {
         const int x = f.box.x0 + static_cast<int>(i % f.box.w()), y = f.box.y0 + static_cast<int>(i / f.box.w());
         const double xa = static_cast<double>(x) + origin_x, ya = static_cast<double>(y) + origin_y;
         float rgb[3] = {0, 0, 0};
@@ -149,11 +180,6 @@ __global__ void __launch_bounds__(kThreads) chain_warp(Level0 f, int origin_x, i
         f.score[i] = m ? s : 0.0;
     }
 }
-
-struct Frames {
-    Level0 f[kMaxChainFrames];
-    int n;
-};
 
 // Owner and valid count per canvas pixel; the mask, the statistics, and the
 // output of every pixel the band mix does not decide (feather: all of them).
@@ -220,62 +246,114 @@ __global__ void chain_clear_stats(unsigned long long* stats) {
 }
 
 // Pull into level 1: 5x5 reduce of colour x validity and validity from the
-// frame's level 0 (zero outside its box; zero taps are skipped, which leaves
-// every sum as the full filter gives it).
-__global__ void __launch_bounds__(kThreads) chain_pull1(Level0 f, int W0, int H0, Plane dst) {
-    GRID_STRIDE(i, dst.b.n()) {
-        const int x = dst.b.x0 + static_cast<int>(i % dst.b.w()), y = dst.b.y0 + static_cast<int>(i / dst.b.w());
-        float v[4] = {0, 0, 0, 0};
-        for (int dy = -2; dy <= 2; ++dy) {
-            const int yy = reflect101(2 * y + dy, H0);
-            for (int dx = -2; dx <= 2; ++dx) {
-                const int xx = reflect101(2 * x + dx, W0);
-                if (!f.box.has(xx, yy)) continue;
+// frame's level 0. A block makes a kG1W x kG1H tile of level-1 pixels from
+// the (2 kG1W + 3) x (2 kG1H + 3) level-0 pixels it reads, staged in LDS with
+// zeros outside the frame's box (a zero tap adds exactly nothing to these
+// non-negative sums, so the result is the full filter's).
+constexpr int kG1W = 64, kG1H = 8;
+constexpr int kG1InW = 2 * kG1W + 3, kG1InH = 2 * kG1H + 3;
+
+__global__ void __launch_bounds__(kThreads) chain_pull1(const Frames* fr, int W0, int H0, const Plane* dsts) {
+    __shared__ float s_v[4][kG1InH][kG1InW];
+    const Level0 f = fr->f[blockIdx.y];
+    const Plane dst = dsts[blockIdx.y];
+    const int tiles_x = (dst.b.w() + kG1W - 1) / kG1W;
+    const int tiles = tiles_x * ((dst.b.h() + kG1H - 1) / kG1H);
+    for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int ox0 = dst.b.x0 + (t % tiles_x) * kG1W;
+        const int oy0 = dst.b.y0 + (t / tiles_x) * kG1H;
+        const int ix0 = 2 * ox0 - 2, iy0 = 2 * oy0 - 2;
+        for (int i = threadIdx.x; i < kG1InW * kG1InH; i += blockDim.x) {
+            const int lx = i % kG1InW, ly = i / kG1InW;
+            const int xx = reflect101(min(ix0 + lx, W0 + 1), W0), yy = reflect101(min(iy0 + ly, H0 + 1), H0);
+            float v[4] = {0, 0, 0, 0};
+            if (f.box.has(xx, yy)) {
                 const size_t j = static_cast<size_t>(yy - f.box.y0) * f.box.w() + (xx - f.box.x0);
-                const float t = tap(dx) * tap(dy);
-                const float ok = f.valid[j] ? 1.0f : 0.0f;
-                for (int c = 0; c < 3; ++c) v[c] = fmaf(t, f.color[j * 3 + c], v[c]);
-                v[3] = fmaf(t, ok, v[3]);
+                for (int c = 0; c < 3; ++c) v[c] = f.color[j * 3 + c];
+                v[3] = f.valid[j] ? 1.0f : 0.0f;
             }
+            for (int c = 0; c < 4; ++c) s_v[c][ly][lx] = v[c];
         }
-        float* d = dst.ptr(x, y);
-        for (int c = 0; c < 4; ++c) d[c] = v[c];
+        __syncthreads();
+        for (int i = threadIdx.x; i < kG1W * kG1H; i += blockDim.x) {
+            const int lx = i % kG1W, ly = i / kG1W;
+            const int x = ox0 + lx, y = oy0 + ly;
+            if (x >= dst.b.x1 || y >= dst.b.y1) continue;
+            float v[4] = {0, 0, 0, 0};
+            for (int dy = -2; dy <= 2; ++dy)
+                for (int dx = -2; dx <= 2; ++dx) {
+                    const float tp = tap(dx) * tap(dy);
+                    const int sy = 2 * ly + dy + 2, sx = 2 * lx + dx + 2;
+                    for (int c = 0; c < 4; ++c) v[c] = fmaf(tp, s_v[c][sy][sx], v[c]);
+                }
+            float* d = dst.ptr(x, y);
+            for (int c = 0; c < 4; ++c) d[c] = v[c];
+        }
+        __syncthreads();
     }
 }
 
 // Pull into level l >= 2 from level l - 1 (zero outside the source window).
-__global__ void __launch_bounds__(kThreads) chain_pull(Plane src, int fw, int fh, Plane dst) {
-    GRID_STRIDE(i, dst.b.n()) {
-        const int x = dst.b.x0 + static_cast<int>(i % dst.b.w()), y = dst.b.y0 + static_cast<int>(i / dst.b.w());
+__global__ void __launch_bounds__(kThreads) chain_pull(const PullJob* jobs) {
+    const PullJob J = jobs[blockIdx.y];
+    FRAME_STRIDE(i, J.dst.b.n()) {
+        const int x = J.dst.b.x0 + static_cast<int>(i % J.dst.b.w()), y = J.dst.b.y0 + static_cast<int>(i / J.dst.b.w());
         float v[4] = {0, 0, 0, 0};
         for (int dy = -2; dy <= 2; ++dy) {
-            const int yy = reflect101(2 * y + dy, fh);
+            const int yy = reflect101(2 * y + dy, J.fh);
             for (int dx = -2; dx <= 2; ++dx) {
-                const int xx = reflect101(2 * x + dx, fw);
-                if (!src.b.has(xx, yy)) continue;
+                const int xx = reflect101(2 * x + dx, J.fw);
+                if (!J.src.b.has(xx, yy)) continue;
                 const float t = tap(dx) * tap(dy);
-                for (int c = 0; c < 4; ++c) v[c] = fmaf(t, src.at(xx, yy, c), v[c]);
+                const float* sp = J.src.ptr(xx, yy);
+                for (int c = 0; c < 4; ++c) v[c] = fmaf(t, sp[c], v[c]);
             }
         }
-        float* d = dst.ptr(x, y);
+        float* d = J.dst.ptr(x, y);
         for (int c = 0; c < 4; ++c) d[c] = v[c];
     }
 }
 
-// Expansion of a coarse plane (cw x ch level) at fine pixel (x, y): the taps
-// whose reflected fine position is even, times 4.
-__device__ __forceinline__ float expand_at(const Plane& src, int cw, int ch, int x, int y, int c) {
-    float sum = 0;
+// Expansion of the three channels of coarse planes (cw x ch level) at fine
+// pixel (x, y): the taps whose reflected fine position is even, times 4,
+// each channel summed in the taps' order. Tap positions are found once for
+// all channels and, in expand3x2, for both planes (each read clamped to its
+// own window).
+__device__ __forceinline__ void expand3(const Plane& a, int cw, int ch, int x, int y, float* ea) {
+    float sa[3] = {0, 0, 0};
     for (int dy = -2; dy <= 2; ++dy) {
         const int yy = reflect101(y + dy, 2 * ch);
         if (yy & 1) continue;
         for (int dx = -2; dx <= 2; ++dx) {
             const int xx = reflect101(x + dx, 2 * cw);
             if (xx & 1) continue;
-            sum = fmaf(tap(dx) * tap(dy), src.at(xx / 2, yy / 2, c), sum);
+            const float t = tap(dx) * tap(dy);
+            for (int c = 0; c < 3; ++c) sa[c] = fmaf(t, a.at(xx / 2, yy / 2, c), sa[c]);
         }
     }
-    return 4 * sum;
+    for (int c = 0; c < 3; ++c) ea[c] = 4 * sa[c];
+}
+
+__device__ __forceinline__ void expand3x2(const Plane& a, const Plane& b, int cw, int ch, int x, int y, float* ea,
+                                          float* eb) {
+    float sa[3] = {0, 0, 0}, sb[3] = {0, 0, 0};
+    for (int dy = -2; dy <= 2; ++dy) {
+        const int yy = reflect101(y + dy, 2 * ch);
+        if (yy & 1) continue;
+        for (int dx = -2; dx <= 2; ++dx) {
+            const int xx = reflect101(x + dx, 2 * cw);
+            if (xx & 1) continue;
+            const float t = tap(dx) * tap(dy);
+            for (int c = 0; c < 3; ++c) {
+                sa[c] = fmaf(t, a.at(xx / 2, yy / 2, c), sa[c]);
+                sb[c] = fmaf(t, b.at(xx / 2, yy / 2, c), sb[c]);
+            }
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        ea[c] = 4 * sa[c];
+        eb[c] = 4 * sb[c];
+    }
 }
 
 // Pull value (colour sum c or weight 3) of a windowed pull plane: zero outside.
@@ -285,106 +363,132 @@ __device__ __forceinline__ float pulled(const Plane& p, int x, int y, int c) {
 
 // Normalised pull where the level has weight, else the expansion of the
 // filled level above (none at the top level: zero).
-__global__ void __launch_bounds__(kThreads) chain_push(Plane pull, Plane dst, Plane coarse, int cw, int ch,
-                                                      bool top) {
-    GRID_STRIDE(i, dst.b.n()) {
-        const int x = dst.b.x0 + static_cast<int>(i % dst.b.w()), y = dst.b.y0 + static_cast<int>(i / dst.b.w());
-        const float wgt = pulled(pull, x, y, 3);
-        float* d = dst.ptr(x, y);
-        for (int c = 0; c < 3; ++c) {
-            if (wgt > 0) d[c] = pulled(pull, x, y, c) / wgt;
-            else d[c] = top ? 0.0f : expand_at(coarse, cw, ch, x, y, c);
+__global__ void __launch_bounds__(kThreads) chain_push(const PushJob* jobs, bool top) {
+    const PushJob J = jobs[blockIdx.y];
+    FRAME_STRIDE(i, J.dst.b.n()) {
+        const int x = J.dst.b.x0 + static_cast<int>(i % J.dst.b.w()), y = J.dst.b.y0 + static_cast<int>(i / J.dst.b.w());
+        const float wgt = pulled(J.pull, x, y, 3);
+        float* d = J.dst.ptr(x, y);
+        if (wgt > 0) {
+            for (int c = 0; c < 3; ++c) d[c] = pulled(J.pull, x, y, c) / wgt;
+        } else if (top) {
+            for (int c = 0; c < 3; ++c) d[c] = 0.0f;
+        } else {
+            float e[3];
+            expand3(J.coarse, J.cw, J.ch, x, y, e);
+            for (int c = 0; c < 3; ++c) d[c] = e[c];
         }
     }
 }
 
-// Filled level-0 colour of a frame (3 channels): its own colour where valid,
-// else the expansion of its filled level 1 (each channel summed in the
-// expansion's tap order).
-__device__ __forceinline__ void filled0(const Level0& f, const Plane& f1, int w1, int h1, int x, int y, float* v) {
-    if (f.box.has(x, y)) {
-        const size_t j = static_cast<size_t>(y - f.box.y0) * f.box.w() + (x - f.box.x0);
-        if (f.valid[j]) {
-            for (int c = 0; c < 3; ++c) v[c] = f.color[j * 3 + c];
-            return;
-        }
-    }
-    float sum[3] = {0, 0, 0};
-    for (int dy = -2; dy <= 2; ++dy) {
-        const int yy = reflect101(y + dy, 2 * h1);
-        if (yy & 1) continue;
-        for (int dx = -2; dx <= 2; ++dx) {
-            const int xx = reflect101(x + dx, 2 * w1);
-            if (xx & 1) continue;
-            const float t = tap(dx) * tap(dy);
-            for (int c = 0; c < 3; ++c) sum[c] = fmaf(t, f1.at(xx / 2, yy / 2, c), sum[c]);
-        }
-    }
-    for (int c = 0; c < 3; ++c) v[c] = 4 * sum[c];
-}
-
-// Gaussian level 1 of the filled frame and its ownership weight. A block
-// makes a kG1W x kG1H tile of level-1 pixels from the (2 kG1W + 3) x
-// (2 kG1H + 3) level-0 pixels the 5x5 reduce reads, staged once in LDS
-// (filled colour, and ownership as 0/1), so each filled value is formed once
-// per tile instead of once per tap. Input positions are reflected into the
+// Gaussian level 1 of the filled frame and its ownership weight. The filled
+// level-0 colour is the frame's own where valid, else the expansion of its
+// filled level 1. A block makes a kG1W x kG1H tile of level-1 pixels from
+// the (2 kG1W + 3) x (2 kG1H + 3) level-0 pixels the 5x5 reduce reads,
+// staged once in LDS (filled colour, and ownership as 0/1), so each filled
+// value is formed once per tile instead of once per tap. Input positions are reflected into the
 // canvas as they are staged; positions past what the tile's real outputs
 // read are clamped (their values are never used).
-constexpr int kG1W = 64, kG1H = 8;
-constexpr int kG1InW = 2 * kG1W + 3, kG1InH = 2 * kG1H + 3;
+// The filled level 1 that the expansions read is staged too: a
+// kF1W x kF1H region starting one coarse pixel before the first input's
+// half, which holds every tap of every staged input (reflected inputs lie
+// inside the staged input range); a tap outside it, which the layout never
+// produces, would be read from memory.
+constexpr int kF1W = (kG1InW + 4) / 2 + 3, kF1H = (kG1InH + 4) / 2 + 3;
 
-__global__ void __launch_bounds__(kThreads) chain_gauss1(Level0 f, int k, const std::int8_t* owner, int W0, int H0,
-                                                        Plane f1, int w1, int h1, Plane g, Plane u) {
+__global__ void __launch_bounds__(kThreads) chain_gauss1(const Frames* fr, const Gauss1Job* jobs,
+                                                        const std::int8_t* owner, int W0, int H0, int w1, int h1) {
     __shared__ float s_c[3][kG1InH][kG1InW];
     __shared__ float s_o[kG1InH][kG1InW];
-    const int tiles_x = (g.b.w() + kG1W - 1) / kG1W;
-    const int ox0 = g.b.x0 + static_cast<int>(blockIdx.x % tiles_x) * kG1W;
-    const int oy0 = g.b.y0 + static_cast<int>(blockIdx.x / tiles_x) * kG1H;
-    const int ix0 = 2 * ox0 - 2, iy0 = 2 * oy0 - 2;
-    for (int i = threadIdx.x; i < kG1InW * kG1InH; i += blockDim.x) {
-        const int lx = i % kG1InW, ly = i / kG1InW;
-        const int xx = reflect101(min(ix0 + lx, W0 + 1), W0), yy = reflect101(min(iy0 + ly, H0 + 1), H0);
-        float v[3];
-        filled0(f, f1, w1, h1, xx, yy, v);
-        for (int c = 0; c < 3; ++c) s_c[c][ly][lx] = v[c];
-        s_o[ly][lx] = owner[static_cast<size_t>(yy) * W0 + xx] == k ? 1.0f : 0.0f;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kG1W * kG1H; i += blockDim.x) {
-        const int lx = i % kG1W, ly = i / kG1W;
-        const int x = ox0 + lx, y = oy0 + ly;
-        if (x >= g.b.x1 || y >= g.b.y1) continue;
-        float v[3] = {0, 0, 0}, o = 0;
-        for (int dy = -2; dy <= 2; ++dy)
-            for (int dx = -2; dx <= 2; ++dx) {
-                const float t = tap(dx) * tap(dy);
-                const int sy = 2 * ly + dy + 2, sx = 2 * lx + dx + 2;
-                for (int c = 0; c < 3; ++c) v[c] = fmaf(t, s_c[c][sy][sx], v[c]);
-                o = fmaf(t, s_o[sy][sx], o);
+    __shared__ float s_f[3][kF1H][kF1W];
+    const int k = static_cast<int>(blockIdx.y);
+    const Level0& f = fr->f[k];
+    const Gauss1Job& J = jobs[k];
+    const int tiles_x = (J.g.b.w() + kG1W - 1) / kG1W;
+    const int tiles = tiles_x * ((J.g.b.h() + kG1H - 1) / kG1H);
+    for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int ox0 = J.g.b.x0 + (t % tiles_x) * kG1W;
+        const int oy0 = J.g.b.y0 + (t / tiles_x) * kG1H;
+        const int ix0 = 2 * ox0 - 2, iy0 = 2 * oy0 - 2;
+        // Staged filled level 1: [fx0, fx0 + kF1W) x [fy0, fy0 + kF1H).
+        const int fx0 = max(ix0, 0) / 2 - 2, fy0 = max(iy0, 0) / 2 - 2;
+        for (int i = threadIdx.x; i < kF1W * kF1H; i += blockDim.x) {
+            const int lx = i % kF1W, ly = i / kF1W;
+            for (int c = 0; c < 3; ++c) s_f[c][ly][lx] = J.f1.at(fx0 + lx, fy0 + ly, c);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kG1InW * kG1InH; i += blockDim.x) {
+            const int lx = i % kG1InW, ly = i / kG1InW;
+            const int xx = reflect101(min(ix0 + lx, W0 + 1), W0), yy = reflect101(min(iy0 + ly, H0 + 1), H0);
+            float v[3];
+            bool own_colour = false;
+            if (f.box.has(xx, yy)) {
+                const size_t j = static_cast<size_t>(yy - f.box.y0) * f.box.w() + (xx - f.box.x0);
+                if (f.valid[j]) {
+                    for (int c = 0; c < 3; ++c) v[c] = f.color[j * 3 + c];
+                    own_colour = true;
+                }
             }
-        float* d = g.ptr(x, y);
-        for (int c = 0; c < 3; ++c) d[c] = v[c];
-        *u.ptr(x, y) = o;
+            if (!own_colour) {
+                // The expansion of the filled level 1 (expand3's taps and order).
+                float sum[3] = {0, 0, 0};
+                for (int dy = -2; dy <= 2; ++dy) {
+                    const int ry = reflect101(yy + dy, 2 * h1);
+                    if (ry & 1) continue;
+                    for (int dx = -2; dx <= 2; ++dx) {
+                        const int rx = reflect101(xx + dx, 2 * w1);
+                        if (rx & 1) continue;
+                        const float tp = tap(dx) * tap(dy);
+                        const int cx = rx / 2 - fx0, cy = ry / 2 - fy0;
+                        const bool staged = cx >= 0 && cx < kF1W && cy >= 0 && cy < kF1H;
+                        for (int c = 0; c < 3; ++c)
+                            sum[c] = fmaf(tp, staged ? s_f[c][cy][cx] : J.f1.at(rx / 2, ry / 2, c), sum[c]);
+                    }
+                }
+                for (int c = 0; c < 3; ++c) v[c] = 4 * sum[c];
+            }
+            for (int c = 0; c < 3; ++c) s_c[c][ly][lx] = v[c];
+            s_o[ly][lx] = owner[static_cast<size_t>(yy) * W0 + xx] == k ? 1.0f : 0.0f;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kG1W * kG1H; i += blockDim.x) {
+            const int lx = i % kG1W, ly = i / kG1W;
+            const int x = ox0 + lx, y = oy0 + ly;
+            if (x >= J.g.b.x1 || y >= J.g.b.y1) continue;
+            float v[3] = {0, 0, 0}, o = 0;
+            for (int dy = -2; dy <= 2; ++dy)
+                for (int dx = -2; dx <= 2; ++dx) {
+                    const float tp = tap(dx) * tap(dy);
+                    const int sy = 2 * ly + dy + 2, sx = 2 * lx + dx + 2;
+                    for (int c = 0; c < 3; ++c) v[c] = fmaf(tp, s_c[c][sy][sx], v[c]);
+                    o = fmaf(tp, s_o[sy][sx], o);
+                }
+            float* d = J.g.ptr(x, y);
+            for (int c = 0; c < 3; ++c) d[c] = v[c];
+            *J.u.ptr(x, y) = o;
+        }
+        __syncthreads();
     }
 }
 
 // Gaussian level l >= 2: reduce of the level below (colour and ownership).
-__global__ void __launch_bounds__(kThreads) chain_gauss(Plane gs, Plane us, int fw, int fh, Plane g, Plane u) {
-    GRID_STRIDE(i, g.b.n()) {
-        const int x = g.b.x0 + static_cast<int>(i % g.b.w()), y = g.b.y0 + static_cast<int>(i / g.b.w());
+__global__ void __launch_bounds__(kThreads) chain_gauss(const GaussJob* jobs) {
+    const GaussJob J = jobs[blockIdx.y];
+    FRAME_STRIDE(i, J.g.b.n()) {
+        const int x = J.g.b.x0 + static_cast<int>(i % J.g.b.w()), y = J.g.b.y0 + static_cast<int>(i / J.g.b.w());
         float v[3] = {0, 0, 0}, o = 0;
         for (int dy = -2; dy <= 2; ++dy) {
-            const int yy = reflect101(2 * y + dy, fh);
+            const int yy = reflect101(2 * y + dy, J.fh);
             for (int dx = -2; dx <= 2; ++dx) {
-                const int xx = reflect101(2 * x + dx, fw);
+                const int xx = reflect101(2 * x + dx, J.fw);
                 const float t = tap(dx) * tap(dy);
-                for (int c = 0; c < 3; ++c) v[c] = fmaf(t, gs.at(xx, yy, c), v[c]);
-                o = fmaf(t, us.at(xx, yy, 0), o);
+                for (int c = 0; c < 3; ++c) v[c] = fmaf(t, J.gs.at(xx, yy, c), v[c]);
+                o = fmaf(t, J.us.at(xx, yy, 0), o);
             }
         }
-        float* d = g.ptr(x, y);
+        float* d = J.g.ptr(x, y);
         for (int c = 0; c < 3; ++c) d[c] = v[c];
-        *u.ptr(x, y) = o;
+        *J.u.ptr(x, y) = o;
     }
 }
 
@@ -411,8 +515,10 @@ __global__ void __launch_bounds__(kThreads) chain_mix(const MixLevel* m, int lw,
             if (!f.zone.has(x, y)) continue;
             const float uk = f.u.at(x, y, 0);
             if (uk == 0) continue;
+            float e[3] = {0, 0, 0};
+            if (coarse) expand3(f.g_up, cw, ch, x, y, e);
             for (int c = 0; c < 3; ++c) {
-                const float lk = f.g.at(x, y, c) - (coarse ? expand_at(f.g_up, cw, ch, x, y, c) : 0.0f);
+                const float lk = f.g.at(x, y, c) - (coarse ? e[c] : 0.0f);
                 acc[c] = any ? fmaf(uk, lk, acc[c]) : uk * lk;
             }
             any = true;
@@ -422,10 +528,12 @@ __global__ void __launch_bounds__(kThreads) chain_mix(const MixLevel* m, int lw,
             const MixFrame& f = m->f[k];
             if (f.zone.has(x, y)) total += f.u.at(x, y, 0);
         }
+        float ez[3] = {0, 0, 0};
+        if (coarse) expand3(z_up, cw, ch, x, y, ez);
         float* d = z.ptr(x, y);
         for (int c = 0; c < 3; ++c) {
             const float mix = total > 0 ? acc[c] / total : 0.0f;
-            d[c] = coarse ? mix + expand_at(z_up, cw, ch, x, y, c) : mix;
+            d[c] = coarse ? mix + ez[c] : mix;
         }
     }
 }
@@ -441,11 +549,13 @@ __global__ void __launch_bounds__(kThreads) chain_finish(const Frames* fr, const
         const Level0& f = fr->f[k];
         const size_t j = static_cast<size_t>(y - f.box.y0) * f.box.w() + (x - f.box.x0);
         std::uint8_t* o = out + static_cast<size_t>(y) * out_pitch + 3 * x;
+        float eg[3] = {0, 0, 0}, ez[3] = {0, 0, 0};
+        if (coarse) expand3x2(m1->f[k].g, z1, w1, h1, x, y, eg, ez);
         for (int c = 0; c < 3; ++c) {
             const float own = f.color[j * 3 + c];
             // One owner with weight 1: the mix is its Laplacian.
-            const float band = coarse ? own - expand_at(m1->f[k].g, w1, h1, x, y, c) : own;
-            o[c] = quantize(coarse ? band + expand_at(z1, w1, h1, x, y, c) : band);
+            const float band = coarse ? own - eg[c] : own;
+            o[c] = quantize(coarse ? band + ez[c] : band);
         }
     }
 }
@@ -466,11 +576,17 @@ struct FramePlan {
     std::size_t off_pull[kMaxLevels], off_fill[kMaxLevels], off_gauss[kMaxLevels], off_own[kMaxLevels];
 };
 
+// Launch tables, uploaded in one copy: offsets within the table region.
+struct TableLayout {
+    std::size_t frames, mix[kMaxLevels], dst1, pull, push, gauss1, gauss, bytes;
+};
+
 struct Plan {
     int n, top, blend_top;
     int lw[kMaxLevels], lh[kMaxLevels];
     std::vector<FramePlan> f;
-    std::size_t off_owner, off_count, off_z[kMaxLevels], off_frames, off_mix[kMaxLevels], bytes;
+    std::size_t off_owner, off_count, off_z[kMaxLevels], off_tables, bytes;
+    TableLayout t;
 };
 
 Box clip(Box b, int w, int h) {
@@ -615,11 +731,26 @@ Status make_plan(int n, const Geometry* g, BlendMode mode, Plan* plan) {
     }
     P.off_owner = take(canvas);
     P.off_count = take(canvas);
-    P.off_frames = take(sizeof(Frames));
-    for (int l = 1; l <= P.blend_top; ++l) {
-        P.off_z[l] = take(static_cast<std::size_t>(P.lw[l]) * P.lh[l] * 12);
-        P.off_mix[l] = take(sizeof(MixLevel));
-    }
+    for (int l = 1; l <= P.blend_top; ++l) P.off_z[l] = take(static_cast<std::size_t>(P.lw[l]) * P.lh[l] * 12);
+    // Tables: the frames, one mix table per blended level, and per level one
+    // job per frame for the pulls, pushes and Gaussians.
+    std::size_t t = 0;
+    const auto part = [&](std::size_t bytes) {
+        t = (t + 255) / 256 * 256;
+        const std::size_t at = t;
+        t += std::max<std::size_t>(bytes, 1);
+        return at;
+    };
+    const std::size_t levels = static_cast<std::size_t>(P.top) + 1;
+    P.t.frames = part(sizeof(Frames));
+    for (int l = 1; l <= P.blend_top; ++l) P.t.mix[l] = part(sizeof(MixLevel));
+    P.t.dst1 = part(n * sizeof(Plane));
+    P.t.pull = part(levels * n * sizeof(PullJob));
+    P.t.push = part(levels * n * sizeof(PushJob));
+    P.t.gauss1 = part(n * sizeof(Gauss1Job));
+    P.t.gauss = part(levels * n * sizeof(GaussJob));
+    P.t.bytes = t;
+    P.off_tables = take(t);
     P.bytes = cur;
     return Status::OK;
 }
@@ -684,8 +815,12 @@ Status render_chain(int n, const ImageView* frames, const Geometry* g, BlendMode
     const auto fptr = [&](std::size_t off) { return reinterpret_cast<float*>(base + off); };
     const int W = P.lw[0], H = P.lh[0];
     const bool bands = mode != BlendMode::Feather;
-    // Host copies of the per-launch tables, uploaded once.
-    Frames fr{};
+    // Host image of the launch tables, uploaded in one copy.
+    std::vector<std::uint8_t> host(P.t.bytes, 0);
+    auto* tables = base + P.off_tables;
+    const auto at = [&](std::size_t off) { return host.data() + off; };
+    const auto dev = [&](std::size_t off) { return tables + off; };
+    Frames& fr = *new (at(P.t.frames)) Frames{};
     fr.n = n;
     for (int k = 0; k < n; ++k) {
         Level0& f = fr.f[k];
@@ -703,69 +838,102 @@ Status render_chain(int n, const ImageView* frames, const Geometry* g, BlendMode
         f.valid = base + F.off_valid;
         f.score = reinterpret_cast<double*>(base + F.off_score);
     }
-    std::vector<MixLevel> mix(P.blend_top + 1);
-    for (int l = 1; l <= P.blend_top; ++l) {
-        MixLevel& m = mix[l];
-        m.n = n;
+    const auto pull = [&](int k, int l) { return Plane{fptr(P.f[k].off_pull[l]), P.f[k].pull[l], 4}; };
+    const auto fill = [&](int k, int l) { return Plane{fptr(P.f[k].off_fill[l]), P.f[k].fill[l], 3}; };
+    const auto gauss = [&](int k, int l) { return Plane{fptr(P.f[k].off_gauss[l]), P.f[k].gauss[l], 3}; };
+    const auto own = [&](int k, int l) { return Plane{fptr(P.f[k].off_own[l]), P.f[k].gauss[l], 1}; };
+    auto* dst1 = reinterpret_cast<Plane*>(at(P.t.dst1));
+    auto* pulls = reinterpret_cast<PullJob*>(at(P.t.pull));
+    auto* pushes = reinterpret_cast<PushJob*>(at(P.t.push));
+    auto* g1s = reinterpret_cast<Gauss1Job*>(at(P.t.gauss1));
+    auto* gs = reinterpret_cast<GaussJob*>(at(P.t.gauss));
+    // Largest per-frame extent of each launch, for its grid.
+    long long n_warp = 0, n_pull[kMaxLevels] = {}, n_push[kMaxLevels] = {}, n_gauss[kMaxLevels] = {}, g1_tiles = 0,
+              p1_tiles = 0;
+    for (int k = 0; k < n; ++k) n_warp = std::max(n_warp, P.f[k].box.n());
+    if (bands) {
+        for (int l = 1; l <= P.blend_top; ++l) {
+            MixLevel& m = *new (at(P.t.mix[l])) MixLevel{};
+            m.n = n;
+            for (int k = 0; k < n; ++k) {
+                m.f[k].g = gauss(k, l);
+                m.f[k].u = own(k, l);
+                m.f[k].g_up = l < P.blend_top ? gauss(k, l + 1) : m.f[k].g;
+                m.f[k].zone = P.f[k].zone[l];
+            }
+        }
         for (int k = 0; k < n; ++k) {
             const FramePlan& F = P.f[k];
-            m.f[k].g = {fptr(F.off_gauss[l]), F.gauss[l], 3};
-            m.f[k].u = {fptr(F.off_own[l]), F.gauss[l], 1};
-            m.f[k].g_up = l < P.blend_top ? Plane{fptr(F.off_gauss[l + 1]), F.gauss[l + 1], 3} : m.f[k].g;
-            m.f[k].zone = F.zone[l];
+            dst1[k] = pull(k, 1);
+            p1_tiles = std::max<long long>(p1_tiles, static_cast<long long>((F.pull[1].w() + kG1W - 1) / kG1W) *
+                                                         ((F.pull[1].h() + kG1H - 1) / kG1H));
+            for (int l = 2; l <= P.top; ++l) {
+                pulls[l * n + k] = {pull(k, l - 1), pull(k, l), P.lw[l - 1], P.lh[l - 1]};
+                n_pull[l] = std::max(n_pull[l], F.pull[l].n());
+            }
+            for (int l = 1; l <= P.top; ++l) {
+                const bool top = l == P.top;
+                pushes[l * n + k] = {pull(k, l), fill(k, l), top ? fill(k, l) : fill(k, l + 1), top ? 1 : P.lw[l + 1],
+                                     top ? 1 : P.lh[l + 1]};
+                n_push[l] = std::max(n_push[l], F.fill[l].n());
+            }
+            if (P.blend_top >= 1) {
+                g1s[k] = {fill(k, 1), gauss(k, 1), own(k, 1)};
+                g1_tiles = std::max<long long>(g1_tiles, static_cast<long long>((F.gauss[1].w() + kG1W - 1) / kG1W) *
+                                                             ((F.gauss[1].h() + kG1H - 1) / kG1H));
+                for (int l = 2; l <= P.blend_top; ++l) {
+                    gs[l * n + k] = {gauss(k, l - 1), own(k, l - 1), gauss(k, l), own(k, l), P.lw[l - 1], P.lh[l - 1]};
+                    n_gauss[l] = std::max(n_gauss[l], F.gauss[l].n());
+                }
+            }
         }
     }
-    auto* d_frames = reinterpret_cast<Frames*>(base + P.off_frames);
-    if (hipMemcpyAsync(d_frames, &fr, sizeof fr, hipMemcpyHostToDevice, stream) != hipSuccess) return Status::CUDA_ERROR;
-    for (int l = 1; l <= P.blend_top; ++l)
-        if (hipMemcpyAsync(base + P.off_mix[l], &mix[l], sizeof(MixLevel), hipMemcpyHostToDevice, stream) != hipSuccess)
-            return Status::CUDA_ERROR;
+    if (hipMemcpyAsync(tables, host.data(), host.size(), hipMemcpyHostToDevice, stream) != hipSuccess)
+        return Status::CUDA_ERROR;
+    const auto* d_frames = reinterpret_cast<const Frames*>(dev(P.t.frames));
+    const auto d_mix = [&](int l) { return reinterpret_cast<const MixLevel*>(dev(P.t.mix[l])); };
+    // A launch over every frame: up to 2048 blocks per frame, striding over its window.
+    const auto grid = [&](long long extent) { return dim3(grid_for(extent, 2048), static_cast<unsigned>(n)); };
     auto* owner = reinterpret_cast<std::int8_t*>(base + P.off_owner);
     auto* count = base + P.off_count;
     auto* stats = reinterpret_cast<unsigned long long*>(statistics);
     if (stats) chain_clear_stats<<<1, 64, 0, stream>>>(stats);
-    for (int k = 0; k < n; ++k)
-        chain_warp<<<grid_for(fr.f[k].box.n()), kThreads, 0, stream>>>(fr.f[k], g[0].origin_x, g[0].origin_y);
+    // The warp is latency-bound per pixel: one pixel per thread where the
+    // grid allows (65536 blocks over all frames).
+    chain_warp<<<dim3(grid_for(n_warp, std::max(2048, 65536 / n)), static_cast<unsigned>(n)), kThreads, 0, stream>>>(
+        d_frames, g[0].origin_x, g[0].origin_y);
     chain_own<<<grid_for(static_cast<long long>(W) * H, 256 * 8), kThreads, 0, stream>>>(
         d_frames, W, H, !bands, owner, count, output.data, output.pitch_bytes, mask.data, mask.pitch_bytes, stats);
     if (bands) {
-        for (int k = 0; k < n; ++k) {
-            const FramePlan& F = P.f[k];
-            const auto pull = [&](int l) { return Plane{fptr(F.off_pull[l]), F.pull[l], 4}; };
-            const auto fill = [&](int l) { return Plane{fptr(F.off_fill[l]), F.fill[l], 3}; };
-            chain_pull1<<<grid_for(F.pull[1].n()), kThreads, 0, stream>>>(fr.f[k], W, H, pull(1));
-            for (int l = 2; l <= P.top; ++l)
-                chain_pull<<<grid_for(F.pull[l].n()), kThreads, 0, stream>>>(pull(l - 1), P.lw[l - 1], P.lh[l - 1],
-                                                                             pull(l));
-            for (int l = P.top; l >= 1; --l) {
-                const bool top = l == P.top;
-                const Plane up = top ? fill(l) : fill(l + 1);
-                chain_push<<<grid_for(F.fill[l].n()), kThreads, 0, stream>>>(
-                    pull(l), fill(l), up, top ? 1 : P.lw[l + 1], top ? 1 : P.lh[l + 1], top);
-            }
-            if (P.blend_top >= 1) {
-                const unsigned g1_tiles = static_cast<unsigned>(((F.gauss[1].w() + kG1W - 1) / kG1W) *
-                                                                ((F.gauss[1].h() + kG1H - 1) / kG1H));
-                chain_gauss1<<<g1_tiles, kThreads, 0, stream>>>(
-                    fr.f[k], k, owner, W, H, fill(1), P.lw[1], P.lh[1], mix[1].f[k].g, mix[1].f[k].u);
-                for (int l = 2; l <= P.blend_top; ++l)
-                    chain_gauss<<<grid_for(F.gauss[l].n()), kThreads, 0, stream>>>(
-                        mix[l - 1].f[k].g, mix[l - 1].f[k].u, P.lw[l - 1], P.lh[l - 1], mix[l].f[k].g, mix[l].f[k].u);
-            }
+        chain_pull1<<<dim3(static_cast<unsigned>(std::min<long long>(p1_tiles, 2048)), static_cast<unsigned>(n)),
+                      kThreads, 0, stream>>>(d_frames, W, H, reinterpret_cast<const Plane*>(dev(P.t.dst1)));
+        for (int l = 2; l <= P.top; ++l)
+            chain_pull<<<grid(n_pull[l]), kThreads, 0, stream>>>(
+                reinterpret_cast<const PullJob*>(dev(P.t.pull)) + static_cast<std::size_t>(l) * n);
+        for (int l = P.top; l >= 1; --l)
+            chain_push<<<grid(n_push[l]), kThreads, 0, stream>>>(
+                reinterpret_cast<const PushJob*>(dev(P.t.push)) + static_cast<std::size_t>(l) * n, l == P.top);
+        if (P.blend_top >= 1) {
+            chain_gauss1<<<dim3(static_cast<unsigned>(std::min<long long>(g1_tiles, 2048)), static_cast<unsigned>(n)),
+                           kThreads, 0, stream>>>(d_frames, reinterpret_cast<const Gauss1Job*>(dev(P.t.gauss1)), owner,
+                                                  W, H, P.lw[1], P.lh[1]);
+            for (int l = 2; l <= P.blend_top; ++l)
+                chain_gauss<<<grid(n_gauss[l]), kThreads, 0, stream>>>(
+                    reinterpret_cast<const GaussJob*>(dev(P.t.gauss)) + static_cast<std::size_t>(l) * n);
         }
         Plane z_up{};
         for (int l = P.blend_top; l >= 1; --l) {
             const Plane z{fptr(P.off_z[l]), {0, 0, P.lw[l], P.lh[l]}, 3};
             const bool coarse = l < P.blend_top;
             chain_mix<<<grid_for(static_cast<long long>(P.lw[l]) * P.lh[l]), kThreads, 0, stream>>>(
-                reinterpret_cast<const MixLevel*>(base + P.off_mix[l]), P.lw[l], P.lh[l], coarse ? P.lw[l + 1] : 1,
-                coarse ? P.lh[l + 1] : 1, coarse, z, coarse ? z_up : z);
+                d_mix(l), P.lw[l], P.lh[l], coarse ? P.lw[l + 1] : 1, coarse ? P.lh[l + 1] : 1, coarse, z,
+                coarse ? z_up : z);
             z_up = z;
         }
         const Plane z1 = P.blend_top >= 1 ? z_up : Plane{fptr(P.off_count), {0, 0, 1, 1}, 1};
         chain_finish<<<grid_for(static_cast<long long>(W) * H), kThreads, 0, stream>>>(
-            d_frames, P.blend_top >= 1 ? reinterpret_cast<const MixLevel*>(base + P.off_mix[1]) : nullptr, owner, count,
-            W, H, P.lw[1], P.lh[1], P.blend_top >= 1, z1, output.data, output.pitch_bytes);
+            d_frames, P.blend_top >= 1 ? d_mix(1) : nullptr, owner, count, W, H, P.lw[1], P.lh[1], P.blend_top >= 1,
+            z1, output.data, output.pitch_bytes);
     }
     return hipGetLastError() == hipSuccess ? Status::OK : Status::CUDA_ERROR;
 }
