@@ -250,7 +250,7 @@ __global__ void chain_clear_stats(unsigned long long* stats) {
 // the (2 kG1W + 3) x (2 kG1H + 3) level-0 pixels it reads, staged in LDS with
 // zeros outside the frame's box (a zero tap adds exactly nothing to these
 // non-negative sums, so the result is the full filter's).
-constexpr int kG1W = 64, kG1H = 8;
+constexpr int kG1W = 32, kG1H = 8;  // 256 outputs: one per thread
 constexpr int kG1InW = 2 * kG1W + 3, kG1InH = 2 * kG1H + 3;
 
 __global__ void __launch_bounds__(kThreads) chain_pull1(const Frames* fr, int W0, int H0, const Plane* dsts) {
@@ -399,7 +399,7 @@ constexpr int kF1W = (kG1InW + 4) / 2 + 3, kF1H = (kG1InH + 4) / 2 + 3;
 __global__ void __launch_bounds__(kThreads) chain_gauss1(const Frames* fr, const Gauss1Job* jobs,
                                                         const std::int8_t* owner, int W0, int H0, int w1, int h1) {
     __shared__ float s_c[3][kG1InH][kG1InW];
-    __shared__ float s_o[kG1InH][kG1InW];
+    __shared__ std::uint8_t s_o[kG1InH][kG1InW];
     __shared__ float s_f[3][kF1H][kF1W];
     const int k = static_cast<int>(blockIdx.y);
     const Level0& f = fr->f[k];
@@ -448,7 +448,7 @@ __global__ void __launch_bounds__(kThreads) chain_gauss1(const Frames* fr, const
                 for (int c = 0; c < 3; ++c) v[c] = 4 * sum[c];
             }
             for (int c = 0; c < 3; ++c) s_c[c][ly][lx] = v[c];
-            s_o[ly][lx] = owner[static_cast<size_t>(yy) * W0 + xx] == k ? 1.0f : 0.0f;
+            s_o[ly][lx] = owner[static_cast<size_t>(yy) * W0 + xx] == k ? 1 : 0;
         }
         __syncthreads();
         for (int i = threadIdx.x; i < kG1W * kG1H; i += blockDim.x) {
@@ -461,7 +461,7 @@ __global__ void __launch_bounds__(kThreads) chain_gauss1(const Frames* fr, const
                     const float tp = tap(dx) * tap(dy);
                     const int sy = 2 * ly + dy + 2, sx = 2 * lx + dx + 2;
                     for (int c = 0; c < 3; ++c) v[c] = fmaf(tp, s_c[c][sy][sx], v[c]);
-                    o = fmaf(tp, s_o[sy][sx], o);
+                    o = fmaf(tp, s_o[sy][sx] ? 1.0f : 0.0f, o);
                 }
             float* d = J.g.ptr(x, y);
             for (int c = 0; c < 3; ++c) d[c] = v[c];
