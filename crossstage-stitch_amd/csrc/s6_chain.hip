@@ -181,20 +181,42 @@ __global__ void __launch_bounds__(kThreads) chain_warp(const Frames* fr, int ori
     }
 }
 
+// What the per-pixel kernels read of each frame, staged in LDS once per
+// block (reads through the global table would be repeated after every store
+// the compiler cannot prove apart from it).
+struct FrameRef {
+    Box box;
+    const float* color;
+    const std::uint8_t* valid;
+    const double* score;
+};
+
+__device__ __forceinline__ int stage_frames(const Frames* fr, FrameRef* refs) {
+    const int n = fr->n;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const Level0& f = fr->f[k];
+        refs[k] = {f.box, f.color, f.valid, f.score};
+    }
+    __syncthreads();
+    return n;
+}
+
 // Owner and valid count per canvas pixel; the mask, the statistics, and the
 // output of every pixel the band mix does not decide (feather: all of them).
 __global__ void __launch_bounds__(kThreads) chain_own(const Frames* fr, int W, int H, bool feather, std::int8_t* owner,
                                                      std::uint8_t* count, std::uint8_t* out, int out_pitch,
                                                      std::uint8_t* mask, int mask_pitch,
                                                      unsigned long long* stats) {
+    __shared__ FrameRef refs[kMaxChainFrames];
+    const int n = stage_frames(fr, refs);
     unsigned long long uni = 0, ovl = 0;
     GRID_STRIDE(i, static_cast<long long>(W) * H) {
         const int x = static_cast<int>(i % W), y = static_cast<int>(i / W);
         int own = -1, valid = 0;
         double best = 0, fs = 0, fc[3] = {0, 0, 0};
         float single[3] = {0, 0, 0};
-        for (int k = 0; k < fr->n; ++k) {
-            const Level0& f = fr->f[k];
+        for (int k = 0; k < n; ++k) {
+            const FrameRef& f = refs[k];
             if (!f.box.has(x, y)) continue;
             const size_t j = static_cast<size_t>(y - f.box.y0) * f.box.w() + (x - f.box.x0);
             if (!f.valid[j]) continue;
@@ -206,10 +228,11 @@ __global__ void __launch_bounds__(kThreads) chain_own(const Frames* fr, int W, i
             }
             for (int c = 0; c < 3; ++c) {
                 const float v = f.color[j * 3 + c];
-                fc[c] = valid == 1 ? s * v : fc[c] + s * v;
+                // The score-weighted sums only feather uses.
+                if (feather) fc[c] = valid == 1 ? s * v : fc[c] + s * v;
                 single[c] = v;
             }
-            fs = valid == 1 ? s : fs + s;
+            if (feather) fs = valid == 1 ? s : fs + s;
         }
         owner[i] = static_cast<std::int8_t>(own);
         count[i] = static_cast<std::uint8_t>(min(valid, 255));
@@ -222,7 +245,7 @@ __global__ void __launch_bounds__(kThreads) chain_own(const Frames* fr, int W, i
                 o[c] = quantize(valid > 1 ? static_cast<float>(fc[c] / fs) : single[c]);
     }
     if (stats) {
-        // One atomic pair per block (the grid is capped at a few blocks per CU).
+        // One atomic pair per block (the grid is capped at 64 blocks per CU).
         __shared__ unsigned long long part[2][kThreads / 64];
         for (int off = 32; off > 0; off >>= 1) {
             uni += __shfl_xor(uni, off, 64);
@@ -506,12 +529,17 @@ struct MixLevel {
 // the expansion of the collapsed level above.
 __global__ void __launch_bounds__(kThreads) chain_mix(const MixLevel* m, int lw, int lh, int cw, int ch, bool coarse,
                                                      Plane z, Plane z_up) {
+    // The frames' planes staged in LDS (see FrameRef).
+    __shared__ MixFrame frames[kMaxChainFrames];
+    const int n = m->n;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) frames[k] = m->f[k];
+    __syncthreads();
     GRID_STRIDE(i, static_cast<long long>(lw) * lh) {
         const int x = static_cast<int>(i % lw), y = static_cast<int>(i / lw);
         float total = 0, acc[3] = {0, 0, 0};
         bool any = false;
-        for (int k = m->n - 1; k >= 0; --k) {
-            const MixFrame& f = m->f[k];
+        for (int k = n - 1; k >= 0; --k) {
+            const MixFrame& f = frames[k];
             if (!f.zone.has(x, y)) continue;
             const float uk = f.u.at(x, y, 0);
             if (uk == 0) continue;
@@ -524,8 +552,8 @@ __global__ void __launch_bounds__(kThreads) chain_mix(const MixLevel* m, int lw,
             any = true;
         }
         // The denominator in frame order.
-        for (int k = 0; k < m->n; ++k) {
-            const MixFrame& f = m->f[k];
+        for (int k = 0; k < n; ++k) {
+            const MixFrame& f = frames[k];
             if (f.zone.has(x, y)) total += f.u.at(x, y, 0);
         }
         float ez[3] = {0, 0, 0};
@@ -542,11 +570,13 @@ __global__ void __launch_bounds__(kThreads) chain_mix(const MixLevel* m, int lw,
 __global__ void __launch_bounds__(kThreads) chain_finish(const Frames* fr, const MixLevel* m1, const std::int8_t* owner,
                                                         const std::uint8_t* count, int W, int H, int w1, int h1,
                                                         bool coarse, Plane z1, std::uint8_t* out, int out_pitch) {
+    __shared__ FrameRef refs[kMaxChainFrames];
+    stage_frames(fr, refs);
     GRID_STRIDE(i, static_cast<long long>(W) * H) {
         if (count[i] < 2) continue;
         const int x = static_cast<int>(i % W), y = static_cast<int>(i / W);
         const int k = owner[i];
-        const Level0& f = fr->f[k];
+        const FrameRef& f = refs[k];
         const size_t j = static_cast<size_t>(y - f.box.y0) * f.box.w() + (x - f.box.x0);
         std::uint8_t* o = out + static_cast<size_t>(y) * out_pitch + 3 * x;
         float eg[3] = {0, 0, 0}, ez[3] = {0, 0, 0};
@@ -902,7 +932,7 @@ Status render_chain(int n, const ImageView* frames, const Geometry* g, BlendMode
     // grid allows (65536 blocks over all frames).
     chain_warp<<<dim3(grid_for(n_warp, std::max(2048, 65536 / n)), static_cast<unsigned>(n)), kThreads, 0, stream>>>(
         d_frames, g[0].origin_x, g[0].origin_y);
-    chain_own<<<grid_for(static_cast<long long>(W) * H, 256 * 8), kThreads, 0, stream>>>(
+    chain_own<<<grid_for(static_cast<long long>(W) * H, 256 * 64), kThreads, 0, stream>>>(
         d_frames, W, H, !bands, owner, count, output.data, output.pitch_bytes, mask.data, mask.pitch_bytes, stats);
     if (bands) {
         chain_pull1<<<dim3(static_cast<unsigned>(std::min<long long>(p1_tiles, 2048)), static_cast<unsigned>(n)),
