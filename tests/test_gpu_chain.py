@@ -1,7 +1,8 @@
 """The N-frame render (cs_s6_render_chain) on the GPU against the oracle's
 full-canvas chain render (orc_s6_render_chain): a strip of 4 frames of
 640x480 in 5 bands and the other blend modes, ragged frame sizes, a chain of
-two against the pair render, and a chain of one. The bar is the pair
+two against the pair render, a chain of one, and 3 frames of 3840x2160 in 7
+bands (the large strip's frame size and depth). The bar is the pair
 render's: the mask and the union/overlap counts exact, every 8-bit value
 within 1 of the oracle's."""
 import numpy as np
@@ -96,3 +97,14 @@ def test_chain_of_one_is_the_frame(T):
         torch.cuda.synchronize()
         assert np.array_equal(out.cpu().numpy(), frame), mode
         assert bool(mask.all()) and tuple(int(v) for v in counters.cpu()) == (130 * 90, 0)
+
+
+def test_full_size_frames_seven_bands(T):
+    # The large-strip frame size: at 7 bands the per-frame windows reach
+    # about a thousand pixels past each frame, and frames' windows overlap
+    # deeply; the oracle renders the 9203x2233 canvas on full planes (tens of
+    # seconds on the host).
+    frames, hs = strip(60, 3, 3840, 2160, overlap=0.3)
+    out, ref, g = render_both(T, list(frames), hs, "multiband_7")
+    assert (g[0].width, g[0].height) == (9203, 2233)
+    assert_close(out, ref, "multiband_7 3x3840x2160")
