@@ -483,11 +483,21 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
         int sxs[kPer], sys[kPer];
         unsigned fls[kPer], abytes[kPer][3];
         float raw[kPer][kCh];
+        // A tile whose region lies inside the level needs no reflection or
+        // clamping (a block-uniform branch); elements past the region take
+        // its last row, which is inside as well.
+        const bool interior = ix0 >= 0 && iy0 >= 0 && ix0 + kRIW <= F.w && iy0 + kRIH <= F.h;
 #pragma unroll
         for (int e = 0; e < kPer; ++e) {
             const int i = static_cast<int>(threadIdx.x) + e * kBlock;
-            sxs[e] = clampi(reflect101(ix0 + i % kRIW, F.w), 0, F.w - 1);
-            sys[e] = clampi(reflect101(iy0 + i / kRIW, F.h), 0, F.h - 1);
+            const int px = ix0 + i % kRIW, py = iy0 + min(i / kRIW, kRIH - 1);
+            if (interior) {
+                sxs[e] = px;
+                sys[e] = py;
+            } else {
+                sxs[e] = clampi(reflect101(px, F.w), 0, F.w - 1);
+                sys[e] = clampi(reflect101(py, F.h), 0, F.h - 1);
+            }
             const unsigned q = static_cast<unsigned>(sys[e]) * static_cast<unsigned>(F.w) + static_cast<unsigned>(sxs[e]);
             if (kFromZero) {
                 // A's bytes only on A's rectangle and B's colour only inside
