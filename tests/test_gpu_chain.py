@@ -1,8 +1,9 @@
 """The N-frame render (cs_s6_render_chain) on the GPU against the oracle's
 full-canvas chain render (orc_s6_render_chain): a strip of 4 frames of
 640x480 in 5 bands and the other blend modes, ragged frame sizes, a chain of
-two against the pair render, a chain of one, and 3 frames of 3840x2160 in 7
-bands (the large strip's frame size and depth). The bar is the pair
+two against the pair render, a chain of one, 3 frames of 3840x2160 in 7
+bands (the large strip's frame size and depth), and the longest chain (64
+frames). The bar is the pair
 render's: the mask and the union/overlap counts exact, every 8-bit value
 within 1 of the oracle's."""
 import numpy as np
@@ -108,3 +109,13 @@ def test_full_size_frames_seven_bands(T):
     out, ref, g = render_both(T, list(frames), hs, "multiband_7")
     assert (g[0].width, g[0].height) == (9203, 2233)
     assert_close(out, ref, "multiband_7 3x3840x2160")
+
+
+def test_longest_chain(T):
+    # The most frames a chain may hold: the per-launch tables are largest and
+    # most canvas pixels are covered by two frames.
+    frames, hs = strip(70, 64, 96, 64, overlap=0.5)
+    for mode in ("multiband_5", "feather"):
+        out, ref, g = render_both(T, list(frames), hs, mode)
+        assert (g[0].width, g[0].height) == (2792, 110)
+        assert_close(out, ref, mode)
