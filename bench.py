@@ -25,8 +25,8 @@ once with the adaptive legacy-v1 controller, once with fixed knobs (the
 batch's), and saturated (every request at t = 0) so that the budget binds;
 `pairs_per_s` there counts OK completions. "strip" stitches 8 frames of
 3840x2160 in a row (30 % overlap, K 16000, RANSAC 8192, 7 bands) onto one
-canvas of about 22000 x 2450: features once per frame, S4/S5 per adjacent
-pair, one chain render.
+canvas of about 22000 x 2450: S1-S5 of the adjacent pairs in one batched
+phase, one chain render.
 
 "roofline": the S6 stage of one batch (one render-graph replay: every S6
 kernel of every canvas) against the 8 TB/s HBM peak, with the compulsory
@@ -476,9 +476,9 @@ STRIP_K, STRIP_R, STRIP_BLEND = 16000, 8192, "multiband_7"
 def strip_bench(steps, warmup):
     """The large-frame strip: 8 frames of 3840x2160 in a row, 30 % of each
     frame under the next, K 16000, cross-checked Hamming, RANSAC 8192 and a
-    7-band blend of every frame onto one canvas. Features once per frame, S4
-    and S5 per adjacent pair, one chain render; wall-clock per strip, with the
-    event-timed phases beside it."""
+    7-band blend of every frame onto one canvas. S1-S5 of the seven adjacent
+    pairs in one batched phase, one chain render; wall-clock per strip, with
+    the event-timed phases beside it."""
     import torch
     from cs_stitch import pipeline as P
     from cs_stitch import stages as S
@@ -529,8 +529,8 @@ def strip_bench(steps, warmup):
         workload=f"{n} frames of {w}x{h} in a strip, overlap {STRIP_OVERLAP}, K {STRIP_K}, cross-check, "
                  f"RANSAC {STRIP_R}, {STRIP_BLEND} onto one canvas", status=status[-1],
         fail_stage=rec.fail_stage.decode(), strips_per_s=round(steps / total, 3),
-        ms_per_strip=round(1e3 * total / steps, 2), t_features_ms=mean("t_features"), t_match_ms=mean("t_match"),
-        t_estimate_ms=mean("t_estimate"), t_geometry_ms=mean("t_geometry"), t_render_ms=round(render_ms, 3),
+        ms_per_strip=round(1e3 * total / steps, 2), t_pairs_ms=mean("t_pairs"), t_geometry_ms=mean("t_geometry"),
+        t_render_ms=round(render_ms, 3),
         canvas=f"{rec.canvas_width}x{rec.canvas_height}", canvas_mp=round(canvas / 1e6, 2),
         union_pixels=int(rec.union_pixels), overlap_pixels=int(rec.overlap_pixels),
         keypoints=counts, inliers=[p.num_inliers for p in pairs], overlap_error_px=err,

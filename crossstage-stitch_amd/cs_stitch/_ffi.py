@@ -112,8 +112,8 @@ class StripRecord(C.Structure):
     _fields_ = [("status", C.c_int), ("fail_stage", C.c_char * 16), ("fail_pair", C.c_int),
                 ("canvas_width", C.c_int), ("canvas_height", C.c_int), ("origin_x", C.c_int), ("origin_y", C.c_int),
                 ("union_pixels", C.c_uint64), ("overlap_pixels", C.c_uint64),
-                ("t_features", C.c_double), ("t_match", C.c_double), ("t_estimate", C.c_double),
-                ("t_geometry", C.c_double), ("t_render", C.c_double), ("t_total", C.c_double)]
+                ("t_pairs", C.c_double), ("t_geometry", C.c_double), ("t_render", C.c_double),
+                ("t_total", C.c_double)]
 
 
 class RunRecord(C.Structure):

@@ -179,8 +179,8 @@ class PipelineRunner:
 
 
 class StripStitcher:
-    """n frames in a row onto one canvas (cs_strip_*): features once per frame,
-    S4/S5 per adjacent pair, one chain render. Frames are device RGB8 tensors."""
+    """n frames in a row onto one canvas (cs_strip_*): S1-S5 of every adjacent
+    pair in one batched phase, one chain render. Frames are device RGB8 tensors."""
 
     def __init__(self, max_frames, max_width, max_height, max_k=16000, max_r=8192):
         h = C.c_void_p()

@@ -451,9 +451,7 @@ int cs_strip_run(cs_strip* s, int n, const cs_image* frames, const cs_harness_co
         rec->origin_y = r.origin_y;
         rec->union_pixels = r.union_pixels;
         rec->overlap_pixels = r.overlap_pixels;
-        rec->t_features = r.t_features;
-        rec->t_match = r.t_match;
-        rec->t_estimate = r.t_estimate;
+        rec->t_pairs = r.t_pairs;
         rec->t_geometry = r.t_geometry;
         rec->t_render = r.t_render;
         rec->t_total = r.t_total;

@@ -435,11 +435,12 @@ int cs_server_release(cs_server* server, uint64_t id);
 /* Strip stitching (no counterpart in the reference, whose stitching stops at
  * pairs; its large-frame stress driver harness/stress_multires.cu:81-200 runs
  * each adjacent pair on its own canvas of at most 16,777,216 pixels): n
- * frames in a row, S1-S3 once per frame, S4 and S5 per adjacent pair
- * (pairs[k]: frame k + 1 -> frame k), the chain canvas from the composed
- * homographies (budget 2^28 pixels) and one cs_s6_render_chain. Frames are
- * device RGB8; config as for cs_runner_execute_run (single-level S2/S3).
- * Frame, feature and match memory is allocated by cs_strip_create; the
+ * frames in a row, S1-S5 of every adjacent pair in one batched phase (pairs[k]:
+ * frame k + 1 -> frame k; each equal to cs_runner_execute_run's), the chain
+ * canvas from the composed homographies (budget 2^28 pixels) and one
+ * cs_s6_render_chain. Frames are device RGB8; config as for
+ * cs_runner_execute_run (single-level S2/S3). Pair buffers are allocated by
+ * cs_strip_create; the
  * canvas and the render workspace grow on demand and stay until the next
  * growth or cs_strip_destroy. cs_strip_run returns the outcome's status (the
  * record says which stage and pair failed). */
@@ -450,7 +451,7 @@ typedef struct {
     int fail_pair;
     int canvas_width, canvas_height, origin_x, origin_y;
     uint64_t union_pixels, overlap_pixels;
-    double t_features, t_match, t_estimate, t_geometry, t_render, t_total; /* ms */
+    double t_pairs, t_geometry, t_render, t_total; /* ms: S1-S5 phase, canvas layout, S6, wall clock */
 } cs_strip_record;
 int cs_strip_create(int max_frames, int max_w, int max_h, int max_k, int max_r, cs_strip** out);
 void cs_strip_destroy(cs_strip* strip);
