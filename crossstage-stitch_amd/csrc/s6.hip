@@ -422,6 +422,42 @@ static_assert(kRIW <= kRPitch && (2 * kRPitch) % 64 == 32, "reduce tile pitch");
 // Integer [1 4 6 4 1] taps.
 __device__ __forceinline__ int itap(int d) { return d == 0 ? 6 : (d == 1 || d == -1 ? 4 : 1); }
 
+// Loads and stores at a 32-bit byte offset from a launch-uniform base: the
+// address is then one scalar base plus one vector offset (no 64-bit address
+// arithmetic per access). Every plane and image here is below 4 GiB.
+template <class T>
+__device__ __forceinline__ T ld_at(const T* base, unsigned off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+template <class T>
+__device__ __forceinline__ void st_at(T* base, unsigned off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
+// Whether pixel (x, y) of level l lies in image i's support (A 0, B 1):
+// outside it the image's raw pull sums and weights are exactly 0 and never
+// stored, so readers take 0 there instead of loading.
+__device__ __forceinline__ bool in_support(const S6Job& j, int l, int i, int x, int y) {
+    const int(&s)[4] = j.sup[l][i];
+    return x >= s[0] && x <= s[2] && y >= s[1] && y <= s[3];
+}
+
+// Raw buffer resource over [p, p + bytes): loads at a byte offset at or past
+// `bytes` return 0 without touching memory (the range check of buffer
+// instructions), so a load that only some lanes need takes no branch.
+constexpr unsigned kNoLoad = 0xfffffffcu;  // an offset past every buffer here
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, static_cast<int>(bytes), 0x00020000);
+}
+__device__ __forceinline__ float ld_buf(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+// Store through a buffer resource: at an offset at or past its size the
+// store is dropped (no branch around stores that only some lanes make).
+__device__ __forceinline__ void st_buf(__amdgpu_buffer_rsrc_t r, unsigned off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
+
 // Modes: 0 pull at l >= 2, 1 Gaussian at l >= 2, 2 pull at l = 1 (from the
 // virtual level 0). The Gaussian at l = 1 is s6_gauss1. From level 0 the
 // tile holds the six colour channels; the four level-1 weights (validity and
@@ -476,12 +512,34 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 planes[kCh - 1] = kMode == 0 ? F.vb : F.ub;
             }
         }
+        // A pull above level 1 reads an image's raw sums only inside its
+        // support at level l - 1 (0 elsewhere, where they are not stored),
+        // through buffer resources whose range check returns the 0.
+        // Elsewhere the load reads the nearest pixel of the support (written
+        // by this render, so the address is in use anyway) and the value is
+        // dropped: plain loads with no branch.
+        int sa0 = 0, sa1 = 0, sa2 = -1, sa3 = -1, sb0 = 0, sb1 = 0, sb2 = -1, sb3 = -1;
+        // Supports at level l, where the pull stores.
+        int oa0 = 0, oa1 = 0, oa2 = -1, oa3 = -1, ob0 = 0, ob1 = 0, ob2 = -1, ob3 = -1;
+        if (kMode != 1) {
+            const int(&ka)[4] = j.sup[l][0];
+            const int(&kb)[4] = j.sup[l][1];
+            oa0 = ka[0], oa1 = ka[1], oa2 = ka[2], oa3 = ka[3];
+            ob0 = kb[0], ob1 = kb[1], ob2 = kb[2], ob3 = kb[3];
+        }
+        if (kMode == 0) {
+            const int(&ka)[4] = j.sup[l - 1][0];
+            const int(&kb)[4] = j.sup[l - 1][1];
+            sa0 = ka[0], sa1 = ka[1], sa2 = ka[2], sa3 = ka[3];
+            sb0 = kb[0], sb1 = kb[1], sb2 = kb[2], sb3 = kb[3];
+        }
         // Pass 1: every load of this thread's tile elements, issued before any
         // is used (positions are clamped into the level, so all are in
         // bounds; level-0 values of invalid pixels are discarded below).
         constexpr int kElems = kRIW * kRIH, kPer = (kElems + kBlock - 1) / kBlock;
         int sxs[kPer], sys[kPer];
         unsigned fls[kPer], abytes[kPer][3];
+        bool ins[kPer][2];  // pulls above level 1: element inside A's / B's support
         float raw[kPer][kCh];
         // A tile whose region lies inside the level needs no reflection or
         // clamping (a block-uniform branch); elements past the region take
@@ -499,6 +557,19 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 sys[e] = clampi(reflect101(py, F.h), 0, F.h - 1);
             }
             const unsigned q = static_cast<unsigned>(sys[e]) * static_cast<unsigned>(F.w) + static_cast<unsigned>(sxs[e]);
+            unsigned qa = q, qb = q;
+            bool& ina = ins[e][0];
+            bool& inb = ins[e][1];
+            ina = inb = true;
+            if (kMode == 0) {
+                // An empty support (never, for a renderable pair) reads pixel 0.
+                const int ax = max(sa0, min(sxs[e], sa2)), ay = max(sa1, min(sys[e], sa3));
+                const int bx2 = max(sb0, min(sxs[e], sb2)), by2 = max(sb1, min(sys[e], sb3));
+                ina = ax == sxs[e] && ay == sys[e];
+                inb = bx2 == sxs[e] && by2 == sys[e];
+                qa = sa2 < sa0 || sa3 < sa1 ? 0u : static_cast<unsigned>(ay) * static_cast<unsigned>(F.w) + ax;
+                qb = sb2 < sb0 || sb3 < sb1 ? 0u : static_cast<unsigned>(by2) * static_cast<unsigned>(F.w) + bx2;
+            }
             if (kFromZero) {
                 // A's bytes only on A's rectangle and B's colour only inside
                 // B's box: both are known from the position alone, so the
@@ -519,7 +590,10 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 }
             } else {
 #pragma unroll
-                for (int c = 0; c < kCh; ++c) raw[e][c] = planes[c][q];
+                for (int c = 0; c < kCh; ++c) {
+                    if (kMode == 0) raw[e][c] = ld_at(planes[c], 4u * (c < 3 || c == kCh - 2 ? qa : qb));
+                    else raw[e][c] = planes[c][q];
+                }
             }
         }
         // Pass 2: level-0 selection, then the LDS tile.
@@ -546,7 +620,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 fl[iy][ix] = static_cast<std::uint8_t>(f);
             } else {
 #pragma unroll
-                for (int c = 0; c < kCh; ++c) v[c] = raw[e][c];
+                for (int c = 0; c < kCh; ++c) v[c] = ins[e][c < 3 || c == kCh - 2 ? 0 : 1] ? raw[e][c] : 0.0f;
             }
 #pragma unroll
             for (int c = 0; c < kCh; ++c) tile[c][iy][cx] = v[c];
@@ -606,10 +680,19 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
             const size_t p = static_cast<size_t>(y) * C.w + x;
             float* da = kMode == 1 ? C.ga : C.a;
             float* db = kMode == 1 ? C.gb : C.b;
+            // Pulls store an image's sums only inside its support at level l:
+            // elsewhere the buffer store's offset is out of range and the
+            // store is dropped.
+            const unsigned pb4 = static_cast<unsigned>(4 * p), nb4 = static_cast<unsigned>(4 * C.n);
+            const unsigned oa = kMode == 1 || (x >= oa0 && x <= oa2 && y >= oa1 && y <= oa3) ? pb4 : kNoLoad;
+            const unsigned ob = kMode == 1 || (x >= ob0 && x <= ob2 && y >= ob1 && y <= ob3) ? pb4 : kNoLoad;
+            const bool own = kFromZero && x >= j.gbox[1][0] && x <= j.gbox[1][2] && y >= j.gbox[1][1] &&
+                             y <= j.gbox[1][3];
+            const __amdgpu_buffer_rsrc_t ra = buffer_of(da, 3 * nb4), rb = buffer_of(db, 3 * nb4);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                da[c * C.n + p] = acc[c];
-                db[c * C.n + p] = acc[3 + c];
+                st_buf(ra, oa == kNoLoad ? kNoLoad : oa + c * nb4, acc[c]);
+                st_buf(rb, ob == kNoLoad ? kNoLoad : ob + c * nb4, acc[3 + c]);
             }
             if (kFromZero) {
                 int va = 0, vb = 0, ua = 0, ub = 0;
@@ -630,17 +713,17 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                         ua += (f & kUA) ? w : 0;
                         ub += (f & kUB) ? w : 0;
                     }
-                C.va[p] = static_cast<float>(va) * (1.0f / 256.0f);
-                C.vb[p] = static_cast<float>(vb) * (1.0f / 256.0f);
+                st_buf(buffer_of(C.va, nb4), oa, static_cast<float>(va) * (1.0f / 256.0f));
+                st_buf(buffer_of(C.vb, nb4), ob, static_cast<float>(vb) * (1.0f / 256.0f));
                 // Ownership weights are read only inside the level-1 Gaussian
                 // region (by the band mix and the Gaussian into level 2).
-                if (x >= j.gbox[1][0] && x <= j.gbox[1][2] && y >= j.gbox[1][1] && y <= j.gbox[1][3]) {
+                if (own) {
                     C.ua[p] = static_cast<float>(ua) * (1.0f / 256.0f);
                     C.ub[p] = static_cast<float>(ub) * (1.0f / 256.0f);
                 }
             } else {
-                (kMode == 0 ? C.va : C.ua)[p] = acc[6];
-                (kMode == 0 ? C.vb : C.ub)[p] = acc[7];
+                st_buf(buffer_of(kMode == 0 ? C.va : C.ua, nb4), oa, acc[6]);
+                st_buf(buffer_of(kMode == 0 ? C.vb : C.ub, nb4), ob, acc[7]);
             }
         }
         __syncthreads();
@@ -820,13 +903,22 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
         // This pixel's raw pull sums and weights first. The raw sums (a / b)
         // are never written here: the result goes to ga / gb, so
         // neighbouring tiles read unchanged halos.
+        // Raw level-1 planes through buffer resources: 0 outside an image's
+        // support, where they are not stored.
+        const unsigned nb = static_cast<unsigned>(4 * C.n);
+        const __amdgpu_buffer_rsrc_t pa[4] = {buffer_of(C.a, nb), buffer_of(C.a + C.n, nb), buffer_of(C.a + 2 * C.n, nb),
+                                              buffer_of(C.va, nb)};
+        const __amdgpu_buffer_rsrc_t pb[4] = {buffer_of(C.b, nb), buffer_of(C.b + C.n, nb), buffer_of(C.b + 2 * C.n, nb),
+                                              buffer_of(C.vb, nb)};
+        const unsigned oa = inside && in_support(j, 1, 0, x, y) ? static_cast<unsigned>(4 * p) : kNoLoad;
+        const unsigned ob = inside && in_support(j, 1, 1, x, y) ? static_cast<unsigned>(4 * p) : kNoLoad;
         float ra[3], rb[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            ra[c] = C.a[c * C.n + p];
-            rb[c] = C.b[c * C.n + p];
+            ra[c] = ld_buf(pa[c], oa);
+            rb[c] = ld_buf(pb[c], ob);
         }
-        const float wa = C.va[p], wb = C.vb[p];
+        const float wa = ld_buf(pa[3], oa), wb = ld_buf(pb[3], ob);
         // An image's pushed level 1 is needed only if some output of the
         // tile misses it somewhere in its footprint (weight below 1); only
         // then are its staged level-1 pixels and level 2 read.
@@ -841,14 +933,14 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
                 const int cx = clampi(x0 - 2 + i % kGPW, 0, C.w - 1), cy = clampi(y0 - 2 + i / kGPW, 0, C.h - 1);
                 const unsigned q = static_cast<unsigned>(cy) * static_cast<unsigned>(C.w) + static_cast<unsigned>(cx);
                 if (need_a) {
+                    const unsigned o = in_support(j, 1, 0, cx, cy) ? 4u * q : kNoLoad;
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) sa[it][c] = C.a[c * C.n + q];
-                    sa[it][3] = C.va[q];
+                    for (int c = 0; c < 4; ++c) sa[it][c] = ld_buf(pa[c], o);
                 }
                 if (need_b) {
+                    const unsigned o = in_support(j, 1, 1, cx, cy) ? 4u * q : kNoLoad;
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) sb[it][c] = C.b[c * C.n + q];
-                    sb[it][3] = C.vb[q];
+                    for (int c = 0; c < 4; ++c) sb[it][c] = ld_buf(pb[c], o);
                 }
             }
             if (coarse) {
@@ -960,19 +1052,27 @@ __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev:
         const bool coarse = l < j.top;
         const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 1;
         const int x = x0 + static_cast<int>(threadIdx.x) % kEW;
+        const unsigned nb = static_cast<unsigned>(4 * L.n);
+        const __amdgpu_buffer_rsrc_t pa[4] = {buffer_of(L.a, nb), buffer_of(L.a + L.n, nb), buffer_of(L.a + 2 * L.n, nb),
+                                              buffer_of(L.va, nb)};
+        const __amdgpu_buffer_rsrc_t pb[4] = {buffer_of(L.b, nb), buffer_of(L.b + L.n, nb), buffer_of(L.b + 2 * L.n, nb),
+                                              buffer_of(L.vb, nb)};
         float wa[kEPer], wb[kEPer], va[kEPer][3], vb[kEPer][3];
         bool missing = false;
 #pragma unroll
         for (int k = 0; k < kEPer; ++k) {
             const int y = y0 + static_cast<int>(threadIdx.x) / kEW + k * (kBlock / kEW);
             const bool inside = x < L.w && y < L.h;
-            const size_t p = inside ? static_cast<size_t>(y) * L.w + x : 0;
-            wa[k] = inside ? L.va[p] : 1.0f;
-            wb[k] = inside ? L.vb[p] : 1.0f;
+            const unsigned p = inside ? static_cast<unsigned>(y) * static_cast<unsigned>(L.w) + x : 0u;
+            // Raw sums: 0 outside an image's support, where they are not stored.
+            const unsigned oa = inside && in_support(j, l, 0, x, y) ? 4u * p : kNoLoad;
+            const unsigned ob = inside && in_support(j, l, 1, x, y) ? 4u * p : kNoLoad;
+            wa[k] = inside ? ld_buf(pa[3], oa) : 1.0f;
+            wb[k] = inside ? ld_buf(pb[3], ob) : 1.0f;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                va[k][c] = L.a[c * L.n + p];
-                vb[k][c] = L.b[c * L.n + p];
+                va[k][c] = ld_buf(pa[c], oa);
+                vb[k][c] = ld_buf(pb[c], ob);
             }
             missing |= !(wa[k] > 0) || !(wb[k] > 0);
         }
@@ -1443,6 +1543,26 @@ static void s6_regions(S6Job* job) {
     }
 }
 
+// Level l support from level l - 1: X such that [2X - 2, 2X + 2] meets
+// [s0, s1], within the level. Reflected footprint positions fold back inside
+// the same clipped span, so reflection adds nothing.
+static void s6_supports(S6Job* job) {
+    S6Job& j = *job;
+    for (int l = 0; l < kS6MaxLevels; ++l)
+        for (int i = 0; i < 2; ++i) put(j.sup[l][i], kEmpty, 1, 1);
+    if (j.top < 1) return;
+    Box s[2] = {clip({-j.origin_x, -j.origin_y, -j.origin_x + j.wa - 1, -j.origin_y + j.ha - 1}, j.width, j.height),
+                clip({j.bx0, j.by0, j.bx1, j.by1}, j.width, j.height)};
+    for (int l = 0; l <= j.top; ++l) {
+        for (int i = 0; i < 2; ++i) {
+            if (l > 0 && !s[i].empty())
+                s[i] = clip({(s[i].x0 - 1) >> 1, (s[i].y0 - 1) >> 1, (s[i].x1 + 2) >> 1, (s[i].y1 + 2) >> 1}, j.lv[l].w,
+                            j.lv[l].h);
+            put(j.sup[l][i], s[i], 1, 1);
+        }
+    }
+}
+
 Status make_s6_job(const ImageView& a, const ImageView& b, const s6::Geometry& g, s6::BlendMode mode,
                    const ImageView& output, const ImageView& mask, void* ws, std::size_t available,
                    s6::RenderStatistics* stats, S6Job* job) {
@@ -1558,6 +1678,7 @@ Status make_s6_job(const ImageView& a, const ImageView& b, const s6::Geometry& g
         }
     }
     s6_regions(&j);
+    s6_supports(&j);
     *job = j;
     return Status::OK;
 }

@@ -68,6 +68,12 @@ struct S6Job {
     int gbox[kS6MaxLevels][4];
     int pbox[kS6MaxLevels][4];
     int ubox[kS6MaxLevels][4];
+    // Support of A (0) and B (1) at each level: the box outside which the
+    // image's raw pull sums and weights are exactly 0 (level 0: A's canvas
+    // rectangle and B's box; level l: every pixel whose 5x5 footprint at
+    // level l - 1 meets the support there). Pulls store nothing outside it
+    // and readers take 0 there without a load.
+    int sup[kS6MaxLevels][2][4];
     long long warp_begin;
     long long rb[kS6MaxLevels];  // pull (reduce into level l) tiles
     long long gb[kS6MaxLevels];  // Gaussian (reduce into level l) tiles
