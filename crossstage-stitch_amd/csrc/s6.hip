@@ -140,9 +140,16 @@ __device__ __forceinline__ int next_job_by(const dev::JobSpan<S6Job>& span, long
 
 constexpr unsigned kVA = 1u, kVB = 2u, kUA = 4u, kUB = 8u;
 
-// float(v / 255.0) for every byte: A's level-0 colour.
-__device__ __forceinline__ void fill_lut(float* lut) {
-    for (int t = threadIdx.x; t < 256; t += blockDim.x) lut[t] = static_cast<float>(static_cast<double>(t) / 255.0);
+// float(v / 255.0) for byte k of w: A's level-0 colour. q = v * RN(1/255)
+// is within an ulp of v / 255; one correction with the residual v - 255 q
+// (exact through a fused multiply-add) rounds it correctly, which an
+// exhaustive check over the 256 bytes confirms. Four VALU instructions and
+// no table, so no LDS bank conflicts between lanes looking up different bytes.
+__device__ __forceinline__ float unit_byte(unsigned w, int k) {
+    const float v = static_cast<float>((w >> (8 * k)) & 0xffu);
+    constexpr float kInv = 1.0f / 255.0f;
+    const float q = v * kInv;
+    return fmaf(fmaf(-q, 255.0f, v), kInv, q);
 }
 
 __device__ __forceinline__ const std::uint8_t* a_pixel(const S6Job& j, int x, int y) {
@@ -204,7 +211,7 @@ __device__ __forceinline__ WarpRow warp_row(const S6Job& j, int y) {
     return {ya, j.inverse[1] * ya, j.inverse[4] * ya, j.inverse[7] * ya, fmin(ya + 0.5, j.ha - 0.5 - ya) / j.ha};
 }
 
-__device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int x, int y, const float* lut,
+__device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int x, int y,
                                            unsigned long long& uni, unsigned long long& ovl) {
     const size_t p = static_cast<size_t>(y) * j.width + x;
     const int ax = x + j.origin_x, ay = y + j.origin_y;
@@ -261,9 +268,9 @@ __device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int
         if (ma) {
             unsigned lo, hi;
             load_rgb<true>(a_pixel(j, x, y), false, lo, hi);
-            ra[0] = lut[byte_of(lo, 0)];
-            ra[1] = lut[byte_of(lo, 1)];
-            ra[2] = lut[byte_of(lo, 2)];
+            ra[0] = unit_byte(lo, 0);
+            ra[1] = unit_byte(lo, 1);
+            ra[2] = unit_byte(lo, 2);
         }
         std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
 #pragma unroll
@@ -284,9 +291,6 @@ constexpr int kWarpSeg = 4 * kBlock;
 __global__ void __launch_bounds__(kBlock) s6_warp(dev::JobSpan<S6Job> span, dev::Total work) {
     const long long total = work.get();
     __shared__ unsigned long long red[2][kBlock / 64];
-    __shared__ float lut[256];
-    fill_lut(lut);
-    __syncthreads();
     int cur = -1, ji = 0;
     const int njobs = span.jobs();
     unsigned long long uni = 0, ovl = 0;
@@ -307,7 +311,7 @@ __global__ void __launch_bounds__(kBlock) s6_warp(dev::JobSpan<S6Job> span, dev:
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int x = x0 + k * kBlock;
-            if (x < j.width) warp_pixel(j, wr, x, static_cast<int>(row), lut, uni, ovl);
+            if (x < j.width) warp_pixel(j, wr, x, static_cast<int>(row), uni, ovl);
         }
     }
     if (cur >= 0 && dev::job_at(span, cur).stats) flush_stats(dev::job_at(span, cur), uni, ovl, red);
@@ -471,13 +475,8 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
     constexpr bool kFromZero = kMode == 2;
     constexpr int kCh = kFromZero ? 6 : 8;
     __shared__ float tile[kCh][kRIH][kRPitch];
-    __shared__ float lut[kFromZero ? 256 : 1];
     __shared__ std::uint8_t fl[kFromZero ? kRIH : 1][kRIW + 1];
     __shared__ unsigned wave_flags[2][kBlock / 64];  // per wave: OR and AND of the staged flags
-    if (kFromZero) {
-        fill_lut(lut);
-        __syncthreads();
-    }
     int ji = 0;
     const int njobs = span.jobs();
     for (dev::TileWalk w = dev::tile_walk(total); w.t < w.end; w.t += w.step) {
@@ -538,7 +537,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
         // bounds; level-0 values of invalid pixels are discarded below).
         constexpr int kElems = kRIW * kRIH, kPer = (kElems + kBlock - 1) / kBlock;
         int sxs[kPer], sys[kPer];
-        unsigned fls[kPer], abytes[kPer][3];
+        unsigned fls[kPer], abytes[kPer];
         bool ins[kPer][2];  // pulls above level 1: element inside A's / B's support
         float raw[kPer][kCh];
         // A tile whose region lies inside the level needs no reflection or
@@ -583,11 +582,9 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 unsigned lo, hi;
                 load_rgb(j.a + static_cast<size_t>(in_a ? ay : 0) * j.pitch_a + 3 * (in_a ? ax : 0), false, lo, hi);
                 lo = in_a ? lo : 0u;
+                abytes[e] = lo;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    abytes[e][c] = byte_of(lo, c);
-                    raw[e][3 + c] = in_b ? planes[3 + c][q] : 0.0f;
-                }
+                for (int c = 0; c < 3; ++c) raw[e][3 + c] = in_b ? planes[3 + c][q] : 0.0f;
             } else {
 #pragma unroll
                 for (int c = 0; c < kCh; ++c) {
@@ -614,7 +611,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 const bool has_a = f & kVA, has_b = f & kVB;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    v[c] = has_a ? lut[abytes[e][c]] : 0.0f;
+                    v[c] = has_a ? unit_byte(abytes[e], c) : 0.0f;
                     v[3 + c] = has_b ? raw[e][3 + c] : 0.0f;
                 }
                 fl[iy][ix] = static_cast<std::uint8_t>(f);
@@ -1114,11 +1111,6 @@ __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev:
 __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::Total work, int l) {
     const long long total = work.get();
     __shared__ float4 cc[3][kCH][kCW];
-    __shared__ float lut[256];
-    if (l == 0) {
-        fill_lut(lut);
-        __syncthreads();
-    }
     int ji = 0;
     const int njobs = span.jobs();
     for (dev::TileWalk w = dev::tile_walk(total); w.t < w.end; w.t += w.step) {
@@ -1145,7 +1137,7 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
             // pixels' output came from the warp), the owner's colour only:
             // an overlap pixel's result is its owner's colour plus the
             // expansion of the owner's difference plane.
-            unsigned ab[kEPer][3];
+            unsigned ab[kEPer];  // A's bytes (R, G, B in bytes 0-2)
 #pragma unroll
             for (int k = 0; k < kEPer; ++k) {
                 const int y = y0 + static_cast<int>(threadIdx.x) / kEW + k * (kBlock / kEW);
@@ -1154,13 +1146,13 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
 #pragma unroll
             for (int k = 0; k < kEPer; ++k) {
                 const int y = y0 + static_cast<int>(threadIdx.x) / kEW + k * (kBlock / kEW);
-                ab[k][0] = ab[k][1] = ab[k][2] = 0;
+                ab[k] = 0;
                 gb[k][0] = gb[k][1] = gb[k][2] = 0;
                 const bool both = (f[k] & (kVA | kVB)) == (kVA | kVB);
                 if (both && (f[k] & kUA)) {
                     const std::uint8_t* src = a_pixel(j, x, y);
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) ab[k][c] = src[c];
+                    for (int c = 0; c < 3; ++c) ab[k] |= static_cast<unsigned>(src[c]) << (8 * c);
                 }
                 if (both && !(f[k] & kUA)) {
                     const unsigned p = static_cast<unsigned>(y) * static_cast<unsigned>(L.w) + x;
@@ -1171,7 +1163,7 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
 #pragma unroll
             for (int k = 0; k < kEPer; ++k) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) ga[k][c] = lut[ab[k][c]];
+                for (int c = 0; c < 3; ++c) ga[k][c] = unit_byte(ab[k], c);
                 ua[k] = (f[k] & kUA) ? 1.0f : 0.0f;
                 ub[k] = (f[k] & kUB) ? 1.0f : 0.0f;
                 const bool both = (f[k] & (kVA | kVB)) == (kVA | kVB);

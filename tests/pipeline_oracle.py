@@ -26,6 +26,35 @@ def run(a, b, S, K, R, blend, cross_check, fast=20, ratio=0.8, refinements=1, th
     return out
 
 
+def replay_attempts(a, b, trace, seed=0, fast=20, ratio=0.8, threshold=3.0, refinements=1):
+    """The adaptive run's attempts (harness/pipeline_runner.cu:531-885) on the
+    oracle, from the knobs and reuse plan the run recorded: an attempt that
+    restarts (the first one, or restart_from S1/S2) runs S1-S4 at its S and
+    K without cross-check, one that restarts at S5 keeps the previous
+    matches; S5 then evaluates hypotheses [previous_R, R) from the state the
+    earlier attempts left (a fresh state when previous_R is 0). Returns the
+    oracle's S5 result per attempt and the matching/feature state of the
+    last one."""
+    results, feats, state = [], None, None
+    for t in trace:
+        restart = t.restart_from.decode()
+        if feats is None or restart in ("none", "S1", "S2"):
+            ga, gb = O.s1_preprocess(a, t.S), O.s1_preprocess(b, t.S)
+            ka, kb = O.s2_detect(ga, fast, t.K), O.s2_detect(gb, fast, t.K)
+            da, db = O.s3_describe(ga, ka), O.s3_describe(gb, kb)
+            ia, ib, _ = O.s4_match(da, db, ratio, False)
+            _, sa = O.scale_info(a.shape[1], a.shape[0], t.S)
+            _, sb = O.scale_info(b.shape[1], b.shape[0], t.S)
+            feats = (ka, kb, ia, ib, sa, sb)
+        ka, kb, ia, ib, sa, sb = feats
+        if t.previous_R == 0:
+            state = None
+        res, _, state = O.s5_estimate_kp(ka[:, 0], ka[:, 1], kb[:, 0], kb[:, 1], ia, ib, sa, sb, t.R, refinements,
+                                         threshold, seed, previous_hypotheses=t.previous_R, state=state)
+        results.append(res)
+    return results
+
+
 def check_outputs(d, ref, what=""):
     """Host copies of one pair's device outputs (keys as tests/test_gpu_batch
     download_outcome: kp_a/kp_b (n, 3) float32, desc_a/desc_b bytes, nm,

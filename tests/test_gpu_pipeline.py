@@ -38,9 +38,13 @@ def test_plain_runner_matches_oracle(seed, w, h, S, K, R, blend, cross):
 
 def test_adaptive_runner_legacy_policy():
     """The adaptive path (probe -> rung -> attempts with S5 reuse -> S6) is
-    deterministic and walks the legacy-v1 ladder; whether a pair is accepted
-    depends on the calibrated self-checks, so every terminal status of the
-    reference's decision logic is allowed."""
+    deterministic and walks the legacy-v1 ladder. Every attempt's S5 result
+    (status, inlier count, H bit for bit) equals the oracle's for the
+    attempt's knobs and reuse plan (pipeline_oracle.replay_attempts), and an
+    accepted pair's panorama is the oracle's render of the accepted H within
+    1 LSB, union mask exact. Which terminal status a pair reaches follows
+    from those results through the controller, whose decisions are pinned
+    against the reference controller on the CPU (tests/test_controller.py)."""
     from cs_stitch import pipeline as P
     ladder = [(.25, 500, 128), (.5, 1000, 128), (.5, 1000, 256), (.75, 2000, 512), (1., 2000, 1024)]
     for seed, ov in ((7, (0.6, 0.8)), (8, (0.85, 0.95))):
@@ -63,17 +67,32 @@ def test_adaptive_runner_legacy_policy():
         # attempt starts from scratch.
         trace = runner.attempt_trace()
         assert len(trace) == rec.attempt_count
-        for i, a in enumerate(trace):
-            assert (a.S, a.K, a.R) in ladder and a.has_homography and a.has_decision
-            assert a.t_s5 > 0 and a.t_attempt >= a.t_s5
+        for i, at in enumerate(trace):
+            assert (at.S, at.K, at.R) in ladder and at.has_homography and at.has_decision
+            assert at.t_s5 > 0 and at.t_attempt >= at.t_s5
             if i + 1 < len(trace):
-                assert a.action == 1 and (a.next_S, a.next_K, a.next_R) == (trace[i + 1].S, trace[i + 1].K,
-                                                                              trace[i + 1].R)
+                assert at.action == 1 and (at.next_S, at.next_K, at.next_R) == (trace[i + 1].S, trace[i + 1].K,
+                                                                                 trace[i + 1].R)
         assert trace[0].restart_from == b"none" and trace[0].previous_R == 0
         assert trace[-1].action == (0 if status == "OK" else 2)
         assert (trace[-1].S, trace[-1].K, trace[-1].R) == (rec.S, rec.K, rec.R)
         if status == "OK":
             assert trace[-1].homography.num_inliers == rec.num_inliers
+        # Attempt by attempt against the oracle.
+        for i, (t, want) in enumerate(zip(trace, pipeline_oracle.replay_attempts(a, b, trace))):
+            got = t.homography
+            assert (got.status, got.num_matches, got.num_inliers) == (want.status, want.num_matches,
+                                                                      want.num_inliers), (seed, i)
+            if want.status == 0:
+                assert list(got.H) == list(want.H), (seed, i)
+        if status == "OK":
+            st_, geo = O.prepare_geometry(640, 480, 640, 480, np.array(rec.H[:]))
+            assert st_ == 0
+            pano, pmask, _ = O.s6_render(a, b, geo, trace[-1].blend_mode)
+            out = runner.outputs()
+            got = P.download_image(out["canvas"], 3)
+            assert np.array_equal(P.download_image(out["mask"], 1), pmask)
+            assert np.abs(got.astype(int) - pano.astype(int)).max() <= 1
         rec2 = runner.execute_adaptive_run(ctl, seed=0)
         assert rec2.status == rec.status and rec2.attempt_count == rec.attempt_count
         assert list(rec2.H) == list(rec.H) or status != "OK"
