@@ -918,10 +918,14 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
         const float wa = ld_buf(pa[3], oa), wb = ld_buf(pb[3], ob);
         // An image's pushed level 1 is needed only if some output of the
         // tile misses it somewhere in its footprint (weight below 1); only
-        // then are its staged level-1 pixels and level 2 read.
-        const bool need_a = __syncthreads_or(inside && wa != 1.0f);
-        const bool need_b = __syncthreads_or(inside && wb != 1.0f);
-        if (need_a || need_b) {
+        // then are its staged level-1 pixels and level 2 read. That can only
+        // happen where the tile's level-0 footprint leaves A's rectangle, or
+        // B's inner box: known from the geometry, so those stagings are
+        // issued with this pixel's own loads (one memory round trip). A vote
+        // over the loaded weights confirms it; an image the geometry ruled
+        // out but some weight needs is staged afterwards.
+        const auto stage = [&](bool need_a, bool need_b) {
+            if (!need_a && !need_b) return;
             float sa[kGSIt][4], sb[kGSIt][4];
 #pragma unroll
             for (int it = 0; it < kGSIt; ++it) {
@@ -986,7 +990,16 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
                 }
             }
             __syncthreads();
-        }
+        };
+        const int ex1 = min(x0 + kGW - 1, C.w - 1), ey1 = min(y0 + kGH - 1, C.h - 1);
+        const int fx0 = 2 * x0 - 2, fx1 = 2 * ex1 + 2, fy0 = 2 * y0 - 2, fy1 = 2 * ey1 + 2;
+        const int rx0 = -j.origin_x, ry0 = -j.origin_y, rx1 = rx0 + j.wa - 1, ry1 = ry0 + j.ha - 1;
+        const bool geo_a = !(fx0 >= rx0 && fx1 <= rx1 && fy0 >= ry0 && fy1 <= ry1);
+        const bool geo_b = !(fx0 >= j.bi[0] && fx1 <= j.bi[2] && fy0 >= j.bi[1] && fy1 <= j.bi[3]);
+        stage(geo_a, geo_b);
+        const bool late_a = __syncthreads_or(inside && wa != 1.0f) && !geo_a;
+        const bool late_b = __syncthreads_or(inside && wb != 1.0f) && !geo_b;
+        stage(late_a, late_b);
         if (inside) {
             // The footprint is all valid where the weight is 1 and all
             // missing where it is 0 (the weights are exact sums of taps).
@@ -1131,34 +1144,27 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
         // kept only where A is valid (B likewise).
         float ga[kEPer][3], gb[kEPer][3], ua[kEPer], ub[kEPer];
         unsigned f[kEPer];
-        bool any_own_a = false, any_own_b = false;
         if (l == 0) {
-            // Flags first; then, where both images are valid (the other
-            // pixels' output came from the warp), the owner's colour only:
-            // an overlap pixel's result is its owner's colour plus the
-            // expansion of the owner's difference plane.
+            // An overlap pixel's result is its owner's colour plus the
+            // expansion of the owner's difference plane (the other pixels'
+            // output came from the warp). Flags, both images' colours and (in
+            // the staging below) both difference planes are loaded together
+            // from in-bounds positions, one memory round trip per tile; what
+            // a pixel does not own is dropped. A's position is clamped into
+            // its rectangle (tiles start on the tile grid, left of or above
+            // the overlap box); B's colour planes cover the canvas.
             unsigned ab[kEPer];  // A's bytes (R, G, B in bytes 0-2)
 #pragma unroll
             for (int k = 0; k < kEPer; ++k) {
                 const int y = y0 + static_cast<int>(threadIdx.x) / kEW + k * (kBlock / kEW);
-                f[k] = x < L.w && y < L.h ? L.flags[static_cast<unsigned>(y) * static_cast<unsigned>(L.w) + x] : 0u;
-            }
+                const bool inside = x < L.w && y < L.h;
+                const unsigned p = inside ? static_cast<unsigned>(y) * static_cast<unsigned>(L.w) + x : 0u;
+                f[k] = inside ? L.flags[p] : 0u;
+                const int ax = clampi(x + j.origin_x, 0, j.wa - 1), ay = clampi(y + j.origin_y, 0, j.ha - 1);
+                unsigned hi;
+                load_rgb(j.a + static_cast<size_t>(ay) * j.pitch_a + 3 * ax, false, ab[k], hi);
 #pragma unroll
-            for (int k = 0; k < kEPer; ++k) {
-                const int y = y0 + static_cast<int>(threadIdx.x) / kEW + k * (kBlock / kEW);
-                ab[k] = 0;
-                gb[k][0] = gb[k][1] = gb[k][2] = 0;
-                const bool both = (f[k] & (kVA | kVB)) == (kVA | kVB);
-                if (both && (f[k] & kUA)) {
-                    const std::uint8_t* src = a_pixel(j, x, y);
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) ab[k] |= static_cast<unsigned>(src[c]) << (8 * c);
-                }
-                if (both && !(f[k] & kUA)) {
-                    const unsigned p = static_cast<unsigned>(y) * static_cast<unsigned>(L.w) + x;
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) gb[k][c] = L.b[c * L.n + p];
-                }
+                for (int c = 0; c < 3; ++c) gb[k][c] = L.b[c * L.n + p];
             }
 #pragma unroll
             for (int k = 0; k < kEPer; ++k) {
@@ -1166,9 +1172,6 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
                 for (int c = 0; c < 3; ++c) ga[k][c] = unit_byte(ab[k], c);
                 ua[k] = (f[k] & kUA) ? 1.0f : 0.0f;
                 ub[k] = (f[k] & kUB) ? 1.0f : 0.0f;
-                const bool both = (f[k] & (kVA | kVB)) == (kVA | kVB);
-                any_own_a |= both && (f[k] & kUA);
-                any_own_b |= both && !(f[k] & kUA);
             }
         }
 #pragma unroll
@@ -1187,33 +1190,27 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
                 ub[k] = L.ub[p];
             }
         }
-        // The coarse level behind the tile: at level 0 only for tiles with
-        // an overlap pixel (known after a first barrier); above it every
-        // tile needs it, so its loads go out with the pixels' own.
-        bool staged = false;
+        // The coarse level behind the tile; its loads go out with the
+        // pixels' own.
+        const bool staged = coarse;
         if (coarse) {
             const S6Level& C = j.lv[l + 1];
             StageRGB<kCW, kCW, kCH> s0, s1, s2;
-            // Level 0: each difference plane only if some pixel's owner uses it.
-            const bool need_a = l == 0 && __syncthreads_or(any_own_a);
-            const bool need_b = l == 0 && __syncthreads_or(any_own_b);
-            staged = l > 0 || need_a || need_b;
-            if (staged && l > 0) {
+            if (l > 0) {
                 s0.load(C.ga, C.n, C.w, C.h, cx0, cy0);
                 s1.load(C.gb, C.n, C.w, C.h, cx0, cy0);
                 s2.load(C.z, C.n, C.w, C.h, cx0, cy0);
                 s0.store(&cc[0][0][0]);
                 s1.store(&cc[1][0][0]);
                 s2.store(&cc[2][0][0]);
-                __syncthreads();
-            } else if (staged) {
+            } else {
                 // Level 0 expands z1 - ga1 (kept in z) and z1 - gb1 (kept in a).
-                if (need_a) s0.load(C.z, C.n, C.w, C.h, cx0, cy0);
-                if (need_b) s1.load(C.a, C.n, C.w, C.h, cx0, cy0);
-                if (need_a) s0.store(&cc[0][0][0]);
-                if (need_b) s1.store(&cc[1][0][0]);
-                __syncthreads();
+                s0.load(C.z, C.n, C.w, C.h, cx0, cy0);
+                s1.load(C.a, C.n, C.w, C.h, cx0, cy0);
+                s0.store(&cc[0][0][0]);
+                s1.store(&cc[1][0][0]);
             }
+            __syncthreads();
         }
         // Pass 2: band mix and collapse where both images are valid.
 #pragma unroll
@@ -1392,11 +1389,13 @@ namespace {
 // canvas pixel outside it maps more than a pixel outside B, which its
 // validity test (exact to ~1e-10 px) rejects; the whole canvas when the
 // corners cannot be mapped.
-void b_box(const s6::Geometry& g, int box[4]) {
+void b_box(const s6::Geometry& g, int box[4], int inner[4]) {
     box[0] = 0;
     box[1] = 0;
     box[2] = g.width - 1;
     box[3] = g.height - 1;
+    inner[0] = inner[1] = 0;
+    inner[2] = inner[3] = -1;
     using R = long double;
     R m[9];
     for (int i = 0; i < 9; ++i) m[i] = g.inverse[i];
@@ -1406,7 +1405,7 @@ void b_box(const s6::Geometry& g, int box[4]) {
     const R f[9] = {c0, m[2] * m[7] - m[1] * m[8], m[1] * m[5] - m[2] * m[4],
                     c1, m[0] * m[8] - m[2] * m[6], m[2] * m[3] - m[0] * m[5],
                     c2, m[1] * m[6] - m[0] * m[7], m[0] * m[4] - m[1] * m[3]};
-    R x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    R x0 = 0, y0 = 0, x1 = 0, y1 = 0, cx[4], cy[4];
     for (int k = 0; k < 4; ++k) {
         const R x = (k & 1) ? g.width_b - 1 : 0, y = (k & 2) ? g.height_b - 1 : 0;
         const R w = f[6] * x + f[7] * y + f[8];
@@ -1414,10 +1413,28 @@ void b_box(const s6::Geometry& g, int box[4]) {
         const R px = (f[0] * x + f[1] * y + f[2]) / w - g.origin_x;
         const R py = (f[3] * x + f[4] * y + f[5]) / w - g.origin_y;
         if (!std::isfinite(px) || !std::isfinite(py)) return;
+        cx[k] = px;
+        cy[k] = py;
         x0 = k ? std::min(x0, px) : px;
         x1 = k ? std::max(x1, px) : px;
         y0 = k ? std::min(y0, py) : py;
         y1 = k ? std::max(y1, py) : py;
+    }
+    // Inner box: B's corners 0 (0, 0), 1 (w-1, 0), 2 (0, h-1), 3 (w-1, h-1)
+    // in their usual order on the canvas make a convex quad (the corners'
+    // denominators share a sign) whose left edge lies left of max(x0, x2),
+    // right edge right of min(x1, x3), top edge above max(y0, y1) and
+    // bottom edge below min(y2, y3); the box between those, less 2 pixels,
+    // maps well inside B. Any other order leaves it empty.
+    if (cx[1] > cx[0] && cx[3] > cx[2] && cy[2] > cy[0] && cy[3] > cy[1]) {
+        const R l = std::max(cx[0], cx[2]) + 2, r = std::min(cx[1], cx[3]) - 2;
+        const R t = std::max(cy[0], cy[1]) + 2, b = std::min(cy[2], cy[3]) - 2;
+        if (l < r && t < b && l >= 0 && t >= 0 && r <= g.width - 1 && b <= g.height - 1) {
+            inner[0] = static_cast<int>(std::ceil(l));
+            inner[1] = static_cast<int>(std::ceil(t));
+            inner[2] = static_cast<int>(std::floor(r));
+            inner[3] = static_cast<int>(std::floor(b));
+        }
     }
     const auto clip = [](R v, int lo, int hi) {
         return static_cast<int>(std::max<R>(lo, std::min<R>(hi, v)));
@@ -1611,7 +1628,7 @@ Status make_s6_job(const ImageView& a, const ImageView& b, const s6::Geometry& g
     j.snap_tolerance = g.snap_tolerance;
     std::memcpy(j.inverse, g.inverse, sizeof j.inverse);
     int box[4];
-    b_box(g, box);
+    b_box(g, box, j.bi);
     j.bx0 = box[0];
     j.by0 = box[1];
     j.bx1 = box[2];

@@ -58,6 +58,7 @@ struct S6Job {
     int top, blend_top;  // -1 for feather
     S6Level lv[kS6MaxLevels];
     int bx0, by0, bx1, by1;  // canvas box outside which B is invalid (inclusive)
+    int bi[4];               // canvas box inside which B is valid (inclusive; empty when unknown)
     int ox0, oy0, ox1, oy1;  // box holding every overlap pixel, start aligned to the mix tile
     // Regions (x0, y0, x1, y1 inclusive, in the level's pixels; empty when
     // x1 < x0) that the level-0 band mix depends on: where the band mix runs
