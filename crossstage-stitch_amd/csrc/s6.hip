@@ -100,7 +100,7 @@ __device__ __forceinline__ unsigned byte_of(unsigned w, int k) { return (w >> (8
 // below what the 8-bit output can show.
 // x and y come back snapped, as the border score needs them.
 __device__ bool sample(const std::uint8_t* img, int w, int h, int pitch, double& x, double& y, float* rgb, double* score,
-                       double tol) {
+                       double tol, float* raw = nullptr) {
     if (!center_coordinate(x, w, tol) || !center_coordinate(y, h, tol)) return false;
     const int x0 = static_cast<int>(floor(x)), y0 = static_cast<int>(floor(y));
     const int x1 = x0 < w - 1 ? x0 + 1 : x0, y1 = y0 < h - 1 ? y0 + 1 : y0;
@@ -118,7 +118,9 @@ __device__ bool sample(const std::uint8_t* img, int w, int h, int pitch, double&
     for (int c = 0; c < 3; ++c) {
         const float v0 = fmaf(ax, static_cast<float>(q01[c]), (1 - ax) * static_cast<float>(q00[c]));
         const float v1 = fmaf(ax, static_cast<float>(q11[c]), (1 - ax) * static_cast<float>(q10[c]));
-        rgb[c] = fmaf(ay, v1, (1 - ay) * v0) * (1.0f / 255.0f);
+        const float v = fmaf(ay, v1, (1 - ay) * v0);
+        if (raw) raw[c] = v;
+        rgb[c] = v * (1.0f / 255.0f);
     }
     // min((x+.5)/w, (w-.5-x)/w) == min(x+.5, w-.5-x)/w exactly: a correctly
     // rounded division by a positive w is monotonic, so it commutes with min.
@@ -128,6 +130,26 @@ __device__ bool sample(const std::uint8_t* img, int w, int h, int pitch, double&
 
 __device__ __forceinline__ std::uint8_t quantize(float x) {
     return static_cast<std::uint8_t>(floorf(fminf(255.0f, fmaxf(0.0f, 255.0f * x)) + 0.5f));
+}
+
+// B's warped colour at level 0, stored for the pull into level 1 and the
+// level-0 band mix: the bilinear values v in [0, 255] of R, G and B as 8.13
+// fixed point in 21-bit fields of one 8-byte word (8 B per pixel instead of
+// three FP32 planes, 12 B). The stored colour is within 2^-14 of a grey level
+// of the computed one, about the size of the pyramid's own FP32 rounding; the
+// warp's own single-source output uses the exact value.
+__device__ __forceinline__ unsigned long long pack_b(const float v[3]) {
+    unsigned long long w = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        w |= static_cast<unsigned long long>(static_cast<unsigned>(rintf(v[c] * 8192.0f))) << (21 * c);
+    return w;
+}
+__device__ __forceinline__ void unpack_b(unsigned long long w, float rgb[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        rgb[c] = static_cast<float>(static_cast<unsigned>(w >> (21 * c)) & 0x1fffffu) * (1.0f / 8192.0f) *
+                 (1.0f / 255.0f);
 }
 
 // Incremental form for tiles visited in increasing order (see dev::next_job).
@@ -216,7 +238,7 @@ __device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int
     const size_t p = static_cast<size_t>(y) * j.width + x;
     const int ax = x + j.origin_x, ay = y + j.origin_y;
     const double xa = static_cast<double>(ax), ya = r.ya;
-    float ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
+    float ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0}, vb[3] = {0, 0, 0};
     const bool ma = ax >= 0 && ax <= j.wa - 1 && ay >= 0 && ay <= j.ha - 1;
     bool mb = false;
     double xb = 0, yb = 0;
@@ -227,7 +249,7 @@ __device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int
         if (isfinite(den) && dsc > 0 && fabs(den) > j.denominator_tolerance * dsc) {
             xb = (inv[0] * xa + r.ty0 + inv[2]) / den;
             yb = (inv[3] * xa + r.ty1 + inv[5]) / den;
-            mb = sample(j.b, j.wb, j.hb, j.pitch_b, xb, yb, rb, nullptr, j.snap_tolerance);
+            mb = sample(j.b, j.wb, j.hb, j.pitch_b, xb, yb, rb, nullptr, j.snap_tolerance, vb);
         }
     }
     double sa = 0, sb = 0;
@@ -243,10 +265,7 @@ __device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int
         if (ma && (!mb || sa >= sb)) f |= kUA;
         if (mb && (!ma || sb > sa)) f |= kUB;
         L.flags[p] = static_cast<std::uint8_t>(f);
-        if (mb) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) L.b[c * L.n + p] = rb[c];
-        }
+        if (mb) reinterpret_cast<unsigned long long*>(L.b)[p] = pack_b(vb);
         // Single-source pixels keep their warped colour (s6.cu:294-306):
         // written here; the band mix writes the overlap.
         j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = ma || mb ? 1 : 0;
@@ -538,6 +557,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
         constexpr int kElems = kRIW * kRIH, kPer = (kElems + kBlock - 1) / kBlock;
         int sxs[kPer], sys[kPer];
         unsigned fls[kPer], abytes[kPer];
+        unsigned long long bpk[kFromZero ? kPer : 1];  // level 0: B's packed colour
         bool ins[kPer][2];  // pulls above level 1: element inside A's / B's support
         float raw[kPer][kCh];
         // A tile whose region lies inside the level needs no reflection or
@@ -583,8 +603,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 load_rgb(j.a + static_cast<size_t>(in_a ? ay : 0) * j.pitch_a + 3 * (in_a ? ax : 0), false, lo, hi);
                 lo = in_a ? lo : 0u;
                 abytes[e] = lo;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) raw[e][3 + c] = in_b ? planes[3 + c][q] : 0.0f;
+                bpk[e] = in_b ? reinterpret_cast<const unsigned long long*>(F.b)[q] : 0ull;
             } else {
 #pragma unroll
                 for (int c = 0; c < kCh; ++c) {
@@ -609,10 +628,12 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
             if (kFromZero) {
                 const unsigned f = fls[e];
                 const bool has_a = f & kVA, has_b = f & kVB;
+                float bc[3];
+                unpack_b(bpk[e], bc);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     v[c] = has_a ? unit_byte(abytes[e], c) : 0.0f;
-                    v[3 + c] = has_b ? raw[e][3 + c] : 0.0f;
+                    v[3 + c] = has_b ? bc[c] : 0.0f;
                 }
                 fl[iy][ix] = static_cast<std::uint8_t>(f);
             } else {
@@ -1152,7 +1173,7 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
             // from in-bounds positions, one memory round trip per tile; what
             // a pixel does not own is dropped. A's position is clamped into
             // its rectangle (tiles start on the tile grid, left of or above
-            // the overlap box); B's colour planes cover the canvas.
+            // the overlap box); B's packed colour plane covers the canvas.
             unsigned ab[kEPer];  // A's bytes (R, G, B in bytes 0-2)
 #pragma unroll
             for (int k = 0; k < kEPer; ++k) {
@@ -1163,8 +1184,7 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
                 const int ax = clampi(x + j.origin_x, 0, j.wa - 1), ay = clampi(y + j.origin_y, 0, j.ha - 1);
                 unsigned hi;
                 load_rgb(j.a + static_cast<size_t>(ay) * j.pitch_a + 3 * ax, false, ab[k], hi);
-#pragma unroll
-                for (int c = 0; c < 3; ++c) gb[k][c] = L.b[c * L.n + p];
+                unpack_b(reinterpret_cast<const unsigned long long*>(L.b)[p], gb[k]);
             }
 #pragma unroll
             for (int k = 0; k < kEPer; ++k) {
@@ -1327,7 +1347,7 @@ std::size_t align256(std::size_t v) { return (v + 255) / 256 * 256; }
 namespace jobs {
 
 // Pyramid layout (s6.cu:60-88): level 0 is the canvas, each level halves with
-// ceil down to 1x1. Level 0: B's warped colour (RGB FP32) and one flag byte;
+// ceil down to 1x1. Level 0: B's warped colour (packed, 8 B) and one flag byte;
 // levels >= 1: A, B (RGB FP32), VA, VB, and on the blended levels UA, UB, Z;
 // level 1 also keeps its Gaussian colour planes apart from the pushed ones,
 // which the level-1 Gaussian reads while it writes.
@@ -1351,7 +1371,7 @@ Status s6_layout(const s6::Geometry& g, s6::BlendMode mode, S6Layout* out) {
         o.w = w;
         o.h = h;
         const std::size_t n = static_cast<std::size_t>(w) * h;
-        add(n, 3, o.b);
+        add(n, l == 0 ? 2 : 3, o.b);  // level 0: B's packed colour, 8 B per pixel
         if (l == 0) {
             cur = align256(cur);
             o.flags = cur;
@@ -1752,7 +1772,7 @@ double s6_algorithmic_bytes(int kernel, int l, const S6Job& j, const s6::RenderS
             // A and B sources once; flags and mask (multiband) or the mask
             // (feather); B's colour where valid; every output byte the warp owns.
             if (j.top < 0) return 3 * nA + 3 * SB + P + 3 * P;
-            return 3 * (nA - O) + 3 * SB + 2 * P + 12 * nB + 3 * (P - O);
+            return 3 * (nA - O) + 3 * SB + 2 * P + 8 * nB + 3 * (P - O);
         case LaunchProbe::kPull:
         case LaunchProbe::kGauss: {
             if (l > (kernel == LaunchProbe::kPull ? j.top : j.blend_top)) return 0;
@@ -1780,7 +1800,7 @@ double s6_algorithmic_bytes(int kernel, int l, const S6Job& j, const s6::RenderS
                 const double fb = bbox > 0 ? static_cast<double>(span(fx0, fx1, j.bx0, j.bx1)) *
                                                  span(fy0, fy1, j.by0, j.by1) / bbox * nB
                                            : 0.0;
-                return foot + 3 * fa + 12 * fb + 32 * u1 + 8 * area(j.gbox[1], 1);
+                return foot + 3 * fa + 8 * fb + 32 * u1 + 8 * area(j.gbox[1], 1);
             }
             return (32 + (j.top >= 2 ? 6 : 0) + 24) * area(j.gbox[1], 1);
         }
@@ -1797,7 +1817,7 @@ double s6_algorithmic_bytes(int kernel, int l, const S6Job& j, const s6::RenderS
                 // level-1 difference planes under the overlap and the output.
                 const double box = j.ox1 >= j.ox0 && j.oy1 >= j.oy0
                                        ? static_cast<double>(j.ox1 - j.ox0 + 1) * (j.oy1 - j.oy0 + 1) : 0.0;
-                return box + 15 * O + 3 * O + (j.blend_top > 0 ? 6 * O : 0);
+                return box + 11 * O + 3 * O + (j.blend_top > 0 ? 6 * O : 0);
             }
         default:
             return 0;

@@ -23,15 +23,15 @@ struct S6Layout {
 };
 
 // Level 0 is never stored as A colours: A sits at an integer offset on the
-// canvas, so its level-0 colour is read from the source bytes through a
-// 256-entry table. Validity and ownership are bits of one byte per pixel
-// (exactly 0 or 1 there) and B's warped colour is stored where B is valid.
+// canvas, so its level-0 colour is read from the source bytes. Validity and
+// ownership are bits of one byte per pixel (exactly 0 or 1 there) and B's
+// warped colour is stored, packed, where B is valid.
 // Coarser levels hold FP32 planes.
 struct S6Level {
     int w, h;
     std::size_t n;  // w * h; colour planes are n floats apart
     float* a;   // R, G, B planes (levels >= 1): pull-push, then Gaussian at levels >= 2
-    float* b;   // R, G, B planes (level 0: warped B where valid)
+    float* b;   // R, G, B planes; level 0: B's warped colour where valid, one packed 8-byte word per pixel
     float* va;  // levels >= 1
     float* vb;
     float* ua;  // levels 1..blend_top
