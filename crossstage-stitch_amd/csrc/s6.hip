@@ -481,15 +481,30 @@ __device__ __forceinline__ void st_buf(__amdgpu_buffer_rsrc_t r, unsigned off, f
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
 }
 
+// Level-1 validity and ownership weights are multiples of 1/256 in [0, 1]
+// (sums of tap products of 0/1 flags), stored exactly as 256 w in 16 bits.
+__device__ __forceinline__ float w16(unsigned k) { return static_cast<float>(k) * (1.0f / 256.0f); }
+__device__ __forceinline__ float ld_w16(const float* plane, unsigned i) {
+    return w16(reinterpret_cast<const std::uint16_t*>(plane)[i]);
+}
+__device__ __forceinline__ float ld_buf_w16(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return w16(__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0));
+}
+__device__ __forceinline__ void st_buf_w16(__amdgpu_buffer_rsrc_t r, unsigned off, int k) {
+    __builtin_amdgcn_raw_buffer_store_b16(static_cast<unsigned short>(k), r, off, 0, 0);
+}
+__device__ __forceinline__ unsigned half_off(unsigned off4) { return off4 == kNoLoad ? kNoLoad : off4 / 2; }
+
 // Modes: 0 pull at l >= 2, 1 Gaussian at l >= 2, 2 pull at l = 1 (from the
 // virtual level 0). The Gaussian at l = 1 is s6_gauss1. From level 0 the
 // tile holds the six colour channels; the four level-1 weights (validity and
 // ownership of A and B) are integer sums of the flag bits, exact (every
 // partial sum of tap products is a multiple of 1/256), and the ownership
 // weights are the Gaussian's own at level 1.
-template <int kMode>
+template <int kMode, bool kW16 = false>
 __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span, dev::Total work, int l) {
     static_assert(kMode >= 0 && kMode <= 2, "reduce mode");
+    // kW16: the finer level is level 1, whose weights are 16-bit.
     const long long total = work.get();
     constexpr bool kFromZero = kMode == 2;
     constexpr int kCh = kFromZero ? 6 : 8;
@@ -607,8 +622,9 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
             } else {
 #pragma unroll
                 for (int c = 0; c < kCh; ++c) {
-                    if (kMode == 0) raw[e][c] = ld_at(planes[c], 4u * (c < 3 || c == kCh - 2 ? qa : qb));
-                    else raw[e][c] = planes[c][q];
+                    const unsigned qq = kMode == 0 ? (c < 3 || c == kCh - 2 ? qa : qb) : q;
+                    if (kW16 && c >= kCh - 2) raw[e][c] = ld_w16(planes[c], qq);
+                    else raw[e][c] = ld_at(planes[c], 4u * qq);
                 }
             }
         }
@@ -731,13 +747,14 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                         ua += (f & kUA) ? w : 0;
                         ub += (f & kUB) ? w : 0;
                     }
-                st_buf(buffer_of(C.va, nb4), oa, static_cast<float>(va) * (1.0f / 256.0f));
-                st_buf(buffer_of(C.vb, nb4), ob, static_cast<float>(vb) * (1.0f / 256.0f));
+                const unsigned nb2 = static_cast<unsigned>(2 * C.n);
+                st_buf_w16(buffer_of(C.va, nb2), half_off(oa), va);
+                st_buf_w16(buffer_of(C.vb, nb2), half_off(ob), vb);
                 // Ownership weights are read only inside the level-1 Gaussian
                 // region (by the band mix and the Gaussian into level 2).
                 if (own) {
-                    C.ua[p] = static_cast<float>(ua) * (1.0f / 256.0f);
-                    C.ub[p] = static_cast<float>(ub) * (1.0f / 256.0f);
+                    reinterpret_cast<std::uint16_t*>(C.ua)[p] = static_cast<std::uint16_t>(ua);
+                    reinterpret_cast<std::uint16_t*>(C.ub)[p] = static_cast<std::uint16_t>(ub);
                 }
             } else {
                 st_buf(buffer_of(kMode == 0 ? C.va : C.ua, nb4), oa, acc[6]);
@@ -923,11 +940,12 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
         // neighbouring tiles read unchanged halos.
         // Raw level-1 planes through buffer resources: 0 outside an image's
         // support, where they are not stored.
-        const unsigned nb = static_cast<unsigned>(4 * C.n);
+        // The weights are 16-bit (256 w) at level 1.
+        const unsigned nb = static_cast<unsigned>(4 * C.n), nb2 = static_cast<unsigned>(2 * C.n);
         const __amdgpu_buffer_rsrc_t pa[4] = {buffer_of(C.a, nb), buffer_of(C.a + C.n, nb), buffer_of(C.a + 2 * C.n, nb),
-                                              buffer_of(C.va, nb)};
+                                              buffer_of(C.va, nb2)};
         const __amdgpu_buffer_rsrc_t pb[4] = {buffer_of(C.b, nb), buffer_of(C.b + C.n, nb), buffer_of(C.b + 2 * C.n, nb),
-                                              buffer_of(C.vb, nb)};
+                                              buffer_of(C.vb, nb2)};
         const unsigned oa = inside && in_support(j, 1, 0, x, y) ? static_cast<unsigned>(4 * p) : kNoLoad;
         const unsigned ob = inside && in_support(j, 1, 1, x, y) ? static_cast<unsigned>(4 * p) : kNoLoad;
         float ra[3], rb[3];
@@ -936,7 +954,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
             ra[c] = ld_buf(pa[c], oa);
             rb[c] = ld_buf(pb[c], ob);
         }
-        const float wa = ld_buf(pa[3], oa), wb = ld_buf(pb[3], ob);
+        const float wa = ld_buf_w16(pa[3], half_off(oa)), wb = ld_buf_w16(pb[3], half_off(ob));
         // An image's pushed level 1 is needed only if some output of the
         // tile misses it somewhere in its footprint (weight below 1); only
         // then are its staged level-1 pixels and level 2 read. That can only
@@ -957,12 +975,14 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
                 if (need_a) {
                     const unsigned o = in_support(j, 1, 0, cx, cy) ? 4u * q : kNoLoad;
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) sa[it][c] = ld_buf(pa[c], o);
+                    for (int c = 0; c < 3; ++c) sa[it][c] = ld_buf(pa[c], o);
+                    sa[it][3] = ld_buf_w16(pa[3], half_off(o));
                 }
                 if (need_b) {
                     const unsigned o = in_support(j, 1, 1, cx, cy) ? 4u * q : kNoLoad;
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) sb[it][c] = ld_buf(pb[c], o);
+                    for (int c = 0; c < 3; ++c) sb[it][c] = ld_buf(pb[c], o);
+                    sb[it][3] = ld_buf_w16(pb[3], half_off(o));
                 }
             }
             if (coarse) {
@@ -1142,6 +1162,8 @@ __global__ void __launch_bounds__(kBlock) s6_push(dev::JobSpan<S6Job> span, dev:
 // (written by the warp) and only overlap pixels need the band mix, so a tile
 // without overlap pixels stages nothing. Level 1 stores z - ga and z - gb, the
 // two planes level 0 expands (one per overlap pixel, its owner's).
+// kL1: level 1, whose ownership weights are 16-bit.
+template <bool kL1 = false>
 __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::Total work, int l) {
     const long long total = work.get();
     __shared__ float4 cc[3][kCH][kCW];
@@ -1206,8 +1228,8 @@ __global__ void __launch_bounds__(kBlock) s6_mix(dev::JobSpan<S6Job> span, dev::
                     ga[k][c] = L.ga[c * L.n + p];
                     gb[k][c] = L.gb[c * L.n + p];
                 }
-                ua[k] = L.ua[p];
-                ub[k] = L.ub[p];
+                ua[k] = kL1 ? ld_w16(L.ua, static_cast<unsigned>(p)) : L.ua[p];
+                ub[k] = kL1 ? ld_w16(L.ub, static_cast<unsigned>(p)) : L.ub[p];
             }
         }
         // The coarse level behind the tile; its loads go out with the
@@ -1366,6 +1388,12 @@ Status s6_layout(const s6::Geometry& g, s6::BlendMode mode, S6Layout* out) {
         off = cur;
         cur += count * sizeof(float) * ch;
     };
+    // Level-1 weights: 16 bits each (256 w, exact).
+    const auto add16 = [&](std::size_t count, std::size_t& off) {
+        cur = align256(cur);
+        off = cur;
+        cur += count * sizeof(std::uint16_t);
+    };
     for (int l = 0; l < kS6MaxLevels; ++l) {
         S6LevelOffsets& o = L.lv[l];
         o.w = w;
@@ -1378,11 +1406,21 @@ Status s6_layout(const s6::Geometry& g, s6::BlendMode mode, S6Layout* out) {
             cur += n;
         } else {
             add(n, 3, o.a);
-            add(n, 1, o.va);
-            add(n, 1, o.vb);
+            if (l == 1) {
+                add16(n, o.va);
+                add16(n, o.vb);
+            } else {
+                add(n, 1, o.va);
+                add(n, 1, o.vb);
+            }
             if (l <= requested) {
-                add(n, 1, o.ua);
-                add(n, 1, o.ub);
+                if (l == 1) {
+                    add16(n, o.ua);
+                    add16(n, o.ub);
+                } else {
+                    add(n, 1, o.ua);
+                    add(n, 1, o.ub);
+                }
                 add(n, 3, o.z);
                 if (l == 1) {
                     add(n, 3, o.ga);
@@ -1800,7 +1838,7 @@ double s6_algorithmic_bytes(int kernel, int l, const S6Job& j, const s6::RenderS
                 const double fb = bbox > 0 ? static_cast<double>(span(fx0, fx1, j.bx0, j.bx1)) *
                                                  span(fy0, fy1, j.by0, j.by1) / bbox * nB
                                            : 0.0;
-                return foot + 3 * fa + 8 * fb + 32 * u1 + 8 * area(j.gbox[1], 1);
+                return foot + 3 * fa + 8 * fb + 28 * u1 + 4 * area(j.gbox[1], 1);
             }
             return (32 + (j.top >= 2 ? 6 : 0) + 24) * area(j.gbox[1], 1);
         }
@@ -1851,6 +1889,7 @@ hipError_t launch_s6(const S6Job* table, int count, const S6Totals& t, const S6J
         const dev::Total w = total(&t.pull[l], d ? &d->pull[l] : nullptr);
         timed(LaunchProbe::kPull, l, [&] {
             if (l == 1) s6_reduce<2><<<persistent_grid(t.pull[l], jobs::Stage::S6), kBlock, 0, stream>>>(span, w, l);
+            else if (l == 2) s6_reduce<0, true><<<persistent_grid(t.pull[l], jobs::Stage::S6), kBlock, 0, stream>>>(span, w, l);
             else s6_reduce<0><<<persistent_grid(t.pull[l], jobs::Stage::S6), kBlock, 0, stream>>>(span, w, l);
         });
     }
@@ -1865,14 +1904,19 @@ hipError_t launch_s6(const S6Job* table, int count, const S6Totals& t, const S6J
         const dev::Total w = total(&t.gauss[l], d ? &d->gauss[l] : nullptr);
         timed(LaunchProbe::kGauss, l, [&] {
             if (l == 1) s6_gauss1<<<persistent_grid(t.gauss[l], jobs::Stage::S6), kBlock, 0, stream>>>(span, w);
+            else if (l == 2) s6_reduce<1, true><<<persistent_grid(t.gauss[l], jobs::Stage::S6), kBlock, 0, stream>>>(span, w, l);
             else s6_reduce<1><<<persistent_grid(t.gauss[l], jobs::Stage::S6), kBlock, 0, stream>>>(span, w, l);
         });
     }
     for (int l = t.blend_top; l >= 0; --l)
         if (t.mix[l] > 0)
             timed(LaunchProbe::kMix, l, [&] {
-                s6_mix<<<persistent_grid(t.mix[l], jobs::Stage::S6), kBlock, 0, stream>>>(
-                    span, total(&t.mix[l], d ? &d->mix[l] : nullptr), l);
+                if (l == 1)
+                    s6_mix<true><<<persistent_grid(t.mix[l], jobs::Stage::S6), kBlock, 0, stream>>>(
+                        span, total(&t.mix[l], d ? &d->mix[l] : nullptr), l);
+                else
+                    s6_mix<<<persistent_grid(t.mix[l], jobs::Stage::S6), kBlock, 0, stream>>>(
+                        span, total(&t.mix[l], d ? &d->mix[l] : nullptr), l);
             });
     return hipGetLastError();
 }
