@@ -495,6 +495,13 @@ __device__ __forceinline__ void st_buf_w16(__amdgpu_buffer_rsrc_t r, unsigned of
 }
 __device__ __forceinline__ unsigned half_off(unsigned off4) { return off4 == kNoLoad ? kNoLoad : off4 / 2; }
 
+// A's level-1 raw pull sums: sum over the 5x5 footprint of the integer tap
+// products (1 .. 36, total 256) times A's bytes where A is valid, an integer
+// N <= 65280 that the pull forms exactly (every partial sum is an integer
+// below 2^24) and stores in 16 bits; the sum is N / (256 * 255).
+constexpr float kInvA1 = 1.0f / 65280.0f;
+__device__ __forceinline__ float a1(unsigned n) { return static_cast<float>(n) * kInvA1; }
+
 // Modes: 0 pull at l >= 2, 1 Gaussian at l >= 2, 2 pull at l = 1 (from the
 // virtual level 0). The Gaussian at l = 1 is s6_gauss1. From level 0 the
 // tile holds the six colour channels; the four level-1 weights (validity and
@@ -566,6 +573,10 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
             sa0 = ka[0], sa1 = ka[1], sa2 = ka[2], sa3 = ka[3];
             sb0 = kb[0], sb1 = kb[1], sb2 = kb[2], sb3 = kb[3];
         }
+        // Level 1 below a pull into level 2: A's sums are 16-bit planes.
+        const std::uint16_t* a16[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a16[c] = reinterpret_cast<const std::uint16_t*>(F.a) + c * F.n;
         // Pass 1: every load of this thread's tile elements, issued before any
         // is used (positions are clamped into the level, so all are in
         // bounds; level-0 values of invalid pixels are discarded below).
@@ -624,6 +635,8 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 for (int c = 0; c < kCh; ++c) {
                     const unsigned qq = kMode == 0 ? (c < 3 || c == kCh - 2 ? qa : qb) : q;
                     if (kW16 && c >= kCh - 2) raw[e][c] = ld_w16(planes[c], qq);
+                    else if (kW16 && kMode == 0 && c < 3)  // A's level-1 sums, 16-bit N
+                        raw[e][c] = a1(ld_at(a16[c], 2u * qq));
                     else raw[e][c] = ld_at(planes[c], 4u * qq);
                 }
             }
@@ -648,7 +661,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 unpack_b(bpk[e], bc);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    v[c] = has_a ? unit_byte(abytes[e], c) : 0.0f;
+                    v[c] = has_a ? static_cast<float>(byte_of(abytes[e], c)) : 0.0f;  // exact byte values
                     v[3 + c] = has_b ? bc[c] : 0.0f;
                 }
                 fl[iy][ix] = static_cast<std::uint8_t>(f);
@@ -694,7 +707,10 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 for (int dy = 0; dy < 5; ++dy)
 #pragma unroll
                     for (int dx = 0; dx < 5; ++dx) {
-                        const float w = tap(dx - 2) * tap(dy - 2);
+                        // A's bytes at level 0 take the integer taps (exact sums).
+                        const float w = kFromZero && decltype(c0)::value == 0
+                                            ? static_cast<float>(itap(dx - 2) * itap(dy - 2))
+                                            : tap(dx - 2) * tap(dy - 2);
 #pragma unroll
                         for (int c = decltype(c0)::value; c < decltype(c1)::value; ++c)
                             acc[c] = fmaf(w, tile[c][2 * ly + dy][(dx & 1) * kREven + lx + (dx >> 1)], acc[c]);
@@ -722,12 +738,21 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
             const unsigned ob = kMode == 1 || (x >= ob0 && x <= ob2 && y >= ob1 && y <= ob3) ? pb4 : kNoLoad;
             const bool own = kFromZero && x >= j.gbox[1][0] && x <= j.gbox[1][2] && y >= j.gbox[1][1] &&
                              y <= j.gbox[1][3];
-            const __amdgpu_buffer_rsrc_t ra = buffer_of(da, 3 * nb4), rb = buffer_of(db, 3 * nb4);
+            const __amdgpu_buffer_rsrc_t rb = buffer_of(db, 3 * nb4);
+            if (kFromZero) {
+                // A's sums at level 1: the exact integers N, 16 bits each.
+                const unsigned nb2 = static_cast<unsigned>(2 * C.n);
+                const __amdgpu_buffer_rsrc_t ra = buffer_of(da, 3 * nb2);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                st_buf(ra, oa == kNoLoad ? kNoLoad : oa + c * nb4, acc[c]);
-                st_buf(rb, ob == kNoLoad ? kNoLoad : ob + c * nb4, acc[3 + c]);
+                for (int c = 0; c < 3; ++c)
+                    st_buf_w16(ra, oa == kNoLoad ? kNoLoad : oa / 2 + c * nb2, static_cast<int>(acc[c]));
+            } else {
+                const __amdgpu_buffer_rsrc_t ra = buffer_of(da, 3 * nb4);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) st_buf(ra, oa == kNoLoad ? kNoLoad : oa + c * nb4, acc[c]);
             }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) st_buf(rb, ob == kNoLoad ? kNoLoad : ob + c * nb4, acc[3 + c]);
             if (kFromZero) {
                 int va = 0, vb = 0, ua = 0, ub = 0;
                 if (uniform) {
@@ -942,8 +967,10 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
         // support, where they are not stored.
         // The weights are 16-bit (256 w) at level 1.
         const unsigned nb = static_cast<unsigned>(4 * C.n), nb2 = static_cast<unsigned>(2 * C.n);
-        const __amdgpu_buffer_rsrc_t pa[4] = {buffer_of(C.a, nb), buffer_of(C.a + C.n, nb), buffer_of(C.a + 2 * C.n, nb),
-                                              buffer_of(C.va, nb2)};
+        // A's raw sums are 16-bit integers N at level 1 (N / 65280).
+        const std::uint16_t* a16 = reinterpret_cast<const std::uint16_t*>(C.a);
+        const __amdgpu_buffer_rsrc_t pa[4] = {buffer_of(a16, nb2), buffer_of(a16 + C.n, nb2),
+                                              buffer_of(a16 + 2 * C.n, nb2), buffer_of(C.va, nb2)};
         const __amdgpu_buffer_rsrc_t pb[4] = {buffer_of(C.b, nb), buffer_of(C.b + C.n, nb), buffer_of(C.b + 2 * C.n, nb),
                                               buffer_of(C.vb, nb2)};
         const unsigned oa = inside && in_support(j, 1, 0, x, y) ? static_cast<unsigned>(4 * p) : kNoLoad;
@@ -951,7 +978,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
         float ra[3], rb[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            ra[c] = ld_buf(pa[c], oa);
+            ra[c] = a1(__builtin_amdgcn_raw_buffer_load_b16(pa[c], half_off(oa), 0, 0));
             rb[c] = ld_buf(pb[c], ob);
         }
         const float wa = ld_buf_w16(pa[3], half_off(oa)), wb = ld_buf_w16(pb[3], half_off(ob));
@@ -975,7 +1002,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_gauss1(dev::JobSpan<S6Job> span,
                 if (need_a) {
                     const unsigned o = in_support(j, 1, 0, cx, cy) ? 4u * q : kNoLoad;
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) sa[it][c] = ld_buf(pa[c], o);
+                    for (int c = 0; c < 3; ++c) sa[it][c] = a1(__builtin_amdgcn_raw_buffer_load_b16(pa[c], half_off(o), 0, 0));
                     sa[it][3] = ld_buf_w16(pa[3], half_off(o));
                 }
                 if (need_b) {
