@@ -231,3 +231,21 @@ def test_s6_render_matches_oracle(T, mode, size):
     assert np.array_equal(mask, omask)
     assert_within_one_lsb(out, oout, (mode, size))
     assert stats == ostats
+
+
+@pytest.mark.parametrize("mode", [0, 2, 4])
+@pytest.mark.parametrize("th,persp", [(0.45, (3e-4, -2e-4)), (-0.8, (-4e-4, 1e-4)), (1.2, (0.0, 5e-4)),
+                                      (np.pi / 2 + 0.05, (1e-4, 1e-4))])
+def test_s6_render_steep_quads(T, mode, th, persp):
+    """B far from axis-aligned (rotations of 26-95 degrees, perspective): its
+    quad covers little of its bounding box, so the warp's per-row bounds from
+    the quad's edges decide which pixels take the inverse map. Mask and
+    counts exact, colour within 1 LSB of the oracle."""
+    h, w = 240, 320
+    a = images.texture_rgb(7 + mode, h, w)
+    b = images.texture_rgb(70 + mode, h, w)
+    H = np.array([[np.cos(th), -np.sin(th), 0.5 * w], [np.sin(th), np.cos(th), 0.2 * h], [persp[0], persp[1], 1.0]])
+    out, mask, stats, oout, omask, ostats = render_both(T, a, b, H, mode)
+    assert np.array_equal(mask, omask)
+    assert stats == ostats
+    assert_within_one_lsb(out, oout, (mode, th))
