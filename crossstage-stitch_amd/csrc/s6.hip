@@ -589,7 +589,23 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
         // A tile whose region lies inside the level needs no reflection or
         // clamping (a block-uniform branch); elements past the region take
         // its last row, which is inside as well.
-        const bool interior = ix0 >= 0 && iy0 >= 0 && ix0 + kRIW <= F.w && iy0 + kRIH <= F.h;
+        // Fields every element reads, held in scalar registers (one wait).
+        const int fw = dev::hold(F.w), fh = dev::hold(F.h);
+        int org_x = 0, org_y = 0, wa = 0, ha = 0, bx0 = 0, bx1 = -1, by0 = 0, by1 = -1, pitch_a = 0;
+        const std::uint8_t* abase = nullptr;
+        const std::uint8_t* flags0 = nullptr;
+        const __amdgpu_buffer_rsrc_t rbpk =
+            buffer_of(kFromZero ? dev::hold(F.b) : nullptr,
+                      kFromZero ? 8u * static_cast<unsigned>(fw) * static_cast<unsigned>(fh) : 0u);
+        if (kFromZero) {
+            org_x = dev::hold(j.origin_x), org_y = dev::hold(j.origin_y);
+            wa = dev::hold(j.wa), ha = dev::hold(j.ha);
+            bx0 = dev::hold(j.bx0), bx1 = dev::hold(j.bx1), by0 = dev::hold(j.by0), by1 = dev::hold(j.by1);
+            pitch_a = dev::hold(j.pitch_a);
+            abase = dev::hold(j.a);
+            flags0 = dev::hold(F.flags);
+        }
+        const bool interior = ix0 >= 0 && iy0 >= 0 && ix0 + kRIW <= fw && iy0 + kRIH <= fh;
 #pragma unroll
         for (int e = 0; e < kPer; ++e) {
             const int i = static_cast<int>(threadIdx.x) + e * kBlock;
@@ -598,10 +614,10 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 sxs[e] = px;
                 sys[e] = py;
             } else {
-                sxs[e] = clampi(reflect101(px, F.w), 0, F.w - 1);
-                sys[e] = clampi(reflect101(py, F.h), 0, F.h - 1);
+                sxs[e] = clampi(reflect101(px, fw), 0, fw - 1);
+                sys[e] = clampi(reflect101(py, fh), 0, fh - 1);
             }
-            const unsigned q = static_cast<unsigned>(sys[e]) * static_cast<unsigned>(F.w) + static_cast<unsigned>(sxs[e]);
+            const unsigned q = static_cast<unsigned>(sys[e]) * static_cast<unsigned>(fw) + static_cast<unsigned>(sxs[e]);
             unsigned qa = q, qb = q;
             bool& ina = ins[e][0];
             bool& inb = ins[e][1];
@@ -612,24 +628,27 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 const int bx2 = max(sb0, min(sxs[e], sb2)), by2 = max(sb1, min(sys[e], sb3));
                 ina = ax == sxs[e] && ay == sys[e];
                 inb = bx2 == sxs[e] && by2 == sys[e];
-                qa = sa2 < sa0 || sa3 < sa1 ? 0u : static_cast<unsigned>(ay) * static_cast<unsigned>(F.w) + ax;
-                qb = sb2 < sb0 || sb3 < sb1 ? 0u : static_cast<unsigned>(by2) * static_cast<unsigned>(F.w) + bx2;
+                qa = sa2 < sa0 || sa3 < sa1 ? 0u : static_cast<unsigned>(ay) * static_cast<unsigned>(fw) + ax;
+                qb = sb2 < sb0 || sb3 < sb1 ? 0u : static_cast<unsigned>(by2) * static_cast<unsigned>(fw) + bx2;
             }
             if (kFromZero) {
                 // A's bytes only on A's rectangle and B's colour only inside
                 // B's box: both are known from the position alone, so the
                 // loads still go out together; elsewhere the image is invalid.
-                fls[e] = F.flags[q];
-                const int ax = sxs[e] + j.origin_x, ay = sys[e] + j.origin_y;
-                const bool in_a = ax >= 0 && ax < j.wa && ay >= 0 && ay < j.ha;
-                const bool in_b = sxs[e] >= j.bx0 && sxs[e] <= j.bx1 && sys[e] >= j.by0 && sys[e] <= j.by1;
+                fls[e] = flags0[q];
+                const int ax = sxs[e] + org_x, ay = sys[e] + org_y;
+                const bool in_a = ax >= 0 && ax < wa && ay >= 0 && ay < ha;
+                const bool in_b = sxs[e] >= bx0 && sxs[e] <= bx1 && sys[e] >= by0 && sys[e] <= by1;
                 // Outside A's rectangle the load reads A's first pixel (in
-                // bounds) and the bytes are dropped.
+                // bounds) and the bytes are dropped; outside B's box the
+                // buffer load of B's colour returns 0.
                 unsigned lo, hi;
-                load_rgb(j.a + static_cast<size_t>(in_a ? ay : 0) * j.pitch_a + 3 * (in_a ? ax : 0), false, lo, hi);
+                load_rgb(abase + (in_a ? static_cast<unsigned>(ay) * static_cast<unsigned>(pitch_a) + 3u * static_cast<unsigned>(ax) : 0u),
+                         false, lo, hi);
                 lo = in_a ? lo : 0u;
                 abytes[e] = lo;
-                bpk[e] = in_b ? reinterpret_cast<const unsigned long long*>(F.b)[q] : 0ull;
+                bpk[e] = __builtin_bit_cast(unsigned long long,
+                                            __builtin_amdgcn_raw_buffer_load_b64(rbpk, in_b ? 8u * q : kNoLoad, 0, 0));
             } else {
 #pragma unroll
                 for (int c = 0; c < kCh; ++c) {

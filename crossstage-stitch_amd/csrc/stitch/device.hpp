@@ -143,6 +143,35 @@ __device__ inline TileWalk tile_walk(long long total) {
 
 __device__ inline int lane_id() { return __lane_id(); }
 
+// A wave-uniform value (a job field) held in scalar registers from here on.
+// Loads of launch-constant data are invariant, so the compiler is free to
+// sink each one into the conditional block that first uses it, where it
+// then waits for that one load on its own; a value passed through here is
+// loaded at this point, together with the others, under one wait.
+template <class T>
+__device__ __forceinline__ T hold(T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "hold: 32- or 64-bit values");
+    if constexpr (sizeof(T) == 4) {
+        int u = __builtin_bit_cast(int, v);
+        asm volatile("" : "+s"(u));
+        return __builtin_bit_cast(T, u);
+    } else {
+        long long u = __builtin_bit_cast(long long, v);
+        asm volatile("" : "+s"(u));
+        return __builtin_bit_cast(T, u);
+    }
+}
+
+// The same for a pointer into device (global) memory, which it stays known
+// to be, so accesses through it remain global rather than flat ones.
+template <class T>
+__device__ __forceinline__ T* hold(T* p) {
+    long long u = reinterpret_cast<long long>(p);
+    asm volatile("" : "+s"(u));
+    using G = __attribute__((address_space(1))) T;
+    return (T*)reinterpret_cast<G*>(u);
+}
+
 // Job owning tile t, looked up by one lane on its own (vector loads, a
 // binary search over the jobs from `from`), for predicates that lanes
 // evaluate over different tiles at once.
