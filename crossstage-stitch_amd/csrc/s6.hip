@@ -233,6 +233,25 @@ __device__ __forceinline__ WarpRow warp_row(const S6Job& j, int y) {
     return {ya, j.inverse[1] * ya, j.inverse[4] * ya, j.inverse[7] * ya, fmin(ya + 0.5, j.ha - 0.5 - ya) / j.ha};
 }
 
+// Whether A owns an overlap pixel: its border score sa is at least B's, sb
+// (s6.cu:206-207). The scores are min(d / n) over the axes (s6.cu:170), d
+// the distance to the nearer border plus a half and n the image size, each
+// a correctly rounded quotient. Products by the rounded reciprocals 1 / n
+// are each within 3 * 2^-53 of their quotient (relative: two roundings of
+// the product and one of the quotient), and so is the minimum; when the two
+// approximate scores differ by more than 2^-48 of the larger, the exact
+// ones differ with the same sign. Only nearer calls (ties among them) take
+// the quotients themselves.
+__device__ __forceinline__ bool a_owns(const S6Job& j, const WarpRow& r, double xa, double xb, double yb) {
+    const double qa = fmin(fmin(xa + 0.5, j.wa - 0.5 - xa) * j.inv_wa, r.sya);
+    const double qb = fmin(fmin(xb + 0.5, j.wb - 0.5 - xb) * j.inv_wb, fmin(yb + 0.5, j.hb - 0.5 - yb) * j.inv_hb);
+    const double d = qa - qb;
+    if (fabs(d) > 0x1p-48 * fmax(qa, qb)) return d > 0;
+    const double sa = fmin(fmin(xa + 0.5, j.wa - 0.5 - xa) / j.wa, r.sya);
+    const double sb = fmin(fmin(xb + 0.5, j.wb - 0.5 - xb) / j.wb, fmin(yb + 0.5, j.hb - 0.5 - yb) / j.hb);
+    return sa >= sb;
+}
+
 __device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int x, int y,
                                            unsigned long long& uni, unsigned long long& ovl) {
     const size_t p = static_cast<size_t>(y) * j.width + x;
@@ -252,18 +271,14 @@ __device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int
             mb = sample(j.b, j.wb, j.hb, j.pitch_b, xb, yb, rb, nullptr, j.snap_tolerance, vb);
         }
     }
-    double sa = 0, sb = 0;
-    if (ma && mb) {
-        sa = fmin(fmin(xa + 0.5, j.wa - 0.5 - xa) / j.wa, r.sya);
-        sb = fmin(fmin(xb + 0.5, j.wb - 0.5 - xb) / j.wb, fmin(yb + 0.5, j.hb - 0.5 - yb) / j.hb);
-    }
     if (j.top >= 0) {
         const S6Level& L = j.lv[0];
+        const bool owns = !(ma && mb) || a_owns(j, r, xa, xb, yb);
         unsigned f = 0;
         if (ma) f |= kVA;
         if (mb) f |= kVB;
-        if (ma && (!mb || sa >= sb)) f |= kUA;
-        if (mb && (!ma || sb > sa)) f |= kUB;
+        if (ma && (!mb || owns)) f |= kUA;
+        if (mb && (!ma || !owns)) f |= kUB;
         L.flags[p] = static_cast<std::uint8_t>(f);
         if (mb) reinterpret_cast<unsigned long long*>(L.b)[p] = pack_b(vb);
         // Single-source pixels keep their warped colour (s6.cu:294-306):
@@ -284,6 +299,11 @@ __device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int
             }
         }
     } else {
+        double sa = 0, sb = 0;
+        if (ma && mb) {
+            sa = fmin(fmin(xa + 0.5, j.wa - 0.5 - xa) / j.wa, r.sya);
+            sb = fmin(fmin(xb + 0.5, j.wb - 0.5 - xb) / j.wb, fmin(yb + 0.5, j.hb - 0.5 - yb) / j.hb);
+        }
         if (ma) {
             unsigned lo, hi;
             load_rgb<true>(a_pixel(j, x, y), false, lo, hi);
@@ -1730,6 +1750,9 @@ Status make_s6_job(const ImageView& a, const ImageView& b, const s6::Geometry& g
     j.origin_y = g.origin_y;
     j.denominator_tolerance = g.denominator_tolerance;
     j.snap_tolerance = g.snap_tolerance;
+    j.inv_wa = 1.0 / a.width;
+    j.inv_wb = 1.0 / b.width;
+    j.inv_hb = 1.0 / b.height;
     std::memcpy(j.inverse, g.inverse, sizeof j.inverse);
     int box[4];
     b_box(g, box, j.bi);

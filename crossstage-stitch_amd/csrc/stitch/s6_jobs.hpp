@@ -49,6 +49,7 @@ struct S6Job {
     int pitch_b, wb, hb;
     int width, height, origin_x, origin_y;
     double denominator_tolerance, snap_tolerance;
+    double inv_wa, inv_wb, inv_hb;  // 1 / wa, 1 / wb, 1 / hb, correctly rounded
     double inverse[9];
     std::uint8_t* out;
     std::uint8_t* mask;

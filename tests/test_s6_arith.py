@@ -54,3 +54,58 @@ def test_unit_byte_equals_the_fp64_quotient_rounded_to_fp32_for_every_byte():
         plain_product_misses += q != want
     # The correction step matters: the product alone misses about half.
     assert plain_product_misses > 100
+
+
+# Ownership of overlap pixels (crossstage-stitch_amd/csrc/s6.hip, a_owns):
+# A owns a pixel where sa >= sb (src/s6/s6.cu:206-207), each score a minimum
+# of correctly rounded quotients d / n (s6.cu:170). The kernel compares
+# products d * RN(1 / n) instead, trusting their order when they differ by
+# more than 2^-48 of the larger, which rests on each product lying within
+# 3 * 2^-53 (relative) of the rounded quotient.
+SIZES = [1, 2, 3, 7, 480, 640, 720, 1080, 1280, 1920, 2160, 3840, 4095, 4096, 8191, 16384]
+
+
+def test_reciprocal_products_within_three_half_ulps_of_the_quotients():
+    for n in SIZES:
+        d = np.arange(1, n + 1, dtype=np.float64) - 0.5   # every d the kernel forms: 0.5 .. n - 0.5
+        exact = d / np.float64(n)                         # IEEE: the correctly rounded quotient
+        fast = d * (np.float64(1.0) / np.float64(n))
+        assert np.all(np.abs(fast - exact) <= 3 * 2.0 ** -53 * exact), n
+    # The bound is tight enough to matter: the products alone miss often.
+    d = np.arange(1, 3841, dtype=np.float64) - 0.5
+    assert np.count_nonzero(d * (1.0 / 3840.0) != d / 3840.0) > 100
+
+
+def _scores(xa, wa, sya, xb, yb, wb, hb, fast):
+    if fast:
+        qa = min(min(xa + 0.5, wa - 0.5 - xa) * (1.0 / wa), sya)
+        qb = min(min(xb + 0.5, wb - 0.5 - xb) * (1.0 / wb), min(yb + 0.5, hb - 0.5 - yb) * (1.0 / hb))
+    else:
+        qa = min(min(xa + 0.5, wa - 0.5 - xa) / wa, sya)
+        qb = min(min(xb + 0.5, wb - 0.5 - xb) / wb, min(yb + 0.5, hb - 0.5 - yb) / hb)
+    return qa, qb
+
+
+def test_fast_ownership_agrees_with_the_quotients_or_defers():
+    rng = np.random.default_rng(5)
+    deferred = 0
+    for _ in range(20000):
+        wa, wb, hb, ha = (int(rng.choice(SIZES[4:])) for _ in range(4))
+        xa = float(rng.integers(0, wa))
+        ya = float(rng.integers(0, ha))
+        sya = min(ya + 0.5, ha - 0.5 - ya) / ha
+        # B positions: general, on the integer grid (translations) and mirrored
+        # onto A's score (exact ties).
+        kind = rng.integers(3)
+        xb = float(rng.uniform(0, wb - 1)) if kind == 0 else float(rng.integers(0, wb))
+        yb = float(rng.uniform(0, hb - 1)) if kind == 0 else float(rng.integers(0, hb))
+        if kind == 2:
+            wb = wa
+            xb = wb - 1 - xa
+        qa, qb = _scores(xa, wa, sya, xb, yb, wb, hb, True)
+        sa, sb = _scores(xa, wa, sya, xb, yb, wb, hb, False)
+        if abs(qa - qb) > 2.0 ** -48 * max(qa, qb):
+            assert (qa - qb > 0) == (sa >= sb) and sa != sb
+        else:
+            deferred += 1
+    assert deferred < 20000 // 3 + 500  # mostly the planted ties
