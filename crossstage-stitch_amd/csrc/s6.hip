@@ -535,7 +535,10 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
     const long long total = work.get();
     constexpr bool kFromZero = kMode == 2;
     constexpr int kCh = kFromZero ? 6 : 8;
-    __shared__ float tile[kCh][kRIH][kRPitch];
+    // Level 0 stages A's bytes as two integer planes (R | B << 16 and G) and
+    // B's colour as three FP32 planes; above it every channel is FP32.
+    constexpr int kPl = kFromZero ? 5 : kCh;
+    __shared__ float tile[kPl][kRIH][kRPitch];
     __shared__ std::uint8_t fl[kFromZero ? kRIH : 1][kRIW + 1];
     __shared__ unsigned wave_flags[2][kBlock / 64];  // per wave: OR and AND of the staged flags
     int ji = 0;
@@ -692,24 +695,22 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
             }
             const int ix = i % kRIW, iy = i / kRIW;
             const int cx = (ix & 1) * kREven + (ix >> 1);
-            float v[kCh];
             if (kFromZero) {
                 const unsigned f = fls[e];
                 const bool has_a = f & kVA, has_b = f & kVB;
                 float bc[3];
                 unpack_b(bpk[e], bc);
+                const unsigned rgb = has_a ? abytes[e] : 0u;
+                tile[0][iy][cx] = __builtin_bit_cast(float, rgb & 0x00ff00ffu);  // R | B << 16
+                tile[1][iy][cx] = __builtin_bit_cast(float, (rgb >> 8) & 0xffu);  // G
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    v[c] = has_a ? static_cast<float>(byte_of(abytes[e], c)) : 0.0f;  // exact byte values
-                    v[3 + c] = has_b ? bc[c] : 0.0f;
-                }
+                for (int c = 0; c < 3; ++c) tile[2 + c][iy][cx] = has_b ? bc[c] : 0.0f;
                 fl[iy][ix] = static_cast<std::uint8_t>(f);
             } else {
 #pragma unroll
-                for (int c = 0; c < kCh; ++c) v[c] = ins[e][c < 3 || c == kCh - 2 ? 0 : 1] ? raw[e][c] : 0.0f;
+                for (int c = 0; c < kCh; ++c)
+                    tile[c][iy][cx] = ins[e][c < 3 || c == kCh - 2 ? 0 : 1] ? raw[e][c] : 0.0f;
             }
-#pragma unroll
-            for (int c = 0; c < kCh; ++c) tile[c][iy][cx] = v[c];
         }
         if (kFromZero) {
             // Which flag bits the staged tile holds somewhere (OR) and
@@ -741,19 +742,41 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
             float acc[kCh];
 #pragma unroll
             for (int c = 0; c < kCh; ++c) acc[c] = 0.0f;
+            // FP32 channels [c0, c1); at level 0 those are B's, in planes c - 1.
             const auto filter = [&](auto c0, auto c1) {
 #pragma unroll
                 for (int dy = 0; dy < 5; ++dy)
 #pragma unroll
                     for (int dx = 0; dx < 5; ++dx) {
-                        // A's bytes at level 0 take the integer taps (exact sums).
-                        const float w = kFromZero && decltype(c0)::value == 0
-                                            ? static_cast<float>(itap(dx - 2) * itap(dy - 2))
-                                            : tap(dx - 2) * tap(dy - 2);
+                        const float w = tap(dx - 2) * tap(dy - 2);
 #pragma unroll
                         for (int c = decltype(c0)::value; c < decltype(c1)::value; ++c)
-                            acc[c] = fmaf(w, tile[c][2 * ly + dy][(dx & 1) * kREven + lx + (dx >> 1)], acc[c]);
+                            acc[c] = fmaf(w, tile[kFromZero ? c - 1 : c][2 * ly + dy][(dx & 1) * kREven + lx + (dx >> 1)],
+                                          acc[c]);
                     }
+            };
+            // A's bytes at level 0 take the integer taps (1 .. 36): exact sums
+            // N <= 255 * 256, two channels to a word (R | B << 16, no carry
+            // between the 16-bit halves) and G alone.
+            unsigned na[3] = {0, 0, 0};
+            const auto filter_a = [&]() {
+                unsigned rb = 0, g = 0;
+#pragma unroll
+                for (int dy = 0; dy < 5; ++dy)
+#pragma unroll
+                    for (int dx = 0; dx < 5; ++dx) {
+                        const unsigned w = static_cast<unsigned>(itap(dx - 2) * itap(dy - 2));
+                        const int col = (dx & 1) * kREven + lx + (dx >> 1);
+                        const unsigned u = __builtin_bit_cast(unsigned, tile[0][2 * ly + dy][col]);
+                        const unsigned v = __builtin_bit_cast(unsigned, tile[1][2 * ly + dy][col]);
+                        __builtin_assume(u < (1u << 24));
+                        __builtin_assume(v < (1u << 24));
+                        rb += u * w;
+                        g += v * w;
+                    }
+                na[0] = rb & 0xffffu;
+                na[1] = g;
+                na[2] = rb >> 16;
             };
             using I0 = std::integral_constant<int, 0>;
             using I3 = std::integral_constant<int, 3>;
@@ -763,7 +786,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
             } else {
                 // A colour sums where some staged pixel has A, B's where some
                 // has B; the others are sums of zeros.
-                if (t_or & kVA) filter(I0{}, I3{});
+                if (t_or & kVA) filter_a();
                 if (t_or & kVB) filter(I3{}, IC{});
             }
             const size_t p = static_cast<size_t>(y) * C.w + x;
@@ -784,7 +807,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 const __amdgpu_buffer_rsrc_t ra = buffer_of(da, 3 * nb2);
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    st_buf_w16(ra, oa == kNoLoad ? kNoLoad : oa / 2 + c * nb2, static_cast<int>(acc[c]));
+                    st_buf_w16(ra, oa == kNoLoad ? kNoLoad : oa / 2 + c * nb2, static_cast<int>(na[c]));
             } else {
                 const __amdgpu_buffer_rsrc_t ra = buffer_of(da, 3 * nb4);
 #pragma unroll
