@@ -536,9 +536,13 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
     constexpr bool kFromZero = kMode == 2;
     constexpr int kCh = kFromZero ? 6 : 8;
     // Level 0 stages A's bytes as two integer planes (R | B << 16 and G) and
-    // B's colour as three FP32 planes; above it every channel is FP32.
-    constexpr int kPl = kFromZero ? 5 : kCh;
+    // B's colour as three FP32 planes. Level 1 under a pull stages A's sums
+    // (16-bit integers N) and the two validity weights (16-bit 256 w) as
+    // 16-bit planes, B's sums as FP32; the Gaussians above stage FP32.
+    constexpr bool kInt16 = kMode == 0 && kW16;
+    constexpr int kPl = kFromZero ? 5 : (kInt16 ? 3 : kCh);
     __shared__ float tile[kPl][kRIH][kRPitch];
+    __shared__ std::uint16_t t16[kInt16 ? 5 : 1][kRIH][kRPitch];
     __shared__ std::uint8_t fl[kFromZero ? kRIH : 1][kRIW + 1];
     __shared__ unsigned wave_flags[2][kBlock / 64];  // per wave: OR and AND of the staged flags
     int ji = 0;
@@ -609,6 +613,7 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
         unsigned long long bpk[kFromZero ? kPer : 1];  // level 0: B's packed colour
         bool ins[kPer][2];  // pulls above level 1: element inside A's / B's support
         float raw[kPer][kCh];
+        unsigned iraw[kInt16 ? kPer : 1][5];  // level 1 under a pull: A's N (3), then 256 va, 256 vb
         // A tile whose region lies inside the level needs no reflection or
         // clamping (a block-uniform branch); elements past the region take
         // its last row, which is inside as well.
@@ -672,6 +677,14 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 abytes[e] = lo;
                 bpk[e] = __builtin_bit_cast(unsigned long long,
                                             __builtin_amdgcn_raw_buffer_load_b64(rbpk, in_b ? 8u * q : kNoLoad, 0, 0));
+            } else if (kInt16) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    iraw[e][c] = ld_at(a16[c], 2u * qa);
+                    raw[e][3 + c] = ld_at(planes[3 + c], 4u * qb);
+                }
+                iraw[e][3] = ld_at(reinterpret_cast<const std::uint16_t*>(planes[kCh - 2]), 2u * qa);
+                iraw[e][4] = ld_at(reinterpret_cast<const std::uint16_t*>(planes[kCh - 1]), 2u * qb);
             } else {
 #pragma unroll
                 for (int c = 0; c < kCh; ++c) {
@@ -706,6 +719,12 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
 #pragma unroll
                 for (int c = 0; c < 3; ++c) tile[2 + c][iy][cx] = has_b ? bc[c] : 0.0f;
                 fl[iy][ix] = static_cast<std::uint8_t>(f);
+            } else if (kInt16) {
+#pragma unroll
+                for (int c = 0; c < 5; ++c)
+                    t16[c][iy][cx] = static_cast<std::uint16_t>(ins[e][c == 4 ? 1 : 0] ? iraw[e][c] : 0u);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) tile[c][iy][cx] = ins[e][1] ? raw[e][3 + c] : 0.0f;
             } else {
 #pragma unroll
                 for (int c = 0; c < kCh; ++c)
@@ -751,7 +770,9 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                         const float w = tap(dx - 2) * tap(dy - 2);
 #pragma unroll
                         for (int c = decltype(c0)::value; c < decltype(c1)::value; ++c)
-                            acc[c] = fmaf(w, tile[kFromZero ? c - 1 : c][2 * ly + dy][(dx & 1) * kREven + lx + (dx >> 1)],
+                            acc[c] = fmaf(w,
+                                          tile[kFromZero ? c - 1 : (kInt16 ? c - 3 : c)][2 * ly + dy]
+                                              [(dx & 1) * kREven + lx + (dx >> 1)],
                                           acc[c]);
                     }
             };
@@ -778,10 +799,33 @@ __global__ void __launch_bounds__(kBlock, 3) s6_reduce(dev::JobSpan<S6Job> span,
                 na[1] = g;
                 na[2] = rb >> 16;
             };
+            // Level 1 under a pull: A's N and the weights 256 w with the
+            // integer taps, exact (N sums to at most 255 * 256^2 < 2^24).
+            unsigned ni[5] = {0, 0, 0, 0, 0};
+            const auto filter_i16 = [&]() {
+#pragma unroll
+                for (int dy = 0; dy < 5; ++dy)
+#pragma unroll
+                    for (int dx = 0; dx < 5; ++dx) {
+                        const unsigned w = static_cast<unsigned>(itap(dx - 2) * itap(dy - 2));
+#pragma unroll
+                        for (int c = 0; c < 5; ++c)
+                            ni[c] += w * static_cast<unsigned>(t16[c][2 * ly + dy][(dx & 1) * kREven + lx + (dx >> 1)]);
+                    }
+            };
             using I0 = std::integral_constant<int, 0>;
             using I3 = std::integral_constant<int, 3>;
+            using I6 = std::integral_constant<int, 6>;
             using IC = std::integral_constant<int, kCh>;
-            if (!kFromZero) {
+            if (kInt16) {
+                filter_i16();
+                filter(I3{}, I6{});
+                // A's sums: N2 / (256 * 256 * 255); weights: W2 / 65536 (exact).
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc[c] = static_cast<float>(ni[c]) * (kInvA1 * (1.0f / 256.0f));
+                acc[6] = static_cast<float>(ni[3]) * (1.0f / 65536.0f);
+                acc[7] = static_cast<float>(ni[4]) * (1.0f / 65536.0f);
+            } else if (!kFromZero) {
                 filter(I0{}, IC{});
             } else {
                 // A colour sums where some staged pixel has A, B's where some
