@@ -138,18 +138,21 @@ __device__ __forceinline__ std::uint8_t quantize(float x) {
 // three FP32 planes, 12 B). The stored colour is within 2^-14 of a grey level
 // of the computed one, about the size of the pyramid's own FP32 rounding; the
 // warp's own single-source output uses the exact value.
+// The fields are rounded half to even, as rintf rounds: v * 8192 is exact,
+// and adding 2^23 rounds it to an integer held in the low mantissa bits
+// (v * 8192 < 2^21). The word is assembled from 32-bit halves.
 __device__ __forceinline__ unsigned long long pack_b(const float v[3]) {
-    unsigned long long w = 0;
+    unsigned f[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-        w |= static_cast<unsigned long long>(static_cast<unsigned>(rintf(v[c] * 8192.0f))) << (21 * c);
-    return w;
+    for (int c = 0; c < 3; ++c) f[c] = __builtin_bit_cast(unsigned, fmaf(v[c], 8192.0f, 8388608.0f)) - 0x4b000000u;
+    const unsigned lo = f[0] | (f[1] << 21), hi = (f[1] >> 11) | (f[2] << 10);
+    return static_cast<unsigned long long>(hi) << 32 | lo;
 }
 __device__ __forceinline__ void unpack_b(unsigned long long w, float rgb[3]) {
+    const unsigned lo = static_cast<unsigned>(w), hi = static_cast<unsigned>(w >> 32);
+    const unsigned f[3] = {lo & 0x1fffffu, __builtin_amdgcn_alignbit(hi, lo, 21) & 0x1fffffu, hi >> 10};
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-        rgb[c] = static_cast<float>(static_cast<unsigned>(w >> (21 * c)) & 0x1fffffu) * (1.0f / 8192.0f) *
-                 (1.0f / 255.0f);
+    for (int c = 0; c < 3; ++c) rgb[c] = static_cast<float>(f[c]) * (1.0f / 8192.0f) * (1.0f / 255.0f);
 }
 
 // Incremental form for tiles visited in increasing order (see dev::next_job).
