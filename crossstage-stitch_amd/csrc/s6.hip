@@ -62,6 +62,17 @@ __device__ bool center_coordinate(double& x, int n, double tol) {
     return x >= 0 && x <= n - 1.0;
 }
 
+// center_coordinate when the snap threshold max(tol, 16 eps max(1, |x|)) is
+// tol for every |x| <= n (tol >= 16 eps max(1, n), checked per job): then
+// the two agree for |x| <= n, and for larger |x| (or an infinite one) both
+// call the sample invalid - snapping moves x by at most tol < 1/2, and
+// 16 eps |x| + tol < 1. A NaN fails every comparison, so needs no test.
+__device__ __forceinline__ bool center_coordinate_tol(double& x, int n, double tol) {
+    const double nearest = rint(x);
+    if (fabs(x - nearest) <= tol) x = nearest;
+    return x >= 0 && x <= n - 1.0;
+}
+
 // Bytes p[0 .. n) of an RGB row, n = 6 (two neighbouring pixels) or 3 (one),
 // through aligned dword loads: lo holds p[0..3], hi p[4..7] (bytes past n
 // undefined). A dword that holds no needed byte is not read: its load reads
@@ -100,16 +111,20 @@ __device__ __forceinline__ unsigned byte_of(unsigned w, int k) { return (w >> (8
 // below what the 8-bit output can show.
 // x and y come back snapped, as the border score needs them.
 __device__ bool sample(const std::uint8_t* img, int w, int h, int pitch, double& x, double& y, float* rgb, double* score,
-                       double tol, float* raw = nullptr) {
-    if (!center_coordinate(x, w, tol) || !center_coordinate(y, h, tol)) return false;
+                       double tol, float* raw = nullptr, bool tol_snap = false) {
+    if (tol_snap ? !center_coordinate_tol(x, w, tol) || !center_coordinate_tol(y, h, tol)
+                 : !center_coordinate(x, w, tol) || !center_coordinate(y, h, tol))
+        return false;
     const int x0 = static_cast<int>(floor(x)), y0 = static_cast<int>(floor(y));
     const int x1 = x0 < w - 1 ? x0 + 1 : x0, y1 = y0 < h - 1 ? y0 + 1 : y0;
     const float ax = static_cast<float>(x - x0), ay = static_cast<float>(y - y0);
     // Row bytes of pixels x0 and x1 (x1 = x0 at the right border).
     const bool two = x1 != x0;
     unsigned lo0, hi0, lo1, hi1;
-    load_rgb<true>(img + static_cast<size_t>(y0) * pitch + 3 * x0, two, lo0, hi0);
-    load_rgb<true>(img + static_cast<size_t>(y1) * pitch + 3 * x0, two, lo1, hi1);
+    // x0, y0 >= 0 here and every image is below 4 GiB: 32-bit offsets.
+    const unsigned cx = 3u * static_cast<unsigned>(x0);
+    load_rgb<true>(img + (static_cast<unsigned>(y0) * static_cast<unsigned>(pitch) + cx), two, lo0, hi0);
+    load_rgb<true>(img + (static_cast<unsigned>(y1) * static_cast<unsigned>(pitch) + cx), two, lo1, hi1);
     const unsigned q00[3] = {byte_of(lo0, 0), byte_of(lo0, 1), byte_of(lo0, 2)};
     const unsigned q10[3] = {byte_of(lo1, 0), byte_of(lo1, 1), byte_of(lo1, 2)};
     const unsigned q01[3] = {two ? byte_of(lo0, 3) : q00[0], two ? byte_of(hi0, 0) : q00[1], two ? byte_of(hi0, 1) : q00[2]};
@@ -271,7 +286,7 @@ __device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int
         if (isfinite(den) && dsc > 0 && fabs(den) > j.denominator_tolerance * dsc) {
             xb = (inv[0] * xa + r.ty0 + inv[2]) / den;
             yb = (inv[3] * xa + r.ty1 + inv[5]) / den;
-            mb = sample(j.b, j.wb, j.hb, j.pitch_b, xb, yb, rb, nullptr, j.snap_tolerance, vb);
+            mb = sample(j.b, j.wb, j.hb, j.pitch_b, xb, yb, rb, nullptr, j.snap_tolerance, vb, j.tol_snap);
         }
     }
     if (j.top >= 0) {
@@ -1823,6 +1838,8 @@ Status make_s6_job(const ImageView& a, const ImageView& b, const s6::Geometry& g
     j.inv_wa = 1.0 / a.width;
     j.inv_wb = 1.0 / b.width;
     j.inv_hb = 1.0 / b.height;
+    // The snap threshold is the tolerance itself over B's whole extent.
+    j.tol_snap = g.snap_tolerance >= 16 * 2.2204460492503131e-16 * std::max(1.0, static_cast<double>(std::max(b.width, b.height)));
     std::memcpy(j.inverse, g.inverse, sizeof j.inverse);
     int box[4];
     b_box(g, box, j.bi);

@@ -50,6 +50,7 @@ struct S6Job {
     int width, height, origin_x, origin_y;
     double denominator_tolerance, snap_tolerance;
     double inv_wa, inv_wb, inv_hb;  // 1 / wa, 1 / wb, 1 / hb, correctly rounded
+    bool tol_snap;                  // snap_tolerance >= 16 eps max(1, wb, hb): B's snap threshold is the tolerance
     double inverse[9];
     std::uint8_t* out;
     std::uint8_t* mask;

@@ -109,3 +109,54 @@ def test_fast_ownership_agrees_with_the_quotients_or_defers():
         else:
             deferred += 1
     assert deferred < 20000 // 3 + 500  # mostly the planted ties
+
+
+# B's sample position snapping (crossstage-stitch_amd/csrc/s6.hip): the
+# reference's rule (src/s6/s6.cu:148-155) snaps x to the nearest integer when
+# within max(tol, 16 eps max(1, |x|)) and then requires 0 <= x <= n - 1. The
+# warp's short form uses tol alone when tol >= 16 eps max(1, n); the two must
+# give the same validity and snapped value wherever the short form is used.
+EPS = 2.2204460492503131e-16
+
+
+def _snap_full(x, n, tol):
+    if not np.isfinite(x):
+        return False, x
+    nearest = np.rint(x)
+    roundoff = 16 * EPS * max(1.0, abs(x))
+    if abs(x - nearest) <= max(tol, roundoff):
+        x = nearest
+    return (x >= 0 and x <= n - 1.0), x
+
+
+def _snap_tol(x, n, tol):
+    with np.errstate(invalid="ignore"):
+        nearest = np.rint(x)
+        if abs(x - nearest) <= tol:
+            x = nearest
+        return bool(x >= 0 and x <= n - 1.0), x
+
+
+def test_short_snap_agrees_with_the_reference_rule():
+    rng = np.random.default_rng(11)
+    checked = 0
+    for n in (1, 2, 480, 1080, 3840, 16384, 28000):
+        for tol in (1e-10, 1e-6, 0.25, 0.49, 16 * EPS * max(1.0, n)):
+            if not tol >= 16 * EPS * max(1.0, float(n)):
+                continue
+            xs = [0.0, -0.0, n - 1.0, float(n), -tol, n - 1.0 + tol, -2 * tol, n - 1.0 + 2 * tol,
+                  np.nextafter(-tol, -1.0), np.nextafter(n - 1 + tol, np.inf), np.inf, -np.inf, np.nan,
+                  1e300, -1e300, n * 1.5, float(n) + 0.49]
+            ks = rng.integers(-3, n + 3, 400).astype(np.float64)
+            for k in ks:
+                for d in (0.0, tol, -tol, np.nextafter(tol, 1.0), -np.nextafter(tol, 1.0), 0.5, 1e-3):
+                    xs.append(k + d)
+            xs += list(rng.uniform(-n, 2 * n, 400))
+            for x in xs:
+                vf, xf = _snap_full(float(x), n, tol)
+                vt, xt = _snap_tol(float(x), n, tol)
+                assert vf == vt, (n, tol, x)
+                if vf:
+                    assert xf == xt, (n, tol, x)
+                checked += 1
+    assert checked > 10000
