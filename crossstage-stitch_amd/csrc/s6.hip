@@ -193,7 +193,9 @@ __device__ __forceinline__ float unit_byte(unsigned w, int k) {
 }
 
 __device__ __forceinline__ const std::uint8_t* a_pixel(const S6Job& j, int x, int y) {
-    return j.a + static_cast<size_t>(y + j.origin_y) * j.pitch_a + 3 * (x + j.origin_x);
+    // A's pixel lies inside A here; every image is below 4 GiB.
+    return j.a + (static_cast<unsigned>(y + j.origin_y) * static_cast<unsigned>(j.pitch_a) +
+                  3u * static_cast<unsigned>(x + j.origin_x));
 }
 
 __device__ __forceinline__ void s6_clear_stats_job(const S6Job& j) {
@@ -270,9 +272,12 @@ __device__ __forceinline__ bool a_owns(const S6Job& j, const WarpRow& r, double 
     return sa >= sb;
 }
 
-__device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int x, int y,
-                                           unsigned long long& uni, unsigned long long& ovl) {
-    const size_t p = static_cast<size_t>(y) * j.width + x;
+__device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int x, int y, unsigned& uni,
+                                           unsigned& ovl) {
+    // Every plane and image is below 4 GiB: 32-bit offsets from their bases.
+    const unsigned p = static_cast<unsigned>(y) * static_cast<unsigned>(j.width) + static_cast<unsigned>(x);
+    const unsigned pm = static_cast<unsigned>(y) * static_cast<unsigned>(j.mask_pitch) + static_cast<unsigned>(x);
+    const unsigned po = static_cast<unsigned>(y) * static_cast<unsigned>(j.out_pitch) + 3u * static_cast<unsigned>(x);
     const int ax = x + j.origin_x, ay = y + j.origin_y;
     const double xa = static_cast<double>(ax), ya = r.ya;
     float ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0}, vb[3] = {0, 0, 0};
@@ -301,9 +306,9 @@ __device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int
         if (mb) reinterpret_cast<unsigned long long*>(L.b)[p] = pack_b(vb);
         // Single-source pixels keep their warped colour (s6.cu:294-306):
         // written here; the band mix writes the overlap.
-        j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = ma || mb ? 1 : 0;
+        j.mask[pm] = ma || mb ? 1 : 0;
         if (!(ma && mb)) {
-            std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
+            std::uint8_t* out = j.out + po;
             if (ma) {
                 // A's level-0 colour is float(v / 255), which quantises back
                 // to v for every byte v: the output is A's bytes.
@@ -329,14 +334,14 @@ __device__ __forceinline__ void warp_pixel(const S6Job& j, const WarpRow& r, int
             ra[1] = unit_byte(lo, 1);
             ra[2] = unit_byte(lo, 2);
         }
-        std::uint8_t* out = j.out + static_cast<size_t>(y) * j.out_pitch + 3 * x;
+        std::uint8_t* out = j.out + po;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float color =
                 ma && mb ? static_cast<float>((sa * ra[c] + sb * rb[c]) / (sa + sb)) : (ma ? ra[c] : rb[c]);
             out[c] = quantize(color);
         }
-        j.mask[static_cast<size_t>(y) * j.mask_pitch + x] = ma || mb ? 1 : 0;
+        j.mask[pm] = ma || mb ? 1 : 0;
     }
     uni += ma || mb;
     ovl += ma && mb;
@@ -350,7 +355,8 @@ __global__ void __launch_bounds__(kBlock) s6_warp(dev::JobSpan<S6Job> span, dev:
     __shared__ unsigned long long red[2][kBlock / 64];
     int cur = -1, ji = 0;
     const int njobs = span.jobs();
-    unsigned long long uni = 0, ovl = 0;
+    // A thread counts at most four pixels per tile of one canvas: 32 bits.
+    unsigned uni = 0, ovl = 0;
     for (dev::TileWalk w = dev::tile_walk(total); w.t < w.end; w.t += w.step) {
         const long long t = w.t;
         ji = next_job_by(span, t, [](const S6Job& j) -> const long long& { return j.warp_begin; }, ji, njobs);
