@@ -1,7 +1,7 @@
 """Writes the golden vectors under tests/golden/ from the reference's own NumPy
 oracles. Run in a checkout where the reference repository is available:
 
-    python tests/golden/make_golden.py --reference /root/reference
+    python tests/golden/make_golden.py --reference /root/reference [--only s6]
 
 It imports two numpy-only modules of the reference,
 tools/s2_s3/reference.py (single-level ORB: detect + describe) and
@@ -120,9 +120,26 @@ def s6_golden(ref_dir: Path):
         ("textured_overlap", tex_a, tex_b,
          np.array([[0.99, 0.05, 31.5], [-0.04, 1.01, 6.25], [0.0002, -0.0001, 1.]])),
     ]
+    shallow = ("feather", "multiband_3", "multiband_5")
+    cases = [(name, a, b, h, shallow) for name, a, b, h in cases]
+    # Larger textured canvases (about 150 x 100, seven or more levels above
+    # level 0) with a projective and a rotated map. The reference's NumPy
+    # renderer implements feather, multiband_3 and multiband_5 only.
+
+    def textured(h, w, cell):
+        t = np.clip(_smooth_noise(rng, h, w, cell)[..., None] + rng.uniform(-30, 30, (h, w, 3)), 0, 255)
+        return np.rint(t).astype(np.uint8)
+
+    th = 0.3
+    cases += [
+        ("deep_projective", textured(80, 112, 9), textured(76, 108, 8),
+         np.array([[1.01, 0.04, 47.25], [-0.03, 0.99, 9.5], [0.0003, -0.0002, 1.]]), shallow),
+        ("deep_rotated", textured(96, 96, 10), textured(90, 100, 7),
+         np.array([[np.cos(th), -np.sin(th), 58.0], [np.sin(th), np.cos(th), -21.5], [0.0, 0.0, 1.]]), shallow),
+    ]
     out = {}
-    for name, a, b, h in cases:
-        for mode in ("feather", "multiband_3", "multiband_5"):
+    for name, a, b, h, modes in cases:
+        for mode in modes:
             r = mod.render(a, b, h, mode)
             g = r.geometry
             key = f"{name}__{mode}"
@@ -138,11 +155,15 @@ def s6_golden(ref_dir: Path):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
+    ap.add_argument("--only", choices=("s2s3", "s6"), help="write one of the two files")
     args = ap.parse_args()
     ref = Path(args.reference)
-    np.savez_compressed(HERE / "s2s3_golden.npz", **s2s3_golden(ref))
-    np.savez_compressed(HERE / "s6_golden.npz", **s6_golden(ref))
-    print("wrote", HERE / "s2s3_golden.npz", HERE / "s6_golden.npz")
+    if args.only != "s6":
+        np.savez_compressed(HERE / "s2s3_golden.npz", **s2s3_golden(ref))
+        print("wrote", HERE / "s2s3_golden.npz")
+    if args.only != "s2s3":
+        np.savez_compressed(HERE / "s6_golden.npz", **s6_golden(ref))
+        print("wrote", HERE / "s6_golden.npz")
 
 
 if __name__ == "__main__":

@@ -215,7 +215,7 @@ def test_s6_golden_vectors(T):
         assert np.array_equal(mask, omask), base
         assert_within_one_lsb(out, oout, base)
         n += 1
-    assert n == 27
+    assert n == 33
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
