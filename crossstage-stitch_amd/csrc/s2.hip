@@ -31,6 +31,9 @@ constexpr int kEdge = s2::kEdge;
 constexpr int kHalo = 4;  // 3 for the FAST circle + 1 for the NMS neighbourhood
 constexpr int kGrayW = kS2TileW + 2 * kHalo, kGrayH = kS2TileH + 2 * kHalo;
 constexpr int kScoreW = kS2TileW + 2, kScoreH = kS2TileH + 2;
+// Score rows in LDS: pitch 72 bytes, score column sx at byte sx + 3, so the
+// tile's pixel x0 + 4t starts word t + 1 (NMS reads whole words).
+constexpr int kScorePitch = kS2TileW + 8, kScoreOff = 3;
 
 __constant__ int kCircleX[16] = {0, 1, 2, 3, 3, 3, 2, 1, 0, -1, -2, -3, -3, -3, -2, -1};
 __constant__ int kCircleY[16] = {3, 3, 2, 1, 0, -1, -2, -3, -3, -3, -2, -1, 0, 1, 2, 3};
@@ -109,7 +112,7 @@ __device__ int fast_score_lds(const std::uint8_t* g, int threshold) {
 __global__ void __launch_bounds__(256) s2_fast_nms(dev::JobSpan<S2Job> span, dev::Total work) {
     const long long total = work.get();
     __shared__ __align__(4) std::uint8_t gray[kGrayH * kGrayW];
-    __shared__ std::uint8_t score[kScoreH * kScoreW];
+    __shared__ __align__(16) std::uint8_t score[kScoreH * kScorePitch];
     __shared__ std::uint32_t hist[256];
     const int tid = threadIdx.x;
     int ji = 0;
@@ -174,23 +177,57 @@ __global__ void __launch_bounds__(256) s2_fast_nms(dev::JobSpan<S2Job> span, dev
             const bool inside = (gx >= 3) & (gx < w - 3) & (gy >= 3) & (gy < h - 3);
             int s = 0;
             if (inside & fast_candidate(g, thr)) s = fast_score_lds(g, thr);
-            if (i < kScoreH * kScoreW) score[i] = static_cast<std::uint8_t>(s);
+            if (i < kScoreH * kScoreW) score[sy * kScorePitch + sx + kScoreOff] = static_cast<std::uint8_t>(s);
         }
         __syncthreads();
-        static_assert(kS2TileW * kS2TileH % 256 == 0, "NMS trips");
+        // Strict 3x3 maxima on strips of four pixels: one word of scores per
+        // strip; where all four are 0 (most strips) the strip writes zeros,
+        // otherwise its eight neighbouring words are read and each pixel is
+        // tested against its eight neighbours.
+        constexpr int kStripsX = kS2TileW / 4;
+        static_assert(kStripsX * kS2TileH % 256 == 0, "NMS trips");
+        const unsigned* sw = reinterpret_cast<const unsigned*>(score);
+        constexpr int kRowWords = kScorePitch / 4;
         #pragma unroll 1
-        for (int base = 0; base < kS2TileW * kS2TileH; base += 256) {
+        for (int base = 0; base < kStripsX * kS2TileH; base += 256) {
             const int i = base + tid;
-            const int lx = i % kS2TileW, ly = i / kS2TileW;
-            const int gx = x0 + lx, gy = y0 + ly;
-            const std::uint8_t* c = &score[(ly + 1) * kScoreW + lx + 1];
-            const int s = c[0];
-            bool keep = (s > 0) & (gx >= kEdge) & (gx < w - kEdge) & (gy >= kEdge) & (gy < h - kEdge);
-            if (keep)
-                keep = s > c[-kScoreW - 1] && s > c[-kScoreW] && s > c[-kScoreW + 1] && s > c[-1] && s > c[1] &&
-                       s > c[kScoreW - 1] && s > c[kScoreW] && s > c[kScoreW + 1];
-            if ((gx < w) & (gy < h)) maxima[static_cast<size_t>(gy) * w + gx] = keep ? static_cast<std::uint8_t>(s) : 0;
-            if (keep) atomicAdd(&hist[s], 1u);
+            const int t = i % kStripsX, ly = i / kStripsX;
+            const int gx = x0 + 4 * t, gy = y0 + ly;
+            const int o = (ly + 1) * kRowWords + t + 1;  // the strip's word in its score row
+            const unsigned m = sw[o];
+            unsigned kept = 0;
+            if (m != 0) {
+                const unsigned ml = sw[o - 1], mr = sw[o + 1];
+                const unsigned u = sw[o - kRowWords], ul = sw[o - kRowWords - 1], ur = sw[o - kRowWords + 1];
+                const unsigned d = sw[o + kRowWords], dl = sw[o + kRowWords - 1], dr = sw[o + kRowWords + 1];
+                // Neighbours at x - 1 and x + 1 of the strip's four pixels.
+                const unsigned l1 = __builtin_amdgcn_alignbyte(m, ml, 3), r1 = __builtin_amdgcn_alignbyte(mr, m, 1);
+                const unsigned l0 = __builtin_amdgcn_alignbyte(u, ul, 3), r0 = __builtin_amdgcn_alignbyte(ur, u, 1);
+                const unsigned l2 = __builtin_amdgcn_alignbyte(d, dl, 3), r2 = __builtin_amdgcn_alignbyte(dr, d, 1);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const unsigned sh = 8 * k;
+                    const unsigned s = (m >> sh) & 0xffu;
+                    const auto b = [sh](unsigned v) { return (v >> sh) & 0xffu; };
+                    const bool keep = (s > 0) & (gx + k >= kEdge) & (gx + k < w - kEdge) & (gy >= kEdge) &
+                                      (gy < h - kEdge) & (s > b(l0)) & (s > b(u)) & (s > b(r0)) & (s > b(l1)) &
+                                      (s > b(r1)) & (s > b(l2)) & (s > b(d)) & (s > b(r2));
+                    if (keep) {
+                        kept |= s << sh;
+                        atomicAdd(&hist[s], 1u);
+                    }
+                }
+            }
+            if (gy < h) {
+                std::uint8_t* dst = maxima + static_cast<size_t>(gy) * w + gx;
+                if (gx + 3 < w && (reinterpret_cast<std::uintptr_t>(dst) & 3u) == 0) {
+                    *reinterpret_cast<unsigned*>(dst) = kept;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        if (gx + k < w) dst[k] = static_cast<std::uint8_t>(kept >> (8 * k));
+                }
+            }
         }
         __syncthreads();
         if (hist[tid]) atomicAdd(&j.hist[tid], hist[tid]);
