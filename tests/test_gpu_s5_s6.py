@@ -218,6 +218,33 @@ def test_s6_golden_vectors(T):
     assert n == 33
 
 
+@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("snap", [0.0, 0.25])
+def test_s6_render_snap_tolerances(T, mode, snap):
+    """B's sample positions snapped under the reference's rule with other
+    tolerances than the default: 0 (the threshold is the FP64 round-off term,
+    the warp's general path) and 0.25 (every position within a quarter pixel
+    of an integer snaps; the warp's short form, which holds when the
+    tolerance exceeds 16 eps max(1, W, H)). A slight scale and a fractional
+    shift put positions at every distance from the integers."""
+    torch, st = T
+    h, w = 120, 160
+    a = images.texture_rgb(70 + mode, h, w)
+    b = images.texture_rgb(90 + mode, h, w)
+    H = np.array([[1.0003, 0.0, 0.45 * w + 0.125], [0.0, 0.9998, 0.07 * h - 0.375], [0.0, 0.0, 1.0]])
+    pol = st.F.GeometryPolicy(1e-12, 1e-12, snap, 16777216)
+    status, g = st.prepare_geometry(w, h, w, h, H, pol)
+    ost, og = O.prepare_geometry(w, h, w, h, H, snap_tol=snap)
+    assert status == ost == 0
+    assert g.snap_tolerance == og.snap_tolerance == snap and list(g.inverse) == list(og.inverse)
+    out, mask, counters = st.s6_render(dev_rgb(torch, st, a), dev_rgb(torch, st, b, pad=5), g, mode)
+    torch.cuda.synchronize()
+    oout, omask, ostats = O.s6_render(a, b, og, mode)
+    assert np.array_equal(mask.cpu().numpy(), omask)
+    assert_within_one_lsb(out.cpu().numpy(), oout, (mode, snap))
+    assert tuple(int(v) for v in counters.cpu()) == ostats
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("size", [(120, 160), (480, 640), (1080, 1920)])
 def test_s6_render_matches_oracle(T, mode, size):
