@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 
 #include "stitch/jobs.hpp"
 
@@ -168,17 +169,26 @@ __global__ void __launch_bounds__(256) s2_fast_nms(dev::JobSpan<S2Job> span, dev
         // the candidate test runs on every lane (indexes clamped into the
         // tile, conditions combined without branches) and the full score only
         // where it passed.
-        #pragma unroll 1
-        for (int base = 0; base < kScoreH * kScoreW; base += 256) {
-            const int i = base + tid, ic = min(i, kScoreH * kScoreW - 1);
-            const int sx = ic % kScoreW, sy = ic / kScoreW;
-            const int gx = x0 - 1 + sx, gy = y0 - 1 + sy;
-            const std::uint8_t* g = &gray[(sy + kHalo - 1) * kGrayW + sx + kHalo - 1];
-            const bool inside = (gx >= 3) & (gx < w - 3) & (gy >= 3) & (gy < h - 3);
-            int s = 0;
-            if (inside & fast_candidate(g, thr)) s = fast_score_lds(g, thr);
-            if (i < kScoreH * kScoreW) score[sy * kScorePitch + sx + kScoreOff] = static_cast<std::uint8_t>(s);
-        }
+        // A tile whose scored pixels all lie 3 or more pixels inside the
+        // image (most tiles) needs no per-pixel bounds test (a block-uniform
+        // branch between two copies of the pass).
+        const auto score_pass = [&](auto all_inside) {
+            #pragma unroll 1
+            for (int base = 0; base < kScoreH * kScoreW; base += 256) {
+                const int i = base + tid, ic = min(i, kScoreH * kScoreW - 1);
+                const int sx = ic % kScoreW, sy = ic / kScoreW;
+                const int gx = x0 - 1 + sx, gy = y0 - 1 + sy;
+                const std::uint8_t* g = &gray[(sy + kHalo - 1) * kGrayW + sx + kHalo - 1];
+                const bool inside = decltype(all_inside)::value || ((gx >= 3) & (gx < w - 3) & (gy >= 3) & (gy < h - 3));
+                int s = 0;
+                if (inside & fast_candidate(g, thr)) s = fast_score_lds(g, thr);
+                if (i < kScoreH * kScoreW) score[sy * kScorePitch + sx + kScoreOff] = static_cast<std::uint8_t>(s);
+            }
+        };
+        if (x0 - 1 >= 3 && x0 + kS2TileW < w - 3 && y0 - 1 >= 3 && y0 + kS2TileH < h - 3)
+            score_pass(std::true_type{});
+        else
+            score_pass(std::false_type{});
         __syncthreads();
         // Strict 3x3 maxima on strips of four pixels: one word of scores per
         // strip; where all four are 0 (most strips) the strip writes zeros,
